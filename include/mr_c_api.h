@@ -1,0 +1,224 @@
+/*
+ * Flat C ABI of the MI355X-native batched temporalCone stepper.
+ *
+ * Drop-in boundary for the stepping subset of the reference engine ABI
+ * (include/metalrobo/c_api.h, implemented in src/c_api.cpp). Conventions
+ * follow the reference:
+ *   - int return: 0 success, -1 failure; constructors return NULL on failure;
+ *   - every entry point catches all exceptions and stores the message in a
+ *     thread-local error string read with mr_last_error (c_api.cpp:183, 321-332);
+ *   - caller spans are copied before return; returned pointers alias
+ *     handle-owned memory valid until the next advance/reset/destroy
+ *     (c_api.h:562-563, 680-681);
+ *   - a handle is not internally serialized for concurrent advance calls.
+ * Only plain pointers and sizes cross this boundary; device pointers are
+ * accepted by the *_device entry points for inputs already resident in HBM.
+ */
+#ifndef MR_C_API_H
+#define MR_C_API_H
+
+#include "mr_world_types.h"
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------- library --- */
+
+/* c_api.h:553 mr_version */
+const char* mr_version(void);
+/* c_api.h:554 mr_runtime_abi_fingerprint */
+uint64_t mr_runtime_abi_fingerprint(void);
+/* c_api.h:555-557 mr_last_error (thread-local, empty string when none) */
+const char* mr_last_error(void);
+
+/* ------------------------------------------------------------- models --- */
+
+typedef struct MRWorldModelHandle MRWorldModelHandle;
+typedef struct MRWorldHandle MRWorldHandle;
+
+enum MRSyntheticWorldKindC {
+    MR_SYNTHETIC_CHAIN_BOXES = 1u,     /* 7-DoF chain + 20 boxes on a plane */
+    MR_SYNTHETIC_HUMANOID_TERRAIN = 2u /* floating 29-DoF humanoid on a heightfield */
+};
+
+typedef struct MRSyntheticWorldConfigC {
+    uint32_t kind;              /* MRSyntheticWorldKindC */
+    uint32_t environment_count;
+    uint64_t seed;
+} MRSyntheticWorldConfigC;
+
+/* Builds a seeded synthetic world set (one model per environment for the
+ * randomized chain worlds, one shared model for the humanoid). Replaces the
+ * robot-factory sources of MRRunManifestC (c_api.h:160-166) for the
+ * benchmark worlds. */
+MRWorldModelHandle* mr_world_model_create_synthetic(const MRSyntheticWorldConfigC* config);
+/* Copies a caller-described model (all arrays are copied). `microstep` is
+ * the physics timestep used to precompute per-body damping scales. */
+MRWorldModelHandle* mr_world_model_create(const MRWorldModelDescC* desc, float microstep);
+void mr_world_model_destroy(MRWorldModelHandle* model);
+/* Pointers in *out alias handle-owned arrays valid until destroy. */
+int mr_world_model_describe(const MRWorldModelHandle* model, MRWorldModelDescC* out);
+/* Initial state arrays: q [E][max_nq], v [E][max_nv], scene [E][max_scene_bodies]. */
+int mr_world_model_initial_state(const MRWorldModelHandle* model, float* q, size_t q_count, float* v,
+                                 size_t v_count, MRBodyStateC* scene, size_t scene_count);
+/* Seeded synthetic command stream [steps][E][max_nv] for control steps
+ * [first_step, first_step + step_count): efforts N(0, 5 N m) for the chain
+ * worlds, PD targets standing pose + U[-0.3, 0.3] rad for the humanoid. */
+int mr_world_model_commands(const MRWorldModelHandle* model, uint32_t first_step, uint32_t step_count, float* out,
+                            size_t count);
+int mr_world_model_default_config(const MRWorldModelHandle* model, MRWorldStepConfigC* out);
+
+/* ------------------------------------------------- world (executor) --- */
+
+/* Physics-only mirror of MetalWorldContext (MetalWorld.hpp:759-818):
+ * compileMetalWorld + initializeResidentState + submitResident + wait. */
+typedef struct MRWorldAdvanceC {
+    uint32_t control_step_count;
+    uint32_t successful_environment_steps;
+    uint32_t failed_environment_steps;
+    uint32_t first_failing_environment;
+    uint32_t first_failing_control_step;
+    uint32_t first_gpu_status_code;
+    uint32_t maximum_active_pairs;
+    uint32_t maximum_constraints;
+    double gpu_milliseconds;
+} MRWorldAdvanceC;
+
+typedef struct MRWorldLayoutC {
+    uint32_t environment_count;
+    uint32_t max_nq;
+    uint32_t max_nv;
+    uint32_t max_scene_bodies;
+    uint32_t pair_capacity;
+    uint32_t manifold_capacity;
+    uint32_t constraint_capacity;
+    uint32_t lds_bytes_per_environment;
+    uint64_t retained_device_bytes;
+} MRWorldLayoutC;
+
+/* compileMetalWorld (MetalWorld.mm:14302): validates the model, allocates the
+ * resident arena and uploads the model's initial state. */
+MRWorldHandle* mr_world_create(const MRWorldModelHandle* model, const MRWorldStepConfigC* config);
+void mr_world_destroy(MRWorldHandle* world);
+int mr_world_layout(const MRWorldHandle* world, MRWorldLayoutC* out);
+/* initializeResidentState (MetalWorld.mm:16409): replaces q/v/scene, clears manifolds. */
+int mr_world_initialize_resident_state(MRWorldHandle* world, const float* q, size_t q_count, const float* v,
+                                       size_t v_count, const MRBodyStateC* scene, size_t scene_count);
+/* Reset state applied where a reset mask is set (MetalWorldBatch resetQ/resetV). */
+int mr_world_set_reset_state(MRWorldHandle* world, const float* q, size_t q_count, const float* v, size_t v_count,
+                             const MRBodyStateC* scene, size_t scene_count);
+/* Replaces the persistent manifolds ([E][manifold_capacity] headers, [E][capacity][4] points, [E] counts). */
+int mr_world_write_manifolds(MRWorldHandle* world, const MRManifoldHeaderC* headers, const MRManifoldPointC* points,
+                             const uint32_t* counts);
+/* submitResident + wait (MetalWorld.mm:16409-16441, 15976): commands
+ * [steps][E][max_nv] (efforts or PD targets), optional reset masks [steps][E]. */
+int mr_world_advance(MRWorldHandle* world, const float* commands, size_t command_count, const uint32_t* reset_masks,
+                     size_t mask_count, uint32_t control_step_count, MRWorldAdvanceC* out);
+/* Same with device pointers already resident in HBM; enqueues without waiting. */
+int mr_world_enqueue_device(MRWorldHandle* world, const float* device_commands, const uint32_t* device_reset_masks,
+                            uint32_t control_step_count);
+int mr_world_wait(MRWorldHandle* world, MRWorldAdvanceC* out);
+int mr_world_read_state(const MRWorldHandle* world, float* q, size_t q_count, float* v, size_t v_count,
+                        MRBodyStateC* scene, size_t scene_count);
+int mr_world_read_manifolds(const MRWorldHandle* world, MRManifoldHeaderC* headers, MRManifoldPointC* points,
+                            uint32_t* counts);
+/* Statuses of the last advance: [steps][E] each. */
+int mr_world_read_statuses(const MRWorldHandle* world, MRWorldStatusC* out, size_t count);
+int mr_world_read_contact_statuses(const MRWorldHandle* world, MRContactStatusC* out, size_t count);
+/* ABA acceleration of the final microstep of the last control step, zero for failed environments. */
+int mr_world_read_acceleration(const MRWorldHandle* world, float* out, size_t count);
+/* Active broadphase pairs of the captured microstep: [E][pair_capacity], counts [E]. */
+int mr_world_read_evidence(const MRWorldHandle* world, MRPairRecordC* pairs, size_t pair_count, uint32_t* counts,
+                           size_t count_count);
+/* Stream the world launches on (hipStream_t), for callers that order work around it. */
+void* mr_world_stream(const MRWorldHandle* world);
+/* Average duration of the last advance's control-step launches (ms) measured with HIP events. */
+double mr_world_last_kernel_milliseconds(const MRWorldHandle* world);
+
+/* ------------------------------------------------- task rollout --- */
+
+typedef struct MRTaskRolloutHandle MRTaskRolloutHandle;
+
+/* c_api.h:62-88 (subset) */
+typedef struct MRTaskRolloutConfigC {
+    uint32_t environment_count;
+    uint32_t physics_substeps;
+    uint32_t velocity_iterations;
+    uint32_t final_velocity_iterations;
+    float control_timestep_seconds;
+    uint32_t reserved;
+    uint64_t seed;
+} MRTaskRolloutConfigC;
+
+enum MRRunManifestSourceC {
+    MR_RUN_SOURCE_SYNTHETIC_CHAIN_BOXES = 100u,
+    MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN = 101u
+};
+
+/* c_api.h:171-190 (subset): a profile plus the world source. */
+typedef struct MRRunManifestC {
+    MRTaskRolloutConfigC profile;
+    uint32_t source;
+    uint32_t reserved;
+} MRRunManifestC;
+
+/* c_api.h:128-158 (subset) */
+typedef struct MRTaskRolloutLayoutC {
+    uint32_t environment_count;
+    uint32_t nq;
+    uint32_t nv;
+    uint32_t action_count;
+    uint32_t scene_body_count;
+    uint32_t reserved;
+    uint64_t submitted_control_steps;
+    uint64_t completed_environment_steps;
+    uint64_t submission_count;
+    size_t retained_buffer_bytes;
+    double total_gpu_milliseconds;
+} MRTaskRolloutLayoutC;
+
+/* c_api.h:222-235 (subset) */
+typedef struct MRTaskRolloutAdvanceC {
+    uint32_t control_step_count;
+    uint32_t successful_environment_steps;
+    uint32_t failed_environment_steps;
+    uint32_t first_failing_environment;
+    uint32_t first_failing_control_step;
+    uint32_t first_gpu_status_code;
+    uint32_t host_requested_resets;
+    uint32_t maximum_active_contacts;
+    double gpu_milliseconds;
+} MRTaskRolloutAdvanceC;
+
+/* c_api.h:583-585 */
+MRTaskRolloutHandle* mr_create_task_rollout(const MRRunManifestC* manifest);
+/* c_api.h:586 */
+void mr_task_rollout_destroy(MRTaskRolloutHandle* handle);
+/* c_api.h:587-590: restores the seeded initial state of every environment. */
+int mr_task_rollout_reset(MRTaskRolloutHandle* handle, uint64_t seed);
+/* c_api.h:631-634 */
+int mr_task_rollout_set_state_readback(MRTaskRolloutHandle* handle, uint32_t enabled);
+/* c_api.h:635-645: normalized actions [steps][envs][actions] in [-1, 1],
+ * optional reset masks [steps][envs]. */
+int mr_task_rollout_advance(MRTaskRolloutHandle* handle, const float* normalized_actions, size_t action_count,
+                            const uint32_t* reset_masks, size_t mask_count, uint32_t control_step_count,
+                            uint64_t policy_revision, uint32_t evaluate_final_policy, MRTaskRolloutAdvanceC* out);
+/* c_api.h:646-648 */
+MRTaskRolloutLayoutC mr_task_rollout_layout(const MRTaskRolloutHandle* handle);
+/* c_api.h:682-687: [steps][envs] of the last advance. */
+const uint32_t* mr_task_rollout_status_codes(const MRTaskRolloutHandle* handle);
+const uint32_t* mr_task_rollout_active_contacts(const MRTaskRolloutHandle* handle);
+/* c_api.h:736-738: [envs][nq] after the last advance (state readback enabled). */
+const float* mr_task_rollout_final_q(const MRTaskRolloutHandle* handle);
+/* c_api.h:739-746: [env][body][pos3, quat4, linv3, angv3]. */
+int mr_task_rollout_copy_final_scene_states(const MRTaskRolloutHandle* handle, float* out, size_t count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MR_C_API_H */

@@ -1,0 +1,520 @@
+// extern "C" boundary: handle ownership, argument validation and exception
+// translation (src/c_api.cpp:98-134, 183, 321-332 conventions).
+#include "../../include/mr_c_api.h"
+
+#include "mr_builder.h"
+#include "mr_world.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+struct MRWorldModelHandle {
+    mr::ModelData model;
+};
+
+struct MRWorldHandle {
+    std::unique_ptr<mr::World> world;
+};
+
+struct MRTaskRolloutHandle {
+    mr::ModelData model;
+    std::unique_ptr<mr::World> world;
+    uint32_t action_count = 0;
+    float action_scale = 1.0f;
+    bool state_readback = true;
+    std::vector<uint32_t> status_codes;
+    std::vector<uint32_t> active_contacts;
+    std::vector<float> final_q;
+    std::vector<MRBodyStateC> final_scene;
+    uint64_t submitted_control_steps = 0;
+    uint64_t completed_environment_steps = 0;
+    uint64_t submission_count = 0;
+    double total_gpu_ms = 0.0;
+};
+
+namespace {
+
+thread_local std::string gLastError;
+
+template <typename F>
+int translate(F&& f) {
+    try {
+        gLastError.clear();
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        gLastError = e.what();
+    } catch (...) {
+        gLastError = "unknown native exception";
+    }
+    return -1;
+}
+
+template <typename T, typename F>
+T* construct(F&& f) {
+    try {
+        gLastError.clear();
+        return f();
+    } catch (const std::exception& e) {
+        gLastError = e.what();
+    } catch (...) {
+        gLastError = "unknown native exception";
+    }
+    return nullptr;
+}
+
+void require(bool c, const char* msg) {
+    if (!c) throw std::runtime_error(msg);
+}
+
+void require_count(size_t got, size_t want, const char* what) {
+    if (got != want)
+        throw std::runtime_error(std::string(what) + ": expected " + std::to_string(want) + " elements, got " +
+                                 std::to_string(got));
+}
+
+void fill_advance(const mr::AdvanceSummary& s, MRWorldAdvanceC* out) {
+    if (out == nullptr) return;
+    out->control_step_count = s.control_step_count;
+    out->successful_environment_steps = s.successful_environment_steps;
+    out->failed_environment_steps = s.failed_environment_steps;
+    out->first_failing_environment = s.first_failing_environment;
+    out->first_failing_control_step = s.first_failing_control_step;
+    out->first_gpu_status_code = s.first_gpu_status_code;
+    out->maximum_active_pairs = s.maximum_active_pairs;
+    out->maximum_constraints = s.maximum_constraints;
+    out->gpu_milliseconds = s.gpu_milliseconds;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* mr_version(void) { return "numi-lab_amd 0.1.0 (gfx950 temporalCone)"; }
+
+uint64_t mr_runtime_abi_fingerprint(void) {
+    uint64_t h = 1469598103934665603ull;
+    const uint64_t parts[] = {MR_WORLD_ABI_VERSION, sizeof(MRBodyC), sizeof(MRShapeC), sizeof(MRBodyStateC),
+                              sizeof(MRManifoldHeaderC), sizeof(MRManifoldPointC), sizeof(MRWorldStatusC),
+                              sizeof(MRContactStatusC), sizeof(MRWorldStepConfigC)};
+    for (uint64_t p : parts) {
+        h ^= p;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+const char* mr_last_error(void) { return gLastError.c_str(); }
+
+MRWorldModelHandle* mr_world_model_create_synthetic(const MRSyntheticWorldConfigC* config) {
+    return construct<MRWorldModelHandle>([&]() {
+        require(config != nullptr, "config is null");
+        auto* h = new MRWorldModelHandle();
+        try {
+            h->model = mr::build_synthetic(config->kind, config->environment_count, config->seed);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        return h;
+    });
+}
+
+MRWorldModelHandle* mr_world_model_create(const MRWorldModelDescC* desc, float microstep) {
+    return construct<MRWorldModelHandle>([&]() {
+        require(desc != nullptr, "desc is null");
+        require(microstep > 0.0f, "microstep must be positive");
+        auto* h = new MRWorldModelHandle();
+        try {
+            h->model = mr::copy_model(*desc);
+            mr::finalize_model(h->model, microstep);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        return h;
+    });
+}
+
+void mr_world_model_destroy(MRWorldModelHandle* model) { delete model; }
+
+int mr_world_model_describe(const MRWorldModelHandle* model, MRWorldModelDescC* out) {
+    return translate([&]() {
+        require(model != nullptr && out != nullptr, "null argument");
+        *out = model->model.describe();
+    });
+}
+
+int mr_world_model_initial_state(const MRWorldModelHandle* model, float* q, size_t q_count, float* v,
+                                 size_t v_count, MRBodyStateC* scene, size_t scene_count) {
+    return translate([&]() {
+        require(model != nullptr, "model is null");
+        const mr::ModelData& m = model->model;
+        require(!m.initial_q.empty(), "model has no initial state");
+        require_count(q_count, m.initial_q.size(), "q");
+        require_count(v_count, m.initial_v.size(), "v");
+        require_count(scene_count, m.initial_scene.size(), "scene");
+        std::memcpy(q, m.initial_q.data(), q_count * sizeof(float));
+        std::memcpy(v, m.initial_v.data(), v_count * sizeof(float));
+        if (scene_count) std::memcpy(scene, m.initial_scene.data(), scene_count * sizeof(MRBodyStateC));
+    });
+}
+
+int mr_world_model_commands(const MRWorldModelHandle* model, uint32_t first_step, uint32_t step_count, float* out,
+                            size_t count) {
+    return translate([&]() {
+        require(model != nullptr && out != nullptr, "null argument");
+        const mr::ModelData& m = model->model;
+        require_count(count, (size_t)step_count * m.environment_count() * m.max_nv, "commands");
+        m.commands(first_step, step_count, out);
+    });
+}
+
+int mr_world_model_default_config(const MRWorldModelHandle* model, MRWorldStepConfigC* out) {
+    return translate([&]() {
+        require(model != nullptr && out != nullptr, "null argument");
+        *out = model->model.default_config;
+    });
+}
+
+MRWorldHandle* mr_world_create(const MRWorldModelHandle* model, const MRWorldStepConfigC* config) {
+    return construct<MRWorldHandle>([&]() {
+        require(model != nullptr && config != nullptr, "null argument");
+        auto* h = new MRWorldHandle();
+        try {
+            h->world = std::make_unique<mr::World>(model->model, *config);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        return h;
+    });
+}
+
+void mr_world_destroy(MRWorldHandle* world) { delete world; }
+
+int mr_world_layout(const MRWorldHandle* world, MRWorldLayoutC* out) {
+    return translate([&]() {
+        require(world != nullptr && out != nullptr, "null argument");
+        const mr::World& w = *world->world;
+        out->environment_count = w.environment_count();
+        out->max_nq = w.model().max_nq;
+        out->max_nv = w.model().max_nv;
+        out->max_scene_bodies = w.model().max_scene_bodies;
+        out->pair_capacity = w.config().pair_capacity;
+        out->manifold_capacity = w.config().manifold_capacity;
+        out->constraint_capacity = w.config().constraint_capacity;
+        out->lds_bytes_per_environment = w.lds().total;
+        out->retained_device_bytes = w.retained_bytes();
+    });
+}
+
+int mr_world_initialize_resident_state(MRWorldHandle* world, const float* q, size_t q_count, const float* v,
+                                       size_t v_count, const MRBodyStateC* scene, size_t scene_count) {
+    return translate([&]() {
+        require(world != nullptr && q != nullptr && v != nullptr, "null argument");
+        const mr::ModelData& m = world->world->model();
+        const size_t E = world->world->environment_count();
+        require_count(q_count, E * m.max_nq, "q");
+        require_count(v_count, E * m.max_nv, "v");
+        require_count(scene_count, E * m.max_scene_bodies, "scene");
+        world->world->set_state(q, v, scene);
+    });
+}
+
+int mr_world_set_reset_state(MRWorldHandle* world, const float* q, size_t q_count, const float* v, size_t v_count,
+                             const MRBodyStateC* scene, size_t scene_count) {
+    return translate([&]() {
+        require(world != nullptr && q != nullptr && v != nullptr, "null argument");
+        const mr::ModelData& m = world->world->model();
+        const size_t E = world->world->environment_count();
+        require_count(q_count, E * m.max_nq, "q");
+        require_count(v_count, E * m.max_nv, "v");
+        require_count(scene_count, E * m.max_scene_bodies, "scene");
+        world->world->set_reset_state(q, v, scene);
+    });
+}
+
+int mr_world_write_manifolds(MRWorldHandle* world, const MRManifoldHeaderC* headers, const MRManifoldPointC* points,
+                             const uint32_t* counts) {
+    return translate([&]() {
+        require(world != nullptr && headers != nullptr && points != nullptr && counts != nullptr, "null argument");
+        world->world->set_manifolds(headers, points, counts);
+    });
+}
+
+int mr_world_advance(MRWorldHandle* world, const float* commands, size_t command_count, const uint32_t* reset_masks,
+                     size_t mask_count, uint32_t control_step_count, MRWorldAdvanceC* out) {
+    return translate([&]() {
+        require(world != nullptr && commands != nullptr, "null argument");
+        mr::World& w = *world->world;
+        const size_t E = w.environment_count();
+        require(control_step_count > 0u, "control_step_count must be positive");
+        require_count(command_count, (size_t)control_step_count * E * w.model().max_nv, "commands");
+        if (reset_masks != nullptr) require_count(mask_count, (size_t)control_step_count * E, "reset masks");
+        for (size_t i = 0; i < command_count; ++i)
+            if (!std::isfinite(commands[i])) throw std::runtime_error("nonfinite command input");
+        w.enqueue(commands, reset_masks, control_step_count, false);
+        fill_advance(w.wait(), out);
+    });
+}
+
+int mr_world_enqueue_device(MRWorldHandle* world, const float* device_commands, const uint32_t* device_reset_masks,
+                            uint32_t control_step_count) {
+    return translate([&]() {
+        require(world != nullptr && device_commands != nullptr, "null argument");
+        world->world->enqueue(device_commands, device_reset_masks, control_step_count, true);
+    });
+}
+
+int mr_world_wait(MRWorldHandle* world, MRWorldAdvanceC* out) {
+    return translate([&]() {
+        require(world != nullptr, "null argument");
+        fill_advance(world->world->wait(), out);
+    });
+}
+
+int mr_world_read_state(const MRWorldHandle* world, float* q, size_t q_count, float* v, size_t v_count,
+                        MRBodyStateC* scene, size_t scene_count) {
+    return translate([&]() {
+        require(world != nullptr, "null argument");
+        const mr::ModelData& m = world->world->model();
+        const size_t E = world->world->environment_count();
+        if (q) require_count(q_count, E * m.max_nq, "q");
+        if (v) require_count(v_count, E * m.max_nv, "v");
+        if (scene) require_count(scene_count, E * m.max_scene_bodies, "scene");
+        world->world->get_state(q, v, scene);
+    });
+}
+
+int mr_world_read_manifolds(const MRWorldHandle* world, MRManifoldHeaderC* headers, MRManifoldPointC* points,
+                            uint32_t* counts) {
+    return translate([&]() {
+        require(world != nullptr && headers != nullptr && points != nullptr && counts != nullptr, "null argument");
+        world->world->get_manifolds(headers, points, counts);
+    });
+}
+
+int mr_world_read_statuses(const MRWorldHandle* world, MRWorldStatusC* out, size_t count) {
+    return translate([&]() {
+        require(world != nullptr && out != nullptr, "null argument");
+        const mr::World& w = *world->world;
+        require_count(count, (size_t)w.last_steps() * w.environment_count(), "statuses");
+        w.read_statuses(out, w.last_steps());
+    });
+}
+
+int mr_world_read_contact_statuses(const MRWorldHandle* world, MRContactStatusC* out, size_t count) {
+    return translate([&]() {
+        require(world != nullptr && out != nullptr, "null argument");
+        const mr::World& w = *world->world;
+        require_count(count, (size_t)w.last_steps() * w.environment_count(), "contact statuses");
+        w.read_contact_statuses(out, w.last_steps());
+    });
+}
+
+int mr_world_read_acceleration(const MRWorldHandle* world, float* out, size_t count) {
+    return translate([&]() {
+        require(world != nullptr && out != nullptr, "null argument");
+        const mr::World& w = *world->world;
+        require_count(count, (size_t)w.environment_count() * w.model().max_nv, "acceleration");
+        w.read_acceleration(out);
+    });
+}
+
+int mr_world_read_evidence(const MRWorldHandle* world, MRPairRecordC* pairs, size_t pair_count, uint32_t* counts,
+                           size_t count_count) {
+    return translate([&]() {
+        require(world != nullptr && pairs != nullptr && counts != nullptr, "null argument");
+        const mr::World& w = *world->world;
+        require(w.config().capture_evidence != 0u, "world was created without evidence capture");
+        require_count(pair_count, (size_t)w.environment_count() * w.config().pair_capacity, "evidence pairs");
+        require_count(count_count, (size_t)w.environment_count(), "evidence counts");
+        w.read_evidence(pairs, counts);
+    });
+}
+
+void* mr_world_stream(const MRWorldHandle* world) {
+    return world != nullptr ? static_cast<void*>(world->world->stream()) : nullptr;
+}
+
+double mr_world_last_kernel_milliseconds(const MRWorldHandle* world) {
+    if (world == nullptr || world->world->last_steps() == 0u) return 0.0;
+    return (double)world->world->last_kernel_milliseconds() / (double)world->world->last_steps();
+}
+
+// ------------------------------------------------------------ task rollout
+
+MRTaskRolloutHandle* mr_create_task_rollout(const MRRunManifestC* manifest) {
+    return construct<MRTaskRolloutHandle>([&]() {
+        require(manifest != nullptr, "manifest is null");
+        uint32_t kind = 0;
+        if (manifest->source == MR_RUN_SOURCE_SYNTHETIC_CHAIN_BOXES) kind = mr::kChainBoxes;
+        else if (manifest->source == MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN) kind = mr::kHumanoidTerrain;
+        else throw std::runtime_error("unsupported run source");
+        const MRTaskRolloutConfigC& p = manifest->profile;
+        auto* h = new MRTaskRolloutHandle();
+        try {
+            h->model = mr::build_synthetic(kind, p.environment_count, p.seed);
+            MRWorldStepConfigC cfg = h->model.default_config;
+            if (p.physics_substeps) cfg.physics_substeps = p.physics_substeps;
+            if (p.velocity_iterations) cfg.velocity_iterations = p.velocity_iterations;
+            if (p.final_velocity_iterations) cfg.final_velocity_iterations = p.final_velocity_iterations;
+            if (p.control_timestep_seconds > 0.0f) cfg.control_timestep = p.control_timestep_seconds;
+            h->world = std::make_unique<mr::World>(h->model, cfg);
+            const MRModelHeaderC& m0 = h->model.models[0];
+            h->action_count = m0.root_type == MR_ROOT_FLOATING ? m0.nv - 6u : m0.nv;
+            h->action_scale = kind == mr::kChainBoxes ? 5.0f : 0.3f;
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        return h;
+    });
+}
+
+void mr_task_rollout_destroy(MRTaskRolloutHandle* handle) { delete handle; }
+
+int mr_task_rollout_reset(MRTaskRolloutHandle* handle, uint64_t seed) {
+    return translate([&]() {
+        require(handle != nullptr, "null handle");
+        if (seed != handle->model.seed) {
+            const uint32_t E = handle->model.environment_count();
+            mr::ModelData rebuilt = mr::build_synthetic(handle->model.kind, E, seed);
+            handle->model = rebuilt;
+            handle->world = std::make_unique<mr::World>(handle->model, handle->world->config());
+        } else {
+            handle->world->set_state(handle->model.initial_q.data(), handle->model.initial_v.data(),
+                                     handle->model.initial_scene.data());
+        }
+    });
+}
+
+int mr_task_rollout_set_state_readback(MRTaskRolloutHandle* handle, uint32_t enabled) {
+    return translate([&]() {
+        require(handle != nullptr, "null handle");
+        handle->state_readback = enabled != 0u;
+    });
+}
+
+int mr_task_rollout_advance(MRTaskRolloutHandle* handle, const float* normalized_actions, size_t action_count,
+                            const uint32_t* reset_masks, size_t mask_count, uint32_t control_step_count,
+                            uint64_t policy_revision, uint32_t evaluate_final_policy, MRTaskRolloutAdvanceC* out) {
+    (void)policy_revision;
+    return translate([&]() {
+        require(handle != nullptr && normalized_actions != nullptr, "null argument");
+        require(evaluate_final_policy == 0u, "this build has no policy program");
+        require(control_step_count > 0u, "control_step_count must be positive");
+        mr::World& w = *handle->world;
+        const mr::ModelData& m = handle->model;
+        const size_t E = w.environment_count(), A = handle->action_count, nv = m.max_nv;
+        require_count(action_count, (size_t)control_step_count * E * A, "actions");
+        if (reset_masks != nullptr) require_count(mask_count, (size_t)control_step_count * E, "reset masks");
+        std::vector<float> commands((size_t)control_step_count * E * nv, 0.0f);
+        const uint32_t first = m.models[0].root_type == MR_ROOT_FLOATING ? 6u : 0u;
+        uint32_t resets = 0;
+        for (size_t s = 0; s < control_step_count; ++s)
+            for (size_t e = 0; e < E; ++e) {
+                if (reset_masks != nullptr && reset_masks[s * E + e] != 0u) ++resets;
+                for (size_t a = 0; a < A; ++a) {
+                    float x = normalized_actions[(s * E + e) * A + a];
+                    if (!std::isfinite(x)) throw std::runtime_error("nonfinite action input");
+                    x = x < -1.0f ? -1.0f : (x > 1.0f ? 1.0f : x);
+                    const size_t j = first + a;
+                    const float base = m.kind == mr::kHumanoidTerrain ? m.standing_pose[j] : 0.0f;
+                    commands[(s * E + e) * nv + j] = base + handle->action_scale * x;
+                }
+            }
+        w.enqueue(commands.data(), reset_masks, control_step_count, false);
+        const mr::AdvanceSummary sum = w.wait();
+        const size_t n = (size_t)control_step_count * E;
+        std::vector<MRWorldStatusC> st(n);
+        std::vector<MRContactStatusC> cs(n);
+        w.read_statuses(st.data(), control_step_count);
+        w.read_contact_statuses(cs.data(), control_step_count);
+        handle->status_codes.resize(n);
+        handle->active_contacts.resize(n);
+        uint32_t max_contacts = 0;
+        for (size_t i = 0; i < n; ++i) {
+            handle->status_codes[i] = st[i].code;
+            handle->active_contacts[i] = cs[i].constraints;
+            max_contacts = std::max(max_contacts, cs[i].max_constraints);
+        }
+        if (handle->state_readback) {
+            handle->final_q.resize(E * m.max_nq);
+            handle->final_scene.resize(E * m.max_scene_bodies);
+            w.get_state(handle->final_q.data(), nullptr, handle->final_scene.data());
+        }
+        handle->submitted_control_steps += control_step_count;
+        handle->completed_environment_steps += sum.successful_environment_steps;
+        handle->submission_count += 1u;
+        handle->total_gpu_ms += sum.gpu_milliseconds;
+        if (out != nullptr) {
+            out->control_step_count = control_step_count;
+            out->successful_environment_steps = sum.successful_environment_steps;
+            out->failed_environment_steps = sum.failed_environment_steps;
+            out->first_failing_environment = sum.first_failing_environment;
+            out->first_failing_control_step = sum.first_failing_control_step;
+            out->first_gpu_status_code = sum.first_gpu_status_code;
+            out->host_requested_resets = resets;
+            out->maximum_active_contacts = max_contacts;
+            out->gpu_milliseconds = sum.gpu_milliseconds;
+        }
+        if (sum.failed_environment_steps > 0u)
+            throw std::runtime_error("GPU environment step failed with status " +
+                                     std::to_string(sum.first_gpu_status_code) + " in environment " +
+                                     std::to_string(sum.first_failing_environment));
+    });
+}
+
+MRTaskRolloutLayoutC mr_task_rollout_layout(const MRTaskRolloutHandle* handle) {
+    MRTaskRolloutLayoutC out;
+    std::memset(&out, 0, sizeof(out));
+    if (handle == nullptr) return out;
+    out.environment_count = handle->world->environment_count();
+    out.nq = handle->model.max_nq;
+    out.nv = handle->model.max_nv;
+    out.action_count = handle->action_count;
+    out.scene_body_count = handle->model.max_scene_bodies;
+    out.submitted_control_steps = handle->submitted_control_steps;
+    out.completed_environment_steps = handle->completed_environment_steps;
+    out.submission_count = handle->submission_count;
+    out.retained_buffer_bytes = handle->world->retained_bytes();
+    out.total_gpu_milliseconds = handle->total_gpu_ms;
+    return out;
+}
+
+const uint32_t* mr_task_rollout_status_codes(const MRTaskRolloutHandle* handle) {
+    return handle != nullptr && !handle->status_codes.empty() ? handle->status_codes.data() : nullptr;
+}
+
+const uint32_t* mr_task_rollout_active_contacts(const MRTaskRolloutHandle* handle) {
+    return handle != nullptr && !handle->active_contacts.empty() ? handle->active_contacts.data() : nullptr;
+}
+
+const float* mr_task_rollout_final_q(const MRTaskRolloutHandle* handle) {
+    return handle != nullptr && !handle->final_q.empty() ? handle->final_q.data() : nullptr;
+}
+
+int mr_task_rollout_copy_final_scene_states(const MRTaskRolloutHandle* handle, float* out, size_t count) {
+    return translate([&]() {
+        require(handle != nullptr && out != nullptr, "null argument");
+        require(!handle->final_scene.empty(), "no final scene state (enable state readback and advance first)");
+        require_count(count, handle->final_scene.size() * 13u, "scene states");
+        for (size_t i = 0; i < handle->final_scene.size(); ++i) {
+            const MRBodyStateC& s = handle->final_scene[i];
+            float* o = out + i * 13u;
+            o[0] = s.position.x; o[1] = s.position.y; o[2] = s.position.z;
+            o[3] = s.orientation.x; o[4] = s.orientation.y; o[5] = s.orientation.z; o[6] = s.orientation.w;
+            o[7] = s.linear_velocity_inv_mass.x; o[8] = s.linear_velocity_inv_mass.y; o[9] = s.linear_velocity_inv_mass.z;
+            o[10] = s.angular_velocity.x; o[11] = s.angular_velocity.y; o[12] = s.angular_velocity.z;
+        }
+    });
+}
+
+} // extern "C"

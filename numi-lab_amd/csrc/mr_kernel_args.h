@@ -1,0 +1,92 @@
+// Launch record of the fused control-step kernel, shared by the host
+// executor and the device code.
+#pragma once
+
+#include "../../include/mr_world_types.h"
+
+#include <stdint.h>
+
+namespace mr {
+
+// Per-constraint data kept resident in LDS during the cone solve.
+struct HotConstraint {
+    uint32_t body_a, body_b;
+    uint32_t flags, art; // art: 1 if an endpoint is an articulation link
+    float dir[9];        // normal, tangent u, tangent v
+    float point[3];      // contact midpoint
+    float target[3];
+    float reg[3];
+    float mu_u, mu_v;
+    float lambda[3];
+    float winv[9];
+};
+
+// Per-constraint data produced by emission and consumed by the Jacobian and
+// IR evaluation passes (global scratch).
+struct ColdConstraint {
+    float pa[3], pb[3];
+    float separation;
+    uint32_t manifold, point, flags;
+    float friction[4];
+    float restitution, restitution_threshold, compliance, dissipation;
+    float lambda[3];
+    float pad;
+};
+
+// Byte offsets of the per-workgroup dynamic LDS arrays.
+struct LdsLayout {
+    uint32_t q, v, vcand, effort, accel, cmd;
+    uint32_t pos, rot, w, vl, sa, sl, ca, cl, r;
+    uint32_t I, pt, pb, Ut, Ub, D, u, alpha, acc, L;
+    uint32_t scene, col, active, npts, moff, hot;
+    uint32_t total;
+};
+
+struct KernelArgs {
+    // immutable model
+    const MRModelHeaderC* models;
+    const uint32_t* environment_model;
+    const MRBodyC* bodies;
+    const MRDofC* dofs;
+    const MRShapeC* shapes;
+    const MRMaterialC* materials;
+    const MRCompiledPairC* pairs;
+    const float* heights;
+    MRHeightfieldC heightfield;
+    float gravity[4];
+    MRWorldStepConfigC cfg;
+    float h;
+    uint32_t environment_count;
+    uint32_t max_nq, max_nv, max_scene;
+    uint32_t control_step;
+    uint32_t pair_capacity, manifold_capacity, constraint_capacity;
+    // persistent state
+    float* q;
+    float* v;
+    MRBodyStateC* scene;
+    MRManifoldHeaderC* mh;   // [3][E][manifold_capacity]
+    MRManifoldPointC* mp;    // [3][E][manifold_capacity][4]
+    uint32_t* mcount;        // [3][E]
+    uint32_t* mcurrent;      // [E]
+    // inputs of this control step
+    const float* command;       // [E][max_nv]
+    const uint32_t* reset_mask; // [E] or null
+    const float* reset_q;
+    const float* reset_v;
+    const MRBodyStateC* reset_scene;
+    // outputs of this control step
+    MRWorldStatusC* status;     // [E]
+    MRContactStatusC* cstatus;  // [E]
+    float* acceleration;        // [E][max_nv] or null
+    MRPairRecordC* evidence_pairs; // [E][pair_capacity] or null
+    uint32_t* evidence_counts;     // [E] or null
+    // global scratch
+    MRManifoldHeaderC* stage_h;   // [E][pair_capacity]
+    MRManifoldPointC* stage_p;    // [E][pair_capacity][4]
+    float* jrow;                  // [E][constraint_capacity][3][max_nv]
+    float* resp;                  // [E][constraint_capacity][3][max_nv]
+    ColdConstraint* cold;         // [E][constraint_capacity]
+    LdsLayout lds;
+};
+
+} // namespace mr

@@ -1,0 +1,2396 @@
+// Fused temporalCone control-step kernel for gfx950.
+//
+// One 64-lane workgroup (one wavefront) owns one world for a whole control
+// step: reset/checkpoint, `physics_substeps` microsteps and the transactional
+// commit. The world's articulation, colliders, active pairs and solver rows
+// live in LDS; persistent state (q, v, scene bodies, manifolds) is read once
+// at the start and written once at the end of the control step, so a failed
+// step leaves HBM untouched (the checkpoint is the unmodified state). Lanes
+// work on independent items (links of one tree level, colliders, compiled
+// pairs, manifolds, constraint rows, generalized velocities); every item is
+// computed with a fixed scalar evaluation order, and every order-dependent
+// accumulation (children into a parent, contact impulses into a velocity)
+// is performed by one owner lane in a fixed sequence. Results are therefore
+// deterministic and independent of launch shape.
+//
+// Stage map to the reference (paths relative to the reference repository):
+//   drive refresh        src/metal/MetalWorld.metal:280-366
+//   kinematics + ABA     src/metal/ArticulatedABA.metal:352-1440
+//   scene prediction     src/metal/MetalWorldContact.metal:2509-2567
+//   collider projection  src/metal/Collision.metal:321-600, 5666-5712
+//   pair flags / scan    src/metal/Collision.metal:796-858, 7905-8176
+//   narrowphase          src/metal/Collision.metal:1003-1720, 4379-4830
+//   manifolds            src/metal/Collision.metal:5108-5530, 9557-9830
+//   emission             src/metal/Collision.metal:9180-9470
+//   responses            src/metal/ArticulatedInverseMass.metal:1652-1700
+//   IR evaluation        src/metal/MetalWorldContact.metal:3800-3995
+//   cone solve           src/metal/MetalWorldContact.metal:482-720, 7856-8316
+//   joint limits         src/metal/MetalWorldContact.metal:2878-2960
+//   integration          src/metal/MetalWorldContact.metal:8876-9043
+//   latch / commit       src/metal/MetalWorldContact.metal:9051-9082, MetalWorld.metal:372-519
+
+#include "mr_device_math.h"
+#include "mr_kernel_args.h"
+
+using namespace mrd;
+using mr::ColdConstraint;
+using mr::HotConstraint;
+using mr::KernelArgs;
+
+namespace {
+
+constexpr float kTiny = 1.0e-14f;
+constexpr float kFloatEps = 1.1920928955078125e-7f;
+constexpr float kAbsolutePivotFloor = 1.0e-12f;
+constexpr float kMatrixFloor = 1.0e-10f;
+constexpr float kConeEpsilon = 1.0e-7f;
+constexpr float kContactRegularization = 1.0e-2f;
+constexpr uint32_t kMaxCandidates = 12u;
+constexpr uint32_t kLanes = 64u;
+
+MRD_INL V3 f3(const mr_float4& v) { return V3{v.x, v.y, v.z}; }
+MRD_INL Q4 q4(const mr_float4& v) { return Q4{v.x, v.y, v.z, v.w}; }
+MRD_INL uint32_t feature_key(uint32_t t, uint32_t l) { return ((t & 0x0fu) << 28u) | (l & 0x0fffffffu); }
+
+struct SpI {
+    float A[6];
+    float B[9];
+    float C[6];
+};
+
+struct Collider {
+    uint32_t type, body, disabled;
+    V3 center;
+    Q4 rotation;
+    V3 lower, upper;
+    float radius, half_length, contact_offset;
+    V3 half_extents, p0, p1, plane_normal;
+};
+
+struct RawContact {
+    V3 normal;
+    float separation;
+    V3 point_a, point_b;
+    uint32_t feature_a, feature_b;
+};
+
+struct RawBatch {
+    RawContact c[MR_RAW_CONTACTS_PER_PAIR];
+    uint32_t count;
+};
+
+struct Candidate {
+    MRManifoldPointC point;
+    V3 world_point;
+    float separation;
+    float drift;
+};
+
+MRD_INL V3 sym_mul(const float* s, V3 w) {
+    return V3{s[0] * w.x + s[1] * w.y + s[2] * w.z, s[1] * w.x + s[3] * w.y + s[4] * w.z,
+              s[2] * w.x + s[4] * w.y + s[5] * w.z};
+}
+MRD_INL V3 mat_mul(const float* b, V3 v) {
+    return V3{b[0] * v.x + b[1] * v.y + b[2] * v.z, b[3] * v.x + b[4] * v.y + b[5] * v.z,
+              b[6] * v.x + b[7] * v.y + b[8] * v.z};
+}
+MRD_INL V3 matT_mul(const float* b, V3 w) {
+    return V3{b[0] * w.x + b[3] * w.y + b[6] * w.z, b[1] * w.x + b[4] * w.y + b[7] * w.z,
+              b[2] * w.x + b[5] * w.y + b[8] * w.z};
+}
+MRD_INL int sym_index(int i, int j) {
+    return (i == 0) ? j : ((i == 1) ? (j == 0 ? 1 : (j == 1 ? 3 : 4)) : (j == 0 ? 2 : (j == 1 ? 4 : 5)));
+}
+
+// I' = X^T I X for a translation r: A' = A - (Q + Q^T) - r~ C r~, B' = B + r~ C.
+MRD_INL void shift_inertia(const SpI& in, V3 r, SpI& out) {
+    float Cm[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Cm[i][j] = in.C[sym_index(i, j)];
+    float T[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        T[0][j] = r.y * Cm[2][j] - r.z * Cm[1][j];
+        T[1][j] = r.z * Cm[0][j] - r.x * Cm[2][j];
+        T[2][j] = r.x * Cm[1][j] - r.y * Cm[0][j];
+    }
+    float Q[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float b0 = in.B[3 * i + 0], b1 = in.B[3 * i + 1], b2 = in.B[3 * i + 2];
+        Q[i][0] = b1 * r.z - b2 * r.y;
+        Q[i][1] = b2 * r.x - b0 * r.z;
+        Q[i][2] = b0 * r.y - b1 * r.x;
+    }
+    float P[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        P[i][0] = T[i][1] * r.z - T[i][2] * r.y;
+        P[i][1] = T[i][2] * r.x - T[i][0] * r.z;
+        P[i][2] = T[i][0] * r.y - T[i][1] * r.x;
+    }
+    const int si[6] = {0, 0, 0, 1, 1, 2};
+    const int sj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out.A[k] = in.A[k] - (Q[si[k]][sj[k]] + Q[sj[k]][si[k]]) - P[si[k]][sj[k]];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) out.B[3 * i + j] = in.B[3 * i + j] + T[i][j];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out.C[k] = in.C[k];
+}
+
+MRD_INL float spi_max_abs(const SpI& I) {
+    float m = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) m = fmaxd(m, fabs_(I.A[k]));
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m = fmaxd(m, fabs_(I.B[k]));
+#pragma unroll
+    for (int k = 0; k < 6; ++k) m = fmaxd(m, fabs_(I.C[k]));
+    return m;
+}
+
+MRD_INL float spi6(const SpI& I, int r, int c) {
+    if (r < 3 && c < 3) return I.A[sym_index(r, c)];
+    if (r < 3) return I.B[3 * r + (c - 3)];
+    if (c < 3) return I.B[3 * c + (r - 3)];
+    return I.C[sym_index(r - 3, c - 3)];
+}
+
+// ------------------------------------------------------------ environment
+
+struct Env {
+    const KernelArgs* a;
+    uint32_t env;
+    const MRModelHeaderC* hd;
+    const MRBodyC* bodies;
+    const MRDofC* dofs;
+    const MRShapeC* shapes;
+    const MRMaterialC* mats;
+    const MRCompiledPairC* pairs;
+    uint32_t nlinks, nbodies, nv, nq, nshapes, npairs, nscene, levels, root_type;
+    float h;
+    V3 gravity;
+    // LDS
+    float *q, *v, *vcand, *effort, *accel, *cmd;
+    V3 *pos, *w, *vl, *sa, *sl, *ca, *cl, *r;
+    Q4* rot;
+    SpI* I;
+    V3 *pt, *pb, *Ut, *Ub, *alpha, *acc;
+    float *D, *u, *L;
+    MRBodyStateC* scene;
+    Collider* col;
+    uint32_t *active, *npts, *moff;
+    HotConstraint* hot;
+    // global scratch of this environment
+    MRManifoldHeaderC* stage_h;
+    MRManifoldPointC* stage_p;
+    float* jrow;
+    float* resp;
+    ColdConstraint* cold;
+};
+
+template <typename T>
+MRD_INL T* lds_at(unsigned char* base, uint32_t off) {
+    return reinterpret_cast<T*>(base + off);
+}
+
+MRD_INL bool is_link(const Env& e, uint32_t b) { return b < e.nlinks; }
+MRD_INL bool is_dynamic_scene(const Env& e, uint32_t b) {
+    return b >= e.nlinks && e.bodies[b].motion_type == MR_MOTION_DYNAMIC;
+}
+
+MRD_INL void body_pose(const Env& e, uint32_t b, V3& p, Q4& q) {
+    if (b < e.nlinks) {
+        p = e.pos[b];
+        q = e.rot[b];
+    } else {
+        const MRBodyStateC& s = e.scene[b - e.nlinks];
+        p = f3(s.position);
+        q = q4(s.orientation);
+    }
+}
+
+// Packed failure key: (order << 32) | (code << 24) | index (24 bits).
+MRD_INL void report_failure(unsigned long long* slot, uint32_t order, uint32_t code, uint32_t index) {
+    const unsigned long long key = ((unsigned long long)order << 32) | ((unsigned long long)(code & 0xffu) << 24) |
+        (unsigned long long)(index & 0xffffffu);
+    atomicMin(slot, key);
+}
+
+// -------------------------------------------------------------- wave ops
+
+MRD_INL uint32_t wave_prefix_sum(uint32_t value, uint32_t lane, uint32_t& total) {
+    uint32_t x = value;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, (unsigned)d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - value;
+}
+
+MRD_INL float wave_max(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxd(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+MRD_INL uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+MRD_INL uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t o = __shfl_xor(v, d, 64);
+        v = v > o ? v : o;
+    }
+    return v;
+}
+
+// ----------------------------------------------------------------- drives
+
+MRD_INL float effective_armature(const Env& e, uint32_t j) {
+    const MRDofC& dof = e.dofs[j];
+    float value = dof.drive.z;
+    if (e.a->cfg.actuation_mode == MR_ACTUATION_IMPLICIT_POSITION_DRIVE && (dof.flags & MR_DOF_FLAG_DRIVE) != 0u)
+        value += e.h * dof.drive.y + e.h * e.h * dof.drive.x;
+    return value;
+}
+
+MRD_INL float drive_effort(const Env& e, uint32_t j) {
+    const MRDofC& dof = e.dofs[j];
+    const float command = e.cmd[j];
+    if (e.a->cfg.actuation_mode == MR_ACTUATION_EFFORT) return command;
+    float cmd = 0.0f;
+    if ((dof.flags & MR_DOF_FLAG_DRIVE) != 0u && dof.q_index != MR_INVALID_INDEX) {
+        float target = command;
+        if ((dof.flags & MR_DOF_FLAG_POSITION_LIMIT) != 0u) target = clampd(target, dof.limits.x, dof.limits.y);
+        const float vel = e.v[j];
+        cmd = dof.drive.x * (target - e.q[dof.q_index] - e.h * vel) - dof.drive.y * vel;
+        const float dry = dof.drive.w;
+        if (dry > 0.0f) {
+            if (fabs_(vel) > 1.0e-4f) cmd -= vel > 0.0f ? dry : -dry;
+            else cmd -= clampd(cmd, -dry, dry);
+        }
+        if ((dof.flags & MR_DOF_FLAG_EFFORT_LIMIT) != 0u && dof.limits.w > 0.0f)
+            cmd = clampd(cmd, -dof.limits.w, dof.limits.w);
+    }
+    return cmd;
+}
+
+// ------------------------------------------------------------- kinematics
+
+// Forward kinematics of one non-root link from its parent (lane-local).
+MRD_INL bool fk_link(const Env& e, uint32_t i) {
+    const MRBodyC& b = e.bodies[i];
+    const uint32_t p = b.parent;
+    const Q4 pjr = qmul(e.rot[p], q4(b.parent_rotation));
+    const bool moving = b.joint_type == MR_JOINT_REVOLUTE || b.joint_type == MR_JOINT_CONTINUOUS;
+    const V3 axis_n = normalize(f3(b.axis));
+    float qi = 0.0f, qd = 0.0f;
+    Q4 motion{0.0f, 0.0f, 0.0f, 1.0f};
+    if (moving) {
+        qi = e.q[b.q_index];
+        qd = e.v[b.v_index];
+        motion = axis_angle(axis_n, qi);
+    }
+    Q4 child;
+    if (!normalized_quat(qmul(qmul(pjr, motion), qconj(q4(b.child_rotation))), child)) return false;
+    e.rot[i] = child;
+    const V3 axis_w = qrot(pjr, axis_n);
+    const V3 joint_pos = add(e.pos[p], qrot(e.rot[p], f3(b.parent_anchor)));
+    const V3 child_anchor = qrot(child, f3(b.child_anchor));
+    const V3 pos = sub(joint_pos, child_anchor);
+    e.pos[i] = pos;
+    e.r[i] = sub(pos, e.pos[p]);
+    const V3 p2j = sub(joint_pos, e.pos[p]);
+    const V3 wp = e.w[p];
+    const V3 zero{0.0f, 0.0f, 0.0f};
+    const V3 wi = moving ? add(wp, mul(qd, axis_w)) : wp;
+    e.w[i] = wi;
+    const V3 vli = sub(add(e.vl[p], cross(wp, p2j)), cross(wi, child_anchor));
+    e.vl[i] = vli;
+    e.sa[i] = moving ? axis_w : zero;
+    e.sl[i] = moving ? neg(cross(axis_w, child_anchor)) : zero;
+    const V3 cai = moving ? mul(qd, cross(wp, axis_w)) : zero;
+    e.ca[i] = cai;
+    const V3 cli = sub(sub(cross(wp, cross(wp, p2j)), cross(cai, child_anchor)), cross(wi, cross(wi, child_anchor)));
+    e.cl[i] = cli;
+    return finite3(pos) && finite4(child) && finite3(wi) && finite3(vli) && finite3(cli) && finite3(cai);
+}
+
+MRD_INL void world_inertia(const MRBodyC& b, Q4 rot, float out[6]) {
+    const V3 c0 = qrot(rot, V3{1.0f, 0.0f, 0.0f});
+    const V3 c1 = qrot(rot, V3{0.0f, 1.0f, 0.0f});
+    const V3 c2 = qrot(rot, V3{0.0f, 0.0f, 1.0f});
+    const float R[3][3] = {{c0.x, c1.x, c2.x}, {c0.y, c1.y, c2.y}, {c0.z, c1.z, c2.z}};
+    const float Ib[3][3] = {{b.inertia_row0.x, b.inertia_row0.y, b.inertia_row0.z},
+                            {b.inertia_row1.x, b.inertia_row1.y, b.inertia_row1.z},
+                            {b.inertia_row2.x, b.inertia_row2.y, b.inertia_row2.z}};
+    const int si[6] = {0, 0, 0, 1, 1, 2};
+    const int sj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int rr = si[k], cc = sj[k];
+        float acc = 0.0f;
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int y = 0; y < 3; ++y) acc += R[rr][x] * Ib[x][y] * R[cc][y];
+        out[k] = acc;
+    }
+}
+
+MRD_INL void world_inverse_inertia(const MRBodyC& b, Q4 rot, MRBodyStateC& s) {
+    const V3 c0 = qrot(rot, V3{1.0f, 0.0f, 0.0f});
+    const V3 c1 = qrot(rot, V3{0.0f, 1.0f, 0.0f});
+    const V3 c2 = qrot(rot, V3{0.0f, 0.0f, 1.0f});
+    const float R[3][3] = {{c0.x, c1.x, c2.x}, {c0.y, c1.y, c2.y}, {c0.z, c1.z, c2.z}};
+    const float Ib[3][3] = {{b.inverse_inertia_row0.x, b.inverse_inertia_row0.y, b.inverse_inertia_row0.z},
+                            {b.inverse_inertia_row1.x, b.inverse_inertia_row1.y, b.inverse_inertia_row1.z},
+                            {b.inverse_inertia_row2.x, b.inverse_inertia_row2.y, b.inverse_inertia_row2.z}};
+    float o[3][3];
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+#pragma unroll
+                for (int y = 0; y < 3; ++y) acc += R[rr][x] * Ib[x][y] * R[cc][y];
+            o[rr][cc] = acc;
+        }
+    s.inverse_inertia_world_row0 = mr_float4{o[0][0], o[0][1], o[0][2], 0.0f};
+    s.inverse_inertia_world_row1 = mr_float4{o[1][0], o[1][1], o[1][2], 0.0f};
+    s.inverse_inertia_world_row2 = mr_float4{o[2][0], o[2][1], o[2][2], 0.0f};
+}
+
+// ABA initialisation of one link (rigid inertia + bias force).
+MRD_INL void aba_init_link(const Env& e, uint32_t i) {
+    const MRBodyC& b = e.bodies[i];
+    SpI I;
+    world_inertia(b, e.rot[i], I.A);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) I.B[k] = 0.0f;
+    const float m = b.mass.x;
+    I.C[0] = m; I.C[1] = 0.0f; I.C[2] = 0.0f;
+    I.C[3] = m; I.C[4] = 0.0f; I.C[5] = m;
+    const V3 wi = e.w[i];
+    V3 bt = cross(wi, sym_mul(I.A, wi));
+    V3 bf = neg(mul(m, e.gravity));
+    if (e.a->cfg.apply_body_damping != 0u) {
+        bt = add(bt, mul(b.damping.y, wi));
+        bf = add(bf, mul(b.damping.x, e.vl[i]));
+    }
+    e.I[i] = I;
+    e.pt[i] = bt;
+    e.pb[i] = bf;
+}
+
+// Joint projection of link i and its contribution to the parent, written in
+// place into I[i], pt[i], pb[i]. Returns false on a pivot failure.
+MRD_INL bool aba_project_link(const Env& e, uint32_t i, float& pivot_out) {
+    const MRBodyC& b = e.bodies[i];
+    const SpI I = e.I[i];
+    const V3 Ica_t = add(sym_mul(I.A, e.ca[i]), mat_mul(I.B, e.cl[i]));
+    const V3 Ica_b = add(matT_mul(I.B, e.ca[i]), sym_mul(I.C, e.cl[i]));
+    V3 pt = add(e.pt[i], Ica_t);
+    V3 pb = add(e.pb[i], Ica_b);
+    SpI Ip = I;
+    pivot_out = -1.0f;
+    bool ok = true;
+    if (b.v_index != MR_INVALID_INDEX) {
+        const V3 sa = e.sa[i], sl = e.sl[i];
+        const V3 Ut = add(sym_mul(I.A, sa), mat_mul(I.B, sl));
+        const V3 Ub = add(matT_mul(I.B, sa), sym_mul(I.C, sl));
+        const float D = (dot(sa, Ut) + dot(sl, Ub)) + effective_armature(e, b.v_index);
+        const float floor_ = fmaxd(kAbsolutePivotFloor, spi_max_abs(I) * 6.0f * kFloatEps);
+        if (!(D > floor_) || !isfin(D)) ok = false;
+        const float u = (e.effort[b.v_index] - dot(sa, pt)) - dot(sl, pb);
+        const float uD = u / D;
+        pt = add(pt, mul(uD, Ut));
+        pb = add(pb, mul(uD, Ub));
+        const float ut[3] = {Ut.x, Ut.y, Ut.z};
+        const float ub[3] = {Ub.x, Ub.y, Ub.z};
+        const int si[6] = {0, 0, 0, 1, 1, 2};
+        const int sj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) Ip.A[k] = I.A[k] - (ut[si[k]] * ut[sj[k]]) / D;
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int y = 0; y < 3; ++y) Ip.B[3 * x + y] = I.B[3 * x + y] - (ut[x] * ub[y]) / D;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) Ip.C[k] = I.C[k] - (ub[si[k]] * ub[sj[k]]) / D;
+        e.Ut[i] = Ut;
+        e.Ub[i] = Ub;
+        e.D[i] = D;
+        e.u[i] = u;
+        pivot_out = sqrt_rn(D);
+    }
+    SpI shifted;
+    shift_inertia(Ip, e.r[i], shifted);
+    e.I[i] = shifted;
+    e.pt[i] = add(pt, cross(e.r[i], pb));
+    e.pb[i] = pb;
+    return ok;
+}
+
+// Parent accumulation: children of `p` (at the level just processed) are
+// added in decreasing link index order.
+MRD_INL void aba_accumulate(const Env& e, uint32_t p) {
+    SpI P = e.I[p];
+    V3 pt = e.pt[p], pb = e.pb[p];
+    for (uint32_t c = e.nlinks - 1u; c > p; --c) {
+        if (e.bodies[c].parent != p) continue;
+        const SpI& S = e.I[c];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) P.A[k] += S.A[k];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) P.B[k] += S.B[k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) P.C[k] += S.C[k];
+        pt = add(pt, e.pt[c]);
+        pb = add(pb, e.pb[c]);
+    }
+    e.I[p] = P;
+    e.pt[p] = pt;
+    e.pb[p] = pb;
+}
+
+MRD_INL bool root_factor(const Env& e, float& minp, float& maxp, uint32_t& fail_row) {
+    const SpI I = e.I[0];
+    const float floor_ = fmaxd(kAbsolutePivotFloor, spi_max_abs(I) * 6.0f * kFloatEps);
+    float* L = e.L;
+    for (int k = 0; k < 36; ++k) L[k] = 0.0f;
+    for (int row = 0; row < 6; ++row) {
+        for (int col = 0; col <= row; ++col) {
+            float val = spi6(I, row, col);
+            for (int k = 0; k < col; ++k) val -= L[row * 6 + k] * L[col * 6 + k];
+            if (row == col) {
+                if (!(val > floor_) || !isfin(val)) {
+                    fail_row = (uint32_t)row;
+                    return false;
+                }
+                const float piv = sqrt_rn(val);
+                L[row * 6 + row] = piv;
+                minp = fmind(minp, piv);
+                maxp = fmaxd(maxp, piv);
+            } else {
+                L[row * 6 + col] = val / L[col * 6 + col];
+            }
+        }
+    }
+    return true;
+}
+
+MRD_INL void root_solve(const float* L, const float rhs[6], float x[6]) {
+    float y[6];
+    for (int row = 0; row < 6; ++row) {
+        float val = rhs[row];
+        for (int col = 0; col < row; ++col) val -= L[row * 6 + col] * y[col];
+        y[row] = val / L[row * 6 + row];
+    }
+    for (int row = 5; row >= 0; --row) {
+        float val = y[row];
+        for (int col = row + 1; col < 6; ++col) val -= L[col * 6 + row] * x[col];
+        x[row] = val / L[row * 6 + row];
+    }
+}
+
+// Inverse ABA for point forces on up to two links (lane-local, private
+// arrays); out[nv] = M_eff^{-1} J^T f.
+MRD_INL void inverse_aba(const Env& e, uint32_t la, V3 xa, V3 fa, uint32_t lb, V3 xb, V3 fb, float* out) {
+    V3 pt[MR_MAX_LINKS], pb[MR_MAX_LINKS];
+    float uu[MR_MAX_LINKS];
+    const uint32_t n = e.nlinks;
+    for (uint32_t i = 0; i < n; ++i) {
+        pt[i] = V3{0.0f, 0.0f, 0.0f};
+        pb[i] = V3{0.0f, 0.0f, 0.0f};
+    }
+    if (la != MR_INVALID_INDEX) {
+        pt[la] = sub(pt[la], cross(sub(xa, e.pos[la]), fa));
+        pb[la] = sub(pb[la], fa);
+    }
+    if (lb != MR_INVALID_INDEX) {
+        pt[lb] = sub(pt[lb], cross(sub(xb, e.pos[lb]), fb));
+        pb[lb] = sub(pb[lb], fb);
+    }
+    for (uint32_t i = n - 1u; i >= 1u; --i) {
+        const MRBodyC& b = e.bodies[i];
+        V3 t = pt[i], f = pb[i];
+        if (b.v_index != MR_INVALID_INDEX) {
+            const float u = (0.0f - dot(e.sa[i], t)) - dot(e.sl[i], f);
+            const float uD = u / e.D[i];
+            t = add(t, mul(uD, e.Ut[i]));
+            f = add(f, mul(uD, e.Ub[i]));
+            uu[i] = u;
+        }
+        pt[b.parent] = add(pt[b.parent], add(t, cross(e.r[i], f)));
+        pb[b.parent] = add(pb[b.parent], f);
+    }
+    // outward: reuse pt as angular, pb as linear acceleration storage
+    if (e.root_type == MR_ROOT_FLOATING) {
+        const float rhs[6] = {-pt[0].x, -pt[0].y, -pt[0].z, -pb[0].x, -pb[0].y, -pb[0].z};
+        float x[6];
+        root_solve(e.L, rhs, x);
+        pt[0] = V3{x[0], x[1], x[2]};
+        pb[0] = V3{x[3], x[4], x[5]};
+        out[0] = x[3]; out[1] = x[4]; out[2] = x[5];
+        out[3] = x[0]; out[4] = x[1]; out[5] = x[2];
+    } else {
+        pt[0] = V3{0.0f, 0.0f, 0.0f};
+        pb[0] = V3{0.0f, 0.0f, 0.0f};
+    }
+    for (uint32_t i = 1; i < n; ++i) {
+        const MRBodyC& b = e.bodies[i];
+        const V3 ap = pt[b.parent];
+        const V3 lp = add(pb[b.parent], cross(ap, e.r[i]));
+        if (b.v_index != MR_INVALID_INDEX) {
+            const float proj = dot(e.Ut[i], ap) + dot(e.Ub[i], lp);
+            const float qdd = (uu[i] - proj) / e.D[i];
+            out[b.v_index] = qdd;
+            pt[i] = add(ap, mul(qdd, e.sa[i]));
+            pb[i] = add(lp, mul(qdd, e.sl[i]));
+        } else {
+            pt[i] = ap;
+            pb[i] = lp;
+        }
+    }
+}
+
+MRD_INL V3 point_jacobian_column(const Env& e, uint32_t link, V3 x, uint32_t j) {
+    const V3 zero{0.0f, 0.0f, 0.0f};
+    if (e.root_type == MR_ROOT_FLOATING && j < 6u) {
+        V3 ev = zero;
+        if (j < 3u) {
+            setcomp(ev, (int)j, 1.0f);
+            return ev;
+        }
+        setcomp(ev, (int)j - 3, 1.0f);
+        return cross(ev, sub(x, e.pos[0]));
+    }
+    const uint32_t owner = e.dofs[j].body;
+    for (uint32_t b = link; b != 0u && b != MR_INVALID_INDEX; b = e.bodies[b].parent)
+        if (b == owner) return add(e.sl[b], cross(e.sa[b], sub(x, e.pos[b])));
+    return zero;
+}
+
+// ------------------------------------------------------------ colliders
+
+MRD_INL void inflate_bounds(Collider& c) {
+    const V3 scale = add(vmax(vabs(c.lower), vabs(c.upper)), V3{1.0f, 1.0f, 1.0f});
+    const V3 pad = mul(MR_COLLISION_AABB_RELATIVE_PAD, scale);
+    c.lower = sub(c.lower, pad);
+    c.upper = add(c.upper, pad);
+}
+
+MRD_INL bool project_collider(const Env& e, uint32_t s, Collider& c) {
+    const MRShapeC& sh = e.shapes[s];
+    V3 bpos;
+    Q4 borient;
+    body_pose(e, sh.body, bpos, borient);
+    Q4 br, lr;
+    if (!checked_quat(borient, br) || !checked_quat(q4(sh.local_rotation), lr)) return false;
+    c.type = sh.shape_type;
+    c.body = sh.body;
+    c.disabled = (sh.flags & MR_SHAPE_FLAG_SIMULATION_DISABLED) != 0u ? 1u : 0u;
+    c.rotation = qmul(br, lr);
+    c.center = add(bpos, qrot(br, f3(sh.local_position)));
+    c.contact_offset = sh.contact_rest_radius.x;
+    c.radius = 0.0f;
+    c.half_length = 0.0f;
+    const V3 zero{0.0f, 0.0f, 0.0f};
+    c.half_extents = zero;
+    c.p0 = zero;
+    c.p1 = zero;
+    c.plane_normal = zero;
+    c.lower = zero;
+    c.upper = zero;
+    if (!finite3(c.center) || !finite4(c.rotation)) return false;
+    if (c.disabled) return true;
+    const float co = c.contact_offset;
+    if (sh.shape_type == MR_SHAPE_SPHERE) {
+        c.radius = sh.dimensions.x;
+        const float ex = c.radius + co;
+        c.lower = sub(c.center, V3{ex, ex, ex});
+        c.upper = add(c.center, V3{ex, ex, ex});
+        inflate_bounds(c);
+    } else if (sh.shape_type == MR_SHAPE_CAPSULE) {
+        c.radius = sh.dimensions.x;
+        c.half_length = sh.dimensions.y;
+        const V3 axis = qrot(c.rotation, V3{0.0f, c.half_length, 0.0f});
+        c.p0 = sub(c.center, axis);
+        c.p1 = add(c.center, axis);
+        const float ex = c.radius + co;
+        c.lower = sub(vmin(c.p0, c.p1), V3{ex, ex, ex});
+        c.upper = add(vmax(c.p0, c.p1), V3{ex, ex, ex});
+        inflate_bounds(c);
+    } else if (sh.shape_type == MR_SHAPE_BOX) {
+        c.half_extents = f3(sh.dimensions);
+        const V3 bx = qrot(c.rotation, V3{1.0f, 0.0f, 0.0f});
+        const V3 by = qrot(c.rotation, V3{0.0f, 1.0f, 0.0f});
+        const V3 bz = qrot(c.rotation, V3{0.0f, 0.0f, 1.0f});
+        const V3 ext = add(add(add(mul(c.half_extents.x, vabs(bx)), mul(c.half_extents.y, vabs(by))),
+                               mul(c.half_extents.z, vabs(bz))),
+                           V3{co, co, co});
+        c.lower = sub(c.center, ext);
+        c.upper = add(c.center, ext);
+        inflate_bounds(c);
+    } else if (sh.shape_type == MR_SHAPE_PLANE) {
+        c.plane_normal = normalize(qrot(c.rotation, V3{0.0f, 1.0f, 0.0f}));
+    } else if (sh.shape_type == MR_SHAPE_HEIGHTFIELD) {
+        const MRHeightfieldC& hf = e.a->heightfield;
+        c.lower = sub(f3(hf.bounds_lower), V3{co, co, co});
+        c.upper = add(f3(hf.bounds_upper), V3{co, co, co});
+        inflate_bounds(c);
+    } else {
+        return false;
+    }
+    return finite3(c.lower) && finite3(c.upper);
+}
+
+MRD_INL float collider_scale(const Collider& c) {
+    return fmaxd(fmaxd(maxabs3(c.center), maxabs3(c.half_extents)),
+                 fmaxd(fmaxd(maxabs3(c.p0), maxabs3(c.p1)),
+                       fmaxd(fmaxd(fabs_(c.radius), fabs_(c.half_length)),
+                             fmaxd(fabs_(c.contact_offset), fmaxd(maxabs3(c.lower), maxabs3(c.upper))))));
+}
+
+MRD_INL float pair_query_padding(const Collider& a, const Collider& b) {
+    return (fmaxd(collider_scale(a), collider_scale(b)) + 1.0f) * MR_COLLISION_QUERY_RELATIVE_PAD;
+}
+
+MRD_INL bool pair_may_overlap(const Collider& a, const Collider& b) {
+    if (a.disabled || b.disabled) return false;
+    if (a.type != MR_SHAPE_PLANE && b.type != MR_SHAPE_PLANE) {
+        return a.lower.x <= b.upper.x && a.lower.y <= b.upper.y && a.lower.z <= b.upper.z &&
+            a.upper.x >= b.lower.x && a.upper.y >= b.lower.y && a.upper.z >= b.lower.z;
+    }
+    const Collider& plane = a.type == MR_SHAPE_PLANE ? a : b;
+    const Collider& fin = a.type == MR_SHAPE_PLANE ? b : a;
+    const V3 center = mul(0.5f, add(fin.lower, fin.upper));
+    const V3 half = mul(0.5f, sub(fin.upper, fin.lower));
+    const float scale = fmaxd(fmaxd(maxabs3(plane.center), maxabs3(fin.lower)),
+                              fmaxd(maxabs3(fin.upper), fmaxd(fabs_(plane.contact_offset), fabs_(fin.contact_offset))));
+    const float pad = (scale + 1.0f) * MR_COLLISION_QUERY_RELATIVE_PAD;
+    const float dist = dot(plane.plane_normal, sub(center, plane.center)) - dot(vabs(plane.plane_normal), half);
+    return dist <= plane.contact_offset + pad;
+}
+
+// ------------------------------------------------------------ narrowphase
+
+MRD_INL RawContact make_contact(V3 n, float sep, V3 pa, V3 pb, uint32_t fa, uint32_t fb) {
+    RawContact c;
+    c.normal = n;
+    c.separation = sep;
+    c.point_a = pa;
+    c.point_b = pb;
+    c.feature_a = fa;
+    c.feature_b = fb;
+    return c;
+}
+
+MRD_INL RawContact swapped(const RawContact& in) {
+    RawContact c = in;
+    c.normal = neg(in.normal);
+    c.point_a = in.point_b;
+    c.point_b = in.point_a;
+    c.feature_a = in.feature_b;
+    c.feature_b = in.feature_a;
+    return c;
+}
+
+MRD_INL V3 coincident_normal(uint32_t a, uint32_t b) {
+    const uint32_t hash = (a * 73856093u) ^ (b * 19349663u);
+    V3 r{0.0f, 0.0f, 0.0f};
+    setcomp(r, (int)(hash % 3u), (hash & 4u) == 0u ? 1.0f : -1.0f);
+    return r;
+}
+
+MRD_INL V3 stable_segment_normal(V3 da, V3 db, uint32_t ca, uint32_t cb) {
+    const V3 cr = cross(da, db);
+    if (dot(cr, cr) > kTiny) return normalize(cr);
+    V3 d;
+    if (dot(da, da) > kTiny) d = normalize(da);
+    else if (dot(db, db) > kTiny) d = normalize(db);
+    else return coincident_normal(ca, cb);
+    const V3 ad = vabs(d);
+    V3 ref;
+    if (ad.x <= ad.y && ad.x <= ad.z) ref = V3{1.0f, 0.0f, 0.0f};
+    else if (ad.y <= ad.z) ref = V3{0.0f, 1.0f, 0.0f};
+    else ref = V3{0.0f, 0.0f, 1.0f};
+    V3 res = normalize(cross(d, ref));
+    if ((((ca * 73856093u) ^ (cb * 19349663u)) & 4u) != 0u) res = neg(res);
+    return res;
+}
+
+struct SegPoint {
+    V3 point;
+    float t;
+};
+
+MRD_INL SegPoint closest_on_segment(V3 p, V3 e0, V3 e1) {
+    const V3 seg = sub(e1, e0);
+    const float l2 = dot(seg, seg);
+    if (!(l2 > kTiny)) return SegPoint{e0, 0.0f};
+    const float t = clampd(dot(sub(p, e0), seg) / l2, 0.0f, 1.0f);
+    return SegPoint{add(e0, mul(t, seg)), t};
+}
+
+struct SegPair {
+    V3 pa, pb;
+    float ta, tb;
+};
+
+MRD_INL SegPair closest_segments(V3 a0, V3 a1, V3 b0, V3 b1) {
+    const V3 da = sub(a1, a0), db = sub(b1, b0), off = sub(a0, b0);
+    const float sa = dot(da, da), sb = dot(db, db);
+    const float pa = dot(da, off), pb = dot(db, off);
+    float ta = 0.0f, tb = 0.0f;
+    if (!(sa > kTiny) && !(sb > kTiny)) return SegPair{a0, b0, 0.0f, 0.0f};
+    if (!(sa > kTiny)) {
+        tb = clampd(pb / sb, 0.0f, 1.0f);
+    } else {
+        const float cp = dot(da, db);
+        if (!(sb > kTiny)) {
+            ta = clampd(-pa / sa, 0.0f, 1.0f);
+        } else {
+            const float den = sa * sb - cp * cp;
+            if (den > kTiny * sa * sb) ta = clampd((cp * pb - pa * sb) / den, 0.0f, 1.0f);
+            tb = (cp * ta + pb) / sb;
+            if (tb < 0.0f) {
+                tb = 0.0f;
+                ta = clampd(-pa / sa, 0.0f, 1.0f);
+            } else if (tb > 1.0f) {
+                tb = 1.0f;
+                ta = clampd((cp - pa) / sa, 0.0f, 1.0f);
+            }
+        }
+    }
+    return SegPair{add(a0, mul(ta, da)), add(b0, mul(tb, db)), ta, tb};
+}
+
+MRD_INL uint32_t capsule_feature(float t) {
+    if (t <= 0.0f) return 0u;
+    return t >= 1.0f ? 1u : 2u;
+}
+
+MRD_INL uint32_t box_region(float c, float h) {
+    if (c < -h) return 0u;
+    return c > h ? 2u : 1u;
+}
+
+MRD_INL void sphere_box(const Collider& sphere, const Collider& box, V3& normal, V3& box_point, float& sep,
+                        uint32_t& feature) {
+    const V3 lc = qrot_inv(box.rotation, sub(sphere.center, box.center));
+    const V3 he = box.half_extents;
+    V3 closest{clampd(lc.x, -he.x, he.x), clampd(lc.y, -he.y, he.y), clampd(lc.z, -he.z, he.z)};
+    const V3 ld = sub(closest, lc);
+    const float dist = length(ld);
+    if (dist > kTiny) {
+        normal = qrot(box.rotation, mul(1.0f / dist, ld));
+        sep = dist - sphere.radius;
+        feature = box_region(lc.x, he.x) + 3u * box_region(lc.y, he.y) + 9u * box_region(lc.z, he.z);
+    } else {
+        const V3 d = sub(he, vabs(lc));
+        int axis = 0;
+        if (d.y < comp(d, axis)) axis = 1;
+        if (d.z < comp(d, axis)) axis = 2;
+        const float sign = comp(lc, axis) >= 0.0f ? 1.0f : -1.0f;
+        V3 outward{0.0f, 0.0f, 0.0f};
+        setcomp(outward, axis, sign);
+        setcomp(closest, axis, sign * comp(he, axis));
+        normal = qrot(box.rotation, neg(outward));
+        sep = -comp(d, axis) - sphere.radius;
+        feature = 27u + 2u * (uint32_t)axis + (sign > 0.0f ? 1u : 0u);
+    }
+    box_point = add(box.center, qrot(box.rotation, closest));
+}
+
+MRD_INL void consider_capsule_box(V3 seg_point, V3 box_point, float t, uint32_t feat, float& best_sq, V3& best_seg,
+                                  V3& best_box, float& best_t, uint32_t& best_feat) {
+    const V3 d = sub(box_point, seg_point);
+    const float sq = dot(d, d);
+    const float tol = 1.0e-12f * (1.0f + fmaxd(best_sq, sq));
+    if (!isfin(best_sq) || sq < best_sq - tol || (fabs_(sq - best_sq) <= tol && feat < best_feat)) {
+        best_sq = sq;
+        best_seg = seg_point;
+        best_box = box_point;
+        best_t = t;
+        best_feat = feat;
+    }
+}
+
+MRD_INL void capsule_box(const Collider& cap, const Collider& box, uint32_t cap_index, uint32_t box_index,
+                         V3& normal, V3& cap_point, V3& box_point, float& sep, float& t_out, uint32_t& feature) {
+    const V3 s0 = qrot_inv(box.rotation, sub(cap.p0, box.center));
+    const V3 s1 = qrot_inv(box.rotation, sub(cap.p1, box.center));
+    const V3 dir = sub(s1, s0);
+    const V3 he = box.half_extents;
+    float enter = 0.0f, exit_ = 1.0f;
+    bool inter = true;
+    for (int axis = 0; axis < 3; ++axis) {
+        const float da = comp(dir, axis), sa = comp(s0, axis), ha = comp(he, axis);
+        if (fabs_(da) <= kTiny) {
+            if (sa < -ha || sa > ha) inter = false;
+            continue;
+        }
+        float first = (-ha - sa) / da;
+        float second = (ha - sa) / da;
+        if (first > second) {
+            const float tmp = first;
+            first = second;
+            second = tmp;
+        }
+        enter = fmaxd(enter, first);
+        exit_ = fmind(exit_, second);
+        if (enter > exit_) inter = false;
+    }
+    if (inter) {
+        t_out = clampd(0.5f * (enter + exit_), 0.0f, 1.0f);
+        const V3 core = add(s0, mul(t_out, dir));
+        const V3 d = sub(he, vabs(core));
+        int axis = 0;
+        if (d.y < d.x) axis = 1;
+        if (d.z < comp(d, axis)) axis = 2;
+        const float sign = comp(core, axis) >= 0.0f ? 1.0f : -1.0f;
+        V3 outward{0.0f, 0.0f, 0.0f};
+        setcomp(outward, axis, sign);
+        V3 bp = core;
+        setcomp(bp, axis, sign * comp(he, axis));
+        const V3 ln = neg(outward);
+        normal = qrot(box.rotation, ln);
+        cap_point = add(box.center, qrot(box.rotation, add(core, mul(cap.radius, ln))));
+        box_point = add(box.center, qrot(box.rotation, bp));
+        sep = -comp(d, axis) - cap.radius;
+        feature = 27u + 2u * (uint32_t)axis + (sign > 0.0f ? 1u : 0u);
+        return;
+    }
+    float best_sq = __builtin_inff();
+    V3 best_seg{0.0f, 0.0f, 0.0f}, best_box{0.0f, 0.0f, 0.0f};
+    float best_t = 0.0f;
+    uint32_t best_feat = 0xffffffffu;
+    for (uint32_t en = 0; en < 2u; ++en) {
+        const V3 sp = en == 0u ? s0 : s1;
+        const V3 bp{clampd(sp.x, -he.x, he.x), clampd(sp.y, -he.y, he.y), clampd(sp.z, -he.z, he.z)};
+        consider_capsule_box(sp, bp, (float)en, en, best_sq, best_seg, best_box, best_t, best_feat);
+    }
+    uint32_t edge_feature = 2u;
+    for (int va = 0; va < 3; ++va) {
+        const int f0 = (va + 1) % 3, f1 = (va + 2) % 3;
+        for (uint32_t signs = 0; signs < 4u; ++signs) {
+            V3 e0{0.0f, 0.0f, 0.0f}, e1{0.0f, 0.0f, 0.0f};
+            setcomp(e0, va, -comp(he, va));
+            setcomp(e1, va, comp(he, va));
+            const float sg0 = (signs & 1u) == 0u ? -1.0f : 1.0f;
+            const float sg1 = (signs & 2u) == 0u ? -1.0f : 1.0f;
+            setcomp(e0, f0, sg0 * comp(he, f0));
+            setcomp(e1, f0, comp(e0, f0));
+            setcomp(e0, f1, sg1 * comp(he, f1));
+            setcomp(e1, f1, comp(e0, f1));
+            const SegPair cl = closest_segments(s0, s1, e0, e1);
+            consider_capsule_box(cl.pa, cl.pb, cl.ta, edge_feature, best_sq, best_seg, best_box, best_t, best_feat);
+            ++edge_feature;
+        }
+    }
+    const float dist = sqrt_rn(fmaxd(best_sq, 0.0f));
+    const V3 ln = dist > kTiny ? mul(1.0f / dist, sub(best_box, best_seg))
+                               : qrot_inv(box.rotation, coincident_normal(cap_index, box_index));
+    normal = qrot(box.rotation, ln);
+    cap_point = add(box.center, qrot(box.rotation, add(best_seg, mul(cap.radius, ln))));
+    box_point = add(box.center, qrot(box.rotation, best_box));
+    sep = dist - cap.radius;
+    t_out = best_t;
+    feature = 64u + best_feat;
+}
+
+MRD_INL V3 box_axis(const Collider& box, int axis) {
+    V3 l{0.0f, 0.0f, 0.0f};
+    setcomp(l, axis, 1.0f);
+    return qrot(box.rotation, l);
+}
+
+MRD_INL float box_projection_radius(const Collider& box, V3 axis) {
+    return fabs_(dot(axis, box_axis(box, 0))) * box.half_extents.x +
+        fabs_(dot(axis, box_axis(box, 1))) * box.half_extents.y +
+        fabs_(dot(axis, box_axis(box, 2))) * box.half_extents.z;
+}
+
+MRD_INL V3 box_vertex(const Collider& box, uint32_t v) {
+    const V3 he = box.half_extents;
+    const V3 l{(v & 1u) == 0u ? -he.x : he.x, (v & 2u) == 0u ? -he.y : he.y, (v & 4u) == 0u ? -he.z : he.z};
+    return add(box.center, qrot(box.rotation, l));
+}
+
+MRD_INL bool point_in_inflated_box(V3 p, const Collider& box, float infl) {
+    const V3 l = vabs(qrot_inv(box.rotation, sub(p, box.center)));
+    return l.x <= box.half_extents.x + infl && l.y <= box.half_extents.y + infl && l.z <= box.half_extents.z + infl;
+}
+
+MRD_INL V3 box_support(const Collider& box, V3 d) {
+    V3 res = box.center;
+    for (int axis = 0; axis < 3; ++axis) {
+        const V3 basis = box_axis(box, axis);
+        const float he = comp(box.half_extents, axis);
+        res = add(res, mul(dot(d, basis) >= 0.0f ? he : -he, basis));
+    }
+    return res;
+}
+
+MRD_INL void append(RawBatch& out, const RawContact& c) {
+    if (out.count < MR_RAW_CONTACTS_PER_PAIR) out.c[out.count++] = c;
+}
+
+MRD_INL void box_box(uint32_t ia, uint32_t ib, const Collider& A, const Collider& B, float accepted, RawBatch& out) {
+    V3 axes[15];
+    uint32_t nax = 0;
+    for (int x = 0; x < 3; ++x) axes[nax++] = box_axis(A, x);
+    for (int x = 0; x < 3; ++x) axes[nax++] = box_axis(B, x);
+    for (int x = 0; x < 3; ++x)
+        for (int y = 0; y < 3; ++y) {
+            const V3 cr = cross(box_axis(A, x), box_axis(B, y));
+            const float sq = dot(cr, cr);
+            if (sq > 1.0e-12f) axes[nax++] = mul(1.0f / sqrt_rn(sq), cr);
+        }
+    const V3 delta = sub(B.center, A.center);
+    float best_sep = -__builtin_inff();
+    V3 best_n = coincident_normal(ia, ib);
+    uint32_t best_axis = 0;
+    for (uint32_t k = 0; k < nax; ++k) {
+        V3 axis = axes[k];
+        const float proj = dot(delta, axis);
+        if (proj < 0.0f || (proj == 0.0f && dot(axis, best_n) < 0.0f)) axis = neg(axis);
+        const float sep = fabs_(proj) - box_projection_radius(A, axis) - box_projection_radius(B, axis);
+        if (sep > accepted) return;
+        const float tie = 1.0e-7f * (1.0f + fmaxd(fabs_(best_sep), fabs_(sep)));
+        if (!isfin(best_sep) || sep > best_sep + tie || (fabs_(sep - best_sep) <= tie && k < best_axis)) {
+            best_sep = sep;
+            best_n = axis;
+            best_axis = k;
+        }
+    }
+    const float ra = box_projection_radius(A, best_n);
+    const float rb = box_projection_radius(B, best_n);
+    const float near_b = dot(B.center, best_n) - rb;
+    const float far_a = dot(A.center, best_n) + ra;
+    for (uint32_t vi = 0; vi < 8u; ++vi) {
+        const V3 pa = box_vertex(A, vi);
+        if (!point_in_inflated_box(pa, B, accepted)) continue;
+        const float sep = near_b - dot(pa, best_n);
+        if (sep <= accepted)
+            append(out, make_contact(best_n, sep, pa, add(pa, mul(sep, best_n)), feature_key(MR_SHAPE_BOX, vi),
+                                     feature_key(MR_SHAPE_BOX, 128u + best_axis)));
+    }
+    for (uint32_t vi = 0; vi < 8u; ++vi) {
+        const V3 pb = box_vertex(B, vi);
+        if (!point_in_inflated_box(pb, A, accepted)) continue;
+        const float sep = dot(pb, best_n) - far_a;
+        if (sep <= accepted)
+            append(out, make_contact(best_n, sep, sub(pb, mul(sep, best_n)), pb,
+                                     feature_key(MR_SHAPE_BOX, 128u + best_axis), feature_key(MR_SHAPE_BOX, vi)));
+    }
+    if (out.count == 0u) {
+        const V3 pa = box_support(A, best_n);
+        const V3 pb = box_support(B, neg(best_n));
+        append(out, make_contact(best_n, dot(sub(pb, pa), best_n), pa, pb, feature_key(MR_SHAPE_BOX, 256u + best_axis),
+                                 feature_key(MR_SHAPE_BOX, 256u + best_axis)));
+    }
+}
+
+MRD_INL void append_surface_contact(RawBatch& out, bool surface_is_a, V3 n, V3 finite_point, float sep,
+                                    uint32_t finite_feature, uint32_t surface_feature) {
+    const V3 sp = sub(finite_point, mul(sep, n));
+    RawContact c = make_contact(neg(n), sep, finite_point, sp, finite_feature, surface_feature);
+    if (surface_is_a) c = swapped(c);
+    append(out, c);
+}
+
+MRD_INL void heightfield_point(const Env& e, RawBatch& out, bool hf_is_a, V3 x, float rho, float accepted,
+                               uint32_t finite_feature) {
+    const MRHeightfieldC& hf = e.a->heightfield;
+    const float s = hf.origin_spacing.z;
+    const float fx = (x.x - hf.origin_spacing.x) / s;
+    const float fy = (x.y - hf.origin_spacing.y) / s;
+    if (!isfin(fx) || !isfin(fy)) return;
+    const int ix = (int)__builtin_floorf(clampd(fx, 0.0f, (float)(hf.nx - 2u)));
+    const int iy = (int)__builtin_floorf(clampd(fy, 0.0f, (float)(hf.ny - 2u)));
+    const float tx = fx - (float)ix;
+    const float ty = fy - (float)iy;
+    const uint32_t nx = hf.nx;
+    const float* H = e.a->heights;
+    const float h00 = H[(uint32_t)iy * nx + (uint32_t)ix];
+    const float h10 = H[(uint32_t)iy * nx + (uint32_t)ix + 1u];
+    const float h01 = H[((uint32_t)iy + 1u) * nx + (uint32_t)ix];
+    const float h11 = H[((uint32_t)iy + 1u) * nx + (uint32_t)ix + 1u];
+    const V3 v00{hf.origin_spacing.x + (float)ix * s, hf.origin_spacing.y + (float)iy * s, h00};
+    const uint32_t tri = tx >= ty ? 0u : 1u;
+    V3 e1, e2;
+    if (tri == 0u) {
+        e1 = V3{s, 0.0f, h10 - h00};
+        e2 = V3{s, s, h11 - h00};
+    } else {
+        e1 = V3{s, s, h11 - h00};
+        e2 = V3{0.0f, s, h01 - h00};
+    }
+    const V3 n = normalize(cross(e1, e2));
+    const float d = dot(n, sub(x, v00)) - rho;
+    if (d <= accepted) {
+        const uint32_t cell = (uint32_t)iy * (nx - 1u) + (uint32_t)ix;
+        append_surface_contact(out, hf_is_a, n, sub(x, mul(rho, n)), d, finite_feature,
+                               feature_key(MR_SHAPE_HEIGHTFIELD, 2u * cell + tri));
+    }
+}
+
+MRD_INL void generate_contacts(const Env& e, uint32_t ia, uint32_t ib, uint32_t cls, const Collider& A,
+                               const Collider& B, RawBatch& out) {
+    out.count = 0;
+    const float accepted = (A.contact_offset + B.contact_offset) + pair_query_padding(A, B);
+    if (cls == MR_PAIR_SPHERE_SPHERE) {
+        const V3 d = sub(B.center, A.center);
+        const float dist = length(d);
+        const V3 n = dist > kTiny ? mul(1.0f / dist, d) : coincident_normal(ia, ib);
+        const float sep = dist - A.radius - B.radius;
+        if (sep > accepted) return;
+        append(out, make_contact(n, sep, add(A.center, mul(A.radius, n)), sub(B.center, mul(B.radius, n)),
+                                 feature_key(MR_SHAPE_SPHERE, 0u), feature_key(MR_SHAPE_SPHERE, 0u)));
+    } else if (cls == MR_PAIR_SPHERE_CAPSULE) {
+        const bool sphere_a = A.type == MR_SHAPE_SPHERE;
+        const Collider& sp = sphere_a ? A : B;
+        const Collider& cp = sphere_a ? B : A;
+        const uint32_t si = sphere_a ? ia : ib, ci = sphere_a ? ib : ia;
+        const SegPoint cl = closest_on_segment(sp.center, cp.p0, cp.p1);
+        const V3 d = sub(cl.point, sp.center);
+        const float dist = length(d);
+        const V3 n = dist > kTiny ? mul(1.0f / dist, d)
+                                  : stable_segment_normal(sub(cp.p1, cp.p0), V3{0.0f, 0.0f, 0.0f}, si, ci);
+        const float sep = dist - sp.radius - cp.radius;
+        if (sep <= accepted) {
+            RawContact c = make_contact(n, sep, add(sp.center, mul(sp.radius, n)), sub(cl.point, mul(cp.radius, n)),
+                                        feature_key(MR_SHAPE_SPHERE, 0u),
+                                        feature_key(MR_SHAPE_CAPSULE, capsule_feature(cl.t)));
+            if (!sphere_a) c = swapped(c);
+            append(out, c);
+        }
+    } else if (cls == MR_PAIR_CAPSULE_CAPSULE) {
+        const SegPair cl = closest_segments(A.p0, A.p1, B.p0, B.p1);
+        const V3 d = sub(cl.pb, cl.pa);
+        const float dist = length(d);
+        const V3 n = dist > kTiny ? mul(1.0f / dist, d) : stable_segment_normal(sub(A.p1, A.p0), sub(B.p1, B.p0), ia, ib);
+        const float sep = dist - A.radius - B.radius;
+        if (sep <= accepted)
+            append(out, make_contact(n, sep, add(cl.pa, mul(A.radius, n)), sub(cl.pb, mul(B.radius, n)),
+                                     feature_key(MR_SHAPE_CAPSULE, capsule_feature(cl.ta)),
+                                     feature_key(MR_SHAPE_CAPSULE, capsule_feature(cl.tb))));
+    } else if (cls == MR_PAIR_SPHERE_BOX) {
+        const bool sphere_a = A.type == MR_SHAPE_SPHERE;
+        const Collider& sp = sphere_a ? A : B;
+        const Collider& bx = sphere_a ? B : A;
+        V3 n, bp;
+        float sep;
+        uint32_t feat;
+        sphere_box(sp, bx, n, bp, sep, feat);
+        if (sep <= accepted) {
+            RawContact c = make_contact(n, sep, add(sp.center, mul(sp.radius, n)), bp, feature_key(MR_SHAPE_SPHERE, 0u),
+                                        feature_key(MR_SHAPE_BOX, feat));
+            if (!sphere_a) c = swapped(c);
+            append(out, c);
+        }
+    } else if (cls == MR_PAIR_CAPSULE_BOX) {
+        const bool cap_a = A.type == MR_SHAPE_CAPSULE;
+        const Collider& cp = cap_a ? A : B;
+        const Collider& bx = cap_a ? B : A;
+        V3 n, cpp, bp;
+        float sep, t;
+        uint32_t feat;
+        capsule_box(cp, bx, cap_a ? ia : ib, cap_a ? ib : ia, n, cpp, bp, sep, t, feat);
+        if (sep <= accepted) {
+            RawContact c = make_contact(n, sep, cpp, bp, feature_key(MR_SHAPE_CAPSULE, capsule_feature(t)),
+                                        feature_key(MR_SHAPE_BOX, feat));
+            if (!cap_a) c = swapped(c);
+            append(out, c);
+        }
+    } else if (cls == MR_PAIR_BOX_BOX) {
+        box_box(ia, ib, A, B, accepted, out);
+    } else if (cls == MR_PAIR_SPHERE_PLANE || cls == MR_PAIR_CAPSULE_PLANE || cls == MR_PAIR_BOX_PLANE) {
+        const bool plane_a = A.type == MR_SHAPE_PLANE;
+        const Collider& pl = plane_a ? A : B;
+        const Collider& fs = plane_a ? B : A;
+        const uint32_t pf = feature_key(MR_SHAPE_PLANE, 0u);
+        if (cls == MR_PAIR_SPHERE_PLANE) {
+            const V3 surf = sub(fs.center, mul(fs.radius, pl.plane_normal));
+            const float sep = dot(pl.plane_normal, sub(surf, pl.center));
+            if (sep <= accepted)
+                append_surface_contact(out, plane_a, pl.plane_normal, surf, sep, feature_key(MR_SHAPE_SPHERE, 0u), pf);
+        } else if (cls == MR_PAIR_CAPSULE_PLANE) {
+            for (uint32_t en = 0; en < 2u; ++en) {
+                const V3 ep = en == 0u ? fs.p0 : fs.p1;
+                const V3 surf = sub(ep, mul(fs.radius, pl.plane_normal));
+                const float sep = dot(pl.plane_normal, sub(surf, pl.center));
+                if (sep <= accepted)
+                    append_surface_contact(out, plane_a, pl.plane_normal, surf, sep, feature_key(MR_SHAPE_CAPSULE, en),
+                                           pf);
+            }
+        } else {
+            for (uint32_t vi = 0; vi < 8u; ++vi) {
+                const V3 wv = box_vertex(fs, vi);
+                const float sep = dot(pl.plane_normal, sub(wv, pl.center));
+                if (sep <= accepted)
+                    append_surface_contact(out, plane_a, pl.plane_normal, wv, sep, feature_key(MR_SHAPE_BOX, vi), pf);
+            }
+        }
+    } else if (cls == MR_PAIR_HEIGHTFIELD) {
+        const bool hf_a = A.type == MR_SHAPE_HEIGHTFIELD;
+        const Collider& fs = hf_a ? B : A;
+        if (fs.type == MR_SHAPE_SPHERE) {
+            heightfield_point(e, out, hf_a, fs.center, fs.radius, accepted, feature_key(MR_SHAPE_SPHERE, 0u));
+        } else if (fs.type == MR_SHAPE_CAPSULE) {
+            heightfield_point(e, out, hf_a, fs.p0, fs.radius, accepted, feature_key(MR_SHAPE_CAPSULE, 0u));
+            heightfield_point(e, out, hf_a, fs.p1, fs.radius, accepted, feature_key(MR_SHAPE_CAPSULE, 1u));
+            heightfield_point(e, out, hf_a, fs.center, fs.radius, accepted, feature_key(MR_SHAPE_CAPSULE, 2u));
+        } else if (fs.type == MR_SHAPE_BOX) {
+            for (uint32_t vi = 0; vi < 8u; ++vi)
+                heightfield_point(e, out, hf_a, box_vertex(fs, vi), 0.0f, accepted, feature_key(MR_SHAPE_BOX, vi));
+        }
+    }
+}
+
+// --------------------------------------------------------------- manifold
+
+MRD_INL unsigned long long cand_feature(const Candidate& c) {
+    return ((unsigned long long)c.point.feature_and_life[0] << 32u) | (unsigned long long)c.point.feature_and_life[1];
+}
+
+MRD_INL bool cand_less(const Candidate& l, const Candidate& r) {
+    if (l.separation != r.separation) return l.separation < r.separation;
+    return cand_feature(l) < cand_feature(r);
+}
+
+MRD_INL V3 stable_tangent(V3 n) {
+    const V3 a = vabs(n);
+    V3 ref;
+    if (a.x <= a.y && a.x <= a.z) ref = V3{1.0f, 0.0f, 0.0f};
+    else if (a.y <= a.z) ref = V3{0.0f, 1.0f, 0.0f};
+    else ref = V3{0.0f, 0.0f, 1.0f};
+    return normalize(cross(ref, n));
+}
+
+MRD_INL void reduce_candidates(Candidate* c, uint32_t& count, V3 n) {
+    for (uint32_t i = 1; i < count; ++i) {
+        const Candidate val = c[i];
+        uint32_t d = i;
+        while (d > 0u && cand_less(val, c[d - 1u])) {
+            c[d] = c[d - 1u];
+            --d;
+        }
+        c[d] = val;
+    }
+    if (count > MR_MANIFOLD_POINT_CAPACITY) {
+        bool selected[kMaxCandidates];
+        for (uint32_t i = 0; i < kMaxCandidates; ++i) selected[i] = false;
+        uint32_t chosen[MR_MANIFOLD_POINT_CAPACITY];
+        chosen[0] = 0u;
+        selected[0] = true;
+        uint32_t nch = 1u;
+        for (uint32_t stage = 1; stage < MR_MANIFOLD_POINT_CAPACITY; ++stage) {
+            uint32_t best = MR_INVALID_INDEX;
+            float best_score = -1.0f;
+            unsigned long long best_feat = ~0ull;
+            for (uint32_t i = 0; i < count; ++i) {
+                if (selected[i]) continue;
+                float score;
+                if (stage == 1u) {
+                    const V3 d = sub(c[i].world_point, c[chosen[0]].world_point);
+                    const V3 t = sub(d, mul(dot(d, n), n));
+                    score = dot(t, t);
+                } else if (stage == 2u) {
+                    const V3 aa = sub(c[chosen[1]].world_point, c[chosen[0]].world_point);
+                    const V3 bb = sub(c[i].world_point, c[chosen[0]].world_point);
+                    score = fabs_(dot(cross(aa, bb), n));
+                } else {
+                    score = __builtin_inff();
+                    for (uint32_t s = 0; s < nch; ++s) {
+                        const V3 d = sub(c[i].world_point, c[chosen[s]].world_point);
+                        const V3 t = sub(d, mul(dot(d, n), n));
+                        score = fmind(score, dot(t, t));
+                    }
+                }
+                const unsigned long long feat = cand_feature(c[i]);
+                if (score > best_score + 1.0e-12f || (fabs_(score - best_score) <= 1.0e-12f && feat < best_feat)) {
+                    best = i;
+                    best_score = score;
+                    best_feat = feat;
+                }
+            }
+            if (best != MR_INVALID_INDEX) {
+                chosen[nch++] = best;
+                selected[best] = true;
+            }
+        }
+        Candidate red[MR_MANIFOLD_POINT_CAPACITY];
+        for (uint32_t i = 0; i < nch; ++i) red[i] = c[chosen[i]];
+        count = nch;
+        for (uint32_t i = 0; i < count; ++i) c[i] = red[i];
+    }
+    for (uint32_t i = 1; i < count; ++i) {
+        const Candidate val = c[i];
+        const unsigned long long f = cand_feature(val);
+        uint32_t d = i;
+        while (d > 0u && f < cand_feature(c[d - 1u])) {
+            c[d] = c[d - 1u];
+            --d;
+        }
+        c[d] = val;
+    }
+}
+
+MRD_INL mr_float4 anchor_from_world(V3 bpos, Q4 rot, V3 x) {
+    const V3 l = qrot_inv(rot, sub(x, bpos));
+    return mr_float4{l.x, l.y, l.z, 0.0f};
+}
+
+MRD_INL bool build_manifold(const Env& e, const MRCompiledPairC& pair, const RawBatch& raw,
+                            const MRManifoldHeaderC* old_h, const MRManifoldPointC* old_p, MRManifoldHeaderC& out_h,
+                            MRManifoldPointC* out_p, uint32_t& retained, uint32_t& fresh) {
+    if (raw.count == 0u) return false;
+    const MRShapeC& sa = e.shapes[pair.collider_a];
+    const MRShapeC& sb = e.shapes[pair.collider_b];
+    V3 pa_pos, pb_pos;
+    Q4 pa_q, pb_q;
+    body_pose(e, sa.body, pa_pos, pa_q);
+    body_pose(e, sb.body, pb_pos, pb_q);
+    Q4 ra, rb;
+    if (!checked_quat(pa_q, ra) || !checked_quat(pb_q, rb)) return false;
+    uint32_t deepest = 0;
+    for (uint32_t i = 1; i < raw.count; ++i) {
+        const RawContact& l = raw.c[i];
+        const RawContact& r = raw.c[deepest];
+        if (l.separation < r.separation ||
+            (l.separation == r.separation &&
+             (l.feature_a < r.feature_a || (l.feature_a == r.feature_a && l.feature_b < r.feature_b))))
+            deepest = i;
+    }
+    const V3 n = normalize(raw.c[deepest].normal);
+    if (!finite3(n) || dot(n, n) < 0.9999f) return false;
+    const MRWorldStepConfigC& cfg = e.a->cfg;
+    Candidate cand[kMaxCandidates];
+    uint32_t nc = 0;
+    bool compatible = false;
+    V3 transported{0.0f, 0.0f, 0.0f};
+    float old_age = 0.0f;
+    if (old_h != nullptr) {
+        const MRManifoldHeaderC oh = *old_h;
+        const bool gen = oh.generations_and_flags[0] == sa.slot_generation &&
+            oh.generations_and_flags[1] == sb.slot_generation && oh.pair_and_count[3] <= MR_MANIFOLD_POINT_CAPACITY;
+        if (gen) {
+            const V3 on = normalize(qrot(ra, f3(oh.normal_and_age)));
+            compatible = finite3(on) && dot(on, n) >= cfg.manifold_normal_cosine;
+            if (compatible) {
+                const V3 ot = qrot(ra, f3(oh.tangent_and_metric));
+                transported = sub(ot, mul(dot(ot, n), n));
+                old_age = oh.normal_and_age.w;
+                for (uint32_t p = 0; p < oh.pair_and_count[3]; ++p) {
+                    const MRManifoldPointC op = old_p[p];
+                    const V3 pa = add(pa_pos, qrot(ra, f3(op.local_anchor_a)));
+                    const V3 pb = add(pb_pos, qrot(rb, f3(op.local_anchor_b)));
+                    const V3 d = sub(pb, pa);
+                    const float sep = dot(d, n);
+                    const V3 tan = sub(d, mul(sep, n));
+                    const float drift = length(tan);
+                    if (sep > cfg.manifold_breaking_separation || drift > cfg.manifold_breaking_tangential) continue;
+                    Candidate c;
+                    c.point = op;
+                    c.point.feature_and_life[2] =
+                        op.feature_and_life[2] == 0xffffffffu ? 0xffffffffu : op.feature_and_life[2] + 1u;
+                    c.world_point = mul(0.5f, add(pa, pb));
+                    c.separation = sep;
+                    c.drift = drift;
+                    cand[nc++] = c;
+                    ++retained;
+                }
+            }
+        }
+    }
+    const float merge_sq = cfg.manifold_merge_distance * cfg.manifold_merge_distance;
+    for (uint32_t k = 0; k < raw.count; ++k) {
+        const RawContact& rc = raw.c[k];
+        const mr_float4 la = anchor_from_world(pa_pos, ra, rc.point_a);
+        const mr_float4 lb = anchor_from_world(pb_pos, rb, rc.point_b);
+        uint32_t match = MR_INVALID_INDEX;
+        for (uint32_t i = 0; i < nc; ++i) {
+            if (cand[i].point.feature_and_life[0] == rc.feature_a && cand[i].point.feature_and_life[1] == rc.feature_b) {
+                match = i;
+                break;
+            }
+        }
+        if (match == MR_INVALID_INDEX) {
+            for (uint32_t i = 0; i < nc; ++i) {
+                const V3 da = sub(f3(cand[i].point.local_anchor_a), f3(la));
+                const V3 db = sub(f3(cand[i].point.local_anchor_b), f3(lb));
+                if (dot(da, da) + dot(db, db) <= 2.0f * merge_sq) {
+                    match = i;
+                    break;
+                }
+            }
+        }
+        if (match == MR_INVALID_INDEX) {
+            if (nc >= kMaxCandidates) return false;
+            match = nc++;
+            Candidate z;
+            z.point.local_anchor_a = mr_float4{0.0f, 0.0f, 0.0f, 0.0f};
+            z.point.local_anchor_b = mr_float4{0.0f, 0.0f, 0.0f, 0.0f};
+            z.point.impulses = mr_float4{0.0f, 0.0f, 0.0f, 0.0f};
+            z.point.feature_and_life[0] = 0u;
+            z.point.feature_and_life[1] = 0u;
+            z.point.feature_and_life[2] = 0u;
+            z.point.feature_and_life[3] = 0u;
+            z.world_point = V3{0.0f, 0.0f, 0.0f};
+            z.separation = 0.0f;
+            z.drift = 0.0f;
+            cand[match] = z;
+            ++fresh;
+        }
+        cand[match].point.local_anchor_a = la;
+        cand[match].point.local_anchor_b = lb;
+        cand[match].point.feature_and_life[0] = rc.feature_a;
+        cand[match].point.feature_and_life[1] = rc.feature_b;
+        cand[match].point.feature_and_life[3] = 0u;
+        cand[match].world_point = mul(0.5f, add(rc.point_a, rc.point_b));
+        cand[match].separation = rc.separation;
+        cand[match].drift = 0.0f;
+    }
+    if (nc == 0u) return false;
+    reduce_candidates(cand, nc, n);
+    V3 t = transported;
+    if (!compatible || !finite3(t) || dot(t, t) <= 1.0e-12f) t = stable_tangent(n);
+    else t = normalize(t);
+    out_h.pair_and_count[0] = e.env;
+    out_h.pair_and_count[1] = pair.collider_a;
+    out_h.pair_and_count[2] = pair.collider_b;
+    out_h.pair_and_count[3] = nc;
+    out_h.generations_and_flags[0] = sa.slot_generation;
+    out_h.generations_and_flags[1] = sb.slot_generation;
+    out_h.generations_and_flags[2] = 0u;
+    out_h.generations_and_flags[3] = 0u;
+    const V3 ln = qrot_inv(ra, n);
+    out_h.normal_and_age = mr_float4{ln.x, ln.y, ln.z, compatible ? fmind(old_age + 1.0f, 65535.0f) : 0.0f};
+    float metric = 0.0f;
+    for (uint32_t i = 0; i < nc; ++i) {
+        out_p[i] = cand[i].point;
+        metric = fmaxd(metric, fmaxd(fmaxd(cand[i].separation, 0.0f), cand[i].drift));
+    }
+    for (uint32_t i = nc; i < MR_MANIFOLD_POINT_CAPACITY; ++i) {
+        out_p[i].local_anchor_a = mr_float4{0.0f, 0.0f, 0.0f, 0.0f};
+        out_p[i].local_anchor_b = mr_float4{0.0f, 0.0f, 0.0f, 0.0f};
+        out_p[i].impulses = mr_float4{0.0f, 0.0f, 0.0f, 0.0f};
+        out_p[i].feature_and_life[0] = 0u;
+        out_p[i].feature_and_life[1] = 0u;
+        out_p[i].feature_and_life[2] = 0u;
+        out_p[i].feature_and_life[3] = 0u;
+    }
+    const V3 lt = qrot_inv(ra, t);
+    out_h.tangent_and_metric = mr_float4{lt.x, lt.y, lt.z, metric};
+    return true;
+}
+
+// ------------------------------------------------------------ cone solve
+
+MRD_INL void project_cone(float lam[3], float mu_u, float mu_v) {
+    lam[0] = fmaxd(lam[0], 0.0f);
+    const float lu = mu_u * lam[0];
+    const float lv = mu_v * lam[0];
+    if (!(lu > 0.0f) || !(lv > 0.0f)) {
+        lam[1] = 0.0f;
+        lam[2] = 0.0f;
+        return;
+    }
+    const float ns = (lam[1] * lam[1]) / (lu * lu) + (lam[2] * lam[2]) / (lv * lv);
+    if (ns > 1.0f) {
+        const float s = 1.0f / sqrt_rn(ns);
+        lam[1] *= s;
+        lam[2] *= s;
+    }
+}
+
+MRD_INL bool invert3x3(const float m[3][3], float inv[9]) {
+    float scale = 0.0f;
+    for (int rr = 0; rr < 3; ++rr)
+        for (int cc = 0; cc < 3; ++cc) {
+            if (!isfin(m[rr][cc])) return false;
+            scale = fmaxd(scale, fabs_(m[rr][cc]));
+        }
+    if (!(scale > kMatrixFloor)) return false;
+    float g[3][3];
+    for (int rr = 0; rr < 3; ++rr)
+        for (int cc = 0; cc < 3; ++cc) {
+            g[rr][cc] = 0.5f * (m[rr][cc] + m[cc][rr]) / scale;
+            if (rr == cc) g[rr][cc] += kContactRegularization;
+        }
+    const float c00 = g[1][1] * g[2][2] - g[1][2] * g[2][1];
+    const float c01 = g[1][2] * g[2][0] - g[1][0] * g[2][2];
+    const float c02 = g[1][0] * g[2][1] - g[1][1] * g[2][0];
+    const float det = g[0][0] * c00 + g[0][1] * c01 + g[0][2] * c02;
+    if (!(det > kMatrixFloor) || !isfin(det)) return false;
+    const float rcp = 1.0f / (det * scale);
+    inv[0] = c00 * rcp;
+    inv[1] = (g[0][2] * g[2][1] - g[0][1] * g[2][2]) * rcp;
+    inv[2] = (g[0][1] * g[1][2] - g[0][2] * g[1][1]) * rcp;
+    inv[3] = c01 * rcp;
+    inv[4] = (g[0][0] * g[2][2] - g[0][2] * g[2][0]) * rcp;
+    inv[5] = (g[0][2] * g[1][0] - g[0][0] * g[1][2]) * rcp;
+    inv[6] = c02 * rcp;
+    inv[7] = (g[0][1] * g[2][0] - g[0][0] * g[2][1]) * rcp;
+    inv[8] = (g[0][0] * g[1][1] - g[0][1] * g[1][0]) * rcp;
+    for (int k = 0; k < 9; ++k)
+        if (!isfin(inv[k])) return false;
+    return true;
+}
+
+MRD_INL V3 inv_inertia_mul(const MRBodyStateC& b, V3 x) {
+    return V3{dot(f3(b.inverse_inertia_world_row0), x), dot(f3(b.inverse_inertia_world_row1), x),
+              dot(f3(b.inverse_inertia_world_row2), x)};
+}
+
+MRD_INL V3 scene_response(const MRBodyStateC& b, V3 x, V3 d) {
+    const V3 lever = sub(x, f3(b.position));
+    const V3 dw = inv_inertia_mul(b, cross(lever, d));
+    return add(mul(b.linear_velocity_inv_mass.w, d), cross(dw, lever));
+}
+
+MRD_INL V3 scene_point_velocity(const MRBodyStateC& b, V3 x) {
+    return add(f3(b.linear_velocity_inv_mass), cross(f3(b.angular_velocity), sub(x, f3(b.position))));
+}
+
+MRD_INL void apply_scene_impulse(MRBodyStateC& b, V3 x, V3 imp) {
+    const V3 lever = sub(x, f3(b.position));
+    const V3 dv = mul(b.linear_velocity_inv_mass.w, imp);
+    b.linear_velocity_inv_mass.x += dv.x;
+    b.linear_velocity_inv_mass.y += dv.y;
+    b.linear_velocity_inv_mass.z += dv.z;
+    const V3 dw = inv_inertia_mul(b, cross(lever, imp));
+    b.angular_velocity.x += dw.x;
+    b.angular_velocity.y += dw.y;
+    b.angular_velocity.z += dw.z;
+}
+
+MRD_INL V3 hot_dir(const HotConstraint& h, int r) { return V3{h.dir[3 * r], h.dir[3 * r + 1], h.dir[3 * r + 2]}; }
+
+// Relative velocity rows of one constraint from the current velocities.
+MRD_INL float relative_row(const Env& e, const HotConstraint& k, uint32_t c, int row) {
+    float a = 0.0f;
+    if (k.art != 0u) {
+        const float* jr = e.jrow + ((size_t)c * 3u + (size_t)row) * e.a->max_nv;
+        for (uint32_t j = 0; j < e.nv; ++j) a += jr[j] * e.vcand[j];
+    }
+    const V3 x{k.point[0], k.point[1], k.point[2]};
+    const V3 d = hot_dir(k, row);
+    const V3 zero{0.0f, 0.0f, 0.0f};
+    const V3 va = is_dynamic_scene(e, k.body_a) ? scene_point_velocity(e.scene[k.body_a - e.nlinks], x) : zero;
+    const V3 vb = is_dynamic_scene(e, k.body_b) ? scene_point_velocity(e.scene[k.body_b - e.nlinks], x) : zero;
+    return (a + dot(d, vb)) - dot(d, va);
+}
+
+} // namespace
+
+// --------------------------------------------------------------- kernel
+
+extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelArgs args) {
+    extern __shared__ __align__(16) unsigned char lds_base[];
+    __shared__ unsigned long long s_fail;
+    __shared__ uint32_t s_count[8];
+    __shared__ float s_fmax[2];
+    __shared__ float s_root[7];
+
+    const uint32_t lane = threadIdx.x;
+    const uint32_t env = blockIdx.x;
+    if (env >= args.environment_count) return;
+    const KernelArgs& A = args;
+
+    Env e;
+    e.a = &A;
+    e.env = env;
+    e.hd = &A.models[A.environment_model[env]];
+    const MRModelHeaderC& hd = *e.hd;
+    e.bodies = A.bodies + hd.body_offset;
+    e.dofs = A.dofs + hd.dof_offset;
+    e.shapes = A.shapes + hd.shape_offset;
+    e.mats = A.materials + hd.material_offset;
+    e.pairs = A.pairs + hd.pair_offset;
+    e.nlinks = hd.link_count;
+    e.nbodies = hd.body_count;
+    e.nv = hd.nv;
+    e.nq = hd.nq;
+    e.nshapes = hd.shape_count;
+    e.npairs = hd.pair_count;
+    e.nscene = hd.scene_body_count;
+    e.levels = hd.level_count;
+    e.root_type = hd.root_type;
+    e.h = A.h;
+    e.gravity = V3{A.gravity[0], A.gravity[1], A.gravity[2]};
+    const mr::LdsLayout& L = A.lds;
+    e.q = lds_at<float>(lds_base, L.q);
+    e.v = lds_at<float>(lds_base, L.v);
+    e.vcand = lds_at<float>(lds_base, L.vcand);
+    e.effort = lds_at<float>(lds_base, L.effort);
+    e.accel = lds_at<float>(lds_base, L.accel);
+    e.cmd = lds_at<float>(lds_base, L.cmd);
+    e.pos = lds_at<V3>(lds_base, L.pos);
+    e.rot = lds_at<Q4>(lds_base, L.rot);
+    e.w = lds_at<V3>(lds_base, L.w);
+    e.vl = lds_at<V3>(lds_base, L.vl);
+    e.sa = lds_at<V3>(lds_base, L.sa);
+    e.sl = lds_at<V3>(lds_base, L.sl);
+    e.ca = lds_at<V3>(lds_base, L.ca);
+    e.cl = lds_at<V3>(lds_base, L.cl);
+    e.r = lds_at<V3>(lds_base, L.r);
+    e.I = lds_at<SpI>(lds_base, L.I);
+    e.pt = lds_at<V3>(lds_base, L.pt);
+    e.pb = lds_at<V3>(lds_base, L.pb);
+    e.Ut = lds_at<V3>(lds_base, L.Ut);
+    e.Ub = lds_at<V3>(lds_base, L.Ub);
+    e.D = lds_at<float>(lds_base, L.D);
+    e.u = lds_at<float>(lds_base, L.u);
+    e.alpha = lds_at<V3>(lds_base, L.alpha);
+    e.acc = lds_at<V3>(lds_base, L.acc);
+    e.L = lds_at<float>(lds_base, L.L);
+    e.scene = lds_at<MRBodyStateC>(lds_base, L.scene);
+    e.col = lds_at<Collider>(lds_base, L.col);
+    e.active = lds_at<uint32_t>(lds_base, L.active);
+    e.npts = lds_at<uint32_t>(lds_base, L.npts);
+    e.moff = lds_at<uint32_t>(lds_base, L.moff);
+    e.hot = lds_at<HotConstraint>(lds_base, L.hot);
+    const uint32_t pcap = A.pair_capacity, mcap = A.manifold_capacity, ccap = A.constraint_capacity;
+    e.stage_h = A.stage_h + (size_t)env * pcap;
+    e.stage_p = A.stage_p + (size_t)env * pcap * MR_MANIFOLD_POINT_CAPACITY;
+    e.jrow = A.jrow + (size_t)env * ccap * 3u * A.max_nv;
+    e.resp = A.resp + (size_t)env * ccap * 3u * A.max_nv;
+    e.cold = A.cold + (size_t)env * ccap;
+
+    const uint32_t nv = e.nv, nq = e.nq, nl = e.nlinks, nsc = e.nscene;
+    float* gq = A.q + (size_t)env * A.max_nq;
+    float* gv = A.v + (size_t)env * A.max_nv;
+    MRBodyStateC* gs = A.scene + (size_t)env * A.max_scene;
+    const uint32_t cur = A.mcurrent[env];
+    const size_t E = A.environment_count;
+
+    // ---- prepare: reset, load state (the HBM state is the checkpoint)
+    const bool reset = A.reset_mask != nullptr && A.reset_mask[env] != 0u;
+    for (uint32_t i = lane; i < nq; i += kLanes) {
+        const float val = reset ? A.reset_q[(size_t)env * A.max_nq + i] : gq[i];
+        e.q[i] = val;
+        if (reset) gq[i] = val;
+    }
+    for (uint32_t i = lane; i < nv; i += kLanes) {
+        const float val = reset ? A.reset_v[(size_t)env * A.max_nv + i] : gv[i];
+        e.v[i] = val;
+        if (reset) gv[i] = val;
+        e.cmd[i] = A.command[(size_t)env * A.max_nv + i];
+    }
+    for (uint32_t s = lane; s < nsc; s += kLanes) {
+        const MRBodyStateC val = reset ? A.reset_scene[(size_t)env * A.max_scene + s] : gs[s];
+        e.scene[s] = val;
+        if (reset) gs[s] = val;
+    }
+    if (lane == 0u) {
+        if (reset) A.mcount[(size_t)cur * E + env] = 0u;
+        s_fail = ~0ull;
+    }
+    __syncthreads();
+
+    uint32_t src_buf = cur;
+    uint32_t dst_buf = (cur + 1u) % 3u;
+    const uint32_t alt_buf = (cur + 2u) % 3u;
+    uint32_t mcount_src = A.mcount[(size_t)cur * E + env];
+    if (mcount_src > mcap) mcount_src = mcap;
+
+    MRContactStatusC cs;  // lane 0 copy is authoritative
+    {
+        unsigned char* p = reinterpret_cast<unsigned char*>(&cs);
+        for (uint32_t k = 0; k < sizeof(cs); ++k) p[k] = 0;
+    }
+    cs.first_failing_pair = MR_INVALID_INDEX;
+    cs.first_failing_constraint = MR_INVALID_INDEX;
+    uint32_t successful = 0u;
+    float min_piv = __builtin_inff(), max_piv = 0.0f, max_acc = 0.0f;
+    uint32_t failing_substep = MR_INVALID_INDEX;
+    const float h = e.h;
+    const MRWorldStepConfigC& cfg = A.cfg;
+    const bool floating = e.root_type == MR_ROOT_FLOATING;
+
+    for (uint32_t substep = 0; substep < cfg.physics_substeps; ++substep) {
+        const bool final_sub = substep + 1u == cfg.physics_substeps;
+        const bool capture = cfg.capture_evidence != 0u && substep == cfg.capture_substep && A.evidence_pairs != nullptr;
+        // ---------------------------------------------------------- drives
+        for (uint32_t j = lane; j < nv; j += kLanes) e.effort[j] = drive_effort(e, j);
+        __syncthreads();
+        // ------------------------------------------------------ kinematics
+        if (lane == 0u) {
+            if (floating) {
+                Q4 rq;
+                if (!checked_quat(Q4{e.q[3], e.q[4], e.q[5], e.q[6]}, rq)) {
+                    report_failure(&s_fail, 0u, MR_STEP_NONFINITE_INPUT, 3u);
+                    rq = Q4{0.0f, 0.0f, 0.0f, 1.0f};
+                }
+                e.pos[0] = V3{e.q[0], e.q[1], e.q[2]};
+                e.rot[0] = rq;
+                e.vl[0] = V3{e.v[0], e.v[1], e.v[2]};
+                e.w[0] = V3{e.v[3], e.v[4], e.v[5]};
+            } else {
+                e.pos[0] = V3{0.0f, 0.0f, 0.0f};
+                e.rot[0] = Q4{0.0f, 0.0f, 0.0f, 1.0f};
+                e.vl[0] = V3{0.0f, 0.0f, 0.0f};
+                e.w[0] = V3{0.0f, 0.0f, 0.0f};
+            }
+            const V3 zero{0.0f, 0.0f, 0.0f};
+            e.r[0] = zero;
+            e.sa[0] = zero;
+            e.sl[0] = zero;
+            e.ca[0] = zero;
+            e.cl[0] = zero;
+        }
+        __syncthreads();
+        for (uint32_t d = 1; d < e.levels; ++d) {
+            for (uint32_t i = 1u + lane; i < nl; i += kLanes)
+                if (e.bodies[i].depth == d && !fk_link(e, i))
+                    report_failure(&s_fail, 1u + i, MR_STEP_NONFINITE_RESULT, i);
+            __syncthreads();
+        }
+        if (s_fail != ~0ull) break;
+        // -------------------------------------------------------------- ABA
+        for (uint32_t i = lane; i < nl; i += kLanes) aba_init_link(e, i);
+        __syncthreads();
+        if (lane == 0u && floating) {
+            SpI& I = e.I[0];
+            I.C[0] += effective_armature(e, 0);
+            I.C[3] += effective_armature(e, 1);
+            I.C[5] += effective_armature(e, 2);
+            I.A[0] += effective_armature(e, 3);
+            I.A[3] += effective_armature(e, 4);
+            I.A[5] += effective_armature(e, 5);
+            e.pt[0] = sub(e.pt[0], V3{e.effort[3], e.effort[4], e.effort[5]});
+            e.pb[0] = sub(e.pb[0], V3{e.effort[0], e.effort[1], e.effort[2]});
+        }
+        __syncthreads();
+        float lane_min_piv = __builtin_inff(), lane_max_piv = 0.0f;
+        for (uint32_t d = e.levels - 1u; d >= 1u; --d) {
+            for (uint32_t i = 1u + lane; i < nl; i += kLanes) {
+                if (e.bodies[i].depth != d) continue;
+                float piv;
+                if (!aba_project_link(e, i, piv))
+                    report_failure(&s_fail, 0xffffu - i, MR_STEP_FACTORIZATION_FAILED, e.bodies[i].v_index);
+                if (piv >= 0.0f) {
+                    lane_min_piv = fmind(lane_min_piv, piv);
+                    lane_max_piv = fmaxd(lane_max_piv, piv);
+                }
+            }
+            __syncthreads();
+            for (uint32_t p = lane; p < nl; p += kLanes)
+                if (e.bodies[p].depth == d - 1u) aba_accumulate(e, p);
+            __syncthreads();
+        }
+        if (s_fail != ~0ull) break;
+        lane_min_piv = -wave_max(-lane_min_piv);
+        lane_max_piv = wave_max(lane_max_piv);
+        if (lane == 0u) {
+            if (floating) {
+                uint32_t fail_row = 0;
+                float mn = lane_min_piv, mx = lane_max_piv;
+                if (!root_factor(e, mn, mx, fail_row)) {
+                    report_failure(&s_fail, 0x10000u, MR_STEP_FACTORIZATION_FAILED, fail_row);
+                } else {
+                    const float rhs[6] = {-e.pt[0].x, -e.pt[0].y, -e.pt[0].z, -e.pb[0].x, -e.pb[0].y, -e.pb[0].z};
+                    float x[6];
+                    root_solve(e.L, rhs, x);
+                    e.alpha[0] = V3{x[0], x[1], x[2]};
+                    e.acc[0] = V3{x[3], x[4], x[5]};
+                    e.accel[0] = x[3]; e.accel[1] = x[4]; e.accel[2] = x[5];
+                    e.accel[3] = x[0]; e.accel[4] = x[1]; e.accel[5] = x[2];
+                }
+                lane_min_piv = mn;
+                lane_max_piv = mx;
+            } else {
+                e.alpha[0] = V3{0.0f, 0.0f, 0.0f};
+                e.acc[0] = V3{0.0f, 0.0f, 0.0f};
+            }
+            s_fmax[0] = lane_min_piv;
+            s_fmax[1] = lane_max_piv;
+        }
+        __syncthreads();
+        if (s_fail != ~0ull) break;
+        for (uint32_t d = 1; d < e.levels; ++d) {
+            for (uint32_t i = 1u + lane; i < nl; i += kLanes) {
+                if (e.bodies[i].depth != d) continue;
+                const MRBodyC& b = e.bodies[i];
+                const V3 ap = e.alpha[b.parent];
+                const V3 lp = add(e.acc[b.parent], cross(ap, e.r[i]));
+                V3 ai = add(ap, e.ca[i]);
+                V3 li = add(lp, e.cl[i]);
+                if (b.v_index != MR_INVALID_INDEX) {
+                    const float proj = dot(e.Ut[i], ap) + dot(e.Ub[i], lp);
+                    const float qdd = (e.u[i] - proj) / e.D[i];
+                    e.accel[b.v_index] = qdd;
+                    ai = add(ai, mul(qdd, e.sa[i]));
+                    li = add(li, mul(qdd, e.sl[i]));
+                }
+                e.alpha[i] = ai;
+                e.acc[i] = li;
+            }
+            __syncthreads();
+        }
+        for (uint32_t j = lane; j < nv; j += kLanes) {
+            const float a = e.accel[j];
+            const float nvv = e.v[j] + h * a;
+            if (!isfin(a) || !isfin(nvv)) report_failure(&s_fail, j, MR_STEP_NONFINITE_RESULT, j);
+            e.vcand[j] = nvv;
+        }
+        __syncthreads();
+        if (s_fail != ~0ull) break;
+        // ------------------------------------------- scene prediction
+        for (uint32_t s = lane; s < nsc; s += kLanes) {
+            const uint32_t b = nl + s;
+            const MRBodyC& bp = e.bodies[b];
+            MRBodyStateC st = e.scene[s];
+            Q4 rq;
+            if (!normalized_quat(q4(st.orientation), rq) || !finite3(f3(st.position)) ||
+                !finite3(f3(st.linear_velocity_inv_mass)) || !finite3(f3(st.angular_velocity))) {
+                report_failure(&s_fail, b, MR_STEP_NONFINITE_INPUT, b);
+                continue;
+            }
+            st.position.w = 1.0f;
+            st.orientation = mr_float4{rq.x, rq.y, rq.z, rq.w};
+            st.linear_velocity_inv_mass.w = bp.motion_type == MR_MOTION_DYNAMIC ? bp.mass.y : 0.0f;
+            st.angular_velocity.w = 0.0f;
+            st.flags_indices[0] = bp.motion_type;
+            st.flags_indices[1] = MR_INVALID_INDEX;
+            st.flags_indices[2] = b;
+            world_inverse_inertia(bp, rq, st);
+            if (bp.motion_type == MR_MOTION_DYNAMIC) {
+                const V3 lv = add(mul(bp.damping_scale.x, f3(st.linear_velocity_inv_mass)), mul(h, e.gravity));
+                const V3 av = mul(bp.damping_scale.y, f3(st.angular_velocity));
+                st.linear_velocity_inv_mass.x = lv.x;
+                st.linear_velocity_inv_mass.y = lv.y;
+                st.linear_velocity_inv_mass.z = lv.z;
+                st.angular_velocity.x = av.x;
+                st.angular_velocity.y = av.y;
+                st.angular_velocity.z = av.z;
+            } else {
+                st.linear_velocity_inv_mass.x = 0.0f;
+                st.linear_velocity_inv_mass.y = 0.0f;
+                st.linear_velocity_inv_mass.z = 0.0f;
+                st.angular_velocity.x = 0.0f;
+                st.angular_velocity.y = 0.0f;
+                st.angular_velocity.z = 0.0f;
+            }
+            e.scene[s] = st;
+        }
+        __syncthreads();
+        if (s_fail != ~0ull) break;
+        // ------------------------------------------- collider projection
+        for (uint32_t s = lane; s < e.nshapes; s += kLanes) {
+            Collider c;
+            if (!project_collider(e, s, c)) report_failure(&s_fail, s, MR_STEP_NONFINITE_INPUT, s);
+            e.col[s] = c;
+        }
+        __syncthreads();
+        if (s_fail != ~0ull) break;
+        // ----------------------------------------------- broadphase flags
+        uint32_t nact = 0;
+        for (uint32_t base = 0; base < e.npairs; base += kLanes) {
+            const uint32_t p = base + lane;
+            bool flag = false;
+            if (p < e.npairs) {
+                const MRCompiledPairC pr = e.pairs[p];
+                flag = pair_may_overlap(e.col[pr.collider_a], e.col[pr.collider_b]);
+            }
+            const unsigned long long bal = __ballot(flag);
+            const uint32_t before = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+            if (flag) {
+                const uint32_t slot = nact + before;
+                if (slot < pcap) e.active[slot] = p;
+            }
+            nact += (uint32_t)__popcll(bal);
+        }
+        __syncthreads();
+        if (capture) {
+            const uint32_t nrec = nact < pcap ? nact : pcap;
+            for (uint32_t k = lane; k < nrec; k += kLanes) {
+                const MRCompiledPairC pr = e.pairs[e.active[k]];
+                MRPairRecordC rec;
+                rec.environment = env;
+                rec.collider_a = pr.collider_a;
+                rec.collider_b = pr.collider_b;
+                rec.pair_class = pr.pair_class;
+                A.evidence_pairs[(size_t)env * pcap + k] = rec;
+            }
+            if (lane == 0u && A.evidence_counts != nullptr) A.evidence_counts[env] = nact;
+        }
+        cs.required_pairs = nact;
+        cs.active_pairs = nact;
+        if (cs.max_active_pairs < nact) cs.max_active_pairs = nact;
+        if (nact > pcap) {
+            if (lane == 0u) report_failure(&s_fail, 0u, MR_STEP_PAIR_CAPACITY_OVERFLOW, 0u);
+            // first overflowing compiled pair is the (pcap+1)-th flagged pair
+            uint32_t seen = 0, idx = 0;
+            for (uint32_t base = 0; base < e.npairs; base += kLanes) {
+                const uint32_t p = base + lane;
+                bool flag = false;
+                if (p < e.npairs) {
+                    const MRCompiledPairC pr = e.pairs[p];
+                    flag = pair_may_overlap(e.col[pr.collider_a], e.col[pr.collider_b]);
+                }
+                const unsigned long long bal = __ballot(flag);
+                const uint32_t before = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                if (flag && seen + before == pcap) idx = p + 1u;
+                seen += (uint32_t)__popcll(bal);
+            }
+            idx = wave_max_u32(idx);
+            if (lane == 0u) cs.first_failing_pair = idx - 1u;
+            __syncthreads();
+            if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_PAIR_CAPACITY_OVERFLOW << 24) | ((idx - 1u) & 0xffffffu);
+            __syncthreads();
+            break;
+        }
+        // ---------------------------------- narrowphase + manifold refresh
+        const MRManifoldHeaderC* old_h = A.mh + ((size_t)src_buf * E + env) * mcap;
+        const MRManifoldPointC* old_p = A.mp + ((size_t)src_buf * E + env) * mcap * MR_MANIFOLD_POINT_CAPACITY;
+        uint32_t my_raw = 0, my_ret = 0, my_fresh = 0;
+        float my_pen = 0.0f;
+        for (uint32_t k = lane; k < nact; k += kLanes) {
+            const MRCompiledPairC pr = e.pairs[e.active[k]];
+            const Collider ca_ = e.col[pr.collider_a];
+            const Collider cb_ = e.col[pr.collider_b];
+            RawBatch raw;
+            generate_contacts(e, pr.collider_a, pr.collider_b, pr.pair_class, ca_, cb_, raw);
+            bool bad = false;
+            for (uint32_t c = 0; c < raw.count; ++c)
+                if (!finite3(raw.c[c].normal) || !isfin(raw.c[c].separation) || !finite3(raw.c[c].point_a) ||
+                    !finite3(raw.c[c].point_b))
+                    bad = true;
+            uint32_t points = 0;
+            if (bad) {
+                report_failure(&s_fail, k, MR_STEP_NONFINITE_RESULT, e.active[k]);
+            } else if (raw.count > 0u) {
+                my_raw += raw.count;
+                const float rest = e.shapes[pr.collider_a].contact_rest_radius.y + e.shapes[pr.collider_b].contact_rest_radius.y;
+                for (uint32_t c = 0; c < raw.count; ++c) my_pen = fmaxd(my_pen, fmaxd(rest - raw.c[c].separation, 0.0f));
+                const MRManifoldHeaderC* oh = nullptr;
+                const MRManifoldPointC* op = nullptr;
+                for (uint32_t m = 0; m < mcount_src; ++m) {
+                    if (old_h[m].pair_and_count[1] == pr.collider_a && old_h[m].pair_and_count[2] == pr.collider_b) {
+                        oh = &old_h[m];
+                        op = &old_p[(size_t)m * MR_MANIFOLD_POINT_CAPACITY];
+                        break;
+                    }
+                }
+                MRManifoldHeaderC mh;
+                MRManifoldPointC mp[MR_MANIFOLD_POINT_CAPACITY];
+                if (!build_manifold(e, pr, raw, oh, op, mh, mp, my_ret, my_fresh)) {
+                    report_failure(&s_fail, k, MR_STEP_NONFINITE_RESULT, e.active[k]);
+                } else {
+                    points = mh.pair_and_count[3];
+                    e.stage_h[k] = mh;
+                    for (uint32_t p = 0; p < MR_MANIFOLD_POINT_CAPACITY; ++p)
+                        e.stage_p[(size_t)k * MR_MANIFOLD_POINT_CAPACITY + p] = mp[p];
+                }
+            }
+            e.npts[k] = points;
+        }
+        const uint32_t raw_total = wave_sum_u32(my_raw);
+        const uint32_t ret_total = wave_sum_u32(my_ret);
+        const uint32_t fresh_total = wave_sum_u32(my_fresh);
+        const float pen = wave_max(my_pen);
+        __syncthreads();
+        if (s_fail != ~0ull) break;
+        // prefix over manifolds (count>0) and constraint offsets
+        uint32_t nman = 0, ncons = 0;
+        for (uint32_t base = 0; base < nact; base += kLanes) {
+            const uint32_t k = base + lane;
+            const uint32_t pts = k < nact ? e.npts[k] : 0u;
+            const unsigned long long bal = __ballot(pts > 0u);
+            const uint32_t mbefore = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+            uint32_t tot;
+            const uint32_t cbefore = wave_prefix_sum(pts, lane, tot);
+            if (k < nact) e.moff[k] = pts > 0u ? ((nman + mbefore) | ((ncons + cbefore) << 16u)) : 0xffffffffu;
+            nman += (uint32_t)__popcll(bal);
+            ncons += tot;
+        }
+        cs.raw_contacts = raw_total;
+        cs.required_manifolds = nman;
+        cs.manifolds = nman;
+        cs.retained_points = ret_total;
+        cs.new_points = fresh_total;
+        cs.diagnostics.x = pen;
+        if (nman > mcap) {
+            if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_MANIFOLD_CAPACITY_OVERFLOW << 24) | (nman & 0xffffffu);
+            __syncthreads();
+            break;
+        }
+        cs.required_constraints = ncons;
+        if (ncons > ccap) {
+            if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_CONSTRAINT_CAPACITY_OVERFLOW << 24) | (ncons & 0xffffffu);
+            __syncthreads();
+            break;
+        }
+        cs.constraints = ncons;
+        if (cs.max_constraints < ncons) cs.max_constraints = ncons;
+        __syncthreads();
+        // ---------------------------------------- compact + emission
+        MRManifoldHeaderC* new_h = A.mh + ((size_t)dst_buf * E + env) * mcap;
+        MRManifoldPointC* new_p = A.mp + ((size_t)dst_buf * E + env) * mcap * MR_MANIFOLD_POINT_CAPACITY;
+        for (uint32_t k = lane; k < nact; k += kLanes) {
+            const uint32_t mo = e.moff[k];
+            if (mo == 0xffffffffu) continue;
+            const uint32_t m = mo & 0xffffu, c0 = mo >> 16u;
+            const MRManifoldHeaderC mh = e.stage_h[k];
+            new_h[m] = mh;
+            for (uint32_t p = 0; p < MR_MANIFOLD_POINT_CAPACITY; ++p)
+                new_p[(size_t)m * MR_MANIFOLD_POINT_CAPACITY + p] = e.stage_p[(size_t)k * MR_MANIFOLD_POINT_CAPACITY + p];
+            const MRShapeC& sa = e.shapes[mh.pair_and_count[1]];
+            const MRShapeC& sb = e.shapes[mh.pair_and_count[2]];
+            V3 pa_pos, pb_pos;
+            Q4 pa_q, pb_q;
+            body_pose(e, sa.body, pa_pos, pa_q);
+            body_pose(e, sb.body, pb_pos, pb_q);
+            Q4 ra, rb;
+            checked_quat(pa_q, ra);
+            checked_quat(pb_q, rb);
+            const V3 n = normalize(qrot(ra, f3(mh.normal_and_age)));
+            V3 t = qrot(ra, f3(mh.tangent_and_metric));
+            t = sub(t, mul(dot(t, n), n));
+            t = dot(t, t) > 1.0e-12f ? normalize(t) : stable_tangent(n);
+            const V3 bt = cross(n, t);
+            const MRMaterialC& ma = e.mats[sa.material];
+            const MRMaterialC& mb = e.mats[sb.material];
+            const float rest = sa.contact_rest_radius.y + sb.contact_rest_radius.y;
+            for (uint32_t p = 0; p < mh.pair_and_count[3]; ++p) {
+                const MRManifoldPointC mp = e.stage_p[(size_t)k * MR_MANIFOLD_POINT_CAPACITY + p];
+                const uint32_t c = c0 + p;
+                HotConstraint hk;
+                ColdConstraint ck;
+                hk.body_a = sa.body;
+                hk.body_b = sb.body;
+                hk.flags = mp.feature_and_life[2] == 0u ? MR_CONSTRAINT_FLAG_NEW_IMPACT : MR_CONSTRAINT_FLAG_WARM_STARTED;
+                hk.art = (is_link(e, sa.body) || is_link(e, sb.body)) ? 1u : 0u;
+                const V3 pa = add(pa_pos, qrot(ra, f3(mp.local_anchor_a)));
+                const V3 pb = add(pb_pos, qrot(rb, f3(mp.local_anchor_b)));
+                const V3 pw = mul(0.5f, add(pa, pb));
+                hk.point[0] = pw.x; hk.point[1] = pw.y; hk.point[2] = pw.z;
+                hk.dir[0] = n.x; hk.dir[1] = n.y; hk.dir[2] = n.z;
+                hk.dir[3] = t.x; hk.dir[4] = t.y; hk.dir[5] = t.z;
+                hk.dir[6] = bt.x; hk.dir[7] = bt.y; hk.dir[8] = bt.z;
+                ck.pa[0] = pa.x; ck.pa[1] = pa.y; ck.pa[2] = pa.z;
+                ck.pb[0] = pb.x; ck.pb[1] = pb.y; ck.pb[2] = pb.z;
+                ck.separation = dot(sub(pb, pa), n) - rest;
+                ck.manifold = m;
+                ck.point = p;
+                ck.flags = hk.flags;
+                const float fa[4] = {ma.friction.x, ma.friction.y, ma.friction.z, ma.friction.w};
+                const float fb[4] = {mb.friction.x, mb.friction.y, mb.friction.z, mb.friction.w};
+                for (int q = 0; q < 4; ++q) ck.friction[q] = sqrt_rn(fmaxd(0.0f, fa[q] * fb[q]));
+                ck.restitution = fmaxd(ma.response.x, mb.response.x);
+                ck.restitution_threshold = fmaxd(ma.response.y, mb.response.y);
+                ck.compliance = ma.response.z + mb.response.z;
+                ck.dissipation = ma.response.w + mb.response.w;
+                if (cfg.warm_start != 0u) {
+                    hk.lambda[0] = mp.impulses.x;
+                    hk.lambda[1] = mp.impulses.y;
+                    hk.lambda[2] = mp.impulses.z;
+                } else {
+                    hk.lambda[0] = 0.0f;
+                    hk.lambda[1] = 0.0f;
+                    hk.lambda[2] = 0.0f;
+                }
+                e.hot[c] = hk;
+                e.cold[c] = ck;
+            }
+        }
+        __syncthreads();
+        // --------------------------- Jacobian rows + inverse-ABA responses
+        for (uint32_t item = lane; item < 3u * ncons; item += kLanes) {
+            const uint32_t c = item / 3u;
+            const int row = (int)(item - 3u * c);
+            const HotConstraint& hk = e.hot[c];
+            if (hk.art == 0u) continue;
+            const ColdConstraint& ck = e.cold[c];
+            const bool la = is_link(e, hk.body_a), lb = is_link(e, hk.body_b);
+            const V3 pa{ck.pa[0], ck.pa[1], ck.pa[2]};
+            const V3 pb{ck.pb[0], ck.pb[1], ck.pb[2]};
+            const V3 d = hot_dir(hk, row);
+            float* jr = e.jrow + ((size_t)c * 3u + (size_t)row) * A.max_nv;
+            for (uint32_t j = 0; j < nv; ++j) {
+                const V3 cb = lb ? point_jacobian_column(e, hk.body_b, pb, j) : V3{0.0f, 0.0f, 0.0f};
+                const V3 ca = la ? point_jacobian_column(e, hk.body_a, pa, j) : V3{0.0f, 0.0f, 0.0f};
+                jr[j] = dot(d, sub(cb, ca));
+            }
+            float* rs = e.resp + ((size_t)c * 3u + (size_t)row) * A.max_nv;
+            inverse_aba(e, la ? hk.body_a : MR_INVALID_INDEX, pa, neg(d), lb ? hk.body_b : MR_INVALID_INDEX, pb, d, rs);
+        }
+        __syncthreads();
+        // ------------------------------------------ IR evaluation + blocks
+        const float slop = cfg.contact_slop;
+        const float maxdep = cfg.max_depenetration_velocity;
+        for (uint32_t c = lane; c < ncons; c += kLanes) {
+            HotConstraint hk = e.hot[c];
+            const ColdConstraint ck = e.cold[c];
+            float rel[3];
+            for (int r = 0; r < 3; ++r) rel[r] = relative_row(e, hk, c, r);
+            const float slip = sqrt_rn(rel[1] * rel[1] + rel[2] * rel[2]);
+            const bool stat = slip <= 1.0e-3f;
+            hk.mu_u = stat ? ck.friction[0] : ck.friction[2];
+            hk.mu_v = stat ? ck.friction[1] : ck.friction[3];
+            const float src_target = ck.separation > 0.0f ? -ck.separation / h : 0.0f;
+            const float tau = 2.0f * h;
+            const float err = fmind(ck.separation + slop, 0.0f);
+            const float ratio = h / tau;
+            const float den = 1.0f + 2.0f * 1.0f * ratio + ratio * ratio;
+            float stab = -h * err / (tau * tau * den);
+            stab = clampd(fmaxd(stab, 0.0f), 0.0f, maxdep);
+            if ((hk.flags & MR_CONSTRAINT_FLAG_NEW_IMPACT) != 0u) {
+                const float incoming = rel[0] - src_target;
+                if (incoming < -ck.restitution_threshold) stab = fmaxd(stab, -ck.restitution * incoming);
+            }
+            hk.target[0] = src_target + stab;
+            hk.target[1] = 0.0f;
+            hk.target[2] = 0.0f;
+            hk.reg[0] = fmaxd(ck.compliance / (h * h) + ck.dissipation / h, 0.0f);
+            hk.reg[1] = 0.0f;
+            hk.reg[2] = 0.0f;
+            bool ok = true;
+            if (!isfin(hk.target[0]) || !isfin(hk.reg[0])) {
+                report_failure(&s_fail, c, MR_STEP_NONFINITE_RESULT, c);
+                ok = false;
+            }
+            if (ok) {
+                const V3 x{hk.point[0], hk.point[1], hk.point[2]};
+                const bool sa_dyn = is_dynamic_scene(e, hk.body_a), sb_dyn = is_dynamic_scene(e, hk.body_b);
+                float W[3][3];
+                for (int r = 0; r < 3; ++r)
+                    for (int s = 0; s < 3; ++s) {
+                        float val = 0.0f;
+                        if (hk.art != 0u) {
+                            const float* jr = e.jrow + ((size_t)c * 3u + (size_t)r) * A.max_nv;
+                            const float* rs = e.resp + ((size_t)c * 3u + (size_t)s) * A.max_nv;
+                            for (uint32_t j = 0; j < nv; ++j) val += jr[j] * rs[j];
+                        }
+                        if (sa_dyn) val += dot(hot_dir(hk, r), scene_response(e.scene[hk.body_a - nl], x, hot_dir(hk, s)));
+                        if (sb_dyn) val += dot(hot_dir(hk, r), scene_response(e.scene[hk.body_b - nl], x, hot_dir(hk, s)));
+                        if (r == s) val += hk.reg[r];
+                        W[r][s] = val;
+                    }
+                if (!invert3x3(W, hk.winv)) report_failure(&s_fail, c, MR_STEP_FACTORIZATION_FAILED, c);
+            }
+            project_cone(hk.lambda, hk.mu_u, hk.mu_v);
+            e.hot[c] = hk;
+        }
+        __syncthreads();
+        if (s_fail != ~0ull) break;
+        // ---------------------------------------------------- warm start
+        for (uint32_t j = lane; j < nv; j += kLanes) {
+            float vj = e.vcand[j];
+            for (uint32_t c = 0; c < ncons; ++c) {
+                const HotConstraint& hk = e.hot[c];
+                if (hk.art == 0u) continue;
+                const float* r0 = e.resp + ((size_t)c * 3u + 0u) * A.max_nv;
+                vj += (r0[j] * hk.lambda[0] + r0[A.max_nv + j] * hk.lambda[1]) + r0[2u * A.max_nv + j] * hk.lambda[2];
+            }
+            e.vcand[j] = vj;
+        }
+        for (uint32_t s = lane; s < nsc; s += kLanes) {
+            const uint32_t b = nl + s;
+            if (!is_dynamic_scene(e, b)) continue;
+            MRBodyStateC st = e.scene[s];
+            for (uint32_t c = 0; c < ncons; ++c) {
+                const HotConstraint& hk = e.hot[c];
+                if (hk.body_a != b && hk.body_b != b) continue;
+                const V3 imp = add(add(mul(hk.lambda[0], hot_dir(hk, 0)), mul(hk.lambda[1], hot_dir(hk, 1))),
+                                   mul(hk.lambda[2], hot_dir(hk, 2)));
+                const V3 x{hk.point[0], hk.point[1], hk.point[2]};
+                if (hk.body_a == b) apply_scene_impulse(st, x, neg(imp));
+                if (hk.body_b == b) apply_scene_impulse(st, x, imp);
+            }
+            e.scene[s] = st;
+        }
+        __syncthreads();
+        // ------------------------------------------ ordered block sweeps
+        const uint32_t base_iters = cfg.velocity_iterations + (final_sub ? cfg.final_velocity_iterations : 0u);
+        const bool adaptive = cfg.residual_tolerance > 0.0f;
+        const uint32_t max_iters = adaptive ? cfg.max_iterations : base_iters;
+        float sweep_delta = 0.0f;
+        uint32_t iters = 0;
+        bool solve_failed = false;
+        for (uint32_t it = 0; it < max_iters && ncons > 0u; ++it) {
+            sweep_delta = 0.0f;
+            for (uint32_t c = 0; c < ncons; ++c) {
+                const HotConstraint& hk = e.hot[c];
+                float relv = 0.0f;
+                if (lane < 3u) relv = relative_row(e, hk, c, (int)lane);
+                const float rel0 = __shfl(relv, 0, 64);
+                const float rel1 = __shfl(relv, 1, 64);
+                const float rel2 = __shfl(relv, 2, 64);
+                const float rel[3] = {rel0, rel1, rel2};
+                float rhs[3];
+                for (int r = 0; r < 3; ++r) rhs[r] = (hk.target[r] - rel[r]) - hk.reg[r] * hk.lambda[r];
+                float cand[3] = {hk.lambda[0], hk.lambda[1], hk.lambda[2]};
+                for (int r = 0; r < 3; ++r)
+                    for (int s = 0; s < 3; ++s) cand[r] += hk.winv[3 * r + s] * rhs[s];
+                project_cone(cand, hk.mu_u, hk.mu_v);
+                const float d0 = cand[0] - hk.lambda[0], d1 = cand[1] - hk.lambda[1], d2 = cand[2] - hk.lambda[2];
+                if (!isfin(cand[0]) || !isfin(cand[1]) || !isfin(cand[2]) || !isfin(d0) || !isfin(d1) || !isfin(d2)) {
+                    if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_NONFINITE_RESULT << 24) | (c & 0xffffffu);
+                    solve_failed = true;
+                    break;
+                }
+                sweep_delta = fmaxd(sweep_delta, fmaxd(fabs_(d0), fmaxd(fabs_(d1), fabs_(d2))));
+                if (hk.art != 0u) {
+                    const float* r0 = e.resp + ((size_t)c * 3u + 0u) * A.max_nv;
+                    for (uint32_t j = lane; j < nv; j += kLanes)
+                        e.vcand[j] += (r0[j] * d0 + r0[A.max_nv + j] * d1) + r0[2u * A.max_nv + j] * d2;
+                }
+                if (lane == 0u) {
+                    HotConstraint& hw = e.hot[c];
+                    const V3 imp = add(add(mul(d0, hot_dir(hk, 0)), mul(d1, hot_dir(hk, 1))), mul(d2, hot_dir(hk, 2)));
+                    const V3 x{hk.point[0], hk.point[1], hk.point[2]};
+                    if (is_dynamic_scene(e, hk.body_a)) apply_scene_impulse(e.scene[hk.body_a - nl], x, neg(imp));
+                    if (is_dynamic_scene(e, hk.body_b)) apply_scene_impulse(e.scene[hk.body_b - nl], x, imp);
+                    hw.lambda[0] = cand[0];
+                    hw.lambda[1] = cand[1];
+                    hw.lambda[2] = cand[2];
+                }
+                __syncthreads();
+            }
+            if (solve_failed) break;
+            ++iters;
+            if (adaptive && sweep_delta <= cfg.residual_tolerance) break;
+        }
+        __syncthreads();
+        if (s_fail != ~0ull) break;
+        // ------------------------------------- residuals + impulse store
+        float my_normal = 0.0f, my_cone = 0.0f;
+        for (uint32_t c = lane; c < ncons; c += kLanes) {
+            const HotConstraint hk = e.hot[c];
+            const float rel0 = relative_row(e, hk, c, 0);
+            const float eq = (rel0 - hk.target[0]) + hk.reg[0] * hk.lambda[0];
+            const float nres = hk.lambda[0] > kConeEpsilon ? fabs_(eq) : fmaxd(-eq, 0.0f);
+            my_normal = fmaxd(my_normal, nres);
+            const float lu = hk.mu_u * hk.lambda[0], lv = hk.mu_v * hk.lambda[0];
+            float viol;
+            if (lu > 0.0f && lv > 0.0f)
+                viol = fmaxd(sqrt_rn((hk.lambda[1] * hk.lambda[1]) / (lu * lu) + (hk.lambda[2] * hk.lambda[2]) / (lv * lv)) - 1.0f,
+                             0.0f);
+            else
+                viol = sqrt_rn(hk.lambda[1] * hk.lambda[1] + hk.lambda[2] * hk.lambda[2]);
+            my_cone = fmaxd(my_cone, viol);
+            const ColdConstraint& ck = e.cold[c];
+            new_p[(size_t)ck.manifold * MR_MANIFOLD_POINT_CAPACITY + ck.point].impulses =
+                mr_float4{hk.lambda[0], hk.lambda[1], hk.lambda[2], 0.0f};
+        }
+        const float max_normal = wave_max(my_normal);
+        const float max_cone = wave_max(my_cone);
+        if (!isfin(sweep_delta) || !isfin(max_normal) || !isfin(max_cone)) {
+            if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_NONFINITE_RESULT << 24) | 0xffffffu;
+            __syncthreads();
+            break;
+        }
+        cs.solver_iterations = ncons > 0u ? iters : 0u;
+        cs.residuals = mr_float4{ncons > 0u ? sweep_delta : 0.0f, max_normal, max_cone, 0.0f};
+        if (lane == 0u) {
+            uint32_t parent[MR_MAX_BODIES + 1];
+            bool used[MR_MAX_BODIES + 1];
+            const uint32_t nodes = 1u + nsc;
+            for (uint32_t i = 0; i < nodes; ++i) {
+                parent[i] = i;
+                used[i] = false;
+            }
+            for (uint32_t c = 0; c < ncons; ++c) {
+                const HotConstraint& hk = e.hot[c];
+                const uint32_t na = is_link(e, hk.body_a) ? 0u : (is_dynamic_scene(e, hk.body_a) ? 1u + (hk.body_a - nl) : MR_INVALID_INDEX);
+                const uint32_t nb = is_link(e, hk.body_b) ? 0u : (is_dynamic_scene(e, hk.body_b) ? 1u + (hk.body_b - nl) : MR_INVALID_INDEX);
+                if (na != MR_INVALID_INDEX) used[na] = true;
+                if (nb != MR_INVALID_INDEX) used[nb] = true;
+                if (na != MR_INVALID_INDEX && nb != MR_INVALID_INDEX) {
+                    uint32_t ra = na, rb = nb;
+                    while (parent[ra] != ra) ra = parent[ra];
+                    while (parent[rb] != rb) rb = parent[rb];
+                    if (ra != rb) {
+                        if (ra < rb) parent[rb] = ra;
+                        else parent[ra] = rb;
+                    }
+                }
+            }
+            uint32_t islands = 0;
+            for (uint32_t i = 0; i < nodes; ++i) {
+                uint32_t rr = i;
+                while (parent[rr] != rr) rr = parent[rr];
+                if (used[i] && rr == i) ++islands;
+            }
+            s_count[0] = islands;
+        }
+        // ------------------------------------------------- joint limits
+        for (uint32_t j = lane; j < nv; j += kLanes) {
+            const MRDofC& dof = e.dofs[j];
+            float vel = e.vcand[j];
+            if ((dof.flags & MR_DOF_FLAG_VELOCITY_LIMIT) != 0u) vel = clampd(vel, -dof.limits.z, dof.limits.z);
+            if ((dof.flags & MR_DOF_FLAG_POSITION_LIMIT) != 0u && dof.q_index != MR_INVALID_INDEX) {
+                const float pos = e.q[dof.q_index];
+                vel = clampd(vel, (dof.limits.x - pos) / h, (dof.limits.y - pos) / h);
+            }
+            if (!isfin(vel)) report_failure(&s_fail, j, MR_STEP_NONFINITE_RESULT, j);
+            e.vcand[j] = vel;
+        }
+        __syncthreads();
+        cs.islands = s_count[0];
+        if (s_fail != ~0ull) break;
+        // ---------------------------------------------------- integration
+        // New coordinates are staged in registers / s_root and written after
+        // a barrier, because other lanes still read q in this phase.
+        float qnew = 0.0f;
+        uint32_t qslot = MR_INVALID_INDEX;
+        if (lane == 0u && floating) {
+            const Q4 inc = rotation_increment(V3{h * e.vcand[3], h * e.vcand[4], h * e.vcand[5]});
+            Q4 o;
+            if (!normalized_quat(qmul(inc, Q4{e.q[3], e.q[4], e.q[5], e.q[6]}), o)) {
+                report_failure(&s_fail, 0u, MR_STEP_NONFINITE_RESULT, 3u);
+                o = Q4{e.q[3], e.q[4], e.q[5], e.q[6]};
+            }
+            for (uint32_t i = 0; i < 3u; ++i) {
+                s_root[i] = e.q[i] + h * e.vcand[i];
+                if (!isfin(s_root[i])) report_failure(&s_fail, 1u + i, MR_STEP_NONFINITE_RESULT, i);
+            }
+            s_root[3] = o.x;
+            s_root[4] = o.y;
+            s_root[5] = o.z;
+            s_root[6] = o.w;
+        }
+        for (uint32_t i = 1u + lane; i < nl; i += kLanes) {
+            const MRBodyC& b = e.bodies[i];
+            if (b.q_index != MR_INVALID_INDEX) {
+                qslot = b.q_index;
+                qnew = e.q[b.q_index] + h * e.vcand[b.v_index];
+                if (!isfin(qnew)) report_failure(&s_fail, 1u + b.q_index, MR_STEP_NONFINITE_RESULT, b.q_index);
+            }
+        }
+        for (uint32_t s = lane; s < nsc; s += kLanes) {
+            const uint32_t b = nl + s;
+            const MRBodyC& bp = e.bodies[b];
+            if (bp.motion_type != MR_MOTION_DYNAMIC) continue;
+            MRBodyStateC st = e.scene[s];
+            V3 lin = f3(st.linear_velocity_inv_mass);
+            V3 ang = f3(st.angular_velocity);
+            const float ll = length(lin), al = length(ang);
+            if (bp.damping.z > 0.0f && ll > bp.damping.z) lin = mul(bp.damping.z / ll, lin);
+            if (bp.damping.w > 0.0f && al > bp.damping.w) ang = mul(bp.damping.w / al, ang);
+            st.linear_velocity_inv_mass.x = lin.x;
+            st.linear_velocity_inv_mass.y = lin.y;
+            st.linear_velocity_inv_mass.z = lin.z;
+            st.angular_velocity.x = ang.x;
+            st.angular_velocity.y = ang.y;
+            st.angular_velocity.z = ang.z;
+            const V3 np = add(f3(st.position), mul(h, lin));
+            st.position = mr_float4{np.x, np.y, np.z, 1.0f};
+            const Q4 inc = rotation_increment(mul(h, ang));
+            Q4 o;
+            if (!normalized_quat(qmul(inc, q4(st.orientation)), o) || !finite3(np)) {
+                report_failure(&s_fail, 0x10000u + b, MR_STEP_NONFINITE_RESULT, b);
+                continue;
+            }
+            st.orientation = mr_float4{o.x, o.y, o.z, o.w};
+            world_inverse_inertia(bp, o, st);
+            e.scene[s] = st;
+        }
+        __syncthreads();
+        if (s_fail != ~0ull) break;
+        if (qslot != MR_INVALID_INDEX) e.q[qslot] = qnew;
+        if (lane == 0u && floating)
+            for (uint32_t i = 0; i < 7u; ++i) e.q[i] = s_root[i];
+        for (uint32_t j = lane; j < nv; j += kLanes) e.v[j] = e.vcand[j];
+        __syncthreads();
+        if (s_fail != ~0ull) break;
+        // --------------------------------------------- microstep commit
+        if (lane == 0u) A.mcount[(size_t)dst_buf * E + env] = nman;
+        mcount_src = nman;
+        src_buf = dst_buf;
+        dst_buf = dst_buf == (cur + 1u) % 3u ? alt_buf : (cur + 1u) % 3u;
+        successful = substep + 1u;
+        min_piv = fmind(min_piv, s_fmax[0]);
+        max_piv = fmaxd(max_piv, s_fmax[1]);
+        float my_acc = 0.0f;
+        for (uint32_t j = lane; j < nv; j += kLanes) my_acc = fmaxd(my_acc, fabs_(e.accel[j]));
+        max_acc = fmaxd(max_acc, wave_max(my_acc));
+        __syncthreads();
+    }
+
+    // ------------------------------------------- transactional commit
+    __syncthreads();
+    const unsigned long long fk = s_fail;
+    const bool failed = fk != ~0ull;
+    const uint32_t code = failed ? (uint32_t)((fk >> 24) & 0xffull) : MR_STEP_SUCCESS;
+    uint32_t findex = failed ? (uint32_t)(fk & 0xffffffull) : MR_INVALID_INDEX;
+    if (findex == 0xffffffu) findex = MR_INVALID_INDEX;
+    if (failed) failing_substep = successful;
+    if (!failed) {
+        for (uint32_t i = lane; i < nq; i += kLanes) gq[i] = e.q[i];
+        for (uint32_t j = lane; j < nv; j += kLanes) gv[j] = e.v[j];
+        for (uint32_t s = lane; s < nsc; s += kLanes) gs[s] = e.scene[s];
+        if (A.acceleration != nullptr)
+            for (uint32_t j = lane; j < nv; j += kLanes) A.acceleration[(size_t)env * A.max_nv + j] = e.accel[j];
+    } else if (A.acceleration != nullptr) {
+        for (uint32_t j = lane; j < nv; j += kLanes) A.acceleration[(size_t)env * A.max_nv + j] = 0.0f;
+    }
+    if (lane == 0u) {
+        if (!failed && successful > 0u) A.mcurrent[env] = src_buf;
+        MRWorldStatusC ws;
+        ws.code = code;
+        ws.environment = env;
+        ws.control_step = A.control_step;
+        ws.successful_substeps = successful;
+        ws.aba_code = 0u;
+        ws.failing_substep = failed ? failing_substep : MR_INVALID_INDEX;
+        ws.failing_index = findex;
+        ws.flags = 0u;
+        if (!failed) {
+            float qerr = 0.0f;
+            if (floating) {
+                const float n2 = e.q[3] * e.q[3] + e.q[4] * e.q[4] + e.q[5] * e.q[5] + e.q[6] * e.q[6];
+                qerr = fabs_(sqrt_rn(n2) - 1.0f);
+            }
+            ws.diagnostics = mr_float4{isfin(min_piv) ? min_piv : 0.0f, max_piv, max_acc, qerr};
+        } else {
+            ws.diagnostics = mr_float4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+        A.status[env] = ws;
+        cs.code = code;
+        if (failed) {
+            if (code == MR_STEP_PAIR_CAPACITY_OVERFLOW) cs.first_failing_pair = findex;
+            else cs.first_failing_constraint = findex;
+        }
+        A.cstatus[env] = cs;
+    }
+}
+
+namespace mr {
+
+static_assert(sizeof(SpI) == 84, "SpI layout");
+static_assert(sizeof(Collider) == 124, "Collider layout");
+static_assert(sizeof(HotConstraint) == 144, "HotConstraint layout");
+
+hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream) {
+    hipLaunchKernelGGL(mr_control_step_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t configure_control_step(uint32_t lds_bytes) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&mr_control_step_kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+}
+
+} // namespace mr

@@ -1,0 +1,388 @@
+// Host executor: device arenas, LDS layout, per-control-step launches and
+// status publication for the fused temporalCone kernel.
+#include "mr_world.h"
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+namespace mr {
+
+hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream);
+hipError_t configure_control_step(uint32_t lds_bytes);
+
+namespace {
+constexpr uint32_t kSpIBytes = 84u;
+constexpr uint32_t kColliderBytes = 124u;
+constexpr uint32_t kHotBytes = (uint32_t)sizeof(HotConstraint);
+constexpr uint32_t kMaxLdsBytes = 160u * 1024u;
+
+template <typename T>
+void upload(DeviceBuffer& b, const std::vector<T>& v) {
+    const size_t n = std::max<size_t>(v.size() * sizeof(T), 16u);
+    b.reserve(n);
+    if (!v.empty()) hip_check(hipMemcpy(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+}
+} // namespace
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+void DeviceBuffer::reserve(size_t n) {
+    if (n <= bytes) return;
+    release();
+    hip_check(hipMalloc(&ptr, n), "hipMalloc");
+    bytes = n;
+}
+
+void DeviceBuffer::release() {
+    if (ptr != nullptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+World::World(const ModelData& model, const MRWorldStepConfigC& config) : model_(model), cfg_(config) {
+    if (cfg_.physics_substeps == 0u || cfg_.physics_substeps > MR_MAX_PHYSICS_SUBSTEPS)
+        throw std::runtime_error("physics_substeps must be in [1, 64]");
+    if (!(cfg_.control_timestep > 0.0f)) throw std::runtime_error("control_timestep must be positive");
+    if (cfg_.velocity_iterations == 0u && cfg_.residual_tolerance <= 0.0f)
+        throw std::runtime_error("velocity_iterations must be positive");
+    if (cfg_.residual_tolerance > 0.0f && cfg_.max_iterations == 0u)
+        throw std::runtime_error("max_iterations must be positive with a residual tolerance");
+    if (cfg_.pair_capacity == 0u || cfg_.pair_capacity > 4096u) throw std::runtime_error("pair_capacity in [1, 4096]");
+    if (cfg_.manifold_capacity == 0u || cfg_.manifold_capacity > 4096u)
+        throw std::runtime_error("manifold_capacity in [1, 4096]");
+    if (cfg_.constraint_capacity == 0u || cfg_.constraint_capacity > 4096u)
+        throw std::runtime_error("constraint_capacity in [1, 4096]");
+    if (cfg_.capture_evidence != 0u && cfg_.capture_substep >= cfg_.physics_substeps)
+        throw std::runtime_error("capture_substep must be below physics_substeps");
+    finalize_model(model_, cfg_.control_timestep / (float)cfg_.physics_substeps);
+    env_ = model_.environment_count();
+    compute_lds_layout();
+    hip_check(configure_control_step(lds_.total), "hipFuncSetAttribute");
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&ev_begin_), "hipEventCreate");
+    hip_check(hipEventCreate(&ev_end_), "hipEventCreate");
+    upload(d_models_, model_.models);
+    upload(d_env_model_, model_.environment_model);
+    upload(d_bodies_, model_.bodies);
+    upload(d_dofs_, model_.dofs);
+    upload(d_shapes_, model_.shapes);
+    upload(d_materials_, model_.materials);
+    upload(d_pairs_, model_.pairs);
+    upload(d_heights_, model_.heights);
+    const size_t E = env_;
+    d_q_.reserve(E * model_.max_nq * sizeof(float) + 16u);
+    d_v_.reserve(E * model_.max_nv * sizeof(float) + 16u);
+    d_scene_.reserve(E * std::max(1u, model_.max_scene_bodies) * sizeof(MRBodyStateC));
+    d_reset_q_.reserve(d_q_.bytes);
+    d_reset_v_.reserve(d_v_.bytes);
+    d_reset_scene_.reserve(d_scene_.bytes);
+    const size_t mcap = cfg_.manifold_capacity, pcap = cfg_.pair_capacity, ccap = cfg_.constraint_capacity;
+    d_mh_.reserve(3u * E * mcap * sizeof(MRManifoldHeaderC));
+    d_mp_.reserve(3u * E * mcap * MR_MANIFOLD_POINT_CAPACITY * sizeof(MRManifoldPointC));
+    d_mcount_.reserve(3u * E * sizeof(uint32_t));
+    d_mcurrent_.reserve(E * sizeof(uint32_t));
+    hip_check(hipMemset(d_mcount_.ptr, 0, 3u * E * sizeof(uint32_t)), "hipMemset");
+    hip_check(hipMemset(d_mcurrent_.ptr, 0, E * sizeof(uint32_t)), "hipMemset");
+    d_stage_h_.reserve(E * pcap * sizeof(MRManifoldHeaderC));
+    d_stage_p_.reserve(E * pcap * MR_MANIFOLD_POINT_CAPACITY * sizeof(MRManifoldPointC));
+    d_jrow_.reserve(E * ccap * 3u * model_.max_nv * sizeof(float));
+    d_resp_.reserve(E * ccap * 3u * model_.max_nv * sizeof(float));
+    d_cold_.reserve(E * ccap * sizeof(ColdConstraint));
+    d_accel_.reserve(E * model_.max_nv * sizeof(float) + 16u);
+    hip_check(hipMemset(d_accel_.ptr, 0, d_accel_.bytes), "hipMemset");
+    d_ev_pairs_.reserve(E * pcap * sizeof(MRPairRecordC));
+    d_ev_counts_.reserve(E * sizeof(uint32_t));
+    hip_check(hipMemset(d_ev_counts_.ptr, 0, E * sizeof(uint32_t)), "hipMemset");
+    if (!model_.initial_q.empty()) {
+        set_state(model_.initial_q.data(), model_.initial_v.data(), model_.initial_scene.data());
+        set_reset_state(model_.initial_q.data(), model_.initial_v.data(), model_.initial_scene.data());
+    }
+}
+
+World::~World() {
+    if (stream_ != nullptr) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
+    if (ev_begin_ != nullptr) (void)hipEventDestroy(ev_begin_);
+    if (ev_end_ != nullptr) (void)hipEventDestroy(ev_end_);
+}
+
+void World::compute_lds_layout() {
+    uint32_t links = 1u, shapes = 1u;
+    for (const MRModelHeaderC& h : model_.models) {
+        links = std::max(links, h.link_count);
+        shapes = std::max(shapes, h.shape_count);
+    }
+    const uint32_t nq = std::max(1u, model_.max_nq), nv = std::max(1u, model_.max_nv);
+    const uint32_t nscene = std::max(1u, model_.max_scene_bodies);
+    uint32_t off = 0;
+    auto take = [&](uint32_t bytes, uint32_t align) {
+        off = (off + align - 1u) / align * align;
+        const uint32_t at = off;
+        off += bytes;
+        return at;
+    };
+    LdsLayout& L = lds_;
+    L.q = take(4u * nq, 4u);
+    L.v = take(4u * nv, 4u);
+    L.vcand = take(4u * nv, 4u);
+    L.effort = take(4u * nv, 4u);
+    L.accel = take(4u * nv, 4u);
+    L.cmd = take(4u * nv, 4u);
+    L.pos = take(12u * links, 4u);
+    L.rot = take(16u * links, 4u);
+    L.w = take(12u * links, 4u);
+    L.vl = take(12u * links, 4u);
+    L.sa = take(12u * links, 4u);
+    L.sl = take(12u * links, 4u);
+    L.ca = take(12u * links, 4u);
+    L.cl = take(12u * links, 4u);
+    L.r = take(12u * links, 4u);
+    L.I = take(kSpIBytes * links, 4u);
+    L.pt = take(12u * links, 4u);
+    L.pb = take(12u * links, 4u);
+    L.Ut = take(12u * links, 4u);
+    L.Ub = take(12u * links, 4u);
+    L.D = take(4u * links, 4u);
+    L.u = take(4u * links, 4u);
+    L.alpha = take(12u * links, 4u);
+    L.acc = take(12u * links, 4u);
+    L.L = take(4u * 36u, 4u);
+    L.scene = take((uint32_t)sizeof(MRBodyStateC) * nscene, 16u);
+    L.col = take(kColliderBytes * shapes, 4u);
+    L.active = take(4u * cfg_.pair_capacity, 4u);
+    L.npts = take(4u * cfg_.pair_capacity, 4u);
+    L.moff = take(4u * cfg_.pair_capacity, 4u);
+    L.hot = take(kHotBytes * cfg_.constraint_capacity, 4u);
+    L.total = (off + 15u) / 16u * 16u;
+    if (L.total > kMaxLdsBytes)
+        throw std::runtime_error("world needs " + std::to_string(L.total) +
+                                 " bytes of LDS per environment; lower pair/constraint capacity");
+}
+
+size_t World::retained_bytes() const {
+    const DeviceBuffer* all[] = {&d_models_, &d_env_model_, &d_bodies_, &d_dofs_, &d_shapes_, &d_materials_,
+                                 &d_pairs_, &d_heights_, &d_q_, &d_v_, &d_scene_, &d_mh_, &d_mp_, &d_mcount_,
+                                 &d_mcurrent_, &d_reset_q_, &d_reset_v_, &d_reset_scene_, &d_commands_, &d_masks_,
+                                 &d_status_, &d_cstatus_, &d_accel_, &d_ev_pairs_, &d_ev_counts_, &d_stage_h_,
+                                 &d_stage_p_, &d_jrow_, &d_resp_, &d_cold_};
+    size_t total = 0;
+    for (const DeviceBuffer* b : all) total += b->bytes;
+    return total;
+}
+
+void World::set_state(const float* q, const float* v, const MRBodyStateC* scene) {
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    const size_t E = env_;
+    hip_check(hipMemcpy(d_q_.ptr, q, E * model_.max_nq * sizeof(float), hipMemcpyHostToDevice), "set_state q");
+    hip_check(hipMemcpy(d_v_.ptr, v, E * model_.max_nv * sizeof(float), hipMemcpyHostToDevice), "set_state v");
+    if (model_.max_scene_bodies > 0u)
+        hip_check(hipMemcpy(d_scene_.ptr, scene, E * model_.max_scene_bodies * sizeof(MRBodyStateC),
+                            hipMemcpyHostToDevice),
+                  "set_state scene");
+    hip_check(hipMemset(d_mcount_.ptr, 0, 3u * E * sizeof(uint32_t)), "hipMemset");
+    hip_check(hipMemset(d_mcurrent_.ptr, 0, E * sizeof(uint32_t)), "hipMemset");
+}
+
+void World::set_reset_state(const float* q, const float* v, const MRBodyStateC* scene) {
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    const size_t E = env_;
+    hip_check(hipMemcpy(d_reset_q_.ptr, q, E * model_.max_nq * sizeof(float), hipMemcpyHostToDevice), "reset q");
+    hip_check(hipMemcpy(d_reset_v_.ptr, v, E * model_.max_nv * sizeof(float), hipMemcpyHostToDevice), "reset v");
+    if (model_.max_scene_bodies > 0u)
+        hip_check(hipMemcpy(d_reset_scene_.ptr, scene, E * model_.max_scene_bodies * sizeof(MRBodyStateC),
+                            hipMemcpyHostToDevice),
+                  "reset scene");
+}
+
+void World::set_manifolds(const MRManifoldHeaderC* h, const MRManifoldPointC* p, const uint32_t* count) {
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    const size_t E = env_, mcap = cfg_.manifold_capacity;
+    for (size_t e = 0; e < E; ++e)
+        if (count[e] > mcap) throw std::runtime_error("manifold count above capacity");
+    hip_check(hipMemset(d_mcurrent_.ptr, 0, E * sizeof(uint32_t)), "hipMemset");
+    hip_check(hipMemcpy(d_mh_.ptr, h, E * mcap * sizeof(MRManifoldHeaderC), hipMemcpyHostToDevice), "set mh");
+    hip_check(hipMemcpy(d_mp_.ptr, p, E * mcap * MR_MANIFOLD_POINT_CAPACITY * sizeof(MRManifoldPointC),
+                        hipMemcpyHostToDevice),
+              "set mp");
+    hip_check(hipMemcpy(d_mcount_.ptr, count, E * sizeof(uint32_t), hipMemcpyHostToDevice), "set mcount");
+}
+
+void World::get_state(float* q, float* v, MRBodyStateC* scene) const {
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    const size_t E = env_;
+    if (q) hip_check(hipMemcpy(q, d_q_.ptr, E * model_.max_nq * sizeof(float), hipMemcpyDeviceToHost), "get q");
+    if (v) hip_check(hipMemcpy(v, d_v_.ptr, E * model_.max_nv * sizeof(float), hipMemcpyDeviceToHost), "get v");
+    if (scene && model_.max_scene_bodies > 0u)
+        hip_check(hipMemcpy(scene, d_scene_.ptr, E * model_.max_scene_bodies * sizeof(MRBodyStateC),
+                            hipMemcpyDeviceToHost),
+                  "get scene");
+}
+
+void World::get_manifolds(MRManifoldHeaderC* h, MRManifoldPointC* p, uint32_t* count) const {
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    const size_t E = env_, mcap = cfg_.manifold_capacity;
+    std::vector<uint32_t> cur(E), counts(3u * E);
+    hip_check(hipMemcpy(cur.data(), d_mcurrent_.ptr, E * sizeof(uint32_t), hipMemcpyDeviceToHost), "mcurrent");
+    hip_check(hipMemcpy(counts.data(), d_mcount_.ptr, 3u * E * sizeof(uint32_t), hipMemcpyDeviceToHost), "mcount");
+    const auto* bh = static_cast<const MRManifoldHeaderC*>(d_mh_.ptr);
+    const auto* bp = static_cast<const MRManifoldPointC*>(d_mp_.ptr);
+    for (size_t e = 0; e < E; ++e) {
+        const size_t b = cur[e];
+        count[e] = counts[b * E + e];
+        hip_check(hipMemcpy(h + e * mcap, bh + (b * E + e) * mcap, mcap * sizeof(MRManifoldHeaderC),
+                            hipMemcpyDeviceToHost),
+                  "get mh");
+        hip_check(hipMemcpy(p + e * mcap * MR_MANIFOLD_POINT_CAPACITY,
+                            bp + (b * E + e) * mcap * MR_MANIFOLD_POINT_CAPACITY,
+                            mcap * MR_MANIFOLD_POINT_CAPACITY * sizeof(MRManifoldPointC), hipMemcpyDeviceToHost),
+                  "get mp");
+    }
+}
+
+KernelArgs World::base_args(uint32_t step) const {
+    KernelArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.models = static_cast<const MRModelHeaderC*>(d_models_.ptr);
+    a.environment_model = static_cast<const uint32_t*>(d_env_model_.ptr);
+    a.bodies = static_cast<const MRBodyC*>(d_bodies_.ptr);
+    a.dofs = static_cast<const MRDofC*>(d_dofs_.ptr);
+    a.shapes = static_cast<const MRShapeC*>(d_shapes_.ptr);
+    a.materials = static_cast<const MRMaterialC*>(d_materials_.ptr);
+    a.pairs = static_cast<const MRCompiledPairC*>(d_pairs_.ptr);
+    a.heights = static_cast<const float*>(d_heights_.ptr);
+    a.heightfield = model_.heightfield;
+    a.gravity[0] = model_.gravity.x;
+    a.gravity[1] = model_.gravity.y;
+    a.gravity[2] = model_.gravity.z;
+    a.cfg = cfg_;
+    a.h = cfg_.control_timestep / (float)cfg_.physics_substeps;
+    a.environment_count = env_;
+    a.max_nq = model_.max_nq;
+    a.max_nv = model_.max_nv;
+    a.max_scene = model_.max_scene_bodies;
+    a.control_step = step;
+    a.pair_capacity = cfg_.pair_capacity;
+    a.manifold_capacity = cfg_.manifold_capacity;
+    a.constraint_capacity = cfg_.constraint_capacity;
+    a.q = static_cast<float*>(d_q_.ptr);
+    a.v = static_cast<float*>(d_v_.ptr);
+    a.scene = static_cast<MRBodyStateC*>(d_scene_.ptr);
+    a.mh = static_cast<MRManifoldHeaderC*>(d_mh_.ptr);
+    a.mp = static_cast<MRManifoldPointC*>(d_mp_.ptr);
+    a.mcount = static_cast<uint32_t*>(d_mcount_.ptr);
+    a.mcurrent = static_cast<uint32_t*>(d_mcurrent_.ptr);
+    a.reset_q = static_cast<const float*>(d_reset_q_.ptr);
+    a.reset_v = static_cast<const float*>(d_reset_v_.ptr);
+    a.reset_scene = static_cast<const MRBodyStateC*>(d_reset_scene_.ptr);
+    a.acceleration = static_cast<float*>(d_accel_.ptr);
+    a.evidence_pairs = cfg_.capture_evidence != 0u ? static_cast<MRPairRecordC*>(d_ev_pairs_.ptr) : nullptr;
+    a.evidence_counts = cfg_.capture_evidence != 0u ? static_cast<uint32_t*>(d_ev_counts_.ptr) : nullptr;
+    a.stage_h = static_cast<MRManifoldHeaderC*>(d_stage_h_.ptr);
+    a.stage_p = static_cast<MRManifoldPointC*>(d_stage_p_.ptr);
+    a.jrow = static_cast<float*>(d_jrow_.ptr);
+    a.resp = static_cast<float*>(d_resp_.ptr);
+    a.cold = static_cast<ColdConstraint*>(d_cold_.ptr);
+    a.lds = lds_;
+    return a;
+}
+
+void World::enqueue(const float* commands, const uint32_t* masks, uint32_t steps, bool device_inputs) {
+    if (steps == 0u) throw std::runtime_error("control_step_count must be positive");
+    if (commands == nullptr) throw std::runtime_error("commands are required");
+    const size_t E = env_;
+    const size_t per_step = E * model_.max_nv;
+    const float* cmd = commands;
+    const uint32_t* msk = masks;
+    if (!device_inputs) {
+        d_commands_.reserve((size_t)steps * per_step * sizeof(float));
+        hip_check(hipMemcpyAsync(d_commands_.ptr, commands, (size_t)steps * per_step * sizeof(float),
+                                 hipMemcpyHostToDevice, stream_),
+                  "commands");
+        cmd = static_cast<const float*>(d_commands_.ptr);
+        if (masks != nullptr) {
+            d_masks_.reserve((size_t)steps * E * sizeof(uint32_t));
+            hip_check(hipMemcpyAsync(d_masks_.ptr, masks, (size_t)steps * E * sizeof(uint32_t),
+                                     hipMemcpyHostToDevice, stream_),
+                      "masks");
+            msk = static_cast<const uint32_t*>(d_masks_.ptr);
+        }
+    }
+    d_status_.reserve((size_t)steps * E * sizeof(MRWorldStatusC));
+    d_cstatus_.reserve((size_t)steps * E * sizeof(MRContactStatusC));
+    hip_check(hipEventRecord(ev_begin_, stream_), "event");
+    for (uint32_t s = 0; s < steps; ++s) {
+        KernelArgs a = base_args(s);
+        a.command = cmd + (size_t)s * per_step;
+        a.reset_mask = msk != nullptr ? msk + (size_t)s * E : nullptr;
+        a.status = static_cast<MRWorldStatusC*>(d_status_.ptr) + (size_t)s * E;
+        a.cstatus = static_cast<MRContactStatusC*>(d_cstatus_.ptr) + (size_t)s * E;
+        hip_check(launch_control_step(a, lds_.total, stream_), "control step launch");
+    }
+    hip_check(hipEventRecord(ev_end_, stream_), "event");
+    last_steps_ = steps;
+}
+
+AdvanceSummary World::wait() {
+    hip_check(hipStreamSynchronize(stream_), "stream synchronize");
+    hip_check(hipEventElapsedTime(&last_kernel_ms_, ev_begin_, ev_end_), "event elapsed");
+    AdvanceSummary s;
+    s.control_step_count = last_steps_;
+    s.gpu_milliseconds = last_kernel_ms_;
+    const size_t E = env_, n = (size_t)last_steps_ * E;
+    std::vector<MRWorldStatusC> st(n);
+    std::vector<MRContactStatusC> cs(n);
+    hip_check(hipMemcpy(st.data(), d_status_.ptr, n * sizeof(MRWorldStatusC), hipMemcpyDeviceToHost), "status");
+    hip_check(hipMemcpy(cs.data(), d_cstatus_.ptr, n * sizeof(MRContactStatusC), hipMemcpyDeviceToHost), "cstatus");
+    for (size_t i = 0; i < n; ++i) {
+        if (st[i].code == MR_STEP_SUCCESS) {
+            ++s.successful_environment_steps;
+        } else {
+            ++s.failed_environment_steps;
+            if (s.first_failing_environment == MR_INVALID_INDEX) {
+                s.first_failing_environment = (uint32_t)(i % E);
+                s.first_failing_control_step = (uint32_t)(i / E);
+                s.first_gpu_status_code = st[i].code;
+            }
+        }
+        s.maximum_active_pairs = std::max(s.maximum_active_pairs, cs[i].max_active_pairs);
+        s.maximum_constraints = std::max(s.maximum_constraints, cs[i].max_constraints);
+    }
+    return s;
+}
+
+void World::read_statuses(MRWorldStatusC* out, uint32_t steps) const {
+    if (steps > last_steps_) throw std::runtime_error("fewer control steps were executed");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    hip_check(hipMemcpy(out, d_status_.ptr, (size_t)steps * env_ * sizeof(MRWorldStatusC), hipMemcpyDeviceToHost),
+              "read statuses");
+}
+
+void World::read_contact_statuses(MRContactStatusC* out, uint32_t steps) const {
+    if (steps > last_steps_) throw std::runtime_error("fewer control steps were executed");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    hip_check(hipMemcpy(out, d_cstatus_.ptr, (size_t)steps * env_ * sizeof(MRContactStatusC), hipMemcpyDeviceToHost),
+              "read contact statuses");
+}
+
+void World::read_acceleration(float* out) const {
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    hip_check(hipMemcpy(out, d_accel_.ptr, (size_t)env_ * model_.max_nv * sizeof(float), hipMemcpyDeviceToHost),
+              "read acceleration");
+}
+
+void World::read_evidence(MRPairRecordC* pairs, uint32_t* counts) const {
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    hip_check(hipMemcpy(pairs, d_ev_pairs_.ptr, (size_t)env_ * cfg_.pair_capacity * sizeof(MRPairRecordC),
+                        hipMemcpyDeviceToHost),
+              "read evidence");
+    hip_check(hipMemcpy(counts, d_ev_counts_.ptr, (size_t)env_ * sizeof(uint32_t), hipMemcpyDeviceToHost),
+              "read evidence counts");
+}
+
+} // namespace mr

@@ -1,0 +1,96 @@
+// Host executor of the batched temporalCone world on one MI355X GPU.
+#pragma once
+
+#include "mr_builder.h"
+#include "mr_kernel_args.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace mr {
+
+// Device arena that only grows.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t n);
+    void release();
+    ~DeviceBuffer() { release(); }
+};
+
+struct AdvanceSummary {
+    uint32_t control_step_count = 0;
+    uint32_t successful_environment_steps = 0;
+    uint32_t failed_environment_steps = 0;
+    uint32_t first_failing_environment = MR_INVALID_INDEX;
+    uint32_t first_failing_control_step = MR_INVALID_INDEX;
+    uint32_t first_gpu_status_code = 0;
+    uint32_t maximum_active_pairs = 0;
+    uint32_t maximum_constraints = 0;
+    double gpu_milliseconds = 0.0;
+};
+
+class World {
+public:
+    World(const ModelData& model, const MRWorldStepConfigC& config);
+    ~World();
+    World(const World&) = delete;
+    World& operator=(const World&) = delete;
+
+    uint32_t environment_count() const { return env_; }
+    const ModelData& model() const { return model_; }
+    const MRWorldStepConfigC& config() const { return cfg_; }
+    const LdsLayout& lds() const { return lds_; }
+    size_t retained_bytes() const;
+    hipStream_t stream() const { return stream_; }
+
+    // Host-array state boundary (all environment-major with model strides).
+    void set_state(const float* q, const float* v, const MRBodyStateC* scene);
+    void set_reset_state(const float* q, const float* v, const MRBodyStateC* scene);
+    void set_manifolds(const MRManifoldHeaderC* h, const MRManifoldPointC* p, const uint32_t* count);
+    void get_state(float* q, float* v, MRBodyStateC* scene) const;
+    void get_manifolds(MRManifoldHeaderC* h, MRManifoldPointC* p, uint32_t* count) const;
+
+    // Enqueues `steps` control steps on the world's stream. commands is
+    // [steps][E][max_nv], masks is [steps][E] (or null). With device_inputs
+    // the pointers are device memory already resident in HBM; otherwise they
+    // are host arrays copied before the first launch.
+    void enqueue(const float* commands, const uint32_t* masks, uint32_t steps, bool device_inputs);
+    // Waits for the stream and summarises the statuses of the last enqueue.
+    AdvanceSummary wait();
+
+    void read_statuses(MRWorldStatusC* out, uint32_t steps) const;  // [steps][E] of the last enqueue
+    void read_contact_statuses(MRContactStatusC* out, uint32_t steps) const;
+    void read_acceleration(float* out) const;                       // [E][max_nv]
+    void read_evidence(MRPairRecordC* pairs, uint32_t* counts) const; // [E][pair_capacity], [E]
+    uint32_t last_steps() const { return last_steps_; }
+    float last_kernel_milliseconds() const { return last_kernel_ms_; }
+
+private:
+    KernelArgs base_args(uint32_t step) const;
+    void compute_lds_layout();
+
+    ModelData model_;
+    MRWorldStepConfigC cfg_;
+    uint32_t env_ = 0;
+    LdsLayout lds_{};
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev_begin_ = nullptr, ev_end_ = nullptr;
+    // model
+    DeviceBuffer d_models_, d_env_model_, d_bodies_, d_dofs_, d_shapes_, d_materials_, d_pairs_, d_heights_;
+    // state
+    DeviceBuffer d_q_, d_v_, d_scene_, d_mh_, d_mp_, d_mcount_, d_mcurrent_;
+    DeviceBuffer d_reset_q_, d_reset_v_, d_reset_scene_;
+    // per-advance inputs / outputs
+    DeviceBuffer d_commands_, d_masks_, d_status_, d_cstatus_, d_accel_, d_ev_pairs_, d_ev_counts_;
+    // scratch
+    DeviceBuffer d_stage_h_, d_stage_p_, d_jrow_, d_resp_, d_cold_;
+    uint32_t last_steps_ = 0;
+    float last_kernel_ms_ = 0.0f;
+};
+
+void hip_check(hipError_t e, const char* what);
+
+} // namespace mr

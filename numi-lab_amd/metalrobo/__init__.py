@@ -1,0 +1,24 @@
+"""MI355X-native batched temporalCone stepping (drop-in for the reference's
+python/metalrobo stepping surface). Physics runs in the native library
+libmetalrobo_amd.so; this package is the ctypes boundary."""
+
+from .native import (  # noqa: F401
+    BODY_STATE_DTYPE,
+    CONTACT_STATUS_DTYPE,
+    MANIFOLD_HEADER_DTYPE,
+    MANIFOLD_POINT_DTYPE,
+    MR_RUN_SOURCE_SYNTHETIC_CHAIN_BOXES,
+    MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN,
+    MR_SYNTHETIC_CHAIN_BOXES,
+    MR_SYNTHETIC_HUMANOID_TERRAIN,
+    PAIR_RECORD_DTYPE,
+    STATUS_NAMES,
+    WORLD_STATUS_DTYPE,
+    MetalRoboError,
+    TaskRollout,
+    World,
+    WorldModel,
+    WorldStepConfigC,
+    load,
+    version,
+)

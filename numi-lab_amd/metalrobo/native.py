@@ -1,0 +1,626 @@
+"""Thin ctypes binding for the MI355X temporalCone C ABI (include/mr_c_api.h).
+
+Conventions follow the reference binding (python/metalrobo/native.py:16-20,
+225-264, 311-313, 589-590): the library is resolved from ``METALROBO_LIBRARY``
+or the in-tree build, ``argtypes``/``restype`` are declared once per loaded
+library, native failures raise :class:`MetalRoboError` carrying
+``mr_last_error()``, and C structs are mirrored by ctypes ``Structure``s with
+size checks. PyTorch tensors are accepted only at this edge, as device
+inputs already resident in HBM.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+import os
+from pathlib import Path
+from typing import Final, Optional
+
+import numpy as np
+
+_LIBRARY_ENV: Final = "METALROBO_LIBRARY"
+
+MR_INVALID_INDEX: Final = 0xFFFFFFFF
+MR_MANIFOLD_POINT_CAPACITY: Final = 4
+
+MR_SYNTHETIC_CHAIN_BOXES: Final = 1
+MR_SYNTHETIC_HUMANOID_TERRAIN: Final = 2
+MR_RUN_SOURCE_SYNTHETIC_CHAIN_BOXES: Final = 100
+MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN: Final = 101
+
+MR_ACTUATION_EFFORT: Final = 0
+MR_ACTUATION_IMPLICIT_POSITION_DRIVE: Final = 1
+
+STATUS_NAMES: Final = {
+    0: "success", 1: "fixed_budget_complete", 2: "nonfinite_input", 3: "nonfinite_result",
+    4: "pair_capacity_overflow", 5: "contact_capacity_overflow", 6: "manifold_capacity_overflow",
+    7: "constraint_capacity_overflow", 8: "island_capacity_overflow", 9: "ccd_event_budget_exhausted",
+    10: "factorization_failed", 11: "did_not_converge", 12: "unsupported", 13: "ccd_capacity_overflow",
+}
+
+
+class MetalRoboError(RuntimeError):
+    """Raised when the native runtime reports an error."""
+
+
+# ----------------------------------------------------------------- records
+
+class _F4(ct.Structure):
+    _fields_ = [("x", ct.c_float), ("y", ct.c_float), ("z", ct.c_float), ("w", ct.c_float)]
+
+
+class HeightfieldC(ct.Structure):
+    _fields_ = [("nx", ct.c_uint32), ("ny", ct.c_uint32), ("present", ct.c_uint32), ("reserved", ct.c_uint32),
+                ("origin_spacing", _F4), ("bounds_lower", _F4), ("bounds_upper", _F4)]
+
+
+class WorldModelDescC(ct.Structure):
+    _fields_ = [
+        ("model_count", ct.c_uint32), ("environment_count", ct.c_uint32), ("max_nq", ct.c_uint32),
+        ("max_nv", ct.c_uint32), ("max_scene_bodies", ct.c_uint32), ("max_shapes", ct.c_uint32),
+        ("max_pairs", ct.c_uint32), ("reserved", ct.c_uint32), ("gravity", _F4),
+        ("models", ct.c_void_p), ("environment_model", ct.c_void_p), ("bodies", ct.c_void_p),
+        ("dofs", ct.c_void_p), ("shapes", ct.c_void_p), ("materials", ct.c_void_p), ("pairs", ct.c_void_p),
+        ("body_total", ct.c_uint32), ("dof_total", ct.c_uint32), ("shape_total", ct.c_uint32),
+        ("material_total", ct.c_uint32), ("pair_total", ct.c_uint32), ("heightfield_vertex_count", ct.c_uint32),
+        ("heightfield", HeightfieldC), ("heightfield_heights", ct.c_void_p),
+    ]
+
+
+class WorldStepConfigC(ct.Structure):
+    _fields_ = [
+        ("control_timestep", ct.c_float), ("physics_substeps", ct.c_uint32),
+        ("velocity_iterations", ct.c_uint32), ("final_velocity_iterations", ct.c_uint32),
+        ("actuation_mode", ct.c_uint32), ("warm_start", ct.c_uint32), ("apply_body_damping", ct.c_uint32),
+        ("capture_evidence", ct.c_uint32), ("capture_substep", ct.c_uint32), ("pair_capacity", ct.c_uint32),
+        ("manifold_capacity", ct.c_uint32), ("constraint_capacity", ct.c_uint32),
+        ("manifold_breaking_separation", ct.c_float), ("manifold_breaking_tangential", ct.c_float),
+        ("manifold_merge_distance", ct.c_float), ("manifold_normal_cosine", ct.c_float),
+        ("contact_slop", ct.c_float), ("max_depenetration_velocity", ct.c_float),
+        ("residual_tolerance", ct.c_float), ("max_iterations", ct.c_uint32),
+    ]
+
+    def copy(self) -> "WorldStepConfigC":
+        out = WorldStepConfigC()
+        ct.memmove(ct.byref(out), ct.byref(self), ct.sizeof(self))
+        return out
+
+
+class SyntheticWorldConfigC(ct.Structure):
+    _fields_ = [("kind", ct.c_uint32), ("environment_count", ct.c_uint32), ("seed", ct.c_uint64)]
+
+
+class WorldAdvanceC(ct.Structure):
+    _fields_ = [
+        ("control_step_count", ct.c_uint32), ("successful_environment_steps", ct.c_uint32),
+        ("failed_environment_steps", ct.c_uint32), ("first_failing_environment", ct.c_uint32),
+        ("first_failing_control_step", ct.c_uint32), ("first_gpu_status_code", ct.c_uint32),
+        ("maximum_active_pairs", ct.c_uint32), ("maximum_constraints", ct.c_uint32),
+        ("gpu_milliseconds", ct.c_double),
+    ]
+
+
+class WorldLayoutC(ct.Structure):
+    _fields_ = [
+        ("environment_count", ct.c_uint32), ("max_nq", ct.c_uint32), ("max_nv", ct.c_uint32),
+        ("max_scene_bodies", ct.c_uint32), ("pair_capacity", ct.c_uint32), ("manifold_capacity", ct.c_uint32),
+        ("constraint_capacity", ct.c_uint32), ("lds_bytes_per_environment", ct.c_uint32),
+        ("retained_device_bytes", ct.c_uint64),
+    ]
+
+
+class TaskRolloutConfigC(ct.Structure):
+    _fields_ = [
+        ("environment_count", ct.c_uint32), ("physics_substeps", ct.c_uint32),
+        ("velocity_iterations", ct.c_uint32), ("final_velocity_iterations", ct.c_uint32),
+        ("control_timestep_seconds", ct.c_float), ("reserved", ct.c_uint32), ("seed", ct.c_uint64),
+    ]
+
+
+class RunManifestC(ct.Structure):
+    _fields_ = [("profile", TaskRolloutConfigC), ("source", ct.c_uint32), ("reserved", ct.c_uint32)]
+
+
+class TaskRolloutLayoutC(ct.Structure):
+    _fields_ = [
+        ("environment_count", ct.c_uint32), ("nq", ct.c_uint32), ("nv", ct.c_uint32),
+        ("action_count", ct.c_uint32), ("scene_body_count", ct.c_uint32), ("reserved", ct.c_uint32),
+        ("submitted_control_steps", ct.c_uint64), ("completed_environment_steps", ct.c_uint64),
+        ("submission_count", ct.c_uint64), ("retained_buffer_bytes", ct.c_size_t),
+        ("total_gpu_milliseconds", ct.c_double),
+    ]
+
+
+class TaskRolloutAdvanceC(ct.Structure):
+    _fields_ = [
+        ("control_step_count", ct.c_uint32), ("successful_environment_steps", ct.c_uint32),
+        ("failed_environment_steps", ct.c_uint32), ("first_failing_environment", ct.c_uint32),
+        ("first_failing_control_step", ct.c_uint32), ("first_gpu_status_code", ct.c_uint32),
+        ("host_requested_resets", ct.c_uint32), ("maximum_active_contacts", ct.c_uint32),
+        ("gpu_milliseconds", ct.c_double),
+    ]
+
+
+# numpy mirrors of the bulk records (include/mr_world_types.h)
+F4 = ("<f4", (4,))
+MODEL_HEADER_DTYPE: Final = np.dtype([(n, "<u4") for n in (
+    "body_offset", "body_count", "link_count", "root_type", "dof_offset", "nq", "nv", "shape_offset",
+    "shape_count", "pair_offset", "pair_count", "level_count", "scene_body_count", "material_offset",
+    "material_count", "reserved")])
+BODY_DTYPE: Final = np.dtype([
+    ("parent", "<u4"), ("joint_type", "<u4"), ("q_index", "<u4"), ("v_index", "<u4"), ("motion_type", "<u4"),
+    ("depth", "<u4"), ("flags", "<u4"), ("reserved", "<u4"), ("mass", *F4), ("inertia", "<f4", (3, 4)),
+    ("inverse_inertia", "<f4", (3, 4)), ("axis", *F4), ("parent_anchor", *F4), ("child_anchor", *F4),
+    ("parent_rotation", *F4), ("child_rotation", *F4), ("damping", *F4), ("damping_scale", *F4)])
+DOF_DTYPE: Final = np.dtype([("body", "<u4"), ("q_index", "<u4"), ("flags", "<u4"), ("reserved", "<u4"),
+                             ("limits", *F4), ("drive", *F4)])
+SHAPE_DTYPE: Final = np.dtype([
+    ("body", "<u4"), ("shape_type", "<u4"), ("material", "<u4"), ("flags", "<u4"), ("collision_group", "<u4"),
+    ("collision_mask", "<u4"), ("slot_generation", "<u4"), ("reserved", "<u4"), ("local_position", *F4),
+    ("local_rotation", *F4), ("dimensions", *F4), ("contact_rest_radius", *F4)])
+MATERIAL_DTYPE: Final = np.dtype([("friction", *F4), ("response", *F4)])
+PAIR_DTYPE: Final = np.dtype([("collider_a", "<u4"), ("collider_b", "<u4"), ("pair_class", "<u4"),
+                              ("reserved", "<u4")])
+BODY_STATE_DTYPE: Final = np.dtype([
+    ("position", *F4), ("orientation", *F4), ("linear_velocity_inv_mass", *F4), ("angular_velocity", *F4),
+    ("inverse_inertia_world", "<f4", (3, 4)), ("flags_indices", "<u4", (4,))])
+MANIFOLD_HEADER_DTYPE: Final = np.dtype([
+    ("pair_and_count", "<u4", (4,)), ("generations_and_flags", "<u4", (4,)), ("normal_and_age", *F4),
+    ("tangent_and_metric", *F4)])
+MANIFOLD_POINT_DTYPE: Final = np.dtype([
+    ("local_anchor_a", *F4), ("local_anchor_b", *F4), ("impulses", *F4), ("feature_and_life", "<u4", (4,))])
+WORLD_STATUS_DTYPE: Final = np.dtype([
+    ("code", "<u4"), ("environment", "<u4"), ("control_step", "<u4"), ("successful_substeps", "<u4"),
+    ("aba_code", "<u4"), ("failing_substep", "<u4"), ("failing_index", "<u4"), ("flags", "<u4"),
+    ("diagnostics", *F4)])
+CONTACT_STATUS_DTYPE: Final = np.dtype([(n, "<u4") for n in (
+    "code", "active_pairs", "raw_contacts", "manifolds", "constraints", "islands", "retained_points",
+    "new_points", "required_pairs", "required_manifolds", "required_constraints", "solver_iterations",
+    "first_failing_pair", "first_failing_constraint", "max_active_pairs", "max_constraints")] +
+    [("residuals", *F4), ("diagnostics", *F4)])
+PAIR_RECORD_DTYPE: Final = np.dtype([("environment", "<u4"), ("collider_a", "<u4"), ("collider_b", "<u4"),
+                                     ("pair_class", "<u4")])
+
+_EXPECTED_SIZES: Final = {
+    MODEL_HEADER_DTYPE: 64, BODY_DTYPE: 256, DOF_DTYPE: 48, SHAPE_DTYPE: 96, MATERIAL_DTYPE: 32, PAIR_DTYPE: 16,
+    BODY_STATE_DTYPE: 128, MANIFOLD_HEADER_DTYPE: 64, MANIFOLD_POINT_DTYPE: 64, WORLD_STATUS_DTYPE: 48,
+    CONTACT_STATUS_DTYPE: 96, PAIR_RECORD_DTYPE: 16,
+}
+for _dt, _size in _EXPECTED_SIZES.items():
+    assert _dt.itemsize == _size, (_dt, _dt.itemsize, _size)
+assert ct.sizeof(WorldStepConfigC) == 80
+assert ct.sizeof(WorldModelDescC) == 200
+assert ct.sizeof(TaskRolloutLayoutC) == 64
+assert ct.sizeof(WorldAdvanceC) == 40
+assert ct.sizeof(WorldLayoutC) == 40
+assert ct.sizeof(RunManifestC) == 40
+
+
+def _ptr(arr: Optional[np.ndarray]) -> Optional[int]:
+    if arr is None:
+        return None
+    if not arr.flags["C_CONTIGUOUS"]:
+        raise ValueError("arrays crossing the C ABI must be C-contiguous")
+    return arr.ctypes.data
+
+
+# ------------------------------------------------------------ library load
+
+def _package_directory() -> Path:
+    return Path(__file__).resolve().parent
+
+
+def resolve_library_path(path: str | os.PathLike[str] | None = None) -> Path:
+    """Resolve the native library, preferring explicit and environment paths."""
+    if path is not None:
+        explicit = Path(path).expanduser()
+        if not explicit.is_file():
+            raise FileNotFoundError(f"MetalRobo library does not exist: {explicit}")
+        return explicit.resolve()
+    configured = os.environ.get(_LIBRARY_ENV)
+    if configured:
+        configured_path = Path(configured).expanduser()
+        if not configured_path.is_file():
+            raise FileNotFoundError(f"{_LIBRARY_ENV} points to a missing file: {configured_path}")
+        return configured_path.resolve()
+    default = (_package_directory() / ".." / "libmetalrobo_amd.so").resolve()
+    if default.is_file():
+        return default
+    raise FileNotFoundError(
+        "Could not find libmetalrobo_amd.so. Build it with `python numi-lab_amd/build.py` or set "
+        f"{_LIBRARY_ENV}. Expected the in-tree build at {default}")
+
+
+_SIGNATURES: Final = {
+    "mr_version": ([], ct.c_char_p),
+    "mr_runtime_abi_fingerprint": ([], ct.c_uint64),
+    "mr_last_error": ([], ct.c_char_p),
+    "mr_world_model_create_synthetic": ([ct.POINTER(SyntheticWorldConfigC)], ct.c_void_p),
+    "mr_world_model_create": ([ct.POINTER(WorldModelDescC), ct.c_float], ct.c_void_p),
+    "mr_world_model_destroy": ([ct.c_void_p], None),
+    "mr_world_model_describe": ([ct.c_void_p, ct.POINTER(WorldModelDescC)], ct.c_int),
+    "mr_world_model_initial_state": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_void_p, ct.c_size_t,
+                                      ct.c_void_p, ct.c_size_t], ct.c_int),
+    "mr_world_model_commands": ([ct.c_void_p, ct.c_uint32, ct.c_uint32, ct.c_void_p, ct.c_size_t], ct.c_int),
+    "mr_world_model_default_config": ([ct.c_void_p, ct.POINTER(WorldStepConfigC)], ct.c_int),
+    "mr_world_create": ([ct.c_void_p, ct.POINTER(WorldStepConfigC)], ct.c_void_p),
+    "mr_world_destroy": ([ct.c_void_p], None),
+    "mr_world_layout": ([ct.c_void_p, ct.POINTER(WorldLayoutC)], ct.c_int),
+    "mr_world_initialize_resident_state": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_void_p, ct.c_size_t,
+                                            ct.c_void_p, ct.c_size_t], ct.c_int),
+    "mr_world_set_reset_state": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_void_p, ct.c_size_t,
+                                  ct.c_void_p, ct.c_size_t], ct.c_int),
+    "mr_world_write_manifolds": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p], ct.c_int),
+    "mr_world_advance": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_void_p, ct.c_size_t, ct.c_uint32,
+                          ct.POINTER(WorldAdvanceC)], ct.c_int),
+    "mr_world_enqueue_device": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32], ct.c_int),
+    "mr_world_wait": ([ct.c_void_p, ct.POINTER(WorldAdvanceC)], ct.c_int),
+    "mr_world_read_state": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_void_p, ct.c_size_t, ct.c_void_p,
+                             ct.c_size_t], ct.c_int),
+    "mr_world_read_manifolds": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p], ct.c_int),
+    "mr_world_read_statuses": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
+    "mr_world_read_contact_statuses": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
+    "mr_world_read_acceleration": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
+    "mr_world_read_evidence": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_void_p, ct.c_size_t], ct.c_int),
+    "mr_world_stream": ([ct.c_void_p], ct.c_void_p),
+    "mr_world_last_kernel_milliseconds": ([ct.c_void_p], ct.c_double),
+    "mr_create_task_rollout": ([ct.POINTER(RunManifestC)], ct.c_void_p),
+    "mr_task_rollout_destroy": ([ct.c_void_p], None),
+    "mr_task_rollout_reset": ([ct.c_void_p, ct.c_uint64], ct.c_int),
+    "mr_task_rollout_set_state_readback": ([ct.c_void_p, ct.c_uint32], ct.c_int),
+    "mr_task_rollout_advance": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_void_p, ct.c_size_t, ct.c_uint32,
+                                 ct.c_uint64, ct.c_uint32, ct.POINTER(TaskRolloutAdvanceC)], ct.c_int),
+    "mr_task_rollout_layout": ([ct.c_void_p], TaskRolloutLayoutC),
+    "mr_task_rollout_status_codes": ([ct.c_void_p], ct.POINTER(ct.c_uint32)),
+    "mr_task_rollout_active_contacts": ([ct.c_void_p], ct.POINTER(ct.c_uint32)),
+    "mr_task_rollout_final_q": ([ct.c_void_p], ct.POINTER(ct.c_float)),
+    "mr_task_rollout_copy_final_scene_states": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
+}
+
+
+class _Bindings:
+    """Configured ctypes function table for one loaded library."""
+
+    def __init__(self, path: str | os.PathLike[str] | None = None) -> None:
+        self.path = resolve_library_path(path)
+        self.lib = ct.CDLL(str(self.path))
+        for name, (argtypes, restype) in _SIGNATURES.items():
+            fn = getattr(self.lib, name)
+            fn.argtypes = argtypes
+            fn.restype = restype
+
+    def last_error(self) -> str:
+        raw = self.lib.mr_last_error()
+        return (raw.decode() if raw else "") or "unknown native error"
+
+    def check(self, status: int) -> None:
+        if status != 0:
+            raise MetalRoboError(self.last_error())
+
+
+_binding_cache: dict[Path, _Bindings] = {}
+
+
+def load(path: str | os.PathLike[str] | None = None) -> _Bindings:
+    resolved = resolve_library_path(path)
+    b = _binding_cache.get(resolved)
+    if b is None:
+        b = _Bindings(resolved)
+        _binding_cache[resolved] = b
+    return b
+
+
+def version() -> str:
+    return load().lib.mr_version().decode()
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGNATURES)
+
+
+# ------------------------------------------------------------- high level
+
+class WorldModel:
+    """Owned immutable world set (one model block per environment or shared)."""
+
+    def __init__(self, handle: int, bindings: _Bindings) -> None:
+        self._b = bindings
+        self._h = handle
+        self.desc = WorldModelDescC()
+        bindings.check(bindings.lib.mr_world_model_describe(handle, ct.byref(self.desc)))
+
+    @classmethod
+    def synthetic(cls, kind: int, environment_count: int, seed: int = 0,
+                  library: str | os.PathLike[str] | None = None) -> "WorldModel":
+        b = load(library)
+        cfg = SyntheticWorldConfigC(kind, environment_count, seed)
+        h = b.lib.mr_world_model_create_synthetic(ct.byref(cfg))
+        if not h:
+            raise MetalRoboError(b.last_error())
+        return cls(h, b)
+
+    def close(self) -> None:
+        if self._h:
+            self._b.lib.mr_world_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> int:
+        return self._h
+
+    @property
+    def environment_count(self) -> int:
+        return self.desc.environment_count
+
+    @property
+    def max_nq(self) -> int:
+        return self.desc.max_nq
+
+    @property
+    def max_nv(self) -> int:
+        return self.desc.max_nv
+
+    @property
+    def max_scene_bodies(self) -> int:
+        return self.desc.max_scene_bodies
+
+    def _view(self, ptr: int, dtype: np.dtype, count: int) -> np.ndarray:
+        if count == 0:
+            return np.zeros(0, dtype=dtype)
+        buf = (ct.c_char * (dtype.itemsize * count)).from_address(ptr)
+        return np.frombuffer(buf, dtype=dtype, count=count).copy()
+
+    def models(self) -> np.ndarray:
+        return self._view(self.desc.models, MODEL_HEADER_DTYPE, self.desc.model_count)
+
+    def environment_model(self) -> np.ndarray:
+        return self._view(self.desc.environment_model, np.dtype("<u4"), self.desc.environment_count)
+
+    def bodies(self) -> np.ndarray:
+        return self._view(self.desc.bodies, BODY_DTYPE, self.desc.body_total)
+
+    def dofs(self) -> np.ndarray:
+        return self._view(self.desc.dofs, DOF_DTYPE, self.desc.dof_total)
+
+    def shapes(self) -> np.ndarray:
+        return self._view(self.desc.shapes, SHAPE_DTYPE, self.desc.shape_total)
+
+    def materials(self) -> np.ndarray:
+        return self._view(self.desc.materials, MATERIAL_DTYPE, self.desc.material_total)
+
+    def pairs(self) -> np.ndarray:
+        return self._view(self.desc.pairs, PAIR_DTYPE, self.desc.pair_total)
+
+    def heights(self) -> np.ndarray:
+        hf = self.desc.heightfield
+        if not hf.present:
+            return np.zeros((0, 0), np.float32)
+        return self._view(self.desc.heightfield_heights, np.dtype("<f4"), hf.nx * hf.ny).reshape(hf.ny, hf.nx)
+
+    def initial_state(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        E = self.environment_count
+        q = np.zeros((E, self.max_nq), np.float32)
+        v = np.zeros((E, self.max_nv), np.float32)
+        s = np.zeros((E, self.max_scene_bodies), BODY_STATE_DTYPE)
+        self._b.check(self._b.lib.mr_world_model_initial_state(self._h, _ptr(q), q.size, _ptr(v), v.size, _ptr(s),
+                                                               s.size))
+        return q, v, s
+
+    def commands(self, first_step: int, step_count: int) -> np.ndarray:
+        out = np.zeros((step_count, self.environment_count, self.max_nv), np.float32)
+        self._b.check(self._b.lib.mr_world_model_commands(self._h, first_step, step_count, _ptr(out), out.size))
+        return out
+
+    def default_config(self) -> WorldStepConfigC:
+        cfg = WorldStepConfigC()
+        self._b.check(self._b.lib.mr_world_model_default_config(self._h, ct.byref(cfg)))
+        return cfg
+
+
+class World:
+    """Resident batched world on one GPU (physics-only MetalWorldContext mirror)."""
+
+    def __init__(self, model: WorldModel, config: Optional[WorldStepConfigC] = None) -> None:
+        self._b = model._b
+        self.model = model
+        self.config = (config if config is not None else model.default_config()).copy()
+        h = self._b.lib.mr_world_create(model.handle, ct.byref(self.config))
+        if not h:
+            raise MetalRoboError(self._b.last_error())
+        self._h = h
+        self.last_steps = 0
+
+    def close(self) -> None:
+        if self._h:
+            self._b.lib.mr_world_destroy(self._h)
+            self._h = None
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def environment_count(self) -> int:
+        return self.model.environment_count
+
+    def layout(self) -> WorldLayoutC:
+        out = WorldLayoutC()
+        self._b.check(self._b.lib.mr_world_layout(self._h, ct.byref(out)))
+        return out
+
+    def set_state(self, q: np.ndarray, v: np.ndarray, scene: np.ndarray) -> None:
+        q = np.ascontiguousarray(q, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        scene = np.ascontiguousarray(scene, BODY_STATE_DTYPE)
+        self._b.check(self._b.lib.mr_world_initialize_resident_state(self._h, _ptr(q), q.size, _ptr(v), v.size,
+                                                                     _ptr(scene), scene.size))
+
+    def set_reset_state(self, q: np.ndarray, v: np.ndarray, scene: np.ndarray) -> None:
+        q = np.ascontiguousarray(q, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        scene = np.ascontiguousarray(scene, BODY_STATE_DTYPE)
+        self._b.check(self._b.lib.mr_world_set_reset_state(self._h, _ptr(q), q.size, _ptr(v), v.size, _ptr(scene),
+                                                           scene.size))
+
+    def write_manifolds(self, headers: np.ndarray, points: np.ndarray, counts: np.ndarray) -> None:
+        headers = np.ascontiguousarray(headers, MANIFOLD_HEADER_DTYPE)
+        points = np.ascontiguousarray(points, MANIFOLD_POINT_DTYPE)
+        counts = np.ascontiguousarray(counts, np.uint32)
+        self._b.check(self._b.lib.mr_world_write_manifolds(self._h, _ptr(headers), _ptr(points), _ptr(counts)))
+
+    def state(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        E = self.environment_count
+        q = np.zeros((E, self.model.max_nq), np.float32)
+        v = np.zeros((E, self.model.max_nv), np.float32)
+        s = np.zeros((E, self.model.max_scene_bodies), BODY_STATE_DTYPE)
+        self._b.check(self._b.lib.mr_world_read_state(self._h, _ptr(q), q.size, _ptr(v), v.size, _ptr(s), s.size))
+        return q, v, s
+
+    def manifolds(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        E, cap = self.environment_count, self.config.manifold_capacity
+        h = np.zeros((E, cap), MANIFOLD_HEADER_DTYPE)
+        p = np.zeros((E, cap, MR_MANIFOLD_POINT_CAPACITY), MANIFOLD_POINT_DTYPE)
+        c = np.zeros(E, np.uint32)
+        self._b.check(self._b.lib.mr_world_read_manifolds(self._h, _ptr(h), _ptr(p), _ptr(c)))
+        return h, p, c
+
+    def advance(self, commands: np.ndarray, reset_masks: Optional[np.ndarray] = None) -> WorldAdvanceC:
+        commands = np.ascontiguousarray(commands, np.float32)
+        steps = commands.shape[0]
+        masks = None if reset_masks is None else np.ascontiguousarray(reset_masks, np.uint32)
+        out = WorldAdvanceC()
+        self._b.check(self._b.lib.mr_world_advance(self._h, _ptr(commands), commands.size, _ptr(masks),
+                                                   0 if masks is None else masks.size, steps, ct.byref(out)))
+        self.last_steps = steps
+        return out
+
+    def enqueue_device(self, commands, reset_masks=None) -> None:
+        """Enqueue with inputs already in device memory (torch tensors on the GPU)."""
+        steps = int(commands.shape[0])
+        expect = (steps, self.environment_count, self.model.max_nv)
+        if tuple(commands.shape) != expect or not commands.is_contiguous() or not commands.is_cuda:
+            raise ValueError(f"device commands must be a contiguous GPU tensor of shape {expect}")
+        mask_ptr = None
+        if reset_masks is not None:
+            if tuple(reset_masks.shape) != (steps, self.environment_count) or not reset_masks.is_cuda:
+                raise ValueError("device reset masks must be a GPU tensor [steps][environments]")
+            mask_ptr = reset_masks.data_ptr()
+        self._b.check(self._b.lib.mr_world_enqueue_device(self._h, commands.data_ptr(), mask_ptr, steps))
+        self.last_steps = steps
+
+    def wait(self) -> WorldAdvanceC:
+        out = WorldAdvanceC()
+        self._b.check(self._b.lib.mr_world_wait(self._h, ct.byref(out)))
+        return out
+
+    def statuses(self) -> np.ndarray:
+        out = np.zeros((self.last_steps, self.environment_count), WORLD_STATUS_DTYPE)
+        self._b.check(self._b.lib.mr_world_read_statuses(self._h, _ptr(out), out.size))
+        return out
+
+    def contact_statuses(self) -> np.ndarray:
+        out = np.zeros((self.last_steps, self.environment_count), CONTACT_STATUS_DTYPE)
+        self._b.check(self._b.lib.mr_world_read_contact_statuses(self._h, _ptr(out), out.size))
+        return out
+
+    def acceleration(self) -> np.ndarray:
+        out = np.zeros((self.environment_count, self.model.max_nv), np.float32)
+        self._b.check(self._b.lib.mr_world_read_acceleration(self._h, _ptr(out), out.size))
+        return out
+
+    def evidence(self) -> tuple[np.ndarray, np.ndarray]:
+        E, cap = self.environment_count, self.config.pair_capacity
+        pairs = np.zeros((E, cap), PAIR_RECORD_DTYPE)
+        counts = np.zeros(E, np.uint32)
+        self._b.check(self._b.lib.mr_world_read_evidence(self._h, _ptr(pairs), pairs.size, _ptr(counts),
+                                                         counts.size))
+        return pairs, counts
+
+    def stream(self) -> int:
+        return self._b.lib.mr_world_stream(self._h) or 0
+
+    def last_kernel_milliseconds(self) -> float:
+        return float(self._b.lib.mr_world_last_kernel_milliseconds(self._h))
+
+
+class TaskRollout:
+    """Stepping surface named after the reference's task-rollout ABI (c_api.h:583-746)."""
+
+    def __init__(self, source: int, environment_count: int, seed: int = 0, physics_substeps: int = 0,
+                 velocity_iterations: int = 0, final_velocity_iterations: int = 0,
+                 control_timestep_seconds: float = 0.0, library: str | os.PathLike[str] | None = None) -> None:
+        self._b = load(library)
+        manifest = RunManifestC()
+        manifest.profile = TaskRolloutConfigC(environment_count, physics_substeps, velocity_iterations,
+                                              final_velocity_iterations, control_timestep_seconds, 0, seed)
+        manifest.source = source
+        h = self._b.lib.mr_create_task_rollout(ct.byref(manifest))
+        if not h:
+            raise MetalRoboError(self._b.last_error())
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            self._b.lib.mr_task_rollout_destroy(self._h)
+            self._h = None
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def layout(self) -> TaskRolloutLayoutC:
+        return self._b.lib.mr_task_rollout_layout(self._h)
+
+    def reset(self, seed: int) -> None:
+        self._b.check(self._b.lib.mr_task_rollout_reset(self._h, seed))
+
+    def set_state_readback(self, enabled: bool) -> None:
+        self._b.check(self._b.lib.mr_task_rollout_set_state_readback(self._h, 1 if enabled else 0))
+
+    def advance(self, normalized_actions: np.ndarray, reset_masks: Optional[np.ndarray] = None,
+                policy_revision: int = 0) -> TaskRolloutAdvanceC:
+        actions = np.ascontiguousarray(normalized_actions, np.float32)
+        steps = actions.shape[0]
+        masks = None if reset_masks is None else np.ascontiguousarray(reset_masks, np.uint32)
+        out = TaskRolloutAdvanceC()
+        self._b.check(self._b.lib.mr_task_rollout_advance(self._h, _ptr(actions), actions.size, _ptr(masks),
+                                                          0 if masks is None else masks.size, steps,
+                                                          policy_revision, 0, ct.byref(out)))
+        self._last_steps = steps
+        return out
+
+    def status_codes(self) -> np.ndarray:
+        lay = self.layout()
+        ptr = self._b.lib.mr_task_rollout_status_codes(self._h)
+        n = self._last_steps * lay.environment_count
+        return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(self._last_steps, lay.environment_count).copy()
+
+    def active_contacts(self) -> np.ndarray:
+        lay = self.layout()
+        ptr = self._b.lib.mr_task_rollout_active_contacts(self._h)
+        n = self._last_steps * lay.environment_count
+        return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(self._last_steps, lay.environment_count).copy()
+
+    def final_q(self) -> np.ndarray:
+        lay = self.layout()
+        ptr = self._b.lib.mr_task_rollout_final_q(self._h)
+        if not ptr:
+            raise MetalRoboError("no final q (enable state readback and advance first)")
+        return np.ctypeslib.as_array(ptr, shape=(lay.environment_count * lay.nq,)).reshape(
+            lay.environment_count, lay.nq).copy()
+
+    def final_scene_states(self) -> np.ndarray:
+        lay = self.layout()
+        out = np.zeros((lay.environment_count, lay.scene_body_count, 13), np.float32)
+        self._b.check(self._b.lib.mr_task_rollout_copy_final_scene_states(self._h, _ptr(out), out.size))
+        return out

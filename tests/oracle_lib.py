@@ -1,0 +1,142 @@
+"""ctypes loader for the CPU oracle (oracle/liboracle_mr.so) used by the tests.
+
+The oracle is test infrastructure; it is the checker, never the product."""
+
+import ctypes as ct
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "numi-lab_amd"))
+
+from metalrobo.native import (  # noqa: E402
+    BODY_STATE_DTYPE,
+    CONTACT_STATUS_DTYPE,
+    MANIFOLD_HEADER_DTYPE,
+    MANIFOLD_POINT_DTYPE,
+    MR_MANIFOLD_POINT_CAPACITY,
+    PAIR_RECORD_DTYPE,
+    WORLD_STATUS_DTYPE,
+    WorldModelDescC,
+    WorldStepConfigC,
+)
+
+LIBRARY = os.path.join(ROOT, "oracle", "liboracle_mr.so")
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.isfile(LIBRARY):
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import build_oracle  # noqa: E402
+
+            build_oracle.build(verbose=False)
+        lib = ct.CDLL(LIBRARY)
+        P = ct.c_void_p
+        lib.mr_oracle_control_step.argtypes = [ct.POINTER(WorldModelDescC), ct.POINTER(WorldStepConfigC),
+                                               ct.c_uint32, ct.c_uint32, ct.c_uint32] + [P] * 16
+        lib.mr_oracle_control_step.restype = ct.c_int
+        lib.mr_oracle_control_step_parallel.argtypes = [ct.POINTER(WorldModelDescC), ct.POINTER(WorldStepConfigC),
+                                                        ct.c_uint32, ct.c_uint32, ct.c_uint32] + [P] * 10
+        lib.mr_oracle_control_step_parallel.restype = ct.c_int
+        lib.mr_oracle_forward_dynamics.argtypes = [ct.POINTER(WorldModelDescC), ct.POINTER(WorldStepConfigC),
+                                                   ct.c_uint32, P, P, P, P]
+        lib.mr_oracle_forward_dynamics.restype = ct.c_int
+        lib.mr_oracle_point_response.argtypes = [ct.POINTER(WorldModelDescC), ct.POINTER(WorldStepConfigC),
+                                                 ct.c_uint32, P, P, ct.c_uint32, P, P, P]
+        lib.mr_oracle_point_response.restype = ct.c_int
+        lib.mr_oracle_link_poses.argtypes = [ct.POINTER(WorldModelDescC), ct.POINTER(WorldStepConfigC),
+                                             ct.c_uint32, P, P, P]
+        lib.mr_oracle_link_poses.restype = ct.c_int
+        lib.mr_oracle_sincos.argtypes = [P, P, P, ct.c_uint32]
+        lib.mr_oracle_sincos.restype = None
+        lib.mr_oracle_abi_version.argtypes = []
+        lib.mr_oracle_abi_version.restype = ct.c_uint32
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class OracleState:
+    """Host copy of one world's persistent state in the layout the oracle steps."""
+
+    def __init__(self, model, config):
+        E = model.environment_count
+        self.model = model
+        self.config = config
+        self.q, self.v, self.scene = model.initial_state()
+        cap = config.manifold_capacity
+        self.mh = np.zeros((E, cap), MANIFOLD_HEADER_DTYPE)
+        self.mp = np.zeros((E, cap, MR_MANIFOLD_POINT_CAPACITY), MANIFOLD_POINT_DTYPE)
+        self.mcount = np.zeros(E, np.uint32)
+        self.reset_q, self.reset_v, self.reset_scene = model.initial_state()
+
+    def step(self, command, reset_mask=None, control_step=0, env_begin=0, env_end=None):
+        """One control step; returns (status, contact status, acceleration, evidence pairs, counts)."""
+        lib = load()
+        E = self.model.environment_count
+        env_end = E if env_end is None else env_end
+        command = np.ascontiguousarray(command, np.float32)
+        assert command.shape == (E, self.model.max_nv)
+        status = np.zeros(E, WORLD_STATUS_DTYPE)
+        cstatus = np.zeros(E, CONTACT_STATUS_DTYPE)
+        acc = np.zeros((E, self.model.max_nv), np.float32)
+        pairs = np.zeros((E, self.config.pair_capacity), PAIR_RECORD_DTYPE)
+        counts = np.zeros(E, np.uint32)
+        mask = None if reset_mask is None else np.ascontiguousarray(reset_mask, np.uint32)
+        rc = lib.mr_oracle_control_step(ct.byref(self.model.desc), ct.byref(self.config), env_begin, env_end,
+                                        control_step, _p(self.q), _p(self.v), _p(self.scene), _p(self.mh),
+                                        _p(self.mp), _p(self.mcount), _p(command), _p(mask), _p(self.reset_q),
+                                        _p(self.reset_v), _p(self.reset_scene), _p(status), _p(cstatus), _p(acc),
+                                        _p(pairs), _p(counts))
+        assert rc == 0
+        return status, cstatus, acc, pairs, counts
+
+
+def forward_dynamics(model, config, env, q, v, tau):
+    lib = load()
+    out = np.zeros(model.max_nv, np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    tau = np.ascontiguousarray(tau, np.float32)
+    code = lib.mr_oracle_forward_dynamics(ct.byref(model.desc), ct.byref(config), env, _p(q), _p(v), _p(tau),
+                                          _p(out))
+    return code, out
+
+
+def point_response(model, config, env, q, v, link, point, force):
+    lib = load()
+    out = np.zeros(model.max_nv, np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    point = np.ascontiguousarray(point, np.float32)
+    force = np.ascontiguousarray(force, np.float32)
+    code = lib.mr_oracle_point_response(ct.byref(model.desc), ct.byref(config), env, _p(q), _p(v), link, _p(point),
+                                        _p(force), _p(out))
+    return code, out
+
+
+def link_poses(model, config, env, q, v):
+    lib = load()
+    n = int(model.models()[model.environment_model()[env]]["link_count"])
+    out = np.zeros((n, 7), np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    code = lib.mr_oracle_link_poses(ct.byref(model.desc), ct.byref(config), env, _p(q), _p(v), _p(out))
+    return code, out
+
+
+def sincos(x):
+    lib = load()
+    x = np.ascontiguousarray(x, np.float32)
+    s = np.zeros_like(x)
+    c = np.zeros_like(x)
+    lib.mr_oracle_sincos(_p(x), _p(s), _p(c), x.size)
+    return s, c
