@@ -1,0 +1,190 @@
+"""Benchmark of the batched temporalCone control step on one MI355X.
+
+Workload (default): the floating 29-DoF humanoid on a 256x256 Perlin
+heightfield, 4096 worlds, fp32, 4 microsteps of 5 ms per 20 ms control step,
+implicit PD targets U[-0.3, 0.3] rad around a standing pose, elliptic cone
+(mu_u 1.0, mu_v 0.7), 4+2 cone sweeps. One "step" is one control step of all
+worlds (one launch of the fused kernel); the metric is env-steps/s.
+
+Prints one JSON line (see the keys below). Inputs are generated up front and
+resident in HBM before the timed region; the timed region is bracketed by
+device synchronisation on both sides.
+"""
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "numi-lab_amd"))
+
+HBM_PEAK_GBPS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "control_step_humanoid_4096_traffic.md")
+
+
+def bytes_per_env_step(nq, nv, s_dyn, s_static, m):
+    """Compulsory HBM bytes per env-step: the persistent per-environment
+    state (q, v, dynamic bodies, manifolds) read and written every microstep,
+    static bodies read, plus control input, reset mask, world status and
+    contact status (4 microsteps per control step)."""
+    return 4 * (2 * (4 * nq + 4 * nv + 128 * s_dyn + 320 * m) + 128 * s_static) + 4 * nv + 4 + 48 + 288
+
+
+def read_traffic_profile(path):
+    if not os.path.isfile(path):
+        return None
+    text = open(path).read()
+    m = re.search(r"hbm_bytes_per_launch\s*[=:]\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(kind, seed, envs, steps, threads):
+    """Times the oracle (FP32 CPU restatement of the same microstep) on a
+    bounded sample of the same workload: `envs` worlds over `steps` control
+    steps, one contiguous block of worlds per std::thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ctypes as ct
+
+    import metalrobo as mr
+    import oracle_lib as ol
+
+    model = mr.WorldModel.synthetic(kind, envs, seed)
+    cfg = model.default_config()
+    st = ol.OracleState(model, cfg)
+    cmds = model.commands(0, steps)
+    lib = ol.load()
+    E = envs
+    status = np.zeros(E, mr.WORLD_STATUS_DTYPE)
+    cstatus = np.zeros(E, mr.CONTACT_STATUS_DTYPE)
+    acc = np.zeros((E, model.max_nv), np.float32)
+    p = lambda a: a.ctypes.data  # noqa: E731
+    t0 = time.perf_counter()
+    for k in range(steps):
+        rc = lib.mr_oracle_control_step_parallel(ct.byref(model.desc), ct.byref(cfg), E, k, threads, p(st.q),
+                                                 p(st.v), p(st.scene), p(st.mh), p(st.mp), p(st.mcount),
+                                                 p(np.ascontiguousarray(cmds[k])), p(status), p(cstatus), p(acc))
+        if rc != 0:
+            raise RuntimeError("oracle failed")
+    dt = time.perf_counter() - t0
+    return {"value": round(E * steps / dt, 1), "unit": "env-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{E} worlds x {steps} control steps of the same workload (oracle, FP32, "
+                      f"{threads} std::threads)", "seconds": round(dt, 2)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--envs", type=int, default=4096)
+    ap.add_argument("--config", choices=["humanoid", "chain"], default="humanoid")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-envs", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=500)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--skip-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world_size = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world_size != 1:
+        print("bench.py measures one GPU: every GPU box here has one GPU and the worlds shard across GPUs with "
+              "no exchange, so --gpus must be 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import metalrobo as mr
+
+    if not torch.cuda.is_available():
+        print("no GPU available", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.init()
+    kind = mr.MR_SYNTHETIC_HUMANOID_TERRAIN if args.config == "humanoid" else mr.MR_SYNTHETIC_CHAIN_BOXES
+    model = mr.WorldModel.synthetic(kind, args.envs, args.seed)
+    cfg = model.default_config()
+    world = mr.World(model, cfg)
+    world.set_step_timing(True)
+    total = args.warmup + args.steps
+    commands = torch.from_numpy(model.commands(0, total)).to("cuda:0")
+    torch.cuda.synchronize()
+
+    if args.warmup > 0:
+        world.enqueue_device(commands[:args.warmup])
+        world.wait()
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    world.enqueue_device(commands[args.warmup:])
+    adv = world.wait()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = args.envs * args.steps / (t1 - t0)
+
+    step_ms = world.step_milliseconds()
+    cstat = world.contact_statuses()
+    stat = world.statuses()
+    mean_manifolds = float(cstat["manifolds"].mean())
+    h = model.models()[0]
+    nq, nv = int(model.max_nq), int(model.max_nv)
+    bodies = model.bodies()
+    scene = bodies[h["link_count"]:h["body_count"]]
+    s_dyn = int(np.sum(scene["motion_type"] == 2))
+    s_static = int(len(scene) - s_dyn)
+    b_env = bytes_per_env_step(nq, nv, s_dyn, s_static, mean_manifolds)
+    avg_launch_s = float(step_ms.mean()) * 1e-3
+    achieved = args.envs * b_env / avg_launch_s / 1e9
+    traffic = read_traffic_profile(TRAFFIC_PROFILE)
+
+    out = {
+        "metric": "env-steps/s (29-DoF, 4096 worlds) vs plain; p99 step latency at residual 1e-5"
+        if args.config == "humanoid" else "env-steps/s (7-DoF chain + 20 boxes)",
+        "value": round(value, 1),
+        "unit": "env-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded worlds and commands; no datasets)",
+        "config": {
+            "workload": ("humanoid29_heightfield256_pd_elliptic" if args.config == "humanoid"
+                         else "chain7_boxes20_plane_effort"),
+            "worlds": args.envs, "control_steps": args.steps, "microsteps_per_step": int(cfg.physics_substeps),
+            "control_dt_s": float(cfg.control_timestep),
+            "cone_sweeps": f"{cfg.velocity_iterations}+{cfg.final_velocity_iterations}",
+            "eligible_pairs_per_world": int(h["pair_count"]), "plain_path": True,
+        },
+        "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
+                       "p99": round(float(np.percentile(step_ms, 99)), 4),
+                       "max": round(float(step_ms.max()), 4)},
+        "residual": {"final_sweep_max_dlambda_p99": float(np.percentile(cstat["residuals"][..., 0], 99)),
+                     "normal_p99": float(np.percentile(cstat["residuals"][..., 1], 99)),
+                     "cone_violation_max": float(cstat["residuals"][..., 2].max())},
+        "contacts": {"mean_manifolds": round(mean_manifolds, 3),
+                     "mean_constraints": round(float(cstat["constraints"].mean()), 3),
+                     "max_constraints": int(cstat["max_constraints"].max()),
+                     "failed_env_steps": int(np.sum(stat["code"] != 0))},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 6), "traffic": traffic,
+                     "algorithmic_bytes_per_env_step": round(b_env, 1),
+                     "kernel_ms_per_launch": round(avg_launch_s * 1e3, 4)},
+        "cpu_baseline": None,
+    }
+    if not args.skip_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(kind, args.seed, args.cpu_envs, args.cpu_steps, threads)
+    print(json.dumps(out), flush=True)
+    del adv
+
+
+if __name__ == "__main__":
+    main()

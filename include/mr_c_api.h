@@ -138,6 +138,9 @@ int mr_world_read_evidence(const MRWorldHandle* world, MRPairRecordC* pairs, siz
 void* mr_world_stream(const MRWorldHandle* world);
 /* Average duration of the last advance's control-step launches (ms) measured with HIP events. */
 double mr_world_last_kernel_milliseconds(const MRWorldHandle* world);
+/* Per-launch HIP-event timing (off by default); durations of the last advance, [steps] in ms. */
+int mr_world_set_step_timing(MRWorldHandle* world, uint32_t enabled);
+int mr_world_read_step_milliseconds(const MRWorldHandle* world, double* out, size_t count);
 
 /* ------------------------------------------------- task rollout --- */
 

@@ -717,6 +717,37 @@ ModelData copy_model(const MRWorldModelDescC& d) {
     m.max_nq = d.max_nq;
     m.max_nv = d.max_nv;
     m.max_scene_bodies = d.max_scene_bodies;
+    // Default initial state: zero coordinates (identity root orientation),
+    // zero velocities, scene bodies at the origin with their authored mass.
+    const uint32_t E = d.environment_count;
+    m.initial_q.assign((size_t)E * m.max_nq, 0.0f);
+    m.initial_v.assign((size_t)E * m.max_nv, 0.0f);
+    m.initial_scene.assign((size_t)E * m.max_scene_bodies, MRBodyStateC{});
+    for (uint32_t e = 0; e < E; ++e) {
+        const MRModelHeaderC& h = m.models[m.environment_model[e]];
+        if (h.root_type == MR_ROOT_FLOATING && m.max_nq >= 7u) m.initial_q[(size_t)e * m.max_nq + 6u] = 1.0f;
+        for (uint32_t s = 0; s < h.scene_body_count; ++s) {
+            const MRBodyC& b = m.bodies[h.body_offset + h.link_count + s];
+            MRBodyStateC& st = m.initial_scene[(size_t)e * m.max_scene_bodies + s];
+            st = scene_state(D3{0, 0, 0}, kIdentity, b, h.link_count + s);
+        }
+    }
+    m.default_config.control_timestep = 0.02f;
+    m.default_config.physics_substeps = 4u;
+    m.default_config.velocity_iterations = 4u;
+    m.default_config.final_velocity_iterations = 2u;
+    m.default_config.warm_start = 1u;
+    m.default_config.apply_body_damping = 1u;
+    m.default_config.pair_capacity = 64u;
+    m.default_config.manifold_capacity = 32u;
+    m.default_config.constraint_capacity = 64u;
+    m.default_config.manifold_breaking_separation = 0.02f;
+    m.default_config.manifold_breaking_tangential = 0.02f;
+    m.default_config.manifold_merge_distance = 0.002f;
+    m.default_config.manifold_normal_cosine = 0.95f;
+    m.default_config.contact_slop = 0.001f;
+    m.default_config.max_depenetration_velocity = 1.0f;
+    m.default_config.max_iterations = 64u;
     return m;
 }
 

@@ -155,7 +155,8 @@ int mr_world_model_initial_state(const MRWorldModelHandle* model, float* q, size
     return translate([&]() {
         require(model != nullptr, "model is null");
         const mr::ModelData& m = model->model;
-        require(!m.initial_q.empty(), "model has no initial state");
+        require(m.initial_scene.size() == (size_t)m.environment_count() * m.max_scene_bodies,
+                "model has no initial state");
         require_count(q_count, m.initial_q.size(), "q");
         require_count(v_count, m.initial_v.size(), "v");
         require_count(scene_count, m.initial_scene.size(), "scene");
@@ -346,6 +347,21 @@ void* mr_world_stream(const MRWorldHandle* world) {
 double mr_world_last_kernel_milliseconds(const MRWorldHandle* world) {
     if (world == nullptr || world->world->last_steps() == 0u) return 0.0;
     return (double)world->world->last_kernel_milliseconds() / (double)world->world->last_steps();
+}
+
+int mr_world_set_step_timing(MRWorldHandle* world, uint32_t enabled) {
+    return translate([&]() {
+        require(world != nullptr, "null argument");
+        world->world->set_step_timing(enabled != 0u);
+    });
+}
+
+int mr_world_read_step_milliseconds(const MRWorldHandle* world, double* out, size_t count) {
+    return translate([&]() {
+        require(world != nullptr && out != nullptr, "null argument");
+        require_count(count, world->world->last_steps(), "step durations");
+        world->world->read_step_milliseconds(out, world->world->last_steps());
+    });
 }
 
 // ------------------------------------------------------------ task rollout

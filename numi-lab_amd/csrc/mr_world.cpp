@@ -110,6 +110,20 @@ World::~World() {
     }
     if (ev_begin_ != nullptr) (void)hipEventDestroy(ev_begin_);
     if (ev_end_ != nullptr) (void)hipEventDestroy(ev_end_);
+    for (hipEvent_t ev : step_events_) (void)hipEventDestroy(ev);
+}
+
+void World::set_step_timing(bool enabled) { step_timing_ = enabled; }
+
+void World::read_step_milliseconds(double* out, uint32_t steps) const {
+    if (!step_timing_ || steps > last_steps_ || step_events_.size() < (size_t)steps + 1u)
+        throw std::runtime_error("step timing was not enabled for the last advance");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    for (uint32_t s = 0; s < steps; ++s) {
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, step_events_[s], step_events_[s + 1u]), "event elapsed");
+        out[s] = ms;
+    }
 }
 
 void World::compute_lds_layout() {
@@ -315,7 +329,15 @@ void World::enqueue(const float* commands, const uint32_t* masks, uint32_t steps
     }
     d_status_.reserve((size_t)steps * E * sizeof(MRWorldStatusC));
     d_cstatus_.reserve((size_t)steps * E * sizeof(MRContactStatusC));
+    if (step_timing_) {
+        while (step_events_.size() < (size_t)steps + 1u) {
+            hipEvent_t ev;
+            hip_check(hipEventCreate(&ev), "hipEventCreate");
+            step_events_.push_back(ev);
+        }
+    }
     hip_check(hipEventRecord(ev_begin_, stream_), "event");
+    if (step_timing_) hip_check(hipEventRecord(step_events_[0], stream_), "event");
     for (uint32_t s = 0; s < steps; ++s) {
         KernelArgs a = base_args(s);
         a.command = cmd + (size_t)s * per_step;
@@ -323,6 +345,7 @@ void World::enqueue(const float* commands, const uint32_t* masks, uint32_t steps
         a.status = static_cast<MRWorldStatusC*>(d_status_.ptr) + (size_t)s * E;
         a.cstatus = static_cast<MRContactStatusC*>(d_cstatus_.ptr) + (size_t)s * E;
         hip_check(launch_control_step(a, lds_.total, stream_), "control step launch");
+        if (step_timing_) hip_check(hipEventRecord(step_events_[s + 1u], stream_), "event");
     }
     hip_check(hipEventRecord(ev_end_, stream_), "event");
     last_steps_ = steps;

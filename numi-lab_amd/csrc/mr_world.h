@@ -67,6 +67,9 @@ public:
     void read_evidence(MRPairRecordC* pairs, uint32_t* counts) const; // [E][pair_capacity], [E]
     uint32_t last_steps() const { return last_steps_; }
     float last_kernel_milliseconds() const { return last_kernel_ms_; }
+    // Per-launch HIP-event timing of the control-step kernels (off by default).
+    void set_step_timing(bool enabled);
+    void read_step_milliseconds(double* out, uint32_t steps) const;
 
 private:
     KernelArgs base_args(uint32_t step) const;
@@ -89,6 +92,8 @@ private:
     DeviceBuffer d_stage_h_, d_stage_p_, d_jrow_, d_resp_, d_cold_;
     uint32_t last_steps_ = 0;
     float last_kernel_ms_ = 0.0f;
+    bool step_timing_ = false;
+    std::vector<hipEvent_t> step_events_;
 };
 
 void hip_check(hipError_t e, const char* what);

@@ -2363,6 +2363,58 @@ int mr_oracle_link_poses(const MRWorldModelDescC* desc, const MRWorldStepConfigC
     return code;
 }
 
+// Broadphase flags and narrowphase witnesses of every compiled pair of one
+// environment for state (q, v, scene): overlap[npairs], counts[npairs],
+// reals[npairs][8][10] = (normal xyz, separation, point A xyz, point B xyz),
+// features[npairs][8][2]. Returns the number of compiled pairs, or -1.
+int mr_oracle_collide(const MRWorldModelDescC* desc, const MRWorldStepConfigC* cfg, uint32_t env, const float* q,
+                      const float* v, const MRBodyStateC* scene, uint32_t* overlap, uint32_t* counts, float* reals,
+                      uint32_t* features) {
+    StepContext ctx;
+    ctx.desc = desc;
+    ctx.cfg = cfg;
+    ctx.m = env_model(desc, env);
+    ctx.env = env;
+    ctx.h = cfg->control_timestep / (float)cfg->physics_substeps;
+    const MRModelHeaderC& hd = *ctx.m.header;
+    Work* wk = new Work();
+    Failure f;
+    int rc = (int)hd.pair_count;
+    if (!forward_kinematics(ctx, q, v, *wk, f)) {
+        rc = -1;
+    } else {
+        for (uint32_t i = 0; i < hd.link_count; ++i) {
+            MRBodyStateC& b = wk->bodies[i];
+            std::memset(&b, 0, sizeof(b));
+            b.position = mr_float4{wk->pos[i].x, wk->pos[i].y, wk->pos[i].z, 1.0f};
+            b.orientation = mr_float4{wk->rot[i].x, wk->rot[i].y, wk->rot[i].z, wk->rot[i].w};
+        }
+        for (uint32_t b = hd.link_count; b < hd.body_count; ++b) wk->bodies[b] = scene[b - hd.link_count];
+        for (uint32_t s = 0; s < hd.shape_count && rc >= 0; ++s)
+            if (!project_collider(ctx, *wk, s, wk->col[s])) rc = -1;
+        for (uint32_t p = 0; p < hd.pair_count && rc >= 0; ++p) {
+            const MRCompiledPairC& pr = ctx.m.pairs[p];
+            overlap[p] = pair_may_overlap(wk->col[pr.collider_a], wk->col[pr.collider_b]) ? 1u : 0u;
+            RawBatch raw;
+            raw.count = 0;
+            generate_contacts(ctx, pr.collider_a, pr.collider_b, pr.pair_class, wk->col[pr.collider_a],
+                              wk->col[pr.collider_b], raw);
+            counts[p] = raw.count;
+            for (uint32_t c = 0; c < raw.count; ++c) {
+                float* o = reals + ((size_t)p * MR_RAW_CONTACTS_PER_PAIR + c) * 10u;
+                const RawContact& r = raw.c[c];
+                o[0] = r.normal.x; o[1] = r.normal.y; o[2] = r.normal.z; o[3] = r.separation;
+                o[4] = r.point_a.x; o[5] = r.point_a.y; o[6] = r.point_a.z;
+                o[7] = r.point_b.x; o[8] = r.point_b.y; o[9] = r.point_b.z;
+                features[((size_t)p * MR_RAW_CONTACTS_PER_PAIR + c) * 2u] = r.feature_a;
+                features[((size_t)p * MR_RAW_CONTACTS_PER_PAIR + c) * 2u + 1u] = r.feature_b;
+            }
+        }
+    }
+    delete wk;
+    return rc;
+}
+
 void mr_oracle_sincos(const float* x, float* s, float* c, uint32_t n) {
     for (uint32_t i = 0; i < n; ++i) det_sincos(x[i], s[i], c[i]);
 }

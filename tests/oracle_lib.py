@@ -52,6 +52,9 @@ def load():
         lib.mr_oracle_link_poses.argtypes = [ct.POINTER(WorldModelDescC), ct.POINTER(WorldStepConfigC),
                                              ct.c_uint32, P, P, P]
         lib.mr_oracle_link_poses.restype = ct.c_int
+        lib.mr_oracle_collide.argtypes = [ct.POINTER(WorldModelDescC), ct.POINTER(WorldStepConfigC), ct.c_uint32,
+                                          P, P, P, P, P, P, P]
+        lib.mr_oracle_collide.restype = ct.c_int
         lib.mr_oracle_sincos.argtypes = [P, P, P, ct.c_uint32]
         lib.mr_oracle_sincos.restype = None
         lib.mr_oracle_abi_version.argtypes = []
@@ -131,6 +134,25 @@ def link_poses(model, config, env, q, v):
     v = np.ascontiguousarray(v, np.float32)
     code = lib.mr_oracle_link_poses(ct.byref(model.desc), ct.byref(config), env, _p(q), _p(v), _p(out))
     return code, out
+
+
+def collide(model, config, env, q, v, scene):
+    """Per compiled pair: overlap flag, witness count, reals [8][10], features [8][2]."""
+    lib = load()
+    h = model.models()[model.environment_model()[env]]
+    n = max(1, int(h["pair_count"]))
+    overlap = np.zeros(n, np.uint32)
+    counts = np.zeros(n, np.uint32)
+    reals = np.zeros((n, 8, 10), np.float32)
+    feats = np.zeros((n, 8, 2), np.uint32)
+    q = np.ascontiguousarray(q, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    scene = np.ascontiguousarray(scene, BODY_STATE_DTYPE)
+    rc = lib.mr_oracle_collide(ct.byref(model.desc), ct.byref(config), env, _p(q), _p(v), _p(scene), _p(overlap),
+                               _p(counts), _p(reals), _p(feats))
+    assert rc >= 0
+    k = int(h["pair_count"])
+    return overlap[:k], counts[:k], reals[:k], feats[:k]
 
 
 def sincos(x):
