@@ -141,6 +141,12 @@ double mr_world_last_kernel_milliseconds(const MRWorldHandle* world);
 /* Per-launch HIP-event timing (off by default); durations of the last advance, [steps] in ms. */
 int mr_world_set_step_timing(MRWorldHandle* world, uint32_t enabled);
 int mr_world_read_step_milliseconds(const MRWorldHandle* world, double* out, size_t count);
+/* Per-stage shader-clock cycle counters of the fused kernel (diagnostic, off by
+ * default): [E][16] summed over the last advance (stage order: kinematics, ABA,
+ * projection, broadphase, narrowphase, emission, responses, evaluation,
+ * warm start, sweeps, residuals, integration). */
+int mr_world_set_stage_profiling(MRWorldHandle* world, uint32_t enabled);
+int mr_world_read_stage_cycles(const MRWorldHandle* world, uint64_t* out, size_t count);
 
 /* ------------------------------------------------- task rollout --- */
 

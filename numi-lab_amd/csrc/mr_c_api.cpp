@@ -364,6 +364,21 @@ int mr_world_read_step_milliseconds(const MRWorldHandle* world, double* out, siz
     });
 }
 
+int mr_world_set_stage_profiling(MRWorldHandle* world, uint32_t enabled) {
+    return translate([&]() {
+        require(world != nullptr, "null argument");
+        world->world->set_stage_profiling(enabled != 0u);
+    });
+}
+
+int mr_world_read_stage_cycles(const MRWorldHandle* world, uint64_t* out, size_t count) {
+    return translate([&]() {
+        require(world != nullptr && out != nullptr, "null argument");
+        require_count(count, (size_t)world->world->environment_count() * mr::kStageCount, "stage cycles");
+        world->world->read_stage_cycles(reinterpret_cast<unsigned long long*>(out));
+    });
+}
+
 // ------------------------------------------------------------ task rollout
 
 MRTaskRolloutHandle* mr_create_task_rollout(const MRRunManifestC* manifest) {

@@ -8,6 +8,23 @@
 
 namespace mr {
 
+// Stage indices of the optional per-stage cycle counters.
+enum StageIndex : uint32_t {
+    kStageKinematics = 0,
+    kStageABA,
+    kStageProjection,
+    kStageBroadphase,
+    kStageNarrowphase,
+    kStageEmission,
+    kStageResponses,
+    kStageEvaluation,
+    kStageWarmStart,
+    kStageSweeps,
+    kStageResiduals,
+    kStageIntegration,
+    kStageCount = 16
+};
+
 // Per-constraint data kept resident in LDS during the cone solve.
 struct HotConstraint {
     uint32_t body_a, body_b;
@@ -86,6 +103,7 @@ struct KernelArgs {
     float* jrow;                  // [E][constraint_capacity][3][max_nv]
     float* resp;                  // [E][constraint_capacity][3][max_nv]
     ColdConstraint* cold;         // [E][constraint_capacity]
+    unsigned long long* stage_cycles; // [E][kStageCount] shader-clock cycles per stage, or null
     LdsLayout lds;
 };
 

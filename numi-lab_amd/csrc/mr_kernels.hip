@@ -1500,6 +1500,27 @@ MRD_INL float relative_row(const Env& e, const HotConstraint& k, uint32_t c, int
     return (a + dot(d, vb)) - dot(d, va);
 }
 
+struct StageClock {
+    unsigned long long* out;
+    unsigned long long last;
+    unsigned long long acc[mr::kStageCount];
+    MRD_INL void init(unsigned long long* o) {
+        out = o;
+        for (uint32_t i = 0; i < mr::kStageCount; ++i) acc[i] = 0ull;
+        last = out != nullptr ? (unsigned long long)clock64() : 0ull;
+    }
+    MRD_INL void mark(uint32_t stage) {
+        if (out == nullptr) return;
+        const unsigned long long now = (unsigned long long)clock64();
+        acc[stage] += now - last;
+        last = now;
+    }
+    MRD_INL void flush(uint32_t lane) {
+        if (out == nullptr || lane != 0u) return;
+        for (uint32_t i = 0; i < mr::kStageCount; ++i) out[i] += acc[i];
+    }
+};
+
 } // namespace
 
 // --------------------------------------------------------------- kernel
@@ -1607,6 +1628,8 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
     }
     __syncthreads();
 
+    StageClock clk;
+    clk.init(A.stage_cycles != nullptr ? A.stage_cycles + (size_t)env * mr::kStageCount : nullptr);
     uint32_t src_buf = cur;
     uint32_t dst_buf = (cur + 1u) % 3u;
     const uint32_t alt_buf = (cur + 2u) % 3u;
@@ -1665,6 +1688,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                     report_failure(&s_fail, 1u + i, MR_STEP_NONFINITE_RESULT, i);
             __syncthreads();
         }
+        clk.mark(mr::kStageKinematics);
         if (s_fail != ~0ull) break;
         // -------------------------------------------------------------- ABA
         for (uint32_t i = lane; i < nl; i += kLanes) aba_init_link(e, i);
@@ -1754,6 +1778,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
             e.vcand[j] = nvv;
         }
         __syncthreads();
+        clk.mark(mr::kStageABA);
         if (s_fail != ~0ull) break;
         // ------------------------------------------- scene prediction
         for (uint32_t s = lane; s < nsc; s += kLanes) {
@@ -1802,6 +1827,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
             e.col[s] = c;
         }
         __syncthreads();
+        clk.mark(mr::kStageProjection);
         if (s_fail != ~0ull) break;
         // ----------------------------------------------- broadphase flags
         uint32_t nact = 0;
@@ -1860,6 +1886,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
             __syncthreads();
             break;
         }
+        clk.mark(mr::kStageBroadphase);
         // ---------------------------------- narrowphase + manifold refresh
         const MRManifoldHeaderC* old_h = A.mh + ((size_t)src_buf * E + env) * mcap;
         const MRManifoldPointC* old_p = A.mp + ((size_t)src_buf * E + env) * mcap * MR_MANIFOLD_POINT_CAPACITY;
@@ -1910,6 +1937,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         const uint32_t fresh_total = wave_sum_u32(my_fresh);
         const float pen = wave_max(my_pen);
         __syncthreads();
+        clk.mark(mr::kStageNarrowphase);
         if (s_fail != ~0ull) break;
         // prefix over manifolds (count>0) and constraint offsets
         uint32_t nman = 0, ncons = 0;
@@ -2015,6 +2043,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
             }
         }
         __syncthreads();
+        clk.mark(mr::kStageEmission);
         // --------------------------- Jacobian rows + inverse-ABA responses
         for (uint32_t item = lane; item < 3u * ncons; item += kLanes) {
             const uint32_t c = item / 3u;
@@ -2036,6 +2065,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
             inverse_aba(e, la ? hk.body_a : MR_INVALID_INDEX, pa, neg(d), lb ? hk.body_b : MR_INVALID_INDEX, pb, d, rs);
         }
         __syncthreads();
+        clk.mark(mr::kStageResponses);
         // ------------------------------------------ IR evaluation + blocks
         const float slop = cfg.contact_slop;
         const float maxdep = cfg.max_depenetration_velocity;
@@ -2093,6 +2123,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
             e.hot[c] = hk;
         }
         __syncthreads();
+        clk.mark(mr::kStageEvaluation);
         if (s_fail != ~0ull) break;
         // ---------------------------------------------------- warm start
         for (uint32_t j = lane; j < nv; j += kLanes) {
@@ -2121,6 +2152,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
             e.scene[s] = st;
         }
         __syncthreads();
+        clk.mark(mr::kStageWarmStart);
         // ------------------------------------------ ordered block sweeps
         const uint32_t base_iters = cfg.velocity_iterations + (final_sub ? cfg.final_velocity_iterations : 0u);
         const bool adaptive = cfg.residual_tolerance > 0.0f;
@@ -2173,6 +2205,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
             if (adaptive && sweep_delta <= cfg.residual_tolerance) break;
         }
         __syncthreads();
+        clk.mark(mr::kStageSweeps);
         if (s_fail != ~0ull) break;
         // ------------------------------------- residuals + impulse store
         float my_normal = 0.0f, my_cone = 0.0f;
@@ -2249,6 +2282,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         }
         __syncthreads();
         cs.islands = s_count[0];
+        clk.mark(mr::kStageResiduals);
         if (s_fail != ~0ull) break;
         // ---------------------------------------------------- integration
         // New coordinates are staged in registers / s_root and written after
@@ -2327,7 +2361,9 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         for (uint32_t j = lane; j < nv; j += kLanes) my_acc = fmaxd(my_acc, fabs_(e.accel[j]));
         max_acc = fmaxd(max_acc, wave_max(my_acc));
         __syncthreads();
+        clk.mark(mr::kStageIntegration);
     }
+    clk.flush(lane);
 
     // ------------------------------------------- transactional commit
     __syncthreads();

@@ -115,6 +115,19 @@ World::~World() {
 
 void World::set_step_timing(bool enabled) { step_timing_ = enabled; }
 
+void World::set_stage_profiling(bool enabled) {
+    stage_profiling_ = enabled;
+    if (enabled) d_stage_cycles_.reserve((size_t)env_ * kStageCount * sizeof(unsigned long long));
+}
+
+void World::read_stage_cycles(unsigned long long* out) const {
+    if (!stage_profiling_) throw std::runtime_error("stage profiling is not enabled");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    hip_check(hipMemcpy(out, d_stage_cycles_.ptr, (size_t)env_ * kStageCount * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost),
+              "read stage cycles");
+}
+
 void World::read_step_milliseconds(double* out, uint32_t steps) const {
     if (!step_timing_ || steps > last_steps_ || step_events_.size() < (size_t)steps + 1u)
         throw std::runtime_error("step timing was not enabled for the last advance");
@@ -302,6 +315,7 @@ KernelArgs World::base_args(uint32_t step) const {
     a.jrow = static_cast<float*>(d_jrow_.ptr);
     a.resp = static_cast<float*>(d_resp_.ptr);
     a.cold = static_cast<ColdConstraint*>(d_cold_.ptr);
+    a.stage_cycles = stage_profiling_ ? static_cast<unsigned long long*>(d_stage_cycles_.ptr) : nullptr;
     a.lds = lds_;
     return a;
 }
@@ -327,6 +341,9 @@ void World::enqueue(const float* commands, const uint32_t* masks, uint32_t steps
             msk = static_cast<const uint32_t*>(d_masks_.ptr);
         }
     }
+    if (stage_profiling_)
+        hip_check(hipMemsetAsync(d_stage_cycles_.ptr, 0, (size_t)E * kStageCount * sizeof(unsigned long long), stream_),
+                  "clear stage cycles");
     d_status_.reserve((size_t)steps * E * sizeof(MRWorldStatusC));
     d_cstatus_.reserve((size_t)steps * E * sizeof(MRContactStatusC));
     if (step_timing_) {

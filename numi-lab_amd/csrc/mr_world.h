@@ -69,6 +69,9 @@ public:
     float last_kernel_milliseconds() const { return last_kernel_ms_; }
     // Per-launch HIP-event timing of the control-step kernels (off by default).
     void set_step_timing(bool enabled);
+    // Optional per-stage shader-clock counters, summed over an advance: [E][kStageCount].
+    void set_stage_profiling(bool enabled);
+    void read_stage_cycles(unsigned long long* out) const;
     void read_step_milliseconds(double* out, uint32_t steps) const;
 
 private:
@@ -89,7 +92,8 @@ private:
     // per-advance inputs / outputs
     DeviceBuffer d_commands_, d_masks_, d_status_, d_cstatus_, d_accel_, d_ev_pairs_, d_ev_counts_;
     // scratch
-    DeviceBuffer d_stage_h_, d_stage_p_, d_jrow_, d_resp_, d_cold_;
+    DeviceBuffer d_stage_h_, d_stage_p_, d_jrow_, d_resp_, d_cold_, d_stage_cycles_;
+    bool stage_profiling_ = false;
     uint32_t last_steps_ = 0;
     float last_kernel_ms_ = 0.0f;
     bool step_timing_ = false;

@@ -266,6 +266,8 @@ _SIGNATURES: Final = {
     "mr_world_last_kernel_milliseconds": ([ct.c_void_p], ct.c_double),
     "mr_world_set_step_timing": ([ct.c_void_p, ct.c_uint32], ct.c_int),
     "mr_world_read_step_milliseconds": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
+    "mr_world_set_stage_profiling": ([ct.c_void_p, ct.c_uint32], ct.c_int),
+    "mr_world_read_stage_cycles": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
     "mr_create_task_rollout": ([ct.POINTER(RunManifestC)], ct.c_void_p),
     "mr_task_rollout_destroy": ([ct.c_void_p], None),
     "mr_task_rollout_reset": ([ct.c_void_p, ct.c_uint64], ct.c_int),
@@ -554,6 +556,17 @@ class World:
 
     def set_step_timing(self, enabled: bool) -> None:
         self._b.check(self._b.lib.mr_world_set_step_timing(self._h, 1 if enabled else 0))
+
+    STAGE_NAMES = ("kinematics", "aba", "projection", "broadphase", "narrowphase", "emission", "responses",
+                   "evaluation", "warm_start", "sweeps", "residuals", "integration")
+
+    def set_stage_profiling(self, enabled: bool) -> None:
+        self._b.check(self._b.lib.mr_world_set_stage_profiling(self._h, 1 if enabled else 0))
+
+    def stage_cycles(self) -> np.ndarray:
+        out = np.zeros((self.environment_count, 16), np.uint64)
+        self._b.check(self._b.lib.mr_world_read_stage_cycles(self._h, _ptr(out), out.size))
+        return out
 
     def step_milliseconds(self) -> np.ndarray:
         out = np.zeros(self.last_steps, np.float64)
