@@ -50,8 +50,22 @@ struct ColdConstraint {
     float pad;
 };
 
+// Tree topology of one environment's model, cached in LDS once per launch so
+// that tree walks never chase parent indices through global memory. Link and
+// DoF indices fit in a byte (0xff = none); masks are bit sets over links.
+struct TopoCache {
+    unsigned long long anc[MR_MAX_BODIES];      // links on the path link -> root, root excluded
+    unsigned long long children[MR_MAX_BODIES]; // direct child links
+    uint8_t parent[MR_MAX_BODIES];
+    uint8_t vidx[MR_MAX_BODIES];
+    uint8_t depth[MR_MAX_BODIES];
+    uint8_t motion[MR_MAX_BODIES];              // motion type of every body
+    uint8_t dof_body[MR_MAX_BODIES];            // owning link of every DoF
+};
+
 // Byte offsets of the per-workgroup dynamic LDS arrays.
 struct LdsLayout {
+    uint32_t topo;
     uint32_t q, v, vcand, effort, accel, cmd;
     uint32_t pos, rot, w, vl, sa, sl, ca, cl, r;
     uint32_t I, pt, pb, Ut, Ub, D, u, alpha, acc, L;

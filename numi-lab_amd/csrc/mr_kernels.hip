@@ -176,6 +176,7 @@ struct Env {
     float h;
     V3 gravity;
     // LDS
+    mr::TopoCache* topo;
     float *q, *v, *vcand, *effort, *accel, *cmd;
     V3 *pos, *w, *vl, *sa, *sl, *ca, *cl, *r;
     Q4* rot;
@@ -200,8 +201,16 @@ MRD_INL T* lds_at(unsigned char* base, uint32_t off) {
 }
 
 MRD_INL bool is_link(const Env& e, uint32_t b) { return b < e.nlinks; }
+MRD_INL uint32_t link_parent(const Env& e, uint32_t i) {
+    const uint32_t p = e.topo->parent[i];
+    return p == 0xffu ? MR_INVALID_INDEX : p;
+}
+MRD_INL uint32_t link_vidx(const Env& e, uint32_t i) {
+    const uint32_t v = e.topo->vidx[i];
+    return v == 0xffu ? MR_INVALID_INDEX : v;
+}
 MRD_INL bool is_dynamic_scene(const Env& e, uint32_t b) {
-    return b >= e.nlinks && e.bodies[b].motion_type == MR_MOTION_DYNAMIC;
+    return b >= e.nlinks && e.topo->motion[b] == MR_MOTION_DYNAMIC;
 }
 
 MRD_INL void body_pose(const Env& e, uint32_t b, V3& p, Q4& q) {
@@ -233,6 +242,15 @@ MRD_INL uint32_t wave_prefix_sum(uint32_t value, uint32_t lane, uint32_t& total)
     }
     total = __shfl(x, 63, 64);
     return x - value;
+}
+
+// Canonical 64-lane pairwise sum: x[i] + x[i ^ d] for d = 32 ... 1. IEEE
+// addition commutes, so every lane ends with the same bits, equal to the
+// fold x[i] += x[i + d] over lanes 0 .. d-1.
+MRD_INL float wave_fold(float x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x = x + __shfl_xor(x, d, 64);
+    return x;
 }
 
 MRD_INL float wave_max(float v) {
@@ -451,8 +469,10 @@ MRD_INL bool aba_project_link(const Env& e, uint32_t i, float& pivot_out) {
 MRD_INL void aba_accumulate(const Env& e, uint32_t p) {
     SpI P = e.I[p];
     V3 pt = e.pt[p], pb = e.pb[p];
-    for (uint32_t c = e.nlinks - 1u; c > p; --c) {
-        if (e.bodies[c].parent != p) continue;
+    unsigned long long kids = e.topo->children[p];
+    while (kids != 0ull) {
+        const uint32_t c = 63u - (uint32_t)__builtin_clzll(kids);  // decreasing index
+        kids &= ~(1ull << c);
         const SpI& S = e.I[c];
 #pragma unroll
         for (int k = 0; k < 6; ++k) P.A[k] += S.A[k];
@@ -527,17 +547,17 @@ MRD_INL void inverse_aba(const Env& e, uint32_t la, V3 xa, V3 fa, uint32_t lb, V
         pb[lb] = sub(pb[lb], fb);
     }
     for (uint32_t i = n - 1u; i >= 1u; --i) {
-        const MRBodyC& b = e.bodies[i];
+        const uint32_t parent = link_parent(e, i);
         V3 t = pt[i], f = pb[i];
-        if (b.v_index != MR_INVALID_INDEX) {
+        if (link_vidx(e, i) != MR_INVALID_INDEX) {
             const float u = (0.0f - dot(e.sa[i], t)) - dot(e.sl[i], f);
             const float uD = u / e.D[i];
             t = add(t, mul(uD, e.Ut[i]));
             f = add(f, mul(uD, e.Ub[i]));
             uu[i] = u;
         }
-        pt[b.parent] = add(pt[b.parent], add(t, cross(e.r[i], f)));
-        pb[b.parent] = add(pb[b.parent], f);
+        pt[parent] = add(pt[parent], add(t, cross(e.r[i], f)));
+        pb[parent] = add(pb[parent], f);
     }
     // outward: reuse pt as angular, pb as linear acceleration storage
     if (e.root_type == MR_ROOT_FLOATING) {
@@ -553,13 +573,13 @@ MRD_INL void inverse_aba(const Env& e, uint32_t la, V3 xa, V3 fa, uint32_t lb, V
         pb[0] = V3{0.0f, 0.0f, 0.0f};
     }
     for (uint32_t i = 1; i < n; ++i) {
-        const MRBodyC& b = e.bodies[i];
-        const V3 ap = pt[b.parent];
-        const V3 lp = add(pb[b.parent], cross(ap, e.r[i]));
-        if (b.v_index != MR_INVALID_INDEX) {
+        const uint32_t parent = link_parent(e, i), vi = link_vidx(e, i);
+        const V3 ap = pt[parent];
+        const V3 lp = add(pb[parent], cross(ap, e.r[i]));
+        if (vi != MR_INVALID_INDEX) {
             const float proj = dot(e.Ut[i], ap) + dot(e.Ub[i], lp);
             const float qdd = (uu[i] - proj) / e.D[i];
-            out[b.v_index] = qdd;
+            out[vi] = qdd;
             pt[i] = add(ap, mul(qdd, e.sa[i]));
             pb[i] = add(lp, mul(qdd, e.sl[i]));
         } else {
@@ -580,9 +600,9 @@ MRD_INL V3 point_jacobian_column(const Env& e, uint32_t link, V3 x, uint32_t j) 
         setcomp(ev, (int)j - 3, 1.0f);
         return cross(ev, sub(x, e.pos[0]));
     }
-    const uint32_t owner = e.dofs[j].body;
-    for (uint32_t b = link; b != 0u && b != MR_INVALID_INDEX; b = e.bodies[b].parent)
-        if (b == owner) return add(e.sl[b], cross(e.sa[b], sub(x, e.pos[b])));
+    const uint32_t owner = e.topo->dof_body[j];
+    if (owner < 64u && ((e.topo->anc[link] >> owner) & 1ull) != 0ull)
+        return add(e.sl[owner], cross(e.sa[owner], sub(x, e.pos[owner])));
     return zero;
 }
 
@@ -1485,19 +1505,16 @@ MRD_INL void apply_scene_impulse(MRBodyStateC& b, V3 x, V3 imp) {
 
 MRD_INL V3 hot_dir(const HotConstraint& h, int r) { return V3{h.dir[3 * r], h.dir[3 * r + 1], h.dir[3 * r + 2]}; }
 
-// Relative velocity rows of one constraint from the current velocities.
-MRD_INL float relative_row(const Env& e, const HotConstraint& k, uint32_t c, int row) {
-    float a = 0.0f;
-    if (k.art != 0u) {
-        const float* jr = e.jrow + ((size_t)c * 3u + (size_t)row) * e.a->max_nv;
-        for (uint32_t j = 0; j < e.nv; ++j) a += jr[j] * e.vcand[j];
-    }
+// Relative velocity of one constraint row: the articulation part `art_dot`
+// (J v, already folded across the DoF lanes) plus the scene-body point
+// velocities, combined in the oracle's order.
+MRD_INL float relative_from(const Env& e, const HotConstraint& k, int row, float art_dot) {
     const V3 x{k.point[0], k.point[1], k.point[2]};
     const V3 d = hot_dir(k, row);
     const V3 zero{0.0f, 0.0f, 0.0f};
     const V3 va = is_dynamic_scene(e, k.body_a) ? scene_point_velocity(e.scene[k.body_a - e.nlinks], x) : zero;
     const V3 vb = is_dynamic_scene(e, k.body_b) ? scene_point_velocity(e.scene[k.body_b - e.nlinks], x) : zero;
-    return (a + dot(d, vb)) - dot(d, va);
+    return (art_dot + dot(d, vb)) - dot(d, va);
 }
 
 struct StageClock {
@@ -1559,6 +1576,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
     e.h = A.h;
     e.gravity = V3{A.gravity[0], A.gravity[1], A.gravity[2]};
     const mr::LdsLayout& L = A.lds;
+    e.topo = lds_at<mr::TopoCache>(lds_base, L.topo);
     e.q = lds_at<float>(lds_base, L.q);
     e.v = lds_at<float>(lds_base, L.v);
     e.vcand = lds_at<float>(lds_base, L.vcand);
@@ -1596,6 +1614,36 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
     e.jrow = A.jrow + (size_t)env * ccap * 3u * A.max_nv;
     e.resp = A.resp + (size_t)env * ccap * 3u * A.max_nv;
     e.cold = A.cold + (size_t)env * ccap;
+
+    // ---- topology cache (one lane per body / DoF; 64 lanes cover the limits)
+    {
+        mr::TopoCache& T = *e.topo;
+        uint8_t par = 0xffu, vi = 0xffu, dep = 0u, mot = 0u;
+        if (lane < e.nbodies) {
+            const MRBodyC& b = e.bodies[lane];
+            mot = (uint8_t)b.motion_type;
+            if (lane < e.nlinks) {
+                par = b.parent == MR_INVALID_INDEX ? (uint8_t)0xffu : (uint8_t)b.parent;
+                vi = b.v_index == MR_INVALID_INDEX ? (uint8_t)0xffu : (uint8_t)b.v_index;
+                dep = (uint8_t)b.depth;
+            }
+        }
+        T.parent[lane] = par;
+        T.vidx[lane] = vi;
+        T.depth[lane] = dep;
+        T.motion[lane] = mot;
+        T.dof_body[lane] = lane < e.nv ? (uint8_t)e.dofs[lane].body : (uint8_t)0xffu;
+        __syncthreads();
+        // model validation guarantees parent < child, so both walks end
+        unsigned long long anc = 0ull, ch = 0ull;
+        if (lane < e.nlinks) {
+            for (uint32_t b = lane; b != 0u && b != 0xffu; b = T.parent[b]) anc |= 1ull << b;
+            for (uint32_t c = lane + 1u; c < e.nlinks; ++c)
+                if (T.parent[c] == lane) ch |= 1ull << c;
+        }
+        T.anc[lane] = anc;
+        T.children[lane] = ch;
+    }
 
     const uint32_t nv = e.nv, nq = e.nq, nl = e.nlinks, nsc = e.nscene;
     float* gq = A.q + (size_t)env * A.max_nq;
@@ -1684,7 +1732,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         __syncthreads();
         for (uint32_t d = 1; d < e.levels; ++d) {
             for (uint32_t i = 1u + lane; i < nl; i += kLanes)
-                if (e.bodies[i].depth == d && !fk_link(e, i))
+                if (e.topo->depth[i] == d && !fk_link(e, i))
                     report_failure(&s_fail, 1u + i, MR_STEP_NONFINITE_RESULT, i);
             __syncthreads();
         }
@@ -1708,7 +1756,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         float lane_min_piv = __builtin_inff(), lane_max_piv = 0.0f;
         for (uint32_t d = e.levels - 1u; d >= 1u; --d) {
             for (uint32_t i = 1u + lane; i < nl; i += kLanes) {
-                if (e.bodies[i].depth != d) continue;
+                if (e.topo->depth[i] != d) continue;
                 float piv;
                 if (!aba_project_link(e, i, piv))
                     report_failure(&s_fail, 0xffffu - i, MR_STEP_FACTORIZATION_FAILED, e.bodies[i].v_index);
@@ -1719,7 +1767,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
             }
             __syncthreads();
             for (uint32_t p = lane; p < nl; p += kLanes)
-                if (e.bodies[p].depth == d - 1u) aba_accumulate(e, p);
+                if (e.topo->depth[p] == d - 1u) aba_accumulate(e, p);
             __syncthreads();
         }
         if (s_fail != ~0ull) break;
@@ -1753,16 +1801,16 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         if (s_fail != ~0ull) break;
         for (uint32_t d = 1; d < e.levels; ++d) {
             for (uint32_t i = 1u + lane; i < nl; i += kLanes) {
-                if (e.bodies[i].depth != d) continue;
-                const MRBodyC& b = e.bodies[i];
-                const V3 ap = e.alpha[b.parent];
-                const V3 lp = add(e.acc[b.parent], cross(ap, e.r[i]));
+                if (e.topo->depth[i] != d) continue;
+                const uint32_t parent = link_parent(e, i), vi = link_vidx(e, i);
+                const V3 ap = e.alpha[parent];
+                const V3 lp = add(e.acc[parent], cross(ap, e.r[i]));
                 V3 ai = add(ap, e.ca[i]);
                 V3 li = add(lp, e.cl[i]);
-                if (b.v_index != MR_INVALID_INDEX) {
+                if (vi != MR_INVALID_INDEX) {
                     const float proj = dot(e.Ut[i], ap) + dot(e.Ub[i], lp);
                     const float qdd = (e.u[i] - proj) / e.D[i];
-                    e.accel[b.v_index] = qdd;
+                    e.accel[vi] = qdd;
                     ai = add(ai, mul(qdd, e.sa[i]));
                     li = add(li, mul(qdd, e.sl[i]));
                 }
@@ -2067,13 +2115,52 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         __syncthreads();
         clk.mark(mr::kStageResponses);
         // ------------------------------------------ IR evaluation + blocks
+        // Lanes own DoFs for every articulation dot product (canonical fold);
+        // per-constraint scalars are then evaluated lane-parallel over
+        // constraints. target/winv carry rel and J M^-1 J^T between the two
+        // passes.
+        const uint32_t mnv = A.max_nv;
+        const bool dof_lane = lane < nv;
+        float vj = dof_lane ? e.vcand[lane] : 0.0f;
+        for (uint32_t c = 0; c < ncons; ++c) {
+            const HotConstraint& hk = e.hot[c];
+            float rel[3], jm[9];
+            const float* jb = e.jrow + (size_t)c * 3u * mnv + lane;
+            const float* rb = e.resp + (size_t)c * 3u * mnv + lane;
+            float jr[3] = {0.0f, 0.0f, 0.0f}, rr[3] = {0.0f, 0.0f, 0.0f};
+            if (hk.art != 0u && dof_lane) {
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    jr[r] = jb[(size_t)r * mnv];
+                    rr[r] = rb[(size_t)r * mnv];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const float a = hk.art != 0u ? wave_fold(dof_lane ? jr[r] * vj : 0.0f) : 0.0f;
+                rel[r] = relative_from(e, hk, r, a);
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    jm[3 * r + q] = hk.art != 0u ? wave_fold(dof_lane ? jr[r] * rr[q] : 0.0f) : 0.0f;
+            if (lane == 0u) {
+                HotConstraint& hw = e.hot[c];
+                hw.target[0] = rel[0];
+                hw.target[1] = rel[1];
+                hw.target[2] = rel[2];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) hw.winv[k] = jm[k];
+            }
+        }
+        __syncthreads();
         const float slop = cfg.contact_slop;
         const float maxdep = cfg.max_depenetration_velocity;
         for (uint32_t c = lane; c < ncons; c += kLanes) {
             HotConstraint hk = e.hot[c];
             const ColdConstraint ck = e.cold[c];
-            float rel[3];
-            for (int r = 0; r < 3; ++r) rel[r] = relative_row(e, hk, c, r);
+            const float rel[3] = {hk.target[0], hk.target[1], hk.target[2]};
             const float slip = sqrt_rn(rel[1] * rel[1] + rel[2] * rel[2]);
             const bool stat = slip <= 1.0e-3f;
             hk.mu_u = stat ? ck.friction[0] : ck.friction[2];
@@ -2105,17 +2192,12 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                 const bool sa_dyn = is_dynamic_scene(e, hk.body_a), sb_dyn = is_dynamic_scene(e, hk.body_b);
                 float W[3][3];
                 for (int r = 0; r < 3; ++r)
-                    for (int s = 0; s < 3; ++s) {
-                        float val = 0.0f;
-                        if (hk.art != 0u) {
-                            const float* jr = e.jrow + ((size_t)c * 3u + (size_t)r) * A.max_nv;
-                            const float* rs = e.resp + ((size_t)c * 3u + (size_t)s) * A.max_nv;
-                            for (uint32_t j = 0; j < nv; ++j) val += jr[j] * rs[j];
-                        }
-                        if (sa_dyn) val += dot(hot_dir(hk, r), scene_response(e.scene[hk.body_a - nl], x, hot_dir(hk, s)));
-                        if (sb_dyn) val += dot(hot_dir(hk, r), scene_response(e.scene[hk.body_b - nl], x, hot_dir(hk, s)));
-                        if (r == s) val += hk.reg[r];
-                        W[r][s] = val;
+                    for (int q = 0; q < 3; ++q) {
+                        float val = hk.winv[3 * r + q];
+                        if (sa_dyn) val += dot(hot_dir(hk, r), scene_response(e.scene[hk.body_a - nl], x, hot_dir(hk, q)));
+                        if (sb_dyn) val += dot(hot_dir(hk, r), scene_response(e.scene[hk.body_b - nl], x, hot_dir(hk, q)));
+                        if (r == q) val += hk.reg[r];
+                        W[r][q] = val;
                     }
                 if (!invert3x3(W, hk.winv)) report_failure(&s_fail, c, MR_STEP_FACTORIZATION_FAILED, c);
             }
@@ -2126,15 +2208,13 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         clk.mark(mr::kStageEvaluation);
         if (s_fail != ~0ull) break;
         // ---------------------------------------------------- warm start
-        for (uint32_t j = lane; j < nv; j += kLanes) {
-            float vj = e.vcand[j];
-            for (uint32_t c = 0; c < ncons; ++c) {
-                const HotConstraint& hk = e.hot[c];
-                if (hk.art == 0u) continue;
-                const float* r0 = e.resp + ((size_t)c * 3u + 0u) * A.max_nv;
-                vj += (r0[j] * hk.lambda[0] + r0[A.max_nv + j] * hk.lambda[1]) + r0[2u * A.max_nv + j] * hk.lambda[2];
-            }
-            e.vcand[j] = vj;
+        // DoF lanes accumulate the warm-start impulses in constraint order;
+        // scene lanes apply theirs in the same order.
+        for (uint32_t c = 0; c < ncons; ++c) {
+            const HotConstraint& hk = e.hot[c];
+            if (hk.art == 0u || !dof_lane) continue;
+            const float* rb = e.resp + (size_t)c * 3u * mnv + lane;
+            vj += (rb[0] * hk.lambda[0] + rb[mnv] * hk.lambda[1]) + rb[2u * mnv] * hk.lambda[2];
         }
         for (uint32_t s = lane; s < nsc; s += kLanes) {
             const uint32_t b = nl + s;
@@ -2154,27 +2234,46 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         __syncthreads();
         clk.mark(mr::kStageWarmStart);
         // ------------------------------------------ ordered block sweeps
+        // Every lane runs the constraint loop; the block update is uniform,
+        // DoF lanes keep the candidate velocity in a register, and the next
+        // constraint's Jacobian / response rows are prefetched.
         const uint32_t base_iters = cfg.velocity_iterations + (final_sub ? cfg.final_velocity_iterations : 0u);
         const bool adaptive = cfg.residual_tolerance > 0.0f;
         const uint32_t max_iters = adaptive ? cfg.max_iterations : base_iters;
         float sweep_delta = 0.0f;
         uint32_t iters = 0;
         bool solve_failed = false;
+        float nj[3] = {0.0f, 0.0f, 0.0f}, nr[3] = {0.0f, 0.0f, 0.0f};
+        auto load_rows = [&](uint32_t c, float* j3, float* r3) {
+            if (!dof_lane) return;
+            const float* jb = e.jrow + (size_t)c * 3u * mnv + lane;
+            const float* rb = e.resp + (size_t)c * 3u * mnv + lane;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                j3[r] = jb[(size_t)r * mnv];
+                r3[r] = rb[(size_t)r * mnv];
+            }
+        };
+        if (ncons > 0u) load_rows(0u, nj, nr);
         for (uint32_t it = 0; it < max_iters && ncons > 0u; ++it) {
             sweep_delta = 0.0f;
             for (uint32_t c = 0; c < ncons; ++c) {
+                const float jr[3] = {nj[0], nj[1], nj[2]};
+                const float rr[3] = {nr[0], nr[1], nr[2]};
+                load_rows(c + 1u < ncons ? c + 1u : 0u, nj, nr);
                 const HotConstraint& hk = e.hot[c];
-                float relv = 0.0f;
-                if (lane < 3u) relv = relative_row(e, hk, c, (int)lane);
-                const float rel0 = __shfl(relv, 0, 64);
-                const float rel1 = __shfl(relv, 1, 64);
-                const float rel2 = __shfl(relv, 2, 64);
-                const float rel[3] = {rel0, rel1, rel2};
+                const bool art = hk.art != 0u;
+                float rel[3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const float a = art ? wave_fold(dof_lane ? jr[r] * vj : 0.0f) : 0.0f;
+                    rel[r] = relative_from(e, hk, r, a);
+                }
                 float rhs[3];
                 for (int r = 0; r < 3; ++r) rhs[r] = (hk.target[r] - rel[r]) - hk.reg[r] * hk.lambda[r];
                 float cand[3] = {hk.lambda[0], hk.lambda[1], hk.lambda[2]};
                 for (int r = 0; r < 3; ++r)
-                    for (int s = 0; s < 3; ++s) cand[r] += hk.winv[3 * r + s] * rhs[s];
+                    for (int q = 0; q < 3; ++q) cand[r] += hk.winv[3 * r + q] * rhs[q];
                 project_cone(cand, hk.mu_u, hk.mu_v);
                 const float d0 = cand[0] - hk.lambda[0], d1 = cand[1] - hk.lambda[1], d2 = cand[2] - hk.lambda[2];
                 if (!isfin(cand[0]) || !isfin(cand[1]) || !isfin(cand[2]) || !isfin(d0) || !isfin(d1) || !isfin(d2)) {
@@ -2183,38 +2282,40 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                     break;
                 }
                 sweep_delta = fmaxd(sweep_delta, fmaxd(fabs_(d0), fmaxd(fabs_(d1), fabs_(d2))));
-                if (hk.art != 0u) {
-                    const float* r0 = e.resp + ((size_t)c * 3u + 0u) * A.max_nv;
-                    for (uint32_t j = lane; j < nv; j += kLanes)
-                        e.vcand[j] += (r0[j] * d0 + r0[A.max_nv + j] * d1) + r0[2u * A.max_nv + j] * d2;
-                }
+                if (art && dof_lane) vj += (rr[0] * d0 + rr[1] * d1) + rr[2] * d2;
+                const bool sa_dyn = is_dynamic_scene(e, hk.body_a), sb_dyn = is_dynamic_scene(e, hk.body_b);
+                const V3 x{hk.point[0], hk.point[1], hk.point[2]};
+                const V3 imp = add(add(mul(d0, hot_dir(hk, 0)), mul(d1, hot_dir(hk, 1))), mul(d2, hot_dir(hk, 2)));
+                __syncthreads();  // every lane has read hk and the scene bodies
                 if (lane == 0u) {
+                    if (sa_dyn) apply_scene_impulse(e.scene[hk.body_a - nl], x, neg(imp));
+                    if (sb_dyn) apply_scene_impulse(e.scene[hk.body_b - nl], x, imp);
                     HotConstraint& hw = e.hot[c];
-                    const V3 imp = add(add(mul(d0, hot_dir(hk, 0)), mul(d1, hot_dir(hk, 1))), mul(d2, hot_dir(hk, 2)));
-                    const V3 x{hk.point[0], hk.point[1], hk.point[2]};
-                    if (is_dynamic_scene(e, hk.body_a)) apply_scene_impulse(e.scene[hk.body_a - nl], x, neg(imp));
-                    if (is_dynamic_scene(e, hk.body_b)) apply_scene_impulse(e.scene[hk.body_b - nl], x, imp);
                     hw.lambda[0] = cand[0];
                     hw.lambda[1] = cand[1];
                     hw.lambda[2] = cand[2];
                 }
-                __syncthreads();
+                if (sa_dyn || sb_dyn) __syncthreads();
             }
             if (solve_failed) break;
             ++iters;
             if (adaptive && sweep_delta <= cfg.residual_tolerance) break;
+            __syncthreads();
         }
         __syncthreads();
         clk.mark(mr::kStageSweeps);
         if (s_fail != ~0ull) break;
         // ------------------------------------- residuals + impulse store
-        float my_normal = 0.0f, my_cone = 0.0f;
-        for (uint32_t c = lane; c < ncons; c += kLanes) {
-            const HotConstraint hk = e.hot[c];
-            const float rel0 = relative_row(e, hk, c, 0);
+        float max_normal = 0.0f, max_cone = 0.0f;
+        for (uint32_t c = 0; c < ncons; ++c) {
+            const HotConstraint& hk = e.hot[c];
+            float j0 = 0.0f;
+            if (hk.art != 0u && dof_lane) j0 = e.jrow[(size_t)c * 3u * mnv + lane];
+            const float a0 = hk.art != 0u ? wave_fold(dof_lane ? j0 * vj : 0.0f) : 0.0f;
+            const float rel0 = relative_from(e, hk, 0, a0);
             const float eq = (rel0 - hk.target[0]) + hk.reg[0] * hk.lambda[0];
             const float nres = hk.lambda[0] > kConeEpsilon ? fabs_(eq) : fmaxd(-eq, 0.0f);
-            my_normal = fmaxd(my_normal, nres);
+            max_normal = fmaxd(max_normal, nres);
             const float lu = hk.mu_u * hk.lambda[0], lv = hk.mu_v * hk.lambda[0];
             float viol;
             if (lu > 0.0f && lv > 0.0f)
@@ -2222,13 +2323,15 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                              0.0f);
             else
                 viol = sqrt_rn(hk.lambda[1] * hk.lambda[1] + hk.lambda[2] * hk.lambda[2]);
-            my_cone = fmaxd(my_cone, viol);
-            const ColdConstraint& ck = e.cold[c];
-            new_p[(size_t)ck.manifold * MR_MANIFOLD_POINT_CAPACITY + ck.point].impulses =
-                mr_float4{hk.lambda[0], hk.lambda[1], hk.lambda[2], 0.0f};
+            max_cone = fmaxd(max_cone, viol);
+            if (lane == (c & (kLanes - 1u))) {
+                const ColdConstraint& ck = e.cold[c];
+                new_p[(size_t)ck.manifold * MR_MANIFOLD_POINT_CAPACITY + ck.point].impulses =
+                    mr_float4{hk.lambda[0], hk.lambda[1], hk.lambda[2], 0.0f};
+            }
         }
-        const float max_normal = wave_max(my_normal);
-        const float max_cone = wave_max(my_cone);
+        if (dof_lane) e.vcand[lane] = vj;
+        __syncthreads();
         if (!isfin(sweep_delta) || !isfin(max_normal) || !isfin(max_cone)) {
             if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_NONFINITE_RESULT << 24) | 0xffffffu;
             __syncthreads();

@@ -155,6 +155,7 @@ void World::compute_lds_layout() {
         return at;
     };
     LdsLayout& L = lds_;
+    L.topo = take((uint32_t)sizeof(TopoCache), 16u);
     L.q = take(4u * nq, 4u);
     L.v = take(4u * nv, 4u);
     L.vcand = take(4u * nv, 4u);

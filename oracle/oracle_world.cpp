@@ -1522,6 +1522,18 @@ bool build_manifold(const StepContext& ctx, const Work& wk, uint32_t env, const 
 
 // ------------------------------------------------------------- cone solve
 
+// Dot product of two generalized vectors (length n <= 64) in a fixed
+// pairwise order: x[i] = a[i] * b[i] (+0 past n), then x[i] += x[i + d] for
+// d = 32, 16, ..., 1. This is the order of a 64-lane butterfly reduction, so
+// the device reproduces it bit for bit with one DoF per lane.
+float fold_dot(const float* a, const float* b, uint32_t n) {
+    float x[64];
+    for (uint32_t i = 0; i < 64u; ++i) x[i] = i < n ? a[i] * b[i] : 0.0f;
+    for (uint32_t d = 32u; d >= 1u; d >>= 1)
+        for (uint32_t i = 0; i < d; ++i) x[i] = x[i] + x[i + d];
+    return x[0];
+}
+
 // MetalWorldContact.metal:482-509 (projectFrictionCone).
 void project_cone(float lam[3], float mu_u, float mu_v) {
     lam[0] = fmaxd(lam[0], 0.0f);
@@ -1888,11 +1900,7 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
         const V3 vb = sb_dyn ? scene_point_velocity(wk.bodies[k.body_b], k.point_world) : V3{0.0f, 0.0f, 0.0f};
         const bool art = is_link(ctx, k.body_a) || is_link(ctx, k.body_b);
         for (int r = 0; r < 3; ++r) {
-            float a = 0.0f;
-            if (art) {
-                const float* jr = &jrow[((size_t)c * 3u + (size_t)r) * nv];
-                for (uint32_t j = 0; j < nv; ++j) a += jr[j] * vel[j];
-            }
+            const float a = art ? fold_dot(&jrow[((size_t)c * 3u + (size_t)r) * nv], vel, nv) : 0.0f;
             rel[r] = (a + dot(k.dir[r], vb)) - dot(k.dir[r], va);
         }
     };
@@ -1935,12 +1943,8 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
         float W[3][3];
         for (int r = 0; r < 3; ++r)
             for (int s = 0; s < 3; ++s) {
-                float val = 0.0f;
-                if (art) {
-                    const float* jr = &jrow[((size_t)c * 3u + (size_t)r) * nv];
-                    const float* rs = &resp[((size_t)c * 3u + (size_t)s) * nv];
-                    for (uint32_t j = 0; j < nv; ++j) val += jr[j] * rs[j];
-                }
+                float val = art ? fold_dot(&jrow[((size_t)c * 3u + (size_t)r) * nv],
+                                           &resp[((size_t)c * 3u + (size_t)s) * nv], nv) : 0.0f;
                 if (sa_dyn) val += dot(k.dir[r], scene_response(wk.bodies[k.body_a], k.point_world, k.dir[s]));
                 if (sb_dyn) val += dot(k.dir[r], scene_response(wk.bodies[k.body_b], k.point_world, k.dir[s]));
                 if (r == s) val += k.reg[r];
