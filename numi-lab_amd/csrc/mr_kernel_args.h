@@ -61,7 +61,42 @@ struct TopoCache {
     uint8_t depth[MR_MAX_BODIES];
     uint8_t motion[MR_MAX_BODIES];              // motion type of every body
     uint8_t dof_body[MR_MAX_BODIES];            // owning link of every DoF
+    uint8_t slot[MR_MAX_BODIES];                // outward-pass slot of each link (0xff: leaf)
 };
+
+#if defined(__HIPCC__)
+#define MR_HOST_DEVICE __host__ __device__
+#else
+#define MR_HOST_DEVICE
+#endif
+
+// Slot plan of the response outward pass. Links are visited in index order
+// (parents first); a link's spatial acceleration must stay available until
+// its last child has read it, so it takes the lowest free slot and frees it
+// at that child. Leaves take none. `parent(i)` gives the parent link of
+// i >= 1. Returns the number of slots used at the peak.
+template <class ParentFn>
+MR_HOST_DEVICE inline uint32_t plan_response_slots(ParentFn parent, uint32_t n, uint8_t* slot) {
+    uint8_t last_child[MR_MAX_BODIES];
+    for (uint32_t i = 0; i < n; ++i) last_child[i] = 0xffu;
+    for (uint32_t i = 1; i < n; ++i) last_child[parent(i)] = (uint8_t)i;
+    unsigned long long used = 0ull;
+    uint32_t peak = 0;
+    auto take = [&]() -> uint8_t {
+        uint32_t s = 0;
+        while ((used >> s) & 1ull) ++s;
+        used |= 1ull << s;
+        peak = s + 1u > peak ? s + 1u : peak;
+        return (uint8_t)s;
+    };
+    slot[0] = last_child[0] != 0xffu ? take() : (uint8_t)0xffu;
+    for (uint32_t i = 1; i < n; ++i) {
+        const uint32_t p = parent(i);
+        if (last_child[p] == i) used &= ~(1ull << slot[p]);
+        slot[i] = last_child[i] != 0xffu ? take() : (uint8_t)0xffu;
+    }
+    return peak;
+}
 
 // Byte offsets of the per-workgroup dynamic LDS arrays.
 struct LdsLayout {
@@ -70,6 +105,9 @@ struct LdsLayout {
     uint32_t pos, rot, w, vl, sa, sl, ca, cl, r;
     uint32_t I, pt, pb, Ut, Ub, D, u, alpha, acc, L;
     uint32_t scene, col, active, npts, moff, hot;
+    // response scratch, overlaying the collision arrays (dead after emission)
+    uint32_t rslot;  // [slots][6][64] floats: outward spatial accelerations
+    uint32_t rstack; // [stack][64] floats: joint terms of the inward path
     uint32_t total;
 };
 

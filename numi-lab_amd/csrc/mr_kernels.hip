@@ -187,6 +187,7 @@ struct Env {
     Collider* col;
     uint32_t *active, *npts, *moff;
     HotConstraint* hot;
+    float *rslot, *rstack;
     // global scratch of this environment
     MRManifoldHeaderC* stage_h;
     MRManifoldPointC* stage_p;
@@ -244,13 +245,30 @@ MRD_INL uint32_t wave_prefix_sum(uint32_t value, uint32_t lane, uint32_t& total)
     return x - value;
 }
 
-// Canonical 64-lane pairwise sum: x[i] + x[i ^ d] for d = 32 ... 1. IEEE
-// addition commutes, so every lane ends with the same bits, equal to the
-// fold x[i] += x[i + d] over lanes 0 .. d-1.
+// One butterfly step through a DPP lane permutation: x + x[perm(lane)].
+template <int kCtrl>
+MRD_INL float dpp_fold_step(float x) {
+    const int y = __builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, 0xf, 0xf, false);
+    return x + __int_as_float(y);
+}
+
+// Canonical 64-lane pairwise sum x[i] + x[i ^ d] for d = 1, 2, 4, ..., 32.
+// d = 1 and 2 are quad permutations; after them every lane of a quad holds
+// the same bits (IEEE addition commutes), so the half-row mirror pairs quad 0
+// with quad 1 exactly as xor 4 would, and the row mirror then pairs the two
+// halves of a row as xor 8. The four row sums combine as (r0 + r1) + (r2 + r3).
+// Must be called with all 64 lanes active; every lane returns the same value.
 MRD_INL float wave_fold(float x) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) x = x + __shfl_xor(x, d, 64);
-    return x;
+    x = dpp_fold_step<0xB1>(x);   // quad_perm [1,0,3,2]: xor 1
+    x = dpp_fold_step<0x4E>(x);   // quad_perm [2,3,0,1]: xor 2
+    x = dpp_fold_step<0x141>(x);  // row_half_mirror: xor 4 on quad-uniform data
+    x = dpp_fold_step<0x140>(x);  // row_mirror: xor 8 on half-row-uniform data
+    const int b = __float_as_int(x);
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(b, 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(b, 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(b, 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(b, 48));
+    return (r0 + r1) + (r2 + r3);
 }
 
 MRD_INL float wave_max(float v) {
@@ -528,64 +546,127 @@ MRD_INL void root_solve(const float* L, const float rhs[6], float x[6]) {
     }
 }
 
-// Inverse ABA for point forces on up to two links (lane-local, private
-// arrays); out[nv] = M_eff^{-1} J^T f.
-MRD_INL void inverse_aba(const Env& e, uint32_t la, V3 xa, V3 fa, uint32_t lb, V3 xb, V3 fb, float* out) {
-    V3 pt[MR_MAX_LINKS], pb[MR_MAX_LINKS];
-    float uu[MR_MAX_LINKS];
+// Inverse ABA for the point forces of one constraint row on up to two links:
+// out[nv] = M_eff^{-1} J^T f, one row per lane.
+//
+// Inward pass: only links on the root paths of `la` / `lb` carry a bias. Every
+// other link's bias and contribution is exactly +0 in every component (the
+// accumulators start at +0, +0 - y and x + y never produce -0 from operands
+// without -0, and u = (0 - s.t) - s.f is +0 for a +0 bias), so visiting only
+// path links in decreasing index order, and adding the at most two pending
+// child contributions of a merge link in decreasing child order, reproduces
+// the dense pass bit for bit. The joint terms u of path links are pushed on a
+// per-lane LDS stack; the outward pass visits all links in increasing index
+// order and pops them in reverse. Link accelerations live in the LDS slots of
+// the topology cache's plan.
+MRD_INL void inverse_aba(const Env& e, uint32_t lane, uint32_t la, V3 xa, V3 fa, uint32_t lb, V3 xb, V3 fb,
+                         float* out) {
+    const mr::TopoCache& T = *e.topo;
     const uint32_t n = e.nlinks;
-    for (uint32_t i = 0; i < n; ++i) {
-        pt[i] = V3{0.0f, 0.0f, 0.0f};
-        pb[i] = V3{0.0f, 0.0f, 0.0f};
-    }
-    if (la != MR_INVALID_INDEX) {
-        pt[la] = sub(pt[la], cross(sub(xa, e.pos[la]), fa));
-        pb[la] = sub(pb[la], fa);
-    }
-    if (lb != MR_INVALID_INDEX) {
-        pt[lb] = sub(pt[lb], cross(sub(xb, e.pos[lb]), fb));
-        pb[lb] = sub(pb[lb], fb);
-    }
-    for (uint32_t i = n - 1u; i >= 1u; --i) {
-        const uint32_t parent = link_parent(e, i);
-        V3 t = pt[i], f = pb[i];
+    const V3 zero{0.0f, 0.0f, 0.0f};
+    const unsigned long long path = (la != MR_INVALID_INDEX ? T.anc[la] : 0ull) |
+                                    (lb != MR_INVALID_INDEX ? T.anc[lb] : 0ull);
+    float* ustack = e.rstack + lane;
+    uint32_t sp = 0;
+    // pending child contributions: target link, child index, torque, force
+    uint32_t tg0 = MR_INVALID_INDEX, ch0 = 0u, tg1 = MR_INVALID_INDEX, ch1 = 0u;
+    V3 c0t = zero, c0f = zero, c1t = zero, c1f = zero;
+    auto gather = [&](uint32_t i, V3& t, V3& f) {
+        t = zero;
+        f = zero;
+        if (i == la) {
+            t = sub(t, cross(sub(xa, e.pos[i]), fa));
+            f = sub(f, fa);
+        }
+        if (i == lb) {
+            t = sub(t, cross(sub(xb, e.pos[i]), fb));
+            f = sub(f, fb);
+        }
+        const bool m0 = tg0 == i, m1 = tg1 == i;
+        if (m0 && m1) {
+            const bool first0 = ch0 > ch1;
+            t = add(t, first0 ? c0t : c1t);
+            f = add(f, first0 ? c0f : c1f);
+            t = add(t, first0 ? c1t : c0t);
+            f = add(f, first0 ? c1f : c0f);
+        } else if (m0) {
+            t = add(t, c0t);
+            f = add(f, c0f);
+        } else if (m1) {
+            t = add(t, c1t);
+            f = add(f, c1f);
+        }
+        if (m0) tg0 = MR_INVALID_INDEX;
+        if (m1) tg1 = MR_INVALID_INDEX;
+    };
+    unsigned long long rest = path;
+    while (rest != 0ull) {
+        const uint32_t i = 63u - (uint32_t)__builtin_clzll(rest);
+        rest &= ~(1ull << i);
+        V3 t, f;
+        gather(i, t, f);
         if (link_vidx(e, i) != MR_INVALID_INDEX) {
             const float u = (0.0f - dot(e.sa[i], t)) - dot(e.sl[i], f);
             const float uD = u / e.D[i];
             t = add(t, mul(uD, e.Ut[i]));
             f = add(f, mul(uD, e.Ub[i]));
-            uu[i] = u;
+            ustack[sp * 64u] = u;
+            ++sp;
         }
-        pt[parent] = add(pt[parent], add(t, cross(e.r[i], f)));
-        pb[parent] = add(pb[parent], f);
+        const V3 ct = add(t, cross(e.r[i], f));
+        if (tg0 == MR_INVALID_INDEX) {
+            tg0 = link_parent(e, i);
+            ch0 = i;
+            c0t = ct;
+            c0f = f;
+        } else {
+            tg1 = link_parent(e, i);
+            ch1 = i;
+            c1t = ct;
+            c1f = f;
+        }
     }
-    // outward: reuse pt as angular, pb as linear acceleration storage
+    V3 ap, lp;
     if (e.root_type == MR_ROOT_FLOATING) {
-        const float rhs[6] = {-pt[0].x, -pt[0].y, -pt[0].z, -pb[0].x, -pb[0].y, -pb[0].z};
+        V3 t0, f0;
+        gather(0u, t0, f0);
+        const float rhs[6] = {-t0.x, -t0.y, -t0.z, -f0.x, -f0.y, -f0.z};
         float x[6];
         root_solve(e.L, rhs, x);
-        pt[0] = V3{x[0], x[1], x[2]};
-        pb[0] = V3{x[3], x[4], x[5]};
+        ap = V3{x[0], x[1], x[2]};
+        lp = V3{x[3], x[4], x[5]};
         out[0] = x[3]; out[1] = x[4]; out[2] = x[5];
         out[3] = x[0]; out[4] = x[1]; out[5] = x[2];
     } else {
-        pt[0] = V3{0.0f, 0.0f, 0.0f};
-        pb[0] = V3{0.0f, 0.0f, 0.0f};
+        ap = zero;
+        lp = zero;
     }
+    auto slot_store = [&](uint32_t s, V3 a, V3 l) {
+        float* p = e.rslot + s * 6u * 64u + lane;
+        p[0] = a.x; p[64] = a.y; p[128] = a.z;
+        p[192] = l.x; p[256] = l.y; p[320] = l.z;
+    };
+    if (n > 1u) slot_store(T.slot[0], ap, lp);
     for (uint32_t i = 1; i < n; ++i) {
         const uint32_t parent = link_parent(e, i), vi = link_vidx(e, i);
-        const V3 ap = pt[parent];
-        const V3 lp = add(pb[parent], cross(ap, e.r[i]));
+        const float* p = e.rslot + (uint32_t)T.slot[parent] * 6u * 64u + lane;
+        const V3 a_par{p[0], p[64], p[128]};
+        const V3 l_par{p[192], p[256], p[320]};
+        const V3 lpi = add(l_par, cross(a_par, e.r[i]));
+        V3 ai = a_par, li = lpi;
         if (vi != MR_INVALID_INDEX) {
-            const float proj = dot(e.Ut[i], ap) + dot(e.Ub[i], lp);
-            const float qdd = (uu[i] - proj) / e.D[i];
+            float uu = 0.0f;
+            if (((path >> i) & 1ull) != 0ull) {
+                --sp;
+                uu = ustack[sp * 64u];
+            }
+            const float proj = dot(e.Ut[i], a_par) + dot(e.Ub[i], lpi);
+            const float qdd = (uu - proj) / e.D[i];
             out[vi] = qdd;
-            pt[i] = add(ap, mul(qdd, e.sa[i]));
-            pb[i] = add(lp, mul(qdd, e.sl[i]));
-        } else {
-            pt[i] = ap;
-            pb[i] = lp;
+            ai = add(a_par, mul(qdd, e.sa[i]));
+            li = add(lpi, mul(qdd, e.sl[i]));
         }
+        if (T.slot[i] != 0xffu) slot_store(T.slot[i], ai, li);
     }
 }
 
@@ -1608,6 +1689,8 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
     e.npts = lds_at<uint32_t>(lds_base, L.npts);
     e.moff = lds_at<uint32_t>(lds_base, L.moff);
     e.hot = lds_at<HotConstraint>(lds_base, L.hot);
+    e.rslot = lds_at<float>(lds_base, L.rslot);
+    e.rstack = lds_at<float>(lds_base, L.rstack);
     const uint32_t pcap = A.pair_capacity, mcap = A.manifold_capacity, ccap = A.constraint_capacity;
     e.stage_h = A.stage_h + (size_t)env * pcap;
     e.stage_p = A.stage_p + (size_t)env * pcap * MR_MANIFOLD_POINT_CAPACITY;
@@ -1643,6 +1726,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         }
         T.anc[lane] = anc;
         T.children[lane] = ch;
+        if (lane == 0u) mr::plan_response_slots([&](uint32_t i) { return (uint32_t)T.parent[i]; }, e.nlinks, T.slot);
     }
 
     const uint32_t nv = e.nv, nq = e.nq, nl = e.nlinks, nsc = e.nscene;
@@ -2110,7 +2194,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                 jr[j] = dot(d, sub(cb, ca));
             }
             float* rs = e.resp + ((size_t)c * 3u + (size_t)row) * A.max_nv;
-            inverse_aba(e, la ? hk.body_a : MR_INVALID_INDEX, pa, neg(d), lb ? hk.body_b : MR_INVALID_INDEX, pb, d, rs);
+            inverse_aba(e, lane, la ? hk.body_a : MR_INVALID_INDEX, pa, neg(d), lb ? hk.body_b : MR_INVALID_INDEX, pb, d, rs);
         }
         __syncthreads();
         clk.mark(mr::kStageResponses);
@@ -2122,19 +2206,28 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         const uint32_t mnv = A.max_nv;
         const bool dof_lane = lane < nv;
         float vj = dof_lane ? e.vcand[lane] : 0.0f;
-        for (uint32_t c = 0; c < ncons; ++c) {
-            const HotConstraint& hk = e.hot[c];
-            float rel[3], jm[9];
+        // Jacobian / response rows of constraint c for this DoF lane; the
+        // loops below fetch constraint c + 1 while working on c. Rows of
+        // constraints without an articulation endpoint are never written and
+        // their (unused) values are ignored.
+        auto load_rows = [&](uint32_t c, float* j3, float* r3) {
+            if (!dof_lane) return;
             const float* jb = e.jrow + (size_t)c * 3u * mnv + lane;
             const float* rb = e.resp + (size_t)c * 3u * mnv + lane;
-            float jr[3] = {0.0f, 0.0f, 0.0f}, rr[3] = {0.0f, 0.0f, 0.0f};
-            if (hk.art != 0u && dof_lane) {
 #pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    jr[r] = jb[(size_t)r * mnv];
-                    rr[r] = rb[(size_t)r * mnv];
-                }
+            for (int r = 0; r < 3; ++r) {
+                j3[r] = jb[(size_t)r * mnv];
+                r3[r] = rb[(size_t)r * mnv];
             }
+        };
+        float nj[3] = {0.0f, 0.0f, 0.0f}, nr[3] = {0.0f, 0.0f, 0.0f};
+        if (ncons > 0u) load_rows(0u, nj, nr);
+        for (uint32_t c = 0; c < ncons; ++c) {
+            const float jr[3] = {nj[0], nj[1], nj[2]};
+            const float rr[3] = {nr[0], nr[1], nr[2]};
+            if (c + 1u < ncons) load_rows(c + 1u, nj, nr);
+            const HotConstraint& hk = e.hot[c];
+            float rel[3], jm[9];
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
                 const float a = hk.art != 0u ? wave_fold(dof_lane ? jr[r] * vj : 0.0f) : 0.0f;
@@ -2243,17 +2336,6 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         float sweep_delta = 0.0f;
         uint32_t iters = 0;
         bool solve_failed = false;
-        float nj[3] = {0.0f, 0.0f, 0.0f}, nr[3] = {0.0f, 0.0f, 0.0f};
-        auto load_rows = [&](uint32_t c, float* j3, float* r3) {
-            if (!dof_lane) return;
-            const float* jb = e.jrow + (size_t)c * 3u * mnv + lane;
-            const float* rb = e.resp + (size_t)c * 3u * mnv + lane;
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                j3[r] = jb[(size_t)r * mnv];
-                r3[r] = rb[(size_t)r * mnv];
-            }
-        };
         if (ncons > 0u) load_rows(0u, nj, nr);
         for (uint32_t it = 0; it < max_iters && ncons > 0u; ++it) {
             sweep_delta = 0.0f;
@@ -2307,10 +2389,11 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         if (s_fail != ~0ull) break;
         // ------------------------------------- residuals + impulse store
         float max_normal = 0.0f, max_cone = 0.0f;
+        float nj0 = (ncons > 0u && dof_lane) ? e.jrow[lane] : 0.0f;
         for (uint32_t c = 0; c < ncons; ++c) {
             const HotConstraint& hk = e.hot[c];
-            float j0 = 0.0f;
-            if (hk.art != 0u && dof_lane) j0 = e.jrow[(size_t)c * 3u * mnv + lane];
+            const float j0 = nj0;
+            if (c + 1u < ncons && dof_lane) nj0 = e.jrow[(size_t)(c + 1u) * 3u * mnv + lane];
             const float a0 = hk.art != 0u ? wave_fold(dof_lane ? j0 * vj : 0.0f) : 0.0f;
             const float rel0 = relative_from(e, hk, 0, a0);
             const float eq = (rel0 - hk.target[0]) + hk.reg[0] * hk.lambda[0];

@@ -139,6 +139,27 @@ void World::read_step_milliseconds(double* out, uint32_t steps) const {
     }
 }
 
+// Per-lane response scratch every model needs: outward slots (same plan as
+// the kernel) and the inward joint-term stack, bounded by the largest number
+// of DoF links on the union of two root paths.
+void World::response_scratch_bounds(uint32_t& slots, uint32_t& stack) const {
+    for (const MRModelHeaderC& h : model_.models) {
+        const MRBodyC* b = model_.bodies.data() + h.body_offset;
+        const uint32_t n = h.link_count;
+        uint8_t plan[MR_MAX_BODIES];
+        slots = std::max(slots, plan_response_slots([&](uint32_t i) { return b[i].parent; }, n, plan));
+        unsigned long long anc[MR_MAX_BODIES] = {};
+        unsigned long long dof_links = 0ull;
+        for (uint32_t i = 1; i < n; ++i) {
+            anc[i] = anc[b[i].parent] | (1ull << i);
+            if (b[i].v_index != MR_INVALID_INDEX) dof_links |= 1ull << i;
+        }
+        for (uint32_t a = 0; a < n; ++a)
+            for (uint32_t c = a; c < n; ++c)
+                stack = std::max(stack, (uint32_t)__builtin_popcountll((anc[a] | anc[c]) & dof_links));
+    }
+}
+
 void World::compute_lds_layout() {
     uint32_t links = 1u, shapes = 1u;
     for (const MRModelHeaderC& h : model_.models) {
@@ -182,10 +203,18 @@ void World::compute_lds_layout() {
     L.acc = take(12u * links, 4u);
     L.L = take(4u * 36u, 4u);
     L.scene = take((uint32_t)sizeof(MRBodyStateC) * nscene, 16u);
+    const uint32_t collision_start = take(0u, 16u);
     L.col = take(kColliderBytes * shapes, 4u);
     L.active = take(4u * cfg_.pair_capacity, 4u);
     L.npts = take(4u * cfg_.pair_capacity, 4u);
     L.moff = take(4u * cfg_.pair_capacity, 4u);
+    const uint32_t collision_end = off;
+    off = collision_start;
+    uint32_t slots = 1u, stack = 1u;
+    response_scratch_bounds(slots, stack);
+    L.rslot = take(4u * 6u * 64u * slots, 16u);
+    L.rstack = take(4u * 64u * stack, 16u);
+    off = std::max(off, collision_end);
     L.hot = take(kHotBytes * cfg_.constraint_capacity, 4u);
     L.total = (off + 15u) / 16u * 16u;
     if (L.total > kMaxLdsBytes)

@@ -76,6 +76,7 @@ public:
 
 private:
     KernelArgs base_args(uint32_t step) const;
+    void response_scratch_bounds(uint32_t& slots, uint32_t& stack) const;
     void compute_lds_layout();
 
     ModelData model_;

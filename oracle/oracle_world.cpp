@@ -1523,14 +1523,18 @@ bool build_manifold(const StepContext& ctx, const Work& wk, uint32_t env, const 
 // ------------------------------------------------------------- cone solve
 
 // Dot product of two generalized vectors (length n <= 64) in a fixed
-// pairwise order: x[i] = a[i] * b[i] (+0 past n), then x[i] += x[i + d] for
-// d = 32, 16, ..., 1. This is the order of a 64-lane butterfly reduction, so
-// the device reproduces it bit for bit with one DoF per lane.
+// pairwise order: x[i] = a[i] * b[i] (+0 past n), then x[i] += x[i ^ d] for
+// d = 1, 2, 4, ..., 32, i.e. ((r0 + r1) + (r2 + r3)) over the four 16-lane
+// row sums, each row summed pairwise from adjacent lanes outwards. This is
+// the order of a 64-lane butterfly reduction, so the device reproduces it bit
+// for bit with one DoF per lane.
 float fold_dot(const float* a, const float* b, uint32_t n) {
-    float x[64];
+    float x[64], y[64];
     for (uint32_t i = 0; i < 64u; ++i) x[i] = i < n ? a[i] * b[i] : 0.0f;
-    for (uint32_t d = 32u; d >= 1u; d >>= 1)
-        for (uint32_t i = 0; i < d; ++i) x[i] = x[i] + x[i + d];
+    for (uint32_t d = 1u; d < 64u; d <<= 1) {
+        for (uint32_t i = 0; i < 64u; ++i) y[i] = x[i] + x[i ^ d];
+        for (uint32_t i = 0; i < 64u; ++i) x[i] = y[i];
+    }
     return x[0];
 }
 
