@@ -25,10 +25,17 @@ enum StageIndex : uint32_t {
     kStageCount = 16
 };
 
+// Endpoint kinds of a constraint, decoded once at emission.
+enum HotKind : uint32_t {
+    kHotArticulated = 1u, // an endpoint is an articulation link
+    kHotDynamicA = 2u,    // body_a is a dynamic scene body
+    kHotDynamicB = 4u,    // body_b is a dynamic scene body
+};
+
 // Per-constraint data kept resident in LDS during the cone solve.
 struct HotConstraint {
     uint32_t body_a, body_b;
-    uint32_t flags, art; // art: 1 if an endpoint is an articulation link
+    uint32_t flags, kind; // kind: HotKind bits
     float dir[9];        // normal, tangent u, tangent v
     float point[3];      // contact midpoint
     float target[3];

@@ -1586,17 +1586,20 @@ MRD_INL void apply_scene_impulse(MRBodyStateC& b, V3 x, V3 imp) {
 
 MRD_INL V3 hot_dir(const HotConstraint& h, int r) { return V3{h.dir[3 * r], h.dir[3 * r + 1], h.dir[3 * r + 2]}; }
 
+// Point velocities of a constraint's scene endpoints at the contact point
+// (zero for links and static bodies).
+MRD_INL void scene_point_velocities(const Env& e, uint32_t kind, uint32_t ba, uint32_t bb, V3 x, V3& va, V3& vb) {
+    const V3 zero{0.0f, 0.0f, 0.0f};
+    va = (kind & mr::kHotDynamicA) != 0u ? scene_point_velocity(e.scene[ba - e.nlinks], x) : zero;
+    vb = (kind & mr::kHotDynamicB) != 0u ? scene_point_velocity(e.scene[bb - e.nlinks], x) : zero;
+}
+
 // Relative velocity of one constraint row: the articulation part `art_dot`
 // (J v, already folded across the DoF lanes) plus the scene-body point
 // velocities, combined in the oracle's order.
-MRD_INL float relative_from(const Env& e, const HotConstraint& k, int row, float art_dot) {
-    const V3 x{k.point[0], k.point[1], k.point[2]};
-    const V3 d = hot_dir(k, row);
-    const V3 zero{0.0f, 0.0f, 0.0f};
-    const V3 va = is_dynamic_scene(e, k.body_a) ? scene_point_velocity(e.scene[k.body_a - e.nlinks], x) : zero;
-    const V3 vb = is_dynamic_scene(e, k.body_b) ? scene_point_velocity(e.scene[k.body_b - e.nlinks], x) : zero;
-    return (art_dot + dot(d, vb)) - dot(d, va);
-}
+MRD_INL float rel_row(V3 d, V3 va, V3 vb, float art_dot) { return (art_dot + dot(d, vb)) - dot(d, va); }
+
+MRD_INL uint32_t uniform_u32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
 struct StageClock {
     unsigned long long* out;
@@ -2140,7 +2143,9 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                 hk.body_a = sa.body;
                 hk.body_b = sb.body;
                 hk.flags = mp.feature_and_life[2] == 0u ? MR_CONSTRAINT_FLAG_NEW_IMPACT : MR_CONSTRAINT_FLAG_WARM_STARTED;
-                hk.art = (is_link(e, sa.body) || is_link(e, sb.body)) ? 1u : 0u;
+                hk.kind = ((is_link(e, sa.body) || is_link(e, sb.body)) ? mr::kHotArticulated : 0u) |
+                          (is_dynamic_scene(e, sa.body) ? mr::kHotDynamicA : 0u) |
+                          (is_dynamic_scene(e, sb.body) ? mr::kHotDynamicB : 0u);
                 const V3 pa = add(pa_pos, qrot(ra, f3(mp.local_anchor_a)));
                 const V3 pb = add(pb_pos, qrot(rb, f3(mp.local_anchor_b)));
                 const V3 pw = mul(0.5f, add(pa, pb));
@@ -2181,7 +2186,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
             const uint32_t c = item / 3u;
             const int row = (int)(item - 3u * c);
             const HotConstraint& hk = e.hot[c];
-            if (hk.art == 0u) continue;
+            if ((hk.kind & mr::kHotArticulated) == 0u) continue;
             const ColdConstraint& ck = e.cold[c];
             const bool la = is_link(e, hk.body_a), lb = is_link(e, hk.body_b);
             const V3 pa{ck.pa[0], ck.pa[1], ck.pa[2]};
@@ -2226,18 +2231,23 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
             const float jr[3] = {nj[0], nj[1], nj[2]};
             const float rr[3] = {nr[0], nr[1], nr[2]};
             if (c + 1u < ncons) load_rows(c + 1u, nj, nr);
-            const HotConstraint& hk = e.hot[c];
+            const HotConstraint hk = e.hot[c];
+            const uint32_t kind = uniform_u32(hk.kind);
+            const bool art = (kind & mr::kHotArticulated) != 0u;
+            V3 va, vb;
+            scene_point_velocities(e, kind, uniform_u32(hk.body_a), uniform_u32(hk.body_b),
+                                   V3{hk.point[0], hk.point[1], hk.point[2]}, va, vb);
             float rel[3], jm[9];
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
-                const float a = hk.art != 0u ? wave_fold(dof_lane ? jr[r] * vj : 0.0f) : 0.0f;
-                rel[r] = relative_from(e, hk, r, a);
+                const float a = art ? wave_fold(dof_lane ? jr[r] * vj : 0.0f) : 0.0f;
+                rel[r] = rel_row(hot_dir(hk, r), va, vb, a);
             }
 #pragma unroll
             for (int r = 0; r < 3; ++r)
 #pragma unroll
                 for (int q = 0; q < 3; ++q)
-                    jm[3 * r + q] = hk.art != 0u ? wave_fold(dof_lane ? jr[r] * rr[q] : 0.0f) : 0.0f;
+                    jm[3 * r + q] = art ? wave_fold(dof_lane ? jr[r] * rr[q] : 0.0f) : 0.0f;
             if (lane == 0u) {
                 HotConstraint& hw = e.hot[c];
                 hw.target[0] = rel[0];
@@ -2282,7 +2292,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
             }
             if (ok) {
                 const V3 x{hk.point[0], hk.point[1], hk.point[2]};
-                const bool sa_dyn = is_dynamic_scene(e, hk.body_a), sb_dyn = is_dynamic_scene(e, hk.body_b);
+                const bool sa_dyn = (hk.kind & mr::kHotDynamicA) != 0u, sb_dyn = (hk.kind & mr::kHotDynamicB) != 0u;
                 float W[3][3];
                 for (int r = 0; r < 3; ++r)
                     for (int q = 0; q < 3; ++q) {
@@ -2305,7 +2315,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         // scene lanes apply theirs in the same order.
         for (uint32_t c = 0; c < ncons; ++c) {
             const HotConstraint& hk = e.hot[c];
-            if (hk.art == 0u || !dof_lane) continue;
+            if ((hk.kind & mr::kHotArticulated) == 0u || !dof_lane) continue;
             const float* rb = e.resp + (size_t)c * 3u * mnv + lane;
             vj += (rb[0] * hk.lambda[0] + rb[mnv] * hk.lambda[1]) + rb[2u * mnv] * hk.lambda[2];
         }
@@ -2343,13 +2353,19 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                 const float jr[3] = {nj[0], nj[1], nj[2]};
                 const float rr[3] = {nr[0], nr[1], nr[2]};
                 load_rows(c + 1u < ncons ? c + 1u : 0u, nj, nr);
-                const HotConstraint& hk = e.hot[c];
-                const bool art = hk.art != 0u;
+                const HotConstraint hk = e.hot[c];
+                const uint32_t kind = uniform_u32(hk.kind);
+                const uint32_t ba = uniform_u32(hk.body_a), bb = uniform_u32(hk.body_b);
+                const bool art = (kind & mr::kHotArticulated) != 0u;
+                const bool sa_dyn = (kind & mr::kHotDynamicA) != 0u, sb_dyn = (kind & mr::kHotDynamicB) != 0u;
+                const V3 x{hk.point[0], hk.point[1], hk.point[2]};
+                V3 va, vb;
+                scene_point_velocities(e, kind, ba, bb, x, va, vb);
                 float rel[3];
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
                     const float a = art ? wave_fold(dof_lane ? jr[r] * vj : 0.0f) : 0.0f;
-                    rel[r] = relative_from(e, hk, r, a);
+                    rel[r] = rel_row(hot_dir(hk, r), va, vb, a);
                 }
                 float rhs[3];
                 for (int r = 0; r < 3; ++r) rhs[r] = (hk.target[r] - rel[r]) - hk.reg[r] * hk.lambda[r];
@@ -2365,19 +2381,22 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                 }
                 sweep_delta = fmaxd(sweep_delta, fmaxd(fabs_(d0), fmaxd(fabs_(d1), fabs_(d2))));
                 if (art && dof_lane) vj += (rr[0] * d0 + rr[1] * d1) + rr[2] * d2;
-                const bool sa_dyn = is_dynamic_scene(e, hk.body_a), sb_dyn = is_dynamic_scene(e, hk.body_b);
-                const V3 x{hk.point[0], hk.point[1], hk.point[2]};
-                const V3 imp = add(add(mul(d0, hot_dir(hk, 0)), mul(d1, hot_dir(hk, 1))), mul(d2, hot_dir(hk, 2)));
-                __syncthreads();  // every lane has read hk and the scene bodies
+                // The workgroup is one wavefront and its LDS operations
+                // complete in issue order, so lane 0's writes below follow
+                // every lane's reads above and precede the next constraint's
+                // reads; the wave barrier only stops the compiler from
+                // moving memory operations across these points.
+                __builtin_amdgcn_wave_barrier();
                 if (lane == 0u) {
-                    if (sa_dyn) apply_scene_impulse(e.scene[hk.body_a - nl], x, neg(imp));
-                    if (sb_dyn) apply_scene_impulse(e.scene[hk.body_b - nl], x, imp);
+                    const V3 imp = add(add(mul(d0, hot_dir(hk, 0)), mul(d1, hot_dir(hk, 1))), mul(d2, hot_dir(hk, 2)));
+                    if (sa_dyn) apply_scene_impulse(e.scene[ba - nl], x, neg(imp));
+                    if (sb_dyn) apply_scene_impulse(e.scene[bb - nl], x, imp);
                     HotConstraint& hw = e.hot[c];
                     hw.lambda[0] = cand[0];
                     hw.lambda[1] = cand[1];
                     hw.lambda[2] = cand[2];
                 }
-                if (sa_dyn || sb_dyn) __syncthreads();
+                __builtin_amdgcn_wave_barrier();
             }
             if (solve_failed) break;
             ++iters;
@@ -2391,11 +2410,15 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         float max_normal = 0.0f, max_cone = 0.0f;
         float nj0 = (ncons > 0u && dof_lane) ? e.jrow[lane] : 0.0f;
         for (uint32_t c = 0; c < ncons; ++c) {
-            const HotConstraint& hk = e.hot[c];
+            const HotConstraint hk = e.hot[c];
             const float j0 = nj0;
             if (c + 1u < ncons && dof_lane) nj0 = e.jrow[(size_t)(c + 1u) * 3u * mnv + lane];
-            const float a0 = hk.art != 0u ? wave_fold(dof_lane ? j0 * vj : 0.0f) : 0.0f;
-            const float rel0 = relative_from(e, hk, 0, a0);
+            const uint32_t kind = uniform_u32(hk.kind);
+            V3 va, vb;
+            scene_point_velocities(e, kind, uniform_u32(hk.body_a), uniform_u32(hk.body_b),
+                                   V3{hk.point[0], hk.point[1], hk.point[2]}, va, vb);
+            const float a0 = (kind & mr::kHotArticulated) != 0u ? wave_fold(dof_lane ? j0 * vj : 0.0f) : 0.0f;
+            const float rel0 = rel_row(hot_dir(hk, 0), va, vb, a0);
             const float eq = (rel0 - hk.target[0]) + hk.reg[0] * hk.lambda[0];
             const float nres = hk.lambda[0] > kConeEpsilon ? fabs_(eq) : fmaxd(-eq, 0.0f);
             max_normal = fmaxd(max_normal, nres);
