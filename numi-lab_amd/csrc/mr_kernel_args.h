@@ -32,7 +32,8 @@ enum HotKind : uint32_t {
     kHotDynamicB = 4u,    // body_b is a dynamic scene body
 };
 
-// Per-constraint data kept resident in LDS during the cone solve.
+// Per-constraint data of the cone solve (global scratch, read-only during
+// the sweeps); the impulses themselves live in LDS.
 struct HotConstraint {
     uint32_t body_a, body_b;
     uint32_t flags, kind; // kind: HotKind bits
@@ -41,8 +42,8 @@ struct HotConstraint {
     float target[3];
     float reg[3];
     float mu_u, mu_v;
-    float lambda[3];
     float winv[9];
+    float pad[3];
 };
 
 // Per-constraint data produced by emission and consumed by the Jacobian and
@@ -111,7 +112,9 @@ struct LdsLayout {
     uint32_t q, v, vcand, effort, accel, cmd;
     uint32_t pos, rot, w, vl, sa, sl, ca, cl, r;
     uint32_t I, pt, pb, Ut, Ub, D, u, alpha, acc, L;
-    uint32_t scene, col, active, npts, moff, hot;
+    uint32_t scene, col, active, npts, moff, okey;
+    uint32_t lam;    // [constraint_capacity][3] floats: cone impulses
+    uint32_t cpair;  // [constraint_capacity]: body_a | body_b << 16
     // response scratch, overlaying the collision arrays (dead after emission)
     uint32_t rslot;  // [slots][6][64] floats: outward spatial accelerations
     uint32_t rstack; // [stack][64] floats: joint terms of the inward path
@@ -162,6 +165,7 @@ struct KernelArgs {
     float* jrow;                  // [E][constraint_capacity][3][max_nv]
     float* resp;                  // [E][constraint_capacity][3][max_nv]
     ColdConstraint* cold;         // [E][constraint_capacity]
+    HotConstraint* hot;           // [E][constraint_capacity]
     unsigned long long* stage_cycles; // [E][kStageCount] shader-clock cycles per stage, or null
     LdsLayout lds;
 };

@@ -185,8 +185,10 @@ struct Env {
     float *D, *u, *L;
     MRBodyStateC* scene;
     Collider* col;
-    uint32_t *active, *npts, *moff;
-    HotConstraint* hot;
+    uint32_t *active, *npts, *moff, *okey;
+    HotConstraint* hot;  // global scratch
+    float* lam;          // LDS [ccap][3]
+    uint32_t* cpair;     // LDS [ccap]
     float *rslot, *rstack;
     // global scratch of this environment
     MRManifoldHeaderC* stage_h;
@@ -1626,7 +1628,11 @@ struct StageClock {
 
 // --------------------------------------------------------------- kernel
 
-extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelArgs args) {
+// Two wavefronts per SIMD: the register budget is capped at 256 so that two
+// worlds share each SIMD and hide each other's memory and LDS latency (the
+// per-world stages are latency-bound; one wave per SIMD left it idle).
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+mr_control_step_kernel(KernelArgs args) {
     extern __shared__ __align__(16) unsigned char lds_base[];
     __shared__ unsigned long long s_fail;
     __shared__ uint32_t s_count[8];
@@ -1691,7 +1697,9 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
     e.active = lds_at<uint32_t>(lds_base, L.active);
     e.npts = lds_at<uint32_t>(lds_base, L.npts);
     e.moff = lds_at<uint32_t>(lds_base, L.moff);
-    e.hot = lds_at<HotConstraint>(lds_base, L.hot);
+    e.okey = lds_at<uint32_t>(lds_base, L.okey);
+    e.lam = lds_at<float>(lds_base, L.lam);
+    e.cpair = lds_at<uint32_t>(lds_base, L.cpair);
     e.rslot = lds_at<float>(lds_base, L.rslot);
     e.rstack = lds_at<float>(lds_base, L.rstack);
     const uint32_t pcap = A.pair_capacity, mcap = A.manifold_capacity, ccap = A.constraint_capacity;
@@ -1700,6 +1708,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
     e.jrow = A.jrow + (size_t)env * ccap * 3u * A.max_nv;
     e.resp = A.resp + (size_t)env * ccap * 3u * A.max_nv;
     e.cold = A.cold + (size_t)env * ccap;
+    e.hot = A.hot + (size_t)env * ccap;
 
     // ---- topology cache (one lane per body / DoF; 64 lanes cover the limits)
     {
@@ -2027,6 +2036,12 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         const MRManifoldPointC* old_p = A.mp + ((size_t)src_buf * E + env) * mcap * MR_MANIFOLD_POINT_CAPACITY;
         uint32_t my_raw = 0, my_ret = 0, my_fresh = 0;
         float my_pen = 0.0f;
+        // collider keys of the previous manifolds, staged once for the lookups
+        for (uint32_t m = lane; m < mcount_src; m += kLanes) {
+            const uint32_t ka = old_h[m].pair_and_count[1], kb = old_h[m].pair_and_count[2];
+            e.okey[m] = (ka < 0x10000u && kb < 0x10000u) ? (ka | (kb << 16)) : 0xffffffffu;
+        }
+        __syncthreads();
         for (uint32_t k = lane; k < nact; k += kLanes) {
             const MRCompiledPairC pr = e.pairs[e.active[k]];
             const Collider ca_ = e.col[pr.collider_a];
@@ -2047,8 +2062,9 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                 for (uint32_t c = 0; c < raw.count; ++c) my_pen = fmaxd(my_pen, fmaxd(rest - raw.c[c].separation, 0.0f));
                 const MRManifoldHeaderC* oh = nullptr;
                 const MRManifoldPointC* op = nullptr;
+                const uint32_t key = pr.collider_a | (pr.collider_b << 16);
                 for (uint32_t m = 0; m < mcount_src; ++m) {
-                    if (old_h[m].pair_and_count[1] == pr.collider_a && old_h[m].pair_and_count[2] == pr.collider_b) {
+                    if (e.okey[m] == key) {
                         oh = &old_h[m];
                         op = &old_p[(size_t)m * MR_MANIFOLD_POINT_CAPACITY];
                         break;
@@ -2166,15 +2182,11 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                 ck.restitution_threshold = fmaxd(ma.response.y, mb.response.y);
                 ck.compliance = ma.response.z + mb.response.z;
                 ck.dissipation = ma.response.w + mb.response.w;
-                if (cfg.warm_start != 0u) {
-                    hk.lambda[0] = mp.impulses.x;
-                    hk.lambda[1] = mp.impulses.y;
-                    hk.lambda[2] = mp.impulses.z;
-                } else {
-                    hk.lambda[0] = 0.0f;
-                    hk.lambda[1] = 0.0f;
-                    hk.lambda[2] = 0.0f;
-                }
+                const bool warm = cfg.warm_start != 0u;
+                e.lam[3u * c + 0u] = warm ? mp.impulses.x : 0.0f;
+                e.lam[3u * c + 1u] = warm ? mp.impulses.y : 0.0f;
+                e.lam[3u * c + 2u] = warm ? mp.impulses.z : 0.0f;
+                e.cpair[c] = sa.body | (sb.body << 16);
                 e.hot[c] = hk;
                 e.cold[c] = ck;
             }
@@ -2226,12 +2238,19 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
             }
         };
         float nj[3] = {0.0f, 0.0f, 0.0f}, nr[3] = {0.0f, 0.0f, 0.0f};
-        if (ncons > 0u) load_rows(0u, nj, nr);
+        HotConstraint hnext;
+        if (ncons > 0u) {
+            load_rows(0u, nj, nr);
+            hnext = e.hot[0];
+        }
         for (uint32_t c = 0; c < ncons; ++c) {
             const float jr[3] = {nj[0], nj[1], nj[2]};
             const float rr[3] = {nr[0], nr[1], nr[2]};
-            if (c + 1u < ncons) load_rows(c + 1u, nj, nr);
-            const HotConstraint hk = e.hot[c];
+            const HotConstraint hk = hnext;
+            if (c + 1u < ncons) {
+                load_rows(c + 1u, nj, nr);
+                hnext = e.hot[c + 1u];
+            }
             const uint32_t kind = uniform_u32(hk.kind);
             const bool art = (kind & mr::kHotArticulated) != 0u;
             V3 va, vb;
@@ -2304,7 +2323,11 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                     }
                 if (!invert3x3(W, hk.winv)) report_failure(&s_fail, c, MR_STEP_FACTORIZATION_FAILED, c);
             }
-            project_cone(hk.lambda, hk.mu_u, hk.mu_v);
+            float lam[3] = {e.lam[3u * c], e.lam[3u * c + 1u], e.lam[3u * c + 2u]};
+            project_cone(lam, hk.mu_u, hk.mu_v);
+            e.lam[3u * c] = lam[0];
+            e.lam[3u * c + 1u] = lam[1];
+            e.lam[3u * c + 2u] = lam[2];
             e.hot[c] = hk;
         }
         __syncthreads();
@@ -2314,20 +2337,22 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         // DoF lanes accumulate the warm-start impulses in constraint order;
         // scene lanes apply theirs in the same order.
         for (uint32_t c = 0; c < ncons; ++c) {
-            const HotConstraint& hk = e.hot[c];
-            if ((hk.kind & mr::kHotArticulated) == 0u || !dof_lane) continue;
+            const uint32_t kind = uniform_u32(e.hot[c].kind);
+            if ((kind & mr::kHotArticulated) == 0u || !dof_lane) continue;
             const float* rb = e.resp + (size_t)c * 3u * mnv + lane;
-            vj += (rb[0] * hk.lambda[0] + rb[mnv] * hk.lambda[1]) + rb[2u * mnv] * hk.lambda[2];
+            const float* lc = e.lam + 3u * c;
+            vj += (rb[0] * lc[0] + rb[mnv] * lc[1]) + rb[2u * mnv] * lc[2];
         }
         for (uint32_t s = lane; s < nsc; s += kLanes) {
             const uint32_t b = nl + s;
             if (!is_dynamic_scene(e, b)) continue;
             MRBodyStateC st = e.scene[s];
             for (uint32_t c = 0; c < ncons; ++c) {
+                const uint32_t pr = e.cpair[c];
+                if ((pr & 0xffffu) != b && (pr >> 16) != b) continue;
                 const HotConstraint& hk = e.hot[c];
-                if (hk.body_a != b && hk.body_b != b) continue;
-                const V3 imp = add(add(mul(hk.lambda[0], hot_dir(hk, 0)), mul(hk.lambda[1], hot_dir(hk, 1))),
-                                   mul(hk.lambda[2], hot_dir(hk, 2)));
+                const float* lc = e.lam + 3u * c;
+                const V3 imp = add(add(mul(lc[0], hot_dir(hk, 0)), mul(lc[1], hot_dir(hk, 1))), mul(lc[2], hot_dir(hk, 2)));
                 const V3 x{hk.point[0], hk.point[1], hk.point[2]};
                 if (hk.body_a == b) apply_scene_impulse(st, x, neg(imp));
                 if (hk.body_b == b) apply_scene_impulse(st, x, imp);
@@ -2346,14 +2371,20 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         float sweep_delta = 0.0f;
         uint32_t iters = 0;
         bool solve_failed = false;
-        if (ncons > 0u) load_rows(0u, nj, nr);
+        if (ncons > 0u) {
+            load_rows(0u, nj, nr);
+            hnext = e.hot[0];
+        }
         for (uint32_t it = 0; it < max_iters && ncons > 0u; ++it) {
             sweep_delta = 0.0f;
             for (uint32_t c = 0; c < ncons; ++c) {
                 const float jr[3] = {nj[0], nj[1], nj[2]};
                 const float rr[3] = {nr[0], nr[1], nr[2]};
-                load_rows(c + 1u < ncons ? c + 1u : 0u, nj, nr);
-                const HotConstraint hk = e.hot[c];
+                const HotConstraint hk = hnext;
+                const uint32_t cn = c + 1u < ncons ? c + 1u : 0u;
+                load_rows(cn, nj, nr);
+                hnext = e.hot[cn];
+                const float lam0 = e.lam[3u * c], lam1 = e.lam[3u * c + 1u], lam2 = e.lam[3u * c + 2u];
                 const uint32_t kind = uniform_u32(hk.kind);
                 const uint32_t ba = uniform_u32(hk.body_a), bb = uniform_u32(hk.body_b);
                 const bool art = (kind & mr::kHotArticulated) != 0u;
@@ -2368,12 +2399,13 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                     rel[r] = rel_row(hot_dir(hk, r), va, vb, a);
                 }
                 float rhs[3];
-                for (int r = 0; r < 3; ++r) rhs[r] = (hk.target[r] - rel[r]) - hk.reg[r] * hk.lambda[r];
-                float cand[3] = {hk.lambda[0], hk.lambda[1], hk.lambda[2]};
+                const float lamc[3] = {lam0, lam1, lam2};
+                for (int r = 0; r < 3; ++r) rhs[r] = (hk.target[r] - rel[r]) - hk.reg[r] * lamc[r];
+                float cand[3] = {lam0, lam1, lam2};
                 for (int r = 0; r < 3; ++r)
                     for (int q = 0; q < 3; ++q) cand[r] += hk.winv[3 * r + q] * rhs[q];
                 project_cone(cand, hk.mu_u, hk.mu_v);
-                const float d0 = cand[0] - hk.lambda[0], d1 = cand[1] - hk.lambda[1], d2 = cand[2] - hk.lambda[2];
+                const float d0 = cand[0] - lam0, d1 = cand[1] - lam1, d2 = cand[2] - lam2;
                 if (!isfin(cand[0]) || !isfin(cand[1]) || !isfin(cand[2]) || !isfin(d0) || !isfin(d1) || !isfin(d2)) {
                     if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_NONFINITE_RESULT << 24) | (c & 0xffffffu);
                     solve_failed = true;
@@ -2391,10 +2423,9 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                     const V3 imp = add(add(mul(d0, hot_dir(hk, 0)), mul(d1, hot_dir(hk, 1))), mul(d2, hot_dir(hk, 2)));
                     if (sa_dyn) apply_scene_impulse(e.scene[ba - nl], x, neg(imp));
                     if (sb_dyn) apply_scene_impulse(e.scene[bb - nl], x, imp);
-                    HotConstraint& hw = e.hot[c];
-                    hw.lambda[0] = cand[0];
-                    hw.lambda[1] = cand[1];
-                    hw.lambda[2] = cand[2];
+                    e.lam[3u * c] = cand[0];
+                    e.lam[3u * c + 1u] = cand[1];
+                    e.lam[3u * c + 2u] = cand[2];
                 }
                 __builtin_amdgcn_wave_barrier();
             }
@@ -2411,6 +2442,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
         float nj0 = (ncons > 0u && dof_lane) ? e.jrow[lane] : 0.0f;
         for (uint32_t c = 0; c < ncons; ++c) {
             const HotConstraint hk = e.hot[c];
+            const float lamr[3] = {e.lam[3u * c], e.lam[3u * c + 1u], e.lam[3u * c + 2u]};
             const float j0 = nj0;
             if (c + 1u < ncons && dof_lane) nj0 = e.jrow[(size_t)(c + 1u) * 3u * mnv + lane];
             const uint32_t kind = uniform_u32(hk.kind);
@@ -2419,21 +2451,20 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                                    V3{hk.point[0], hk.point[1], hk.point[2]}, va, vb);
             const float a0 = (kind & mr::kHotArticulated) != 0u ? wave_fold(dof_lane ? j0 * vj : 0.0f) : 0.0f;
             const float rel0 = rel_row(hot_dir(hk, 0), va, vb, a0);
-            const float eq = (rel0 - hk.target[0]) + hk.reg[0] * hk.lambda[0];
-            const float nres = hk.lambda[0] > kConeEpsilon ? fabs_(eq) : fmaxd(-eq, 0.0f);
+            const float eq = (rel0 - hk.target[0]) + hk.reg[0] * lamr[0];
+            const float nres = lamr[0] > kConeEpsilon ? fabs_(eq) : fmaxd(-eq, 0.0f);
             max_normal = fmaxd(max_normal, nres);
-            const float lu = hk.mu_u * hk.lambda[0], lv = hk.mu_v * hk.lambda[0];
+            const float lu = hk.mu_u * lamr[0], lv = hk.mu_v * lamr[0];
             float viol;
             if (lu > 0.0f && lv > 0.0f)
-                viol = fmaxd(sqrt_rn((hk.lambda[1] * hk.lambda[1]) / (lu * lu) + (hk.lambda[2] * hk.lambda[2]) / (lv * lv)) - 1.0f,
-                             0.0f);
+                viol = fmaxd(sqrt_rn((lamr[1] * lamr[1]) / (lu * lu) + (lamr[2] * lamr[2]) / (lv * lv)) - 1.0f, 0.0f);
             else
-                viol = sqrt_rn(hk.lambda[1] * hk.lambda[1] + hk.lambda[2] * hk.lambda[2]);
+                viol = sqrt_rn(lamr[1] * lamr[1] + lamr[2] * lamr[2]);
             max_cone = fmaxd(max_cone, viol);
             if (lane == (c & (kLanes - 1u))) {
                 const ColdConstraint& ck = e.cold[c];
                 new_p[(size_t)ck.manifold * MR_MANIFOLD_POINT_CAPACITY + ck.point].impulses =
-                    mr_float4{hk.lambda[0], hk.lambda[1], hk.lambda[2], 0.0f};
+                    mr_float4{lamr[0], lamr[1], lamr[2], 0.0f};
             }
         }
         if (dof_lane) e.vcand[lane] = vj;
@@ -2454,9 +2485,9 @@ extern "C" __global__ void __launch_bounds__(64) mr_control_step_kernel(KernelAr
                 used[i] = false;
             }
             for (uint32_t c = 0; c < ncons; ++c) {
-                const HotConstraint& hk = e.hot[c];
-                const uint32_t na = is_link(e, hk.body_a) ? 0u : (is_dynamic_scene(e, hk.body_a) ? 1u + (hk.body_a - nl) : MR_INVALID_INDEX);
-                const uint32_t nb = is_link(e, hk.body_b) ? 0u : (is_dynamic_scene(e, hk.body_b) ? 1u + (hk.body_b - nl) : MR_INVALID_INDEX);
+                const uint32_t ba = e.cpair[c] & 0xffffu, bb = e.cpair[c] >> 16;
+                const uint32_t na = is_link(e, ba) ? 0u : (is_dynamic_scene(e, ba) ? 1u + (ba - nl) : MR_INVALID_INDEX);
+                const uint32_t nb = is_link(e, bb) ? 0u : (is_dynamic_scene(e, bb) ? 1u + (bb - nl) : MR_INVALID_INDEX);
                 if (na != MR_INVALID_INDEX) used[na] = true;
                 if (nb != MR_INVALID_INDEX) used[nb] = true;
                 if (na != MR_INVALID_INDEX && nb != MR_INVALID_INDEX) {

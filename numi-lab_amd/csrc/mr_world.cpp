@@ -15,7 +15,6 @@ hipError_t configure_control_step(uint32_t lds_bytes);
 namespace {
 constexpr uint32_t kSpIBytes = 84u;
 constexpr uint32_t kColliderBytes = 124u;
-constexpr uint32_t kHotBytes = (uint32_t)sizeof(HotConstraint);
 constexpr uint32_t kMaxLdsBytes = 160u * 1024u;
 
 template <typename T>
@@ -92,6 +91,7 @@ World::World(const ModelData& model, const MRWorldStepConfigC& config) : model_(
     d_jrow_.reserve(E * ccap * 3u * model_.max_nv * sizeof(float));
     d_resp_.reserve(E * ccap * 3u * model_.max_nv * sizeof(float));
     d_cold_.reserve(E * ccap * sizeof(ColdConstraint));
+    d_hot_.reserve(E * ccap * sizeof(HotConstraint));
     d_accel_.reserve(E * model_.max_nv * sizeof(float) + 16u);
     hip_check(hipMemset(d_accel_.ptr, 0, d_accel_.bytes), "hipMemset");
     d_ev_pairs_.reserve(E * pcap * sizeof(MRPairRecordC));
@@ -176,6 +176,7 @@ void World::compute_lds_layout() {
         return at;
     };
     LdsLayout& L = lds_;
+    // arrays live for the whole microstep
     L.topo = take((uint32_t)sizeof(TopoCache), 16u);
     L.q = take(4u * nq, 4u);
     L.v = take(4u * nv, 4u);
@@ -185,37 +186,44 @@ void World::compute_lds_layout() {
     L.cmd = take(4u * nv, 4u);
     L.pos = take(12u * links, 4u);
     L.rot = take(16u * links, 4u);
-    L.w = take(12u * links, 4u);
-    L.vl = take(12u * links, 4u);
     L.sa = take(12u * links, 4u);
     L.sl = take(12u * links, 4u);
-    L.ca = take(12u * links, 4u);
-    L.cl = take(12u * links, 4u);
     L.r = take(12u * links, 4u);
-    L.I = take(kSpIBytes * links, 4u);
-    L.pt = take(12u * links, 4u);
-    L.pb = take(12u * links, 4u);
     L.Ut = take(12u * links, 4u);
     L.Ub = take(12u * links, 4u);
     L.D = take(4u * links, 4u);
+    L.L = take(4u * 36u, 4u);
+    L.scene = take((uint32_t)sizeof(MRBodyStateC) * nscene, 16u);
+    L.lam = take(12u * cfg_.constraint_capacity, 4u);
+    L.cpair = take(4u * cfg_.constraint_capacity, 4u);
+    // Three phase-local regions share one union: the kinematics / ABA
+    // temporaries (dead once the accelerations exist), the collision arrays
+    // (dead after emission) and the response scratch.
+    const uint32_t union_start = take(0u, 16u);
+    L.w = take(12u * links, 4u);
+    L.vl = take(12u * links, 4u);
+    L.ca = take(12u * links, 4u);
+    L.cl = take(12u * links, 4u);
+    L.I = take(kSpIBytes * links, 4u);
+    L.pt = take(12u * links, 4u);
+    L.pb = take(12u * links, 4u);
     L.u = take(4u * links, 4u);
     L.alpha = take(12u * links, 4u);
     L.acc = take(12u * links, 4u);
-    L.L = take(4u * 36u, 4u);
-    L.scene = take((uint32_t)sizeof(MRBodyStateC) * nscene, 16u);
-    const uint32_t collision_start = take(0u, 16u);
+    uint32_t union_end = off;
+    off = union_start;
     L.col = take(kColliderBytes * shapes, 4u);
     L.active = take(4u * cfg_.pair_capacity, 4u);
     L.npts = take(4u * cfg_.pair_capacity, 4u);
     L.moff = take(4u * cfg_.pair_capacity, 4u);
-    const uint32_t collision_end = off;
-    off = collision_start;
+    L.okey = take(4u * cfg_.manifold_capacity, 4u);
+    union_end = std::max(union_end, off);
+    off = union_start;
     uint32_t slots = 1u, stack = 1u;
     response_scratch_bounds(slots, stack);
     L.rslot = take(4u * 6u * 64u * slots, 16u);
     L.rstack = take(4u * 64u * stack, 16u);
-    off = std::max(off, collision_end);
-    L.hot = take(kHotBytes * cfg_.constraint_capacity, 4u);
+    off = std::max(union_end, off);
     L.total = (off + 15u) / 16u * 16u;
     if (L.total > kMaxLdsBytes)
         throw std::runtime_error("world needs " + std::to_string(L.total) +
@@ -227,7 +235,7 @@ size_t World::retained_bytes() const {
                                  &d_pairs_, &d_heights_, &d_q_, &d_v_, &d_scene_, &d_mh_, &d_mp_, &d_mcount_,
                                  &d_mcurrent_, &d_reset_q_, &d_reset_v_, &d_reset_scene_, &d_commands_, &d_masks_,
                                  &d_status_, &d_cstatus_, &d_accel_, &d_ev_pairs_, &d_ev_counts_, &d_stage_h_,
-                                 &d_stage_p_, &d_jrow_, &d_resp_, &d_cold_};
+                                 &d_stage_p_, &d_jrow_, &d_resp_, &d_cold_, &d_hot_};
     size_t total = 0;
     for (const DeviceBuffer* b : all) total += b->bytes;
     return total;
@@ -345,6 +353,7 @@ KernelArgs World::base_args(uint32_t step) const {
     a.jrow = static_cast<float*>(d_jrow_.ptr);
     a.resp = static_cast<float*>(d_resp_.ptr);
     a.cold = static_cast<ColdConstraint*>(d_cold_.ptr);
+    a.hot = static_cast<HotConstraint*>(d_hot_.ptr);
     a.stage_cycles = stage_profiling_ ? static_cast<unsigned long long*>(d_stage_cycles_.ptr) : nullptr;
     a.lds = lds_;
     return a;

@@ -13,8 +13,8 @@
 //   collider projection    Collision.metal:321-600 (inflateFiniteBounds :110-120)
 //   broadphase flags       Collision.metal:796-858, 7905-7962
 //   narrowphase            Collision.metal:1003-1720, 4379-4830
-//   heightfield            first-principles vertex/plane test (no reference oracle
-//                          exists for heightfields, SURVEY hard part 4)
+//   heightfield            first-principles support-point / triangle-plane test
+//                          (the reference's CPU path has no heightfield oracle)
 //   manifold refresh       Collision.metal:5108-5530 (buildPersistentWorldManifold)
 //   constraint emission    Collision.metal:9180-9470
 //   IR evaluation          MetalWorldContact.metal:3800-3995
