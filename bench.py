@@ -76,6 +76,45 @@ def cpu_baseline(kind, seed, envs, steps, threads):
                       f"{threads} std::threads)", "seconds": round(dt, 2)}
 
 
+def residual_leg(mr, model, commands, args):
+    """The same workload with the adaptive cone solve: sweeps continue until
+    the final-sweep max |dlambda| of a microstep is at or below the tolerance
+    (capped at --residual-max-iterations sweeps). Reports throughput, control
+    step latency and how many microsteps reached the tolerance."""
+    import torch
+
+    cfg = model.default_config()
+    cfg.residual_tolerance = args.residual_tolerance
+    cfg.max_iterations = args.residual_max_iterations
+    world = mr.World(model, cfg)
+    world.set_step_timing(True)
+    warm = min(args.warmup, commands.shape[0])
+    if warm > 0:
+        world.enqueue_device(commands[:warm])
+        world.wait()
+    steps = min(args.residual_steps, commands.shape[0] - warm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    world.enqueue_device(commands[warm:warm + steps])
+    world.wait()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    step_ms = world.step_milliseconds()
+    cs = world.contact_statuses()
+    with_contacts = cs["constraints"] > 0
+    res = cs["residuals"][..., 0][with_contacts]
+    iters = cs["solver_iterations"][with_contacts]
+    return {
+        "tolerance": args.residual_tolerance, "max_iterations": args.residual_max_iterations, "steps": int(steps),
+        "env_steps_per_s": round(args.envs * steps / (t1 - t0), 1),
+        "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
+                       "p99": round(float(np.percentile(step_ms, 99)), 4)},
+        "final_microstep_iterations_mean": round(float(iters.mean()), 2) if iters.size else 0.0,
+        "final_microstep_at_tolerance": round(float(np.mean(res <= args.residual_tolerance)), 4) if res.size else 1.0,
+        "final_sweep_max_dlambda_p99": float(np.percentile(res, 99)) if res.size else 0.0,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +127,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=500)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--skip-cpu-baseline", action="store_true")
+    ap.add_argument("--residual-tolerance", type=float, default=1.0e-5,
+                    help="final-sweep max |dlambda| target of the adaptive leg (0 disables the leg)")
+    ap.add_argument("--residual-max-iterations", type=int, default=128)
+    ap.add_argument("--residual-steps", type=int, default=50)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the timed region, run one device-to-device copy of this many MiB (a kernel of "
+                         "known traffic that scales FETCH_SIZE / WRITE_SIZE counter readings)")
     args = ap.parse_args()
 
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
@@ -179,6 +225,16 @@ def main():
                      "kernel_ms_per_launch": round(avg_launch_s * 1e3, 4)},
         "cpu_baseline": None,
     }
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib * 1024 * 1024 // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.residual_tolerance > 0.0 and args.residual_steps > 0:
+        out["at_residual"] = residual_leg(mr, model, commands, args)
     if not args.skip_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(kind, args.seed, args.cpu_envs, args.cpu_steps, threads)

@@ -66,6 +66,8 @@ struct TopoCache {
     unsigned long long children[MR_MAX_BODIES]; // direct child links
     uint8_t parent[MR_MAX_BODIES];
     uint8_t vidx[MR_MAX_BODIES];
+    uint8_t qidx[MR_MAX_BODIES];
+    uint8_t jtype[MR_MAX_BODIES];
     uint8_t depth[MR_MAX_BODIES];
     uint8_t motion[MR_MAX_BODIES];              // motion type of every body
     uint8_t dof_body[MR_MAX_BODIES];            // owning link of every DoF
@@ -106,18 +108,30 @@ MR_HOST_DEVICE inline uint32_t plan_response_slots(ParentFn parent, uint32_t n, 
     return peak;
 }
 
+// Joint constants of one link for forward kinematics, staged in LDS once per
+// launch: parent-side joint rotation, conjugated child-side rotation,
+// normalized joint axis and the two anchors.
+struct LinkConst {
+    float prot[4];
+    float crot_conj[4];
+    float axis[3];
+    float panchor[3];
+    float canchor[3];
+};
+
 // Byte offsets of the per-workgroup dynamic LDS arrays.
 struct LdsLayout {
-    uint32_t topo;
+    uint32_t topo, lconst, arm;
     uint32_t q, v, vcand, effort, accel, cmd;
     uint32_t pos, rot, w, vl, sa, sl, ca, cl, r;
     uint32_t I, pt, pb, Ut, Ub, D, u, alpha, acc, L;
     uint32_t scene, col, active, npts, moff, okey;
     uint32_t lam;    // [constraint_capacity][3] floats: cone impulses
-    uint32_t cpair;  // [constraint_capacity]: body_a | body_b << 16
+    uint32_t cpair;  // [constraint_capacity]: body_a | body_b << 8 | kind << 16
     // response scratch, overlaying the collision arrays (dead after emission)
     uint32_t rslot;  // [slots][6][64] floats: outward spatial accelerations
     uint32_t rstack; // [stack][64] floats: joint terms of the inward path
+    uint32_t isl;    // [MR_MAX_BODIES + 1] u8 island parents + used flags (after responses)
     uint32_t total;
 };
 

@@ -177,6 +177,8 @@ struct Env {
     V3 gravity;
     // LDS
     mr::TopoCache* topo;
+    mr::LinkConst* lconst;
+    float* arm;  // effective armature per DoF (constant for the launch)
     float *q, *v, *vcand, *effort, *accel, *cmd;
     V3 *pos, *w, *vl, *sa, *sl, *ca, *cl, *r;
     Q4* rot;
@@ -188,8 +190,9 @@ struct Env {
     uint32_t *active, *npts, *moff, *okey;
     HotConstraint* hot;  // global scratch
     float* lam;          // LDS [ccap][3]
-    uint32_t* cpair;     // LDS [ccap]
+    uint32_t* cpair;     // LDS [ccap]: body_a | body_b << 8 | kind << 16
     float *rslot, *rstack;
+    uint8_t* isl;
     // global scratch of this environment
     MRManifoldHeaderC* stage_h;
     MRManifoldPointC* stage_p;
@@ -329,24 +332,27 @@ MRD_INL float drive_effort(const Env& e, uint32_t j) {
 
 // Forward kinematics of one non-root link from its parent (lane-local).
 MRD_INL bool fk_link(const Env& e, uint32_t i) {
-    const MRBodyC& b = e.bodies[i];
-    const uint32_t p = b.parent;
-    const Q4 pjr = qmul(e.rot[p], q4(b.parent_rotation));
-    const bool moving = b.joint_type == MR_JOINT_REVOLUTE || b.joint_type == MR_JOINT_CONTINUOUS;
-    const V3 axis_n = normalize(f3(b.axis));
+    const mr::LinkConst lc = e.lconst[i];
+    const uint32_t p = link_parent(e, i);
+    const uint32_t jt = e.topo->jtype[i];
+    const Q4 pjr = qmul(e.rot[p], Q4{lc.prot[0], lc.prot[1], lc.prot[2], lc.prot[3]});
+    const bool moving = jt == MR_JOINT_REVOLUTE || jt == MR_JOINT_CONTINUOUS;
+    const V3 axis_n{lc.axis[0], lc.axis[1], lc.axis[2]};
     float qi = 0.0f, qd = 0.0f;
     Q4 motion{0.0f, 0.0f, 0.0f, 1.0f};
     if (moving) {
-        qi = e.q[b.q_index];
-        qd = e.v[b.v_index];
+        qi = e.q[e.topo->qidx[i]];
+        qd = e.v[link_vidx(e, i)];
         motion = axis_angle(axis_n, qi);
     }
     Q4 child;
-    if (!normalized_quat(qmul(qmul(pjr, motion), qconj(q4(b.child_rotation))), child)) return false;
+    if (!normalized_quat(qmul(qmul(pjr, motion), Q4{lc.crot_conj[0], lc.crot_conj[1], lc.crot_conj[2], lc.crot_conj[3]}),
+                         child))
+        return false;
     e.rot[i] = child;
     const V3 axis_w = qrot(pjr, axis_n);
-    const V3 joint_pos = add(e.pos[p], qrot(e.rot[p], f3(b.parent_anchor)));
-    const V3 child_anchor = qrot(child, f3(b.child_anchor));
+    const V3 joint_pos = add(e.pos[p], qrot(e.rot[p], V3{lc.panchor[0], lc.panchor[1], lc.panchor[2]}));
+    const V3 child_anchor = qrot(child, V3{lc.canchor[0], lc.canchor[1], lc.canchor[2]});
     const V3 pos = sub(joint_pos, child_anchor);
     e.pos[i] = pos;
     e.r[i] = sub(pos, e.pos[p]);
@@ -451,7 +457,7 @@ MRD_INL bool aba_project_link(const Env& e, uint32_t i, float& pivot_out) {
         const V3 sa = e.sa[i], sl = e.sl[i];
         const V3 Ut = add(sym_mul(I.A, sa), mat_mul(I.B, sl));
         const V3 Ub = add(matT_mul(I.B, sa), sym_mul(I.C, sl));
-        const float D = (dot(sa, Ut) + dot(sl, Ub)) + effective_armature(e, b.v_index);
+        const float D = (dot(sa, Ut) + dot(sl, Ub)) + e.arm[b.v_index];
         const float floor_ = fmaxd(kAbsolutePivotFloor, spi_max_abs(I) * 6.0f * kFloatEps);
         if (!(D > floor_) || !isfin(D)) ok = false;
         const float u = (e.effort[b.v_index] - dot(sa, pt)) - dot(sl, pb);
@@ -1288,47 +1294,58 @@ MRD_INL V3 stable_tangent(V3 n) {
     return normalize(cross(ref, n));
 }
 
-MRD_INL void reduce_candidates(Candidate* c, uint32_t& count, V3 n) {
+// Candidate order as an index permutation packed in nibbles (count <= 12), so
+// sorting and selection move 4-bit indices instead of 84-byte records.
+MRD_INL uint32_t perm_get(unsigned long long p, uint32_t i) { return (uint32_t)(p >> (4u * i)) & 0xfu; }
+MRD_INL unsigned long long perm_set(unsigned long long p, uint32_t i, uint32_t v) {
+    return (p & ~(0xfull << (4u * i))) | ((unsigned long long)v << (4u * i));
+}
+
+// Deterministic reduction to at most four points (stable insertion sort by
+// separation then feature, greedy spread selection, final sort by feature);
+// returns the order of the kept candidates.
+MRD_INL unsigned long long reduce_candidates(const Candidate* c, uint32_t& count, V3 n) {
+    unsigned long long perm = 0ull;
+    for (uint32_t i = 0; i < count; ++i) perm = perm_set(perm, i, i);
     for (uint32_t i = 1; i < count; ++i) {
-        const Candidate val = c[i];
+        const uint32_t val = perm_get(perm, i);
         uint32_t d = i;
-        while (d > 0u && cand_less(val, c[d - 1u])) {
-            c[d] = c[d - 1u];
+        while (d > 0u && cand_less(c[val], c[perm_get(perm, d - 1u)])) {
+            perm = perm_set(perm, d, perm_get(perm, d - 1u));
             --d;
         }
-        c[d] = val;
+        perm = perm_set(perm, d, val);
     }
     if (count > MR_MANIFOLD_POINT_CAPACITY) {
-        bool selected[kMaxCandidates];
-        for (uint32_t i = 0; i < kMaxCandidates; ++i) selected[i] = false;
-        uint32_t chosen[MR_MANIFOLD_POINT_CAPACITY];
-        chosen[0] = 0u;
-        selected[0] = true;
+        uint32_t selected = 1u;  // bit i: sorted position i taken
+        uint32_t chosen = 0u;    // nibble k: sorted position of the k-th pick
         uint32_t nch = 1u;
         for (uint32_t stage = 1; stage < MR_MANIFOLD_POINT_CAPACITY; ++stage) {
             uint32_t best = MR_INVALID_INDEX;
             float best_score = -1.0f;
             unsigned long long best_feat = ~0ull;
+            const V3 w0 = c[perm_get(perm, chosen & 0xfu)].world_point;
             for (uint32_t i = 0; i < count; ++i) {
-                if (selected[i]) continue;
+                if ((selected >> i) & 1u) continue;
+                const Candidate& ci = c[perm_get(perm, i)];
                 float score;
                 if (stage == 1u) {
-                    const V3 d = sub(c[i].world_point, c[chosen[0]].world_point);
+                    const V3 d = sub(ci.world_point, w0);
                     const V3 t = sub(d, mul(dot(d, n), n));
                     score = dot(t, t);
                 } else if (stage == 2u) {
-                    const V3 aa = sub(c[chosen[1]].world_point, c[chosen[0]].world_point);
-                    const V3 bb = sub(c[i].world_point, c[chosen[0]].world_point);
+                    const V3 aa = sub(c[perm_get(perm, (chosen >> 4) & 0xfu)].world_point, w0);
+                    const V3 bb = sub(ci.world_point, w0);
                     score = fabs_(dot(cross(aa, bb), n));
                 } else {
                     score = __builtin_inff();
-                    for (uint32_t s = 0; s < nch; ++s) {
-                        const V3 d = sub(c[i].world_point, c[chosen[s]].world_point);
+                    for (uint32_t q = 0; q < nch; ++q) {
+                        const V3 d = sub(ci.world_point, c[perm_get(perm, (chosen >> (4u * q)) & 0xfu)].world_point);
                         const V3 t = sub(d, mul(dot(d, n), n));
                         score = fmind(score, dot(t, t));
                     }
                 }
-                const unsigned long long feat = cand_feature(c[i]);
+                const unsigned long long feat = cand_feature(ci);
                 if (score > best_score + 1.0e-12f || (fabs_(score - best_score) <= 1.0e-12f && feat < best_feat)) {
                     best = i;
                     best_score = score;
@@ -1336,25 +1353,27 @@ MRD_INL void reduce_candidates(Candidate* c, uint32_t& count, V3 n) {
                 }
             }
             if (best != MR_INVALID_INDEX) {
-                chosen[nch++] = best;
-                selected[best] = true;
+                chosen |= best << (4u * nch);
+                ++nch;
+                selected |= 1u << best;
             }
         }
-        Candidate red[MR_MANIFOLD_POINT_CAPACITY];
-        for (uint32_t i = 0; i < nch; ++i) red[i] = c[chosen[i]];
+        unsigned long long red = 0ull;
+        for (uint32_t k = 0; k < nch; ++k) red = perm_set(red, k, perm_get(perm, (chosen >> (4u * k)) & 0xfu));
+        perm = red;
         count = nch;
-        for (uint32_t i = 0; i < count; ++i) c[i] = red[i];
     }
     for (uint32_t i = 1; i < count; ++i) {
-        const Candidate val = c[i];
-        const unsigned long long f = cand_feature(val);
+        const uint32_t val = perm_get(perm, i);
+        const unsigned long long f = cand_feature(c[val]);
         uint32_t d = i;
-        while (d > 0u && f < cand_feature(c[d - 1u])) {
-            c[d] = c[d - 1u];
+        while (d > 0u && f < cand_feature(c[perm_get(perm, d - 1u)])) {
+            perm = perm_set(perm, d, perm_get(perm, d - 1u));
             --d;
         }
-        c[d] = val;
+        perm = perm_set(perm, d, val);
     }
+    return perm;
 }
 
 MRD_INL mr_float4 anchor_from_world(V3 bpos, Q4 rot, V3 x) {
@@ -1473,7 +1492,7 @@ MRD_INL bool build_manifold(const Env& e, const MRCompiledPairC& pair, const Raw
         cand[match].drift = 0.0f;
     }
     if (nc == 0u) return false;
-    reduce_candidates(cand, nc, n);
+    const unsigned long long order = reduce_candidates(cand, nc, n);
     V3 t = transported;
     if (!compatible || !finite3(t) || dot(t, t) <= 1.0e-12f) t = stable_tangent(n);
     else t = normalize(t);
@@ -1489,8 +1508,9 @@ MRD_INL bool build_manifold(const Env& e, const MRCompiledPairC& pair, const Raw
     out_h.normal_and_age = mr_float4{ln.x, ln.y, ln.z, compatible ? fmind(old_age + 1.0f, 65535.0f) : 0.0f};
     float metric = 0.0f;
     for (uint32_t i = 0; i < nc; ++i) {
-        out_p[i] = cand[i].point;
-        metric = fmaxd(metric, fmaxd(fmaxd(cand[i].separation, 0.0f), cand[i].drift));
+        const Candidate& ci = cand[perm_get(order, i)];
+        out_p[i] = ci.point;
+        metric = fmaxd(metric, fmaxd(fmaxd(ci.separation, 0.0f), ci.drift));
     }
     for (uint32_t i = nc; i < MR_MANIFOLD_POINT_CAPACITY; ++i) {
         out_p[i].local_anchor_a = mr_float4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -1667,6 +1687,8 @@ mr_control_step_kernel(KernelArgs args) {
     e.gravity = V3{A.gravity[0], A.gravity[1], A.gravity[2]};
     const mr::LdsLayout& L = A.lds;
     e.topo = lds_at<mr::TopoCache>(lds_base, L.topo);
+    e.lconst = lds_at<mr::LinkConst>(lds_base, L.lconst);
+    e.arm = lds_at<float>(lds_base, L.arm);
     e.q = lds_at<float>(lds_base, L.q);
     e.v = lds_at<float>(lds_base, L.v);
     e.vcand = lds_at<float>(lds_base, L.vcand);
@@ -1702,6 +1724,7 @@ mr_control_step_kernel(KernelArgs args) {
     e.cpair = lds_at<uint32_t>(lds_base, L.cpair);
     e.rslot = lds_at<float>(lds_base, L.rslot);
     e.rstack = lds_at<float>(lds_base, L.rstack);
+    e.isl = lds_at<uint8_t>(lds_base, L.isl);
     const uint32_t pcap = A.pair_capacity, mcap = A.manifold_capacity, ccap = A.constraint_capacity;
     e.stage_h = A.stage_h + (size_t)env * pcap;
     e.stage_p = A.stage_p + (size_t)env * pcap * MR_MANIFOLD_POINT_CAPACITY;
@@ -1713,21 +1736,39 @@ mr_control_step_kernel(KernelArgs args) {
     // ---- topology cache (one lane per body / DoF; 64 lanes cover the limits)
     {
         mr::TopoCache& T = *e.topo;
-        uint8_t par = 0xffu, vi = 0xffu, dep = 0u, mot = 0u;
+        uint8_t par = 0xffu, vi = 0xffu, qi = 0xffu, dep = 0u, mot = 0u, jt = 0u;
         if (lane < e.nbodies) {
             const MRBodyC& b = e.bodies[lane];
             mot = (uint8_t)b.motion_type;
             if (lane < e.nlinks) {
                 par = b.parent == MR_INVALID_INDEX ? (uint8_t)0xffu : (uint8_t)b.parent;
                 vi = b.v_index == MR_INVALID_INDEX ? (uint8_t)0xffu : (uint8_t)b.v_index;
+                qi = b.q_index == MR_INVALID_INDEX ? (uint8_t)0xffu : (uint8_t)b.q_index;
                 dep = (uint8_t)b.depth;
+                jt = (uint8_t)b.joint_type;
             }
         }
+        T.qidx[lane] = qi;
+        T.jtype[lane] = jt;
         T.parent[lane] = par;
         T.vidx[lane] = vi;
         T.depth[lane] = dep;
         T.motion[lane] = mot;
         T.dof_body[lane] = lane < e.nv ? (uint8_t)e.dofs[lane].body : (uint8_t)0xffu;
+        if (lane < e.nlinks) {
+            const MRBodyC& b = e.bodies[lane];
+            mr::LinkConst lc;
+            const Q4 cc = qconj(q4(b.child_rotation));
+            const V3 ax = normalize(f3(b.axis));
+            lc.prot[0] = b.parent_rotation.x; lc.prot[1] = b.parent_rotation.y;
+            lc.prot[2] = b.parent_rotation.z; lc.prot[3] = b.parent_rotation.w;
+            lc.crot_conj[0] = cc.x; lc.crot_conj[1] = cc.y; lc.crot_conj[2] = cc.z; lc.crot_conj[3] = cc.w;
+            lc.axis[0] = ax.x; lc.axis[1] = ax.y; lc.axis[2] = ax.z;
+            lc.panchor[0] = b.parent_anchor.x; lc.panchor[1] = b.parent_anchor.y; lc.panchor[2] = b.parent_anchor.z;
+            lc.canchor[0] = b.child_anchor.x; lc.canchor[1] = b.child_anchor.y; lc.canchor[2] = b.child_anchor.z;
+            e.lconst[lane] = lc;
+        }
+        if (lane < e.nv) e.arm[lane] = effective_armature(e, lane);
         __syncthreads();
         // model validation guarantees parent < child, so both walks end
         unsigned long long anc = 0ull, ch = 0ull;
@@ -1839,12 +1880,12 @@ mr_control_step_kernel(KernelArgs args) {
         __syncthreads();
         if (lane == 0u && floating) {
             SpI& I = e.I[0];
-            I.C[0] += effective_armature(e, 0);
-            I.C[3] += effective_armature(e, 1);
-            I.C[5] += effective_armature(e, 2);
-            I.A[0] += effective_armature(e, 3);
-            I.A[3] += effective_armature(e, 4);
-            I.A[5] += effective_armature(e, 5);
+            I.C[0] += e.arm[0];
+            I.C[3] += e.arm[1];
+            I.C[5] += e.arm[2];
+            I.A[0] += e.arm[3];
+            I.A[3] += e.arm[4];
+            I.A[5] += e.arm[5];
             e.pt[0] = sub(e.pt[0], V3{e.effort[3], e.effort[4], e.effort[5]});
             e.pb[0] = sub(e.pb[0], V3{e.effort[0], e.effort[1], e.effort[2]});
         }
@@ -2186,7 +2227,7 @@ mr_control_step_kernel(KernelArgs args) {
                 e.lam[3u * c + 0u] = warm ? mp.impulses.x : 0.0f;
                 e.lam[3u * c + 1u] = warm ? mp.impulses.y : 0.0f;
                 e.lam[3u * c + 2u] = warm ? mp.impulses.z : 0.0f;
-                e.cpair[c] = sa.body | (sb.body << 16);
+                e.cpair[c] = sa.body | (sb.body << 8) | (hk.kind << 16);
                 e.hot[c] = hk;
                 e.cold[c] = ck;
             }
@@ -2336,12 +2377,13 @@ mr_control_step_kernel(KernelArgs args) {
         // ---------------------------------------------------- warm start
         // DoF lanes accumulate the warm-start impulses in constraint order;
         // scene lanes apply theirs in the same order.
-        for (uint32_t c = 0; c < ncons; ++c) {
-            const uint32_t kind = uniform_u32(e.hot[c].kind);
-            if ((kind & mr::kHotArticulated) == 0u || !dof_lane) continue;
-            const float* rb = e.resp + (size_t)c * 3u * mnv + lane;
-            const float* lc = e.lam + 3u * c;
-            vj += (rb[0] * lc[0] + rb[mnv] * lc[1]) + rb[2u * mnv] * lc[2];
+        if (dof_lane) {
+            for (uint32_t c = 0; c < ncons; ++c) {
+                if (((e.cpair[c] >> 16) & mr::kHotArticulated) == 0u) continue;
+                const float* rb = e.resp + (size_t)c * 3u * mnv + lane;
+                const float* lc = e.lam + 3u * c;
+                vj += (rb[0] * lc[0] + rb[mnv] * lc[1]) + rb[2u * mnv] * lc[2];
+            }
         }
         for (uint32_t s = lane; s < nsc; s += kLanes) {
             const uint32_t b = nl + s;
@@ -2349,7 +2391,7 @@ mr_control_step_kernel(KernelArgs args) {
             MRBodyStateC st = e.scene[s];
             for (uint32_t c = 0; c < ncons; ++c) {
                 const uint32_t pr = e.cpair[c];
-                if ((pr & 0xffffu) != b && (pr >> 16) != b) continue;
+                if ((pr & 0xffu) != b && ((pr >> 8) & 0xffu) != b) continue;
                 const HotConstraint& hk = e.hot[c];
                 const float* lc = e.lam + 3u * c;
                 const V3 imp = add(add(mul(lc[0], hot_dir(hk, 0)), mul(lc[1], hot_dir(hk, 1))), mul(lc[2], hot_dir(hk, 2)));
@@ -2477,15 +2519,15 @@ mr_control_step_kernel(KernelArgs args) {
         cs.solver_iterations = ncons > 0u ? iters : 0u;
         cs.residuals = mr_float4{ncons > 0u ? sweep_delta : 0.0f, max_normal, max_cone, 0.0f};
         if (lane == 0u) {
-            uint32_t parent[MR_MAX_BODIES + 1];
-            bool used[MR_MAX_BODIES + 1];
+            uint8_t* parent = e.isl;                      // LDS, dead response scratch
+            uint8_t* used = e.isl + (MR_MAX_BODIES + 1u);
             const uint32_t nodes = 1u + nsc;
             for (uint32_t i = 0; i < nodes; ++i) {
-                parent[i] = i;
-                used[i] = false;
+                parent[i] = (uint8_t)i;
+                used[i] = 0u;
             }
             for (uint32_t c = 0; c < ncons; ++c) {
-                const uint32_t ba = e.cpair[c] & 0xffffu, bb = e.cpair[c] >> 16;
+                const uint32_t ba = e.cpair[c] & 0xffu, bb = (e.cpair[c] >> 8) & 0xffu;
                 const uint32_t na = is_link(e, ba) ? 0u : (is_dynamic_scene(e, ba) ? 1u + (ba - nl) : MR_INVALID_INDEX);
                 const uint32_t nb = is_link(e, bb) ? 0u : (is_dynamic_scene(e, bb) ? 1u + (bb - nl) : MR_INVALID_INDEX);
                 if (na != MR_INVALID_INDEX) used[na] = true;

@@ -178,6 +178,8 @@ void World::compute_lds_layout() {
     LdsLayout& L = lds_;
     // arrays live for the whole microstep
     L.topo = take((uint32_t)sizeof(TopoCache), 16u);
+    L.lconst = take((uint32_t)sizeof(LinkConst) * links, 4u);
+    L.arm = take(4u * nv, 4u);
     L.q = take(4u * nq, 4u);
     L.v = take(4u * nv, 4u);
     L.vcand = take(4u * nv, 4u);
@@ -223,6 +225,9 @@ void World::compute_lds_layout() {
     response_scratch_bounds(slots, stack);
     L.rslot = take(4u * 6u * 64u * slots, 16u);
     L.rstack = take(4u * 64u * stack, 16u);
+    union_end = std::max(union_end, off);
+    off = union_start;
+    L.isl = take(2u * (MR_MAX_BODIES + 1u), 4u);
     off = std::max(union_end, off);
     L.total = (off + 15u) / 16u * 16u;
     if (L.total > kMaxLdsBytes)

@@ -130,3 +130,26 @@ def test_free_running_horizon_matches_oracle(gpu):
     print(f"free-running q err={rel_err(q, ost.q):.3e} v err={rel_err(v, ost.v):.3e}", flush=True)
     assert rel_err(q, ost.q) <= REL_TOL
     assert rel_err(v, ost.v) <= REL_TOL
+
+
+def test_adaptive_residual_mode_matches_oracle(gpu):
+    """Adaptive cone solve (sweeps until the final-sweep max |dlambda| reaches
+    the tolerance, capped): same iteration counts, residuals and trajectory."""
+    model = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_CHAIN_BOXES, 8, 13)
+    cfg = model.default_config()
+    cfg.residual_tolerance = 1.0e-4
+    cfg.max_iterations = 40
+    world = mr.World(model, cfg)
+    ost = ol.OracleState(model, cfg)
+    steps = 20
+    cmds = model.commands(0, steps)
+    world.advance(cmds)
+    o_cs = None
+    for k in range(steps):
+        _, o_cs, *_ = ost.step(cmds[k], control_step=k)
+    q, v, s = world.state()
+    g_cs = world.contact_statuses()[-1]
+    np.testing.assert_array_equal(g_cs["solver_iterations"], o_cs["solver_iterations"])
+    np.testing.assert_array_equal(g_cs["residuals"], o_cs["residuals"])
+    assert np.any(o_cs["solver_iterations"] > 6), "the tolerance never asked for more sweeps than the default"
+    assert rel_err(q, ost.q) <= REL_TOL and rel_err(v, ost.v) <= REL_TOL

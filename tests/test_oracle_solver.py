@@ -115,3 +115,17 @@ def test_impulses_stay_inside_the_cone():
     assert np.all(lam_n >= 0.0)
     ell = np.sqrt((imp[:, 1] / np.maximum(1.0 * lam_n, 1e-30)) ** 2 + (imp[:, 2] / np.maximum(0.7 * lam_n, 1e-30)) ** 2)
     assert np.all(ell[lam_n > 0] <= 1.0 + 1e-5)
+
+
+def test_adaptive_mode_reaches_the_tolerance_or_the_cap():
+    """With a residual tolerance the sweeps stop at the first sweep whose max
+    |dlambda| is at or below it (MetalWorldContact.metal:8040-8218 iteration
+    control), never later than max_iterations."""
+    w, cfg, st = box_on_plane()
+    cfg.residual_tolerance = 1.0e-7
+    cfg.max_iterations = 64
+    for cs in run(st, 20):
+        it = int(cs["solver_iterations"][0])
+        res = float(cs["residuals"][0, 0])
+        assert 1 <= it <= 64
+        assert res <= 1.0e-7 or it == 64, (it, res)
