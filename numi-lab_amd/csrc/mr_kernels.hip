@@ -1623,24 +1623,21 @@ MRD_INL float rel_row(V3 d, V3 va, V3 vb, float art_dot) { return (art_dot + dot
 
 MRD_INL uint32_t uniform_u32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
+// Optional per-stage shader-clock counters: lane 0 adds each stage's elapsed
+// cycles straight into the environment's global counters (no register array
+// held across the kernel when profiling is off).
 struct StageClock {
     unsigned long long* out;
     unsigned long long last;
-    unsigned long long acc[mr::kStageCount];
-    MRD_INL void init(unsigned long long* o) {
-        out = o;
-        for (uint32_t i = 0; i < mr::kStageCount; ++i) acc[i] = 0ull;
+    MRD_INL void init(unsigned long long* o, uint32_t lane) {
+        out = lane == 0u ? o : nullptr;
         last = out != nullptr ? (unsigned long long)clock64() : 0ull;
     }
     MRD_INL void mark(uint32_t stage) {
         if (out == nullptr) return;
         const unsigned long long now = (unsigned long long)clock64();
-        acc[stage] += now - last;
+        out[stage] += now - last;
         last = now;
-    }
-    MRD_INL void flush(uint32_t lane) {
-        if (out == nullptr || lane != 0u) return;
-        for (uint32_t i = 0; i < mr::kStageCount; ++i) out[i] += acc[i];
     }
 };
 
@@ -1658,6 +1655,7 @@ mr_control_step_kernel(KernelArgs args) {
     __shared__ uint32_t s_count[8];
     __shared__ float s_fmax[2];
     __shared__ float s_root[7];
+    __shared__ MRContactStatusC s_cs;
 
     const uint32_t lane = threadIdx.x;
     const uint32_t env = blockIdx.x;
@@ -1814,20 +1812,23 @@ mr_control_step_kernel(KernelArgs args) {
     __syncthreads();
 
     StageClock clk;
-    clk.init(A.stage_cycles != nullptr ? A.stage_cycles + (size_t)env * mr::kStageCount : nullptr);
+    clk.init(A.stage_cycles != nullptr ? A.stage_cycles + (size_t)env * mr::kStageCount : nullptr, lane);
     uint32_t src_buf = cur;
     uint32_t dst_buf = (cur + 1u) % 3u;
     const uint32_t alt_buf = (cur + 2u) % 3u;
     uint32_t mcount_src = A.mcount[(size_t)cur * E + env];
     if (mcount_src > mcap) mcount_src = mcap;
 
-    MRContactStatusC cs;  // lane 0 copy is authoritative
-    {
+    // Contact status lives in LDS (uniform fields are written with identical
+    // values by every lane; lane-0-only fields by lane 0).
+    MRContactStatusC& cs = s_cs;
+    if (lane == 0u) {
         unsigned char* p = reinterpret_cast<unsigned char*>(&cs);
         for (uint32_t k = 0; k < sizeof(cs); ++k) p[k] = 0;
+        cs.first_failing_pair = MR_INVALID_INDEX;
+        cs.first_failing_constraint = MR_INVALID_INDEX;
     }
-    cs.first_failing_pair = MR_INVALID_INDEX;
-    cs.first_failing_constraint = MR_INVALID_INDEX;
+    __syncthreads();
     uint32_t successful = 0u;
     float min_piv = __builtin_inff(), max_piv = 0.0f, max_acc = 0.0f;
     uint32_t failing_substep = MR_INVALID_INDEX;
@@ -2645,7 +2646,6 @@ mr_control_step_kernel(KernelArgs args) {
         __syncthreads();
         clk.mark(mr::kStageIntegration);
     }
-    clk.flush(lane);
 
     // ------------------------------------------- transactional commit
     __syncthreads();
