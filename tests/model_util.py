@@ -95,6 +95,9 @@ def compile_pairs(bodies, shapes, link_count, exclusion_distance=1):
             sa, sb = shapes[a], shapes[b]
             if sa["body"] == sb["body"]:
                 continue
+            if (int(sa["collision_group"]) & int(sb["collision_mask"])) == 0 or \
+                    (int(sb["collision_group"]) & int(sa["collision_mask"])) == 0:
+                continue
             if bodies[sa["body"]]["motion_type"] != 2 and bodies[sb["body"]]["motion_type"] != 2:
                 continue
             if dist(int(sa["body"]), int(sb["body"])) <= exclusion_distance:
