@@ -253,6 +253,16 @@ struct StepContext {
     uint32_t constraint_capacity;
 };
 
+// Optional capture of one environment's final-microstep contact problem, for
+// the FP64 exact-cone comparison (serial control-step entry point only).
+struct ProblemCapture {
+    uint32_t env = 0xffffffffu;
+    float* out = nullptr;
+    uint32_t capacity = 0;  // floats
+    uint32_t used = 0;
+};
+thread_local ProblemCapture g_capture;
+
 struct Failure {
     uint32_t code = MR_STEP_SUCCESS;
     uint32_t index = MR_INVALID_INDEX;
@@ -1960,6 +1970,44 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
             return false;
         }
     }
+    // problem capture: header, J rows, response rows, free velocity,
+    // targets, regularisation, cone coefficients, warm-start impulses
+    const bool capture_problem = final_sub && g_capture.out != nullptr && ctx.env == g_capture.env;
+    if (capture_problem) {
+        const size_t need = 8u + 2u * (size_t)ncons * 3u * nv + 2u * nv + (size_t)ncons * (3u * 4u + 2u);
+        g_capture.used = 0;
+        if (need <= g_capture.capacity) {
+            float* o = g_capture.out;
+            bool dynamic_scene = false;
+            for (uint32_t c = 0; c < ncons; ++c)
+                dynamic_scene = dynamic_scene || is_dynamic_scene(ctx, cons[c].body_a) || is_dynamic_scene(ctx, cons[c].body_b);
+            const uint32_t hdr[8] = {nv, ncons, dynamic_scene ? 1u : 0u, 0u, 0u, 0u, 0u, 0u};
+            std::memcpy(o, hdr, sizeof(hdr));
+            size_t at = 8;
+            for (uint32_t c = 0; c < ncons; ++c) {
+                const bool art = is_link(ctx, cons[c].body_a) || is_link(ctx, cons[c].body_b);
+                for (uint32_t r = 0; r < 3u; ++r)
+                    for (uint32_t j = 0; j < nv; ++j) o[at++] = art ? jrow[((size_t)c * 3u + r) * nv + j] : 0.0f;
+            }
+            for (uint32_t c = 0; c < ncons; ++c) {
+                const bool art = is_link(ctx, cons[c].body_a) || is_link(ctx, cons[c].body_b);
+                for (uint32_t r = 0; r < 3u; ++r)
+                    for (uint32_t j = 0; j < nv; ++j) o[at++] = art ? resp[((size_t)c * 3u + r) * nv + j] : 0.0f;
+            }
+            for (uint32_t j = 0; j < nv; ++j) o[at++] = wk.vcand[j];
+            for (uint32_t c = 0; c < ncons; ++c)
+                for (int r = 0; r < 3; ++r) o[at++] = cons[c].target[r];
+            for (uint32_t c = 0; c < ncons; ++c)
+                for (int r = 0; r < 3; ++r) o[at++] = cons[c].reg[r];
+            for (uint32_t c = 0; c < ncons; ++c) {
+                o[at++] = cons[c].mu_u;
+                o[at++] = cons[c].mu_v;
+            }
+            for (uint32_t c = 0; c < ncons; ++c)
+                for (int r = 0; r < 3; ++r) o[at++] = cons[c].lambda[r];
+            g_capture.used = (uint32_t)at;
+        }
+    }
     // 11. warm start, then ordered block Gauss-Seidel sweeps
     auto apply_delta = [&](const Constraint& k, uint32_t c, const float d[3]) {
         const bool art = is_link(ctx, k.body_a) || is_link(ctx, k.body_b);
@@ -2010,6 +2058,14 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
         }
         ++iters;
         if (adaptive && sweep_delta <= ctx.cfg->residual_tolerance) break;
+    }
+    if (capture_problem && g_capture.used != 0u) {
+        float* o = g_capture.out;
+        size_t at = g_capture.used;
+        for (uint32_t c = 0; c < ncons; ++c)
+            for (int r = 0; r < 3; ++r) o[at++] = cons[c].lambda[r];
+        for (uint32_t j = 0; j < nv; ++j) o[at++] = wk.vcand[j];
+        g_capture.used = (uint32_t)at;
     }
     // 12. residuals (MetalWorldContact.metal:8220-8315) and impulse write-back
     float max_normal = 0.0f, max_cone = 0.0f;
@@ -2422,6 +2478,18 @@ int mr_oracle_collide(const MRWorldModelDescC* desc, const MRWorldStepConfigC* c
     delete wk;
     return rc;
 }
+
+// Captures the final-microstep contact problem of environment `env` during the
+// next serial mr_oracle_control_step calls (buffer == null turns it off).
+// Returns the number of floats written by the last capture through `used`.
+void mr_oracle_set_problem_capture(uint32_t env, float* buffer, uint32_t capacity_floats) {
+    g_capture.env = env;
+    g_capture.out = buffer;
+    g_capture.capacity = capacity_floats;
+    g_capture.used = 0;
+}
+
+uint32_t mr_oracle_problem_capture_used(void) { return g_capture.used; }
 
 void mr_oracle_sincos(const float* x, float* s, float* c, uint32_t n) {
     for (uint32_t i = 0; i < n; ++i) det_sincos(x[i], s[i], c[i]);

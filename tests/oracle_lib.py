@@ -162,3 +162,91 @@ def sincos(x):
     c = np.zeros_like(x)
     lib.mr_oracle_sincos(_p(x), _p(s), _p(c), x.size)
     return s, c
+
+
+def exact_cone(inverse_mass, free_velocity, jacobian, target, regularization, friction_uv, warm=None,
+               max_iterations=50000, tolerance=1.0e-11):
+    """FP64 exact-cone solve (oracle/exact_cone.cpp). jacobian: [contacts*3][nv];
+    friction_uv: [contacts][2]. Returns (code, impulses, velocity, stats dict)."""
+    lib = load()
+    if not hasattr(lib, "_exact_cone_bound"):
+        P = ct.c_void_p
+        lib.mr_oracle_exact_cone.argtypes = [ct.c_uint32, P, P, ct.c_uint32, P, P, P, P, P, ct.c_uint32,
+                                             ct.c_double, P, P, P]
+        lib.mr_oracle_exact_cone.restype = ct.c_int
+        lib._exact_cone_bound = True
+    M = np.ascontiguousarray(inverse_mass, np.float64)
+    nv = M.shape[0]
+    J = np.ascontiguousarray(jacobian, np.float64).reshape(-1, nv)
+    n = J.shape[0]
+    vf = np.ascontiguousarray(free_velocity, np.float64)
+    tg = np.ascontiguousarray(target, np.float64).reshape(n)
+    rg = np.ascontiguousarray(regularization, np.float64).reshape(n)
+    mu = np.ascontiguousarray(friction_uv, np.float64).reshape(n // 3, 2)
+    w = None if warm is None else np.ascontiguousarray(warm, np.float64).reshape(n)
+    imp = np.zeros(n)
+    vel = np.zeros(nv)
+    stats = np.zeros(5)
+    code = lib.mr_oracle_exact_cone(nv, _p(M), _p(vf), n // 3, _p(J), _p(tg), _p(rg), _p(mu),
+                                    None if w is None else _p(w), max_iterations, tolerance, _p(imp), _p(vel),
+                                    _p(stats))
+    return code, imp, vel, {"iterations": int(stats[0]), "residual": stats[1], "violation": stats[2],
+                            "used_warm": bool(stats[3]), "objective": stats[4]}
+
+
+def exact_cone_program(hessian, linear, friction_uv, warm=None, max_iterations=200000, tolerance=1.0e-10):
+    """FP64 exact-cone solve of a program given by H and l directly."""
+    lib = load()
+    P = ct.c_void_p
+    lib.mr_oracle_exact_cone_program.argtypes = [ct.c_uint32, P, P, P, P, ct.c_uint32, ct.c_double, P, P]
+    lib.mr_oracle_exact_cone_program.restype = ct.c_int
+    H = np.ascontiguousarray(hessian, np.float64)
+    n = H.shape[0]
+    lin = np.ascontiguousarray(linear, np.float64)
+    mu = np.ascontiguousarray(friction_uv, np.float64).reshape(n // 3, 2)
+    w = None if warm is None else np.ascontiguousarray(warm, np.float64)
+    imp = np.zeros(n)
+    stats = np.zeros(5)
+    code = lib.mr_oracle_exact_cone_program(n // 3, _p(H), _p(lin), _p(mu), None if w is None else _p(w),
+                                            max_iterations, tolerance, _p(imp), _p(stats))
+    return code, imp, {"iterations": int(stats[0]), "residual": stats[1], "violation": stats[2],
+                       "used_warm": bool(stats[3]), "objective": stats[4]}
+
+
+def contact_problem_capture(state, command, env, control_step=0, capacity=1 << 20):
+    """One oracle control step that also returns environment `env`'s
+    final-microstep contact problem: dict with J, R (rows x nv), v_free,
+    target, reg, mu (contacts x 2), lambda_warm, lambda_final, v_final."""
+    lib = load()
+    lib.mr_oracle_set_problem_capture.argtypes = [ct.c_uint32, ct.c_void_p, ct.c_uint32]
+    lib.mr_oracle_set_problem_capture.restype = None
+    lib.mr_oracle_problem_capture_used.restype = ct.c_uint32
+    buf = np.zeros(capacity, np.float32)
+    lib.mr_oracle_set_problem_capture(env, _p(buf), capacity)
+    try:
+        out = state.step(command, control_step=control_step)
+        used = int(lib.mr_oracle_problem_capture_used())
+    finally:
+        lib.mr_oracle_set_problem_capture(0xFFFFFFFF, None, 0)
+    if used == 0:
+        return out, None
+    hdr = buf[:8].view(np.uint32)
+    nv, nc, dyn = int(hdr[0]), int(hdr[1]), int(hdr[2])
+    at = 8
+    def take(k):
+        nonlocal at
+        a = buf[at:at + k].astype(np.float64)
+        at += k
+        return a
+    prob = {"nv": nv, "contacts": nc, "dynamic_scene": bool(dyn)}
+    prob["J"] = take(3 * nc * nv).reshape(3 * nc, nv)
+    prob["R"] = take(3 * nc * nv).reshape(3 * nc, nv)
+    prob["v_free"] = take(nv)
+    prob["target"] = take(3 * nc)
+    prob["reg"] = take(3 * nc)
+    prob["mu"] = take(2 * nc).reshape(nc, 2)
+    prob["lambda_warm"] = take(3 * nc)
+    prob["lambda_final"] = take(3 * nc)
+    prob["v_final"] = take(nv)
+    assert at == used, (at, used)
+    return out, prob
