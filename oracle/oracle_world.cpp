@@ -1974,14 +1974,16 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
     // targets, regularisation, cone coefficients, warm-start impulses
     const bool capture_problem = final_sub && g_capture.out != nullptr && ctx.env == g_capture.env;
     if (capture_problem) {
-        const size_t need = 8u + 2u * (size_t)ncons * 3u * nv + 2u * nv + (size_t)ncons * (3u * 4u + 2u);
+        const uint32_t nscene_c = hd.body_count - nlinks;
+        const size_t need = 8u + 2u * (size_t)ncons * 3u * nv + 2u * nv + (size_t)ncons * (3u * 4u + 2u + 16u) +
+                            25u * (size_t)nscene_c;
         g_capture.used = 0;
         if (need <= g_capture.capacity) {
             float* o = g_capture.out;
             bool dynamic_scene = false;
             for (uint32_t c = 0; c < ncons; ++c)
                 dynamic_scene = dynamic_scene || is_dynamic_scene(ctx, cons[c].body_a) || is_dynamic_scene(ctx, cons[c].body_b);
-            const uint32_t hdr[8] = {nv, ncons, dynamic_scene ? 1u : 0u, 0u, 0u, 0u, 0u, 0u};
+            const uint32_t hdr[8] = {nv, ncons, dynamic_scene ? 1u : 0u, nscene_c, 0u, 0u, 0u, 0u};
             std::memcpy(o, hdr, sizeof(hdr));
             size_t at = 8;
             for (uint32_t c = 0; c < ncons; ++c) {
@@ -2005,6 +2007,44 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
             }
             for (uint32_t c = 0; c < ncons; ++c)
                 for (int r = 0; r < 3; ++r) o[at++] = cons[c].lambda[r];
+            // contact geometry: dynamic flags, scene indices, point, directions
+            for (uint32_t c = 0; c < ncons; ++c) {
+                const Constraint& k = cons[c];
+                const bool da = is_dynamic_scene(ctx, k.body_a), db = is_dynamic_scene(ctx, k.body_b);
+                o[at++] = da ? 1.0f : 0.0f;
+                o[at++] = db ? 1.0f : 0.0f;
+                o[at++] = da ? (float)(k.body_a - nlinks) : -1.0f;
+                o[at++] = db ? (float)(k.body_b - nlinks) : -1.0f;
+                o[at++] = k.point_world.x;
+                o[at++] = k.point_world.y;
+                o[at++] = k.point_world.z;
+                for (int r = 0; r < 3; ++r) {
+                    o[at++] = k.dir[r].x;
+                    o[at++] = k.dir[r].y;
+                    o[at++] = k.dir[r].z;
+                }
+            }
+            // scene bodies: position, inverse mass, world inverse inertia, twist
+            for (uint32_t b = 0; b < nscene_c; ++b) {
+                const MRBodyStateC& sb = wk.bodies[nlinks + b];
+                o[at++] = sb.position.x;
+                o[at++] = sb.position.y;
+                o[at++] = sb.position.z;
+                o[at++] = is_dynamic_scene(ctx, nlinks + b) ? sb.linear_velocity_inv_mass.w : 0.0f;
+                const mr_float4 rows[3] = {sb.inverse_inertia_world_row0, sb.inverse_inertia_world_row1,
+                                           sb.inverse_inertia_world_row2};
+                for (int r = 0; r < 3; ++r) {
+                    o[at++] = rows[r].x;
+                    o[at++] = rows[r].y;
+                    o[at++] = rows[r].z;
+                }
+                o[at++] = sb.linear_velocity_inv_mass.x;
+                o[at++] = sb.linear_velocity_inv_mass.y;
+                o[at++] = sb.linear_velocity_inv_mass.z;
+                o[at++] = sb.angular_velocity.x;
+                o[at++] = sb.angular_velocity.y;
+                o[at++] = sb.angular_velocity.z;
+            }
             g_capture.used = (uint32_t)at;
         }
     }
@@ -2065,6 +2105,15 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
         for (uint32_t c = 0; c < ncons; ++c)
             for (int r = 0; r < 3; ++r) o[at++] = cons[c].lambda[r];
         for (uint32_t j = 0; j < nv; ++j) o[at++] = wk.vcand[j];
+        for (uint32_t b = nlinks; b < hd.body_count; ++b) {
+            const MRBodyStateC& sb = wk.bodies[b];
+            o[at++] = sb.linear_velocity_inv_mass.x;
+            o[at++] = sb.linear_velocity_inv_mass.y;
+            o[at++] = sb.linear_velocity_inv_mass.z;
+            o[at++] = sb.angular_velocity.x;
+            o[at++] = sb.angular_velocity.y;
+            o[at++] = sb.angular_velocity.z;
+        }
         g_capture.used = (uint32_t)at;
     }
     // 12. residuals (MetalWorldContact.metal:8220-8315) and impulse write-back

@@ -246,7 +246,53 @@ def contact_problem_capture(state, command, env, control_step=0, capacity=1 << 2
     prob["reg"] = take(3 * nc)
     prob["mu"] = take(2 * nc).reshape(nc, 2)
     prob["lambda_warm"] = take(3 * nc)
+    ns = int(hdr[3])
+    geo = take(16 * nc).reshape(nc, 16)
+    prob["dyn"] = geo[:, 0:2] > 0.5
+    prob["scene_index"] = geo[:, 2:4].astype(np.int64)
+    prob["point"] = geo[:, 4:7]
+    prob["dir"] = geo[:, 7:16].reshape(nc, 3, 3)
+    sc = take(19 * ns).reshape(ns, 19)
+    prob["scene_pos"] = sc[:, 0:3]
+    prob["scene_inv_mass"] = sc[:, 3]
+    prob["scene_inv_inertia"] = sc[:, 4:13].reshape(ns, 3, 3)
+    prob["scene_twist_free"] = sc[:, 13:19]
     prob["lambda_final"] = take(3 * nc)
     prob["v_final"] = take(nv)
+    prob["scene_twist_final"] = take(6 * ns).reshape(ns, 6)
     assert at == used, (at, used)
     return out, prob
+
+
+def contact_program(prob):
+    """The captured problem as one dense cone program over the articulation
+    DoFs and the scene bodies' twists: returns (H, l, apply) where
+    apply(lambda) gives (v, twists) after the impulses. J rows: +d on body B,
+    -d on body A (linear d, angular r x d on scene bodies)."""
+    nc, ns = prob["contacts"], len(prob["scene_inv_mass"])
+    n = 3 * nc
+    Js = np.zeros((n, 6 * ns))
+    for c in range(nc):
+        for side, sign in ((0, -1.0), (1, 1.0)):
+            if not prob["dyn"][c, side]:
+                continue
+            b = int(prob["scene_index"][c, side])
+            lever = prob["point"][c] - prob["scene_pos"][b]
+            for r in range(3):
+                d = prob["dir"][c, r]
+                Js[3 * c + r, 6 * b:6 * b + 3] = sign * d
+                Js[3 * c + r, 6 * b + 3:6 * b + 6] = sign * np.cross(lever, d)
+    Minv = np.zeros((6 * ns, 6 * ns))
+    for b in range(ns):
+        Minv[6 * b:6 * b + 3, 6 * b:6 * b + 3] = prob["scene_inv_mass"][b] * np.eye(3)
+        Minv[6 * b + 3:6 * b + 6, 6 * b + 3:6 * b + 6] = prob["scene_inv_inertia"][b]
+    J, R = prob["J"], prob["R"]
+    H = J @ R.T + Js @ Minv @ Js.T + np.diag(prob["reg"])
+    H = 0.5 * (H + H.T)
+    tw0 = prob["scene_twist_free"].reshape(-1)
+    lin = J @ prob["v_free"] + Js @ tw0 - prob["target"]
+
+    def apply(lam):
+        return prob["v_free"] + R.T @ lam, (tw0 + Minv @ Js.T @ lam).reshape(ns, 6)
+
+    return H, lin, apply

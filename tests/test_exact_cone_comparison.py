@@ -18,36 +18,76 @@ import metalrobo as mr
 import oracle_lib as ol
 
 
-def horizon_errors(kind, envs, steps, seed, env):
+def horizon_errors(kind, envs, steps, seed, env, tolerance=0.0, max_iterations=0, exact_tolerance=1.0e-8,
+                   first=0):
     model = mr.WorldModel.synthetic(kind, envs, seed)
     cfg = model.default_config()
+    if tolerance > 0.0:
+        cfg.residual_tolerance = tolerance
+        cfg.max_iterations = max_iterations
     st = ol.OracleState(model, cfg)
     cmds = model.commands(0, steps)
     rows = []
     for k in range(steps):
-        _, prob = ol.contact_problem_capture(st, cmds[k], env, control_step=k)
-        if prob is None or prob["contacts"] == 0 or prob["dynamic_scene"]:
+        if k < first:
+            st.step(cmds[k], control_step=k)
             continue
-        J, R = prob["J"], prob["R"]
-        H = J @ R.T + np.diag(prob["reg"])
-        H = 0.5 * (H + H.T)
-        lin = J @ prob["v_free"] - prob["target"]
-        code, lam, stats = ol.exact_cone_program(H, lin, prob["mu"])
-        assert code == 0, stats
-        v_exact = prob["v_free"] + R.T @ lam
-        scale = 1.0 + np.linalg.norm(v_exact)
-        rows.append((np.linalg.norm(prob["v_final"] - v_exact) / scale,
-                     np.linalg.norm(prob["v_free"] - v_exact) / scale,
-                     stats["iterations"], prob["contacts"]))
+        _, prob = ol.contact_problem_capture(st, cmds[k], env, control_step=k)
+        if prob is None or prob["contacts"] == 0:
+            continue
+        H, lin, apply = ol.contact_program(prob)
+        code, lam, stats = ol.exact_cone_program(H, lin, prob["mu"], tolerance=exact_tolerance, max_iterations=400000)
+        # stacked boxes make some programs ill-conditioned: accept a budget-limited
+        # solve once its optimality residual is far below the gaps measured here
+        assert code == 0 or (code == 11 and stats["residual"] <= 1.0e-6), stats
+        v_exact, tw_exact = apply(lam)
+        exact = np.concatenate([v_exact, tw_exact.reshape(-1)])
+        final = np.concatenate([prob["v_final"], prob["scene_twist_final"].reshape(-1)])
+        free = np.concatenate([prob["v_free"], prob["scene_twist_free"].reshape(-1)])
+        # the captured solve result must agree with the program's own response map
+        v_tc, tw_tc = apply(prob["lambda_final"])
+        tc_mapped = np.concatenate([v_tc, tw_tc.reshape(-1)])
+        scale = 1.0 + np.linalg.norm(exact)
+        rows.append((np.linalg.norm(final - exact) / scale, np.linalg.norm(free - exact) / scale,
+                     stats["iterations"], prob["contacts"],
+                     np.linalg.norm(tc_mapped - final) / (1.0 + np.linalg.norm(final))))
     return np.array(rows)
+
+
+def report(name, rows):
+    tc, free = rows[:, 0], rows[:, 1]
+    print(f"{name} exact-cone gap over {len(rows)} contact steps: velocity rel err mean {tc.mean():.3e} "
+          f"max {tc.max():.3e} (free-velocity gap mean {free.mean():.3e}); exact iterations mean "
+          f"{rows[:, 2].mean():.0f}; contacts mean {rows[:, 3].mean():.1f}", flush=True)
 
 
 def test_humanoid_temporal_cone_vs_exact_cone_over_horizon():
     rows = horizon_errors(mr.MR_SYNTHETIC_HUMANOID_TERRAIN, 2, 60, 2024, 0)
     assert len(rows) >= 20, "too few contact steps in the horizon"
-    tc, free = rows[:, 0], rows[:, 1]
-    print(f"humanoid exact-cone gap over {len(rows)} contact steps: velocity rel err mean {tc.mean():.3e} "
-          f"max {tc.max():.3e} (free-velocity gap mean {free.mean():.3e}); exact iterations mean "
-          f"{rows[:, 2].mean():.0f}; contacts mean {rows[:, 3].mean():.1f}", flush=True)
-    assert np.all(np.isfinite(tc))
-    assert np.mean(tc < free) >= 0.9
+    report("humanoid (4+2 sweeps)", rows)
+    assert np.all(np.isfinite(rows[:, 0]))
+    assert np.max(rows[:, 4]) <= 1.0e-4, "captured velocities disagree with the program's response map"
+    assert np.mean(rows[:, 0] < rows[:, 1]) >= 0.9
+
+
+def test_chain_with_free_boxes_vs_exact_cone_over_horizon():
+    # stacked boxes give large, ill-conditioned programs: a short horizon and a
+    # 1e-6 exact-solve tolerance (far below the gap measured here) keep the
+    # CPU cost to seconds
+    rows = horizon_errors(mr.MR_SYNTHETIC_CHAIN_BOXES, 1, 44, 7, 0, exact_tolerance=1.0e-6, first=30)
+    assert len(rows) >= 8, "too few contact steps in the horizon"
+    report("chain + 20 boxes (4+2 sweeps)", rows)
+    assert np.all(np.isfinite(rows[:, 0]))
+    assert np.max(rows[:, 4]) <= 1.0e-4, "captured velocities disagree with the program's response map"
+    assert np.mean(rows[:, 0] < rows[:, 1]) >= 0.9
+
+
+def test_converged_sweeps_keep_the_same_gap():
+    """Sweeping until |dlambda| <= 1e-5 converges temporalCone to its own fixed
+    point (block Gauss-Seidel with a radial cone projection per block), which
+    is not the exact-cone solution: the gap stays at the fixed-sweep level."""
+    fixed = horizon_errors(mr.MR_SYNTHETIC_HUMANOID_TERRAIN, 2, 30, 2024, 0)
+    adaptive = horizon_errors(mr.MR_SYNTHETIC_HUMANOID_TERRAIN, 2, 30, 2024, 0, tolerance=1.0e-5, max_iterations=128)
+    report("humanoid (adaptive, |dlambda| <= 1e-5, cap 128)", adaptive)
+    assert np.all(np.isfinite(adaptive[:, 0]))
+    assert abs(adaptive[:, 0].mean() - fixed[:, 0].mean()) <= 0.5 * fixed[:, 0].mean()
