@@ -151,6 +151,7 @@ struct KernelArgs {
     float h;
     uint32_t environment_count;
     uint32_t max_nq, max_nv, max_scene;
+    uint32_t row_stride;  // floats per Jacobian / response row: max(1, max_nv)
     uint32_t control_step;
     uint32_t pair_capacity, manifold_capacity, constraint_capacity;
     // persistent state
@@ -176,8 +177,8 @@ struct KernelArgs {
     // global scratch
     MRManifoldHeaderC* stage_h;   // [E][pair_capacity]
     MRManifoldPointC* stage_p;    // [E][pair_capacity][4]
-    float* jrow;                  // [E][constraint_capacity][3][max_nv]
-    float* resp;                  // [E][constraint_capacity][3][max_nv]
+    float* jrow;                  // [E][constraint_capacity][3][row_stride]
+    float* resp;                  // [E][constraint_capacity][3][row_stride]
     ColdConstraint* cold;         // [E][constraint_capacity]
     HotConstraint* hot;           // [E][constraint_capacity]
     unsigned long long* stage_cycles; // [E][kStageCount] shader-clock cycles per stage, or null

@@ -1726,8 +1726,8 @@ mr_control_step_kernel(KernelArgs args) {
     const uint32_t pcap = A.pair_capacity, mcap = A.manifold_capacity, ccap = A.constraint_capacity;
     e.stage_h = A.stage_h + (size_t)env * pcap;
     e.stage_p = A.stage_p + (size_t)env * pcap * MR_MANIFOLD_POINT_CAPACITY;
-    e.jrow = A.jrow + (size_t)env * ccap * 3u * A.max_nv;
-    e.resp = A.resp + (size_t)env * ccap * 3u * A.max_nv;
+    e.jrow = A.jrow + (size_t)env * ccap * 3u * A.row_stride;
+    e.resp = A.resp + (size_t)env * ccap * 3u * A.row_stride;
     e.cold = A.cold + (size_t)env * ccap;
     e.hot = A.hot + (size_t)env * ccap;
 
@@ -2246,13 +2246,13 @@ mr_control_step_kernel(KernelArgs args) {
             const V3 pa{ck.pa[0], ck.pa[1], ck.pa[2]};
             const V3 pb{ck.pb[0], ck.pb[1], ck.pb[2]};
             const V3 d = hot_dir(hk, row);
-            float* jr = e.jrow + ((size_t)c * 3u + (size_t)row) * A.max_nv;
+            float* jr = e.jrow + ((size_t)c * 3u + (size_t)row) * A.row_stride;
             for (uint32_t j = 0; j < nv; ++j) {
                 const V3 cb = lb ? point_jacobian_column(e, hk.body_b, pb, j) : V3{0.0f, 0.0f, 0.0f};
                 const V3 ca = la ? point_jacobian_column(e, hk.body_a, pa, j) : V3{0.0f, 0.0f, 0.0f};
                 jr[j] = dot(d, sub(cb, ca));
             }
-            float* rs = e.resp + ((size_t)c * 3u + (size_t)row) * A.max_nv;
+            float* rs = e.resp + ((size_t)c * 3u + (size_t)row) * A.row_stride;
             inverse_aba(e, lane, la ? hk.body_a : MR_INVALID_INDEX, pa, neg(d), lb ? hk.body_b : MR_INVALID_INDEX, pb, d, rs);
         }
         __syncthreads();
@@ -2262,17 +2262,20 @@ mr_control_step_kernel(KernelArgs args) {
         // per-constraint scalars are then evaluated lane-parallel over
         // constraints. target/winv carry rel and J M^-1 J^T between the two
         // passes.
-        const uint32_t mnv = A.max_nv;
+        const uint32_t mnv = A.row_stride;  // >= max(1, max_nv)
         const bool dof_lane = lane < nv;
         float vj = dof_lane ? e.vcand[lane] : 0.0f;
         // Jacobian / response rows of constraint c for this DoF lane; the
         // loops below fetch constraint c + 1 while working on c. Rows of
         // constraints without an articulation endpoint are never written and
         // their (unused) values are ignored.
+        // Branch-free on purpose: every lane loads (lanes past the stride read the
+        // row's last element, ignored), so the wait counters can leave the
+        // prefetch of the next constraint in flight.
+        const uint32_t row_lane = lane < mnv ? lane : mnv - 1u;
         auto load_rows = [&](uint32_t c, float* j3, float* r3) {
-            if (!dof_lane) return;
-            const float* jb = e.jrow + (size_t)c * 3u * mnv + lane;
-            const float* rb = e.resp + (size_t)c * 3u * mnv + lane;
+            const float* jb = e.jrow + (size_t)c * 3u * mnv + row_lane;
+            const float* rb = e.resp + (size_t)c * 3u * mnv + row_lane;
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
                 j3[r] = jb[(size_t)r * mnv];
@@ -2482,12 +2485,12 @@ mr_control_step_kernel(KernelArgs args) {
         if (s_fail != ~0ull) break;
         // ------------------------------------- residuals + impulse store
         float max_normal = 0.0f, max_cone = 0.0f;
-        float nj0 = (ncons > 0u && dof_lane) ? e.jrow[lane] : 0.0f;
+        float nj0 = ncons > 0u ? e.jrow[row_lane] : 0.0f;
         for (uint32_t c = 0; c < ncons; ++c) {
             const HotConstraint hk = e.hot[c];
             const float lamr[3] = {e.lam[3u * c], e.lam[3u * c + 1u], e.lam[3u * c + 2u]};
             const float j0 = nj0;
-            if (c + 1u < ncons && dof_lane) nj0 = e.jrow[(size_t)(c + 1u) * 3u * mnv + lane];
+            if (c + 1u < ncons) nj0 = e.jrow[(size_t)(c + 1u) * 3u * mnv + row_lane];
             const uint32_t kind = uniform_u32(hk.kind);
             V3 va, vb;
             scene_point_velocities(e, kind, uniform_u32(hk.body_a), uniform_u32(hk.body_b),

@@ -88,8 +88,9 @@ World::World(const ModelData& model, const MRWorldStepConfigC& config) : model_(
     hip_check(hipMemset(d_mcurrent_.ptr, 0, E * sizeof(uint32_t)), "hipMemset");
     d_stage_h_.reserve(E * pcap * sizeof(MRManifoldHeaderC));
     d_stage_p_.reserve(E * pcap * MR_MANIFOLD_POINT_CAPACITY * sizeof(MRManifoldPointC));
-    d_jrow_.reserve(E * ccap * 3u * model_.max_nv * sizeof(float));
-    d_resp_.reserve(E * ccap * 3u * model_.max_nv * sizeof(float));
+    const size_t row_stride = std::max(1u, model_.max_nv);
+    d_jrow_.reserve(E * ccap * 3u * row_stride * sizeof(float));
+    d_resp_.reserve(E * ccap * 3u * row_stride * sizeof(float));
     d_cold_.reserve(E * ccap * sizeof(ColdConstraint));
     d_hot_.reserve(E * ccap * sizeof(HotConstraint));
     d_accel_.reserve(E * model_.max_nv * sizeof(float) + 16u);
@@ -335,6 +336,7 @@ KernelArgs World::base_args(uint32_t step) const {
     a.environment_count = env_;
     a.max_nq = model_.max_nq;
     a.max_nv = model_.max_nv;
+    a.row_stride = std::max(1u, model_.max_nv);
     a.max_scene = model_.max_scene_bodies;
     a.control_step = step;
     a.pair_capacity = cfg_.pair_capacity;
