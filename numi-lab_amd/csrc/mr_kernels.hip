@@ -261,19 +261,19 @@ MRD_INL float dpp_fold_step(float x) {
 // d = 1 and 2 are quad permutations; after them every lane of a quad holds
 // the same bits (IEEE addition commutes), so the half-row mirror pairs quad 0
 // with quad 1 exactly as xor 4 would, and the row mirror then pairs the two
-// halves of a row as xor 8. The four row sums combine as (r0 + r1) + (r2 + r3).
+// halves of a row as xor 8. A 16-lane row swap then gives every lane the even
+// and the odd row of its row pair (xor 16), and a 32-lane swap the two halves
+// (xor 32): the four row sums combine as (r0 + r1) + (r2 + r3).
 // Must be called with all 64 lanes active; every lane returns the same value.
 MRD_INL float wave_fold(float x) {
     x = dpp_fold_step<0xB1>(x);   // quad_perm [1,0,3,2]: xor 1
     x = dpp_fold_step<0x4E>(x);   // quad_perm [2,3,0,1]: xor 2
     x = dpp_fold_step<0x141>(x);  // row_half_mirror: xor 4 on quad-uniform data
     x = dpp_fold_step<0x140>(x);  // row_mirror: xor 8 on half-row-uniform data
-    const int b = __float_as_int(x);
-    const float r0 = __int_as_float(__builtin_amdgcn_readlane(b, 0));
-    const float r1 = __int_as_float(__builtin_amdgcn_readlane(b, 16));
-    const float r2 = __int_as_float(__builtin_amdgcn_readlane(b, 32));
-    const float r3 = __int_as_float(__builtin_amdgcn_readlane(b, 48));
-    return (r0 + r1) + (r2 + r3);
+    const auto r16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    x = __uint_as_float(r16[0]) + __uint_as_float(r16[1]);  // even row + odd row
+    const auto r32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return __uint_as_float(r32[0]) + __uint_as_float(r32[1]);  // lower half + upper half
 }
 
 MRD_INL float wave_max(float v) {
@@ -1617,9 +1617,15 @@ MRD_INL void scene_point_velocities(const Env& e, uint32_t kind, uint32_t ba, ui
 }
 
 // Relative velocity of one constraint row: the articulation part `art_dot`
-// (J v, already folded across the DoF lanes) plus the scene-body point
-// velocities, combined in the oracle's order.
-MRD_INL float rel_row(V3 d, V3 va, V3 vb, float art_dot) { return (art_dot + dot(d, vb)) - dot(d, va); }
+// (J v, already folded across the DoF lanes) plus the point velocity of a
+// dynamic body B minus that of a dynamic body A (static and link endpoints
+// add nothing), in the oracle's order.
+MRD_INL float rel_row(uint32_t kind, V3 d, V3 va, V3 vb, float art_dot) {
+    float rel = art_dot;
+    if ((kind & mr::kHotDynamicB) != 0u) rel = rel + dot(d, vb);
+    if ((kind & mr::kHotDynamicA) != 0u) rel = rel - dot(d, va);
+    return rel;
+}
 
 MRD_INL uint32_t uniform_u32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
@@ -2305,7 +2311,7 @@ mr_control_step_kernel(KernelArgs args) {
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
                 const float a = art ? wave_fold(dof_lane ? jr[r] * vj : 0.0f) : 0.0f;
-                rel[r] = rel_row(hot_dir(hk, r), va, vb, a);
+                rel[r] = rel_row(kind, hot_dir(hk, r), va, vb, a);
             }
 #pragma unroll
             for (int r = 0; r < 3; ++r)
@@ -2442,7 +2448,7 @@ mr_control_step_kernel(KernelArgs args) {
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
                     const float a = art ? wave_fold(dof_lane ? jr[r] * vj : 0.0f) : 0.0f;
-                    rel[r] = rel_row(hot_dir(hk, r), va, vb, a);
+                    rel[r] = rel_row(kind, hot_dir(hk, r), va, vb, a);
                 }
                 float rhs[3];
                 const float lamc[3] = {lam0, lam1, lam2};
@@ -2466,9 +2472,11 @@ mr_control_step_kernel(KernelArgs args) {
                 // moving memory operations across these points.
                 __builtin_amdgcn_wave_barrier();
                 if (lane == 0u) {
-                    const V3 imp = add(add(mul(d0, hot_dir(hk, 0)), mul(d1, hot_dir(hk, 1))), mul(d2, hot_dir(hk, 2)));
-                    if (sa_dyn) apply_scene_impulse(e.scene[ba - nl], x, neg(imp));
-                    if (sb_dyn) apply_scene_impulse(e.scene[bb - nl], x, imp);
+                    if (sa_dyn || sb_dyn) {
+                        const V3 imp = add(add(mul(d0, hot_dir(hk, 0)), mul(d1, hot_dir(hk, 1))), mul(d2, hot_dir(hk, 2)));
+                        if (sa_dyn) apply_scene_impulse(e.scene[ba - nl], x, neg(imp));
+                        if (sb_dyn) apply_scene_impulse(e.scene[bb - nl], x, imp);
+                    }
                     e.lam[3u * c] = cand[0];
                     e.lam[3u * c + 1u] = cand[1];
                     e.lam[3u * c + 2u] = cand[2];
@@ -2496,7 +2504,7 @@ mr_control_step_kernel(KernelArgs args) {
             scene_point_velocities(e, kind, uniform_u32(hk.body_a), uniform_u32(hk.body_b),
                                    V3{hk.point[0], hk.point[1], hk.point[2]}, va, vb);
             const float a0 = (kind & mr::kHotArticulated) != 0u ? wave_fold(dof_lane ? j0 * vj : 0.0f) : 0.0f;
-            const float rel0 = rel_row(hot_dir(hk, 0), va, vb, a0);
+            const float rel0 = rel_row(kind, hot_dir(hk, 0), va, vb, a0);
             const float eq = (rel0 - hk.target[0]) + hk.reg[0] * lamr[0];
             const float nres = lamr[0] > kConeEpsilon ? fabs_(eq) : fmaxd(-eq, 0.0f);
             max_normal = fmaxd(max_normal, nres);

@@ -1914,8 +1914,12 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
         const V3 vb = sb_dyn ? scene_point_velocity(wk.bodies[k.body_b], k.point_world) : V3{0.0f, 0.0f, 0.0f};
         const bool art = is_link(ctx, k.body_a) || is_link(ctx, k.body_b);
         for (int r = 0; r < 3; ++r) {
-            const float a = art ? fold_dot(&jrow[((size_t)c * 3u + (size_t)r) * nv], vel, nv) : 0.0f;
-            rel[r] = (a + dot(k.dir[r], vb)) - dot(k.dir[r], va);
+            // articulation part, plus the point velocity of a dynamic body B,
+            // minus that of a dynamic body A (other endpoints add nothing)
+            float a = art ? fold_dot(&jrow[((size_t)c * 3u + (size_t)r) * nv], vel, nv) : 0.0f;
+            if (sb_dyn) a = a + dot(k.dir[r], vb);
+            if (sa_dyn) a = a - dot(k.dir[r], va);
+            rel[r] = a;
         }
     };
     const float slop = ctx.cfg->contact_slop;
