@@ -131,6 +131,9 @@ def main():
                     help="final-sweep max |dlambda| target of the adaptive leg (0 disables the leg)")
     ap.add_argument("--residual-max-iterations", type=int, default=128)
     ap.add_argument("--residual-steps", type=int, default=50)
+    ap.add_argument("--stage-profile", type=int, default=0, metavar="STEPS",
+                    help="after the timed region, run STEPS more control steps with the per-stage shader-clock "
+                         "counters on and report each stage's share of the cycles")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the timed region, run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic that scales FETCH_SIZE / WRITE_SIZE counter readings)")
@@ -225,6 +228,20 @@ def main():
                      "kernel_ms_per_launch": round(avg_launch_s * 1e3, 4)},
         "cpu_baseline": None,
     }
+    if args.stage_profile > 0:
+        prof_world = mr.World(model, cfg)
+        prof_world.enqueue_device(commands[:args.warmup])
+        prof_world.wait()
+        prof_world.set_stage_profiling(True)
+        n = min(args.stage_profile, commands.shape[0] - args.warmup)
+        prof_world.enqueue_device(commands[args.warmup:args.warmup + n])
+        prof_world.wait()
+        cyc = prof_world.stage_cycles().astype(np.float64)[:, :len(mr.World.STAGE_NAMES)]
+        out["stage_cycles"] = {
+            "steps": int(n), "cycles_per_env_step": round(float(cyc.sum() / args.envs / n), 0),
+            "fraction": {name: round(float(f), 4) for name, f in zip(mr.World.STAGE_NAMES, cyc.sum(0) / cyc.sum())},
+        }
+        del prof_world
     if args.copy_calibration_mib > 0:
         n = args.copy_calibration_mib * 1024 * 1024 // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda:0")
