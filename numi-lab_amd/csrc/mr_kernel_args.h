@@ -185,4 +185,15 @@ struct KernelArgs {
     LdsLayout lds;
 };
 
+// Device-side summary of one advance (all steps x environments), reduced on
+// the GPU so that waiting for an advance reads back 24 bytes instead of
+// every status record.
+struct AdvanceSummaryDev {
+    unsigned long long first_failure; // (step << 40) | (environment << 8) | code; ~0 if none
+    uint32_t failed;
+    uint32_t max_active_pairs;
+    uint32_t max_constraints;
+    uint32_t pad;
+};
+
 } // namespace mr

@@ -2706,7 +2706,66 @@ mr_control_step_kernel(KernelArgs args) {
     }
 }
 
+// Advance summary: one thread clears it, then a grid-stride pass over the
+// [steps][environments] status records folds it with per-wave reductions and
+// one atomic per wave (kernels rather than a memset, so the sequence is also
+// valid inside a captured graph).
+extern "C" __global__ void mr_summary_clear_kernel(mr::AdvanceSummaryDev* s) {
+    if (threadIdx.x == 0u && blockIdx.x == 0u) {
+        s->first_failure = ~0ull;
+        s->failed = 0u;
+        s->max_active_pairs = 0u;
+        s->max_constraints = 0u;
+        s->pad = 0u;
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+mr_summary_reduce_kernel(const MRWorldStatusC* status, const MRContactStatusC* cstatus, uint64_t count,
+                         uint32_t environments, mr::AdvanceSummaryDev* s) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t failed = 0u, pairs = 0u, cons = 0u;
+    unsigned long long first = ~0ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t code = status[i].code;
+        if (code != MR_STEP_SUCCESS) {
+            ++failed;
+            const unsigned long long step = i / environments, env = i % environments;
+            const unsigned long long key = (step << 40) | (env << 8) | (unsigned long long)(code & 0xffu);
+            first = key < first ? key : first;
+        }
+        const MRContactStatusC& c = cstatus[i];
+        pairs = c.max_active_pairs > pairs ? c.max_active_pairs : pairs;
+        cons = c.max_constraints > cons ? c.max_constraints : cons;
+    }
+    failed = wave_sum_u32(failed);
+    pairs = wave_max_u32(pairs);
+    cons = wave_max_u32(cons);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const unsigned long long o = ((unsigned long long)__shfl_xor((int)(first >> 32), d, 64) << 32) |
+                                     (unsigned long long)(uint32_t)__shfl_xor((int)(first & 0xffffffffull), d, 64);
+        first = o < first ? o : first;
+    }
+    if (lane == 0u) {
+        if (failed != 0u) atomicAdd(&s->failed, failed);
+        atomicMax(&s->max_active_pairs, pairs);
+        atomicMax(&s->max_constraints, cons);
+        if (first != ~0ull) atomicMin(&s->first_failure, first);
+    }
+}
+
 namespace mr {
+
+hipError_t launch_advance_summary(const MRWorldStatusC* status, const MRContactStatusC* cstatus, uint64_t count,
+                                  uint32_t environments, AdvanceSummaryDev* summary, hipStream_t stream) {
+    hipLaunchKernelGGL(mr_summary_clear_kernel, dim3(1), dim3(64), 0, stream, summary);
+    const uint64_t blocks = (count + 255u) / 256u;
+    const uint32_t grid = (uint32_t)(blocks < 1024u ? (blocks > 0u ? blocks : 1u) : 1024u);
+    hipLaunchKernelGGL(mr_summary_reduce_kernel, dim3(grid), dim3(256), 0, stream, status, cstatus, count,
+                       environments, summary);
+    return hipGetLastError();
+}
 
 static_assert(sizeof(SpI) == 84, "SpI layout");
 static_assert(sizeof(Collider) == 124, "Collider layout");

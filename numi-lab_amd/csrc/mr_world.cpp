@@ -11,6 +11,8 @@ namespace mr {
 
 hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream);
 hipError_t configure_control_step(uint32_t lds_bytes);
+hipError_t launch_advance_summary(const MRWorldStatusC* status, const MRContactStatusC* cstatus, uint64_t count,
+                                  uint32_t environments, AdvanceSummaryDev* summary, hipStream_t stream);
 
 namespace {
 constexpr uint32_t kSpIBytes = 84u;
@@ -241,7 +243,7 @@ size_t World::retained_bytes() const {
                                  &d_pairs_, &d_heights_, &d_q_, &d_v_, &d_scene_, &d_mh_, &d_mp_, &d_mcount_,
                                  &d_mcurrent_, &d_reset_q_, &d_reset_v_, &d_reset_scene_, &d_commands_, &d_masks_,
                                  &d_status_, &d_cstatus_, &d_accel_, &d_ev_pairs_, &d_ev_counts_, &d_stage_h_,
-                                 &d_stage_p_, &d_jrow_, &d_resp_, &d_cold_, &d_hot_};
+                                 &d_stage_p_, &d_jrow_, &d_resp_, &d_cold_, &d_hot_, &d_summary_};
     size_t total = 0;
     for (const DeviceBuffer* b : all) total += b->bytes;
     return total;
@@ -411,6 +413,11 @@ void World::enqueue(const float* commands, const uint32_t* masks, uint32_t steps
         if (step_timing_) hip_check(hipEventRecord(step_events_[s + 1u], stream_), "event");
     }
     hip_check(hipEventRecord(ev_end_, stream_), "event");
+    d_summary_.reserve(sizeof(AdvanceSummaryDev));
+    hip_check(launch_advance_summary(static_cast<const MRWorldStatusC*>(d_status_.ptr),
+                                     static_cast<const MRContactStatusC*>(d_cstatus_.ptr), (uint64_t)steps * E,
+                                     (uint32_t)E, static_cast<AdvanceSummaryDev*>(d_summary_.ptr), stream_),
+              "advance summary");
     last_steps_ = steps;
 }
 
@@ -420,25 +427,18 @@ AdvanceSummary World::wait() {
     AdvanceSummary s;
     s.control_step_count = last_steps_;
     s.gpu_milliseconds = last_kernel_ms_;
-    const size_t E = env_, n = (size_t)last_steps_ * E;
-    std::vector<MRWorldStatusC> st(n);
-    std::vector<MRContactStatusC> cs(n);
-    hip_check(hipMemcpy(st.data(), d_status_.ptr, n * sizeof(MRWorldStatusC), hipMemcpyDeviceToHost), "status");
-    hip_check(hipMemcpy(cs.data(), d_cstatus_.ptr, n * sizeof(MRContactStatusC), hipMemcpyDeviceToHost), "cstatus");
-    for (size_t i = 0; i < n; ++i) {
-        if (st[i].code == MR_STEP_SUCCESS) {
-            ++s.successful_environment_steps;
-        } else {
-            ++s.failed_environment_steps;
-            if (s.first_failing_environment == MR_INVALID_INDEX) {
-                s.first_failing_environment = (uint32_t)(i % E);
-                s.first_failing_control_step = (uint32_t)(i / E);
-                s.first_gpu_status_code = st[i].code;
-            }
-        }
-        s.maximum_active_pairs = std::max(s.maximum_active_pairs, cs[i].max_active_pairs);
-        s.maximum_constraints = std::max(s.maximum_constraints, cs[i].max_constraints);
+    AdvanceSummaryDev d;
+    hip_check(hipMemcpy(&d, d_summary_.ptr, sizeof(d), hipMemcpyDeviceToHost), "advance summary");
+    const uint64_t n = (uint64_t)last_steps_ * env_;
+    s.failed_environment_steps = d.failed;
+    s.successful_environment_steps = (uint32_t)(n - d.failed);
+    if (d.first_failure != ~0ull) {
+        s.first_failing_control_step = (uint32_t)(d.first_failure >> 40);
+        s.first_failing_environment = (uint32_t)((d.first_failure >> 8) & 0xffffffffull);
+        s.first_gpu_status_code = (uint32_t)(d.first_failure & 0xffull);
     }
+    s.maximum_active_pairs = d.max_active_pairs;
+    s.maximum_constraints = d.max_constraints;
     return s;
 }
 

@@ -93,7 +93,7 @@ private:
     // per-advance inputs / outputs
     DeviceBuffer d_commands_, d_masks_, d_status_, d_cstatus_, d_accel_, d_ev_pairs_, d_ev_counts_;
     // scratch
-    DeviceBuffer d_stage_h_, d_stage_p_, d_jrow_, d_resp_, d_cold_, d_hot_, d_stage_cycles_;
+    DeviceBuffer d_stage_h_, d_stage_p_, d_jrow_, d_resp_, d_cold_, d_hot_, d_stage_cycles_, d_summary_;
     bool stage_profiling_ = false;
     uint32_t last_steps_ = 0;
     float last_kernel_ms_ = 0.0f;

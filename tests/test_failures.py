@@ -213,3 +213,31 @@ def test_gpu_reset_mask_matches_oracle(gpu):
     q, v, s = world.state()
     np.testing.assert_array_equal(q, st.q)
     np.testing.assert_array_equal(v, st.v)
+
+
+@pytest.mark.gpu
+def test_gpu_advance_summary_matches_status_records(gpu):
+    """The advance summary (reduced on the device) equals a host reduction of
+    the per-step status records: failed count, first failing (step, env) and
+    its code, and the contact maxima."""
+    w, cfg, q, v, scene = crowded_boxes(pair_capacity=64, manifold_capacity=32, constraint_capacity=64)
+    world = mr.World(w.model, cfg)
+    world.set_state(q, v, scene)
+    steps = 6
+    masks = np.zeros((steps, 2), np.uint32)
+    out = world.advance(np.zeros((steps, 2, 0), np.float32), masks)
+    st = world.statuses()
+    cs = world.contact_statuses()
+    codes = st["code"].reshape(-1)
+    assert out.failed_environment_steps == int(np.sum(codes != 0))
+    assert out.successful_environment_steps == int(np.sum(codes == 0))
+    assert out.maximum_active_pairs == int(cs["max_active_pairs"].max())
+    assert out.maximum_constraints == int(cs["max_constraints"].max())
+    # a world that fails: the tiny-pivot chain fails every environment, every step
+    tw = tiny_pivot_chain(envs=3)
+    tworld = mr.World(tw.model, tw.config())
+    tout = tworld.advance(np.zeros((2, 3, 2), np.float32))
+    tst = tworld.statuses()
+    assert tout.failed_environment_steps == 6
+    assert tout.first_failing_control_step == 0 and tout.first_failing_environment == 0
+    assert tout.first_gpu_status_code == int(tst["code"][0, 0]) == FACTORIZATION_FAILED
