@@ -132,6 +132,7 @@ struct LdsLayout {
     uint32_t rslot;  // [slots][6][64] floats: outward spatial accelerations
     uint32_t rstack; // [stack][64] floats: joint terms of the inward path
     uint32_t isl;    // [MR_MAX_BODIES + 1] u8 island parents + used flags (after responses)
+    uint32_t resid;  // [2][constraint_capacity] floats: per-constraint residual terms (after responses)
     uint32_t total;
 };
 

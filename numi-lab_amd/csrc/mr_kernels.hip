@@ -193,6 +193,7 @@ struct Env {
     uint32_t* cpair;     // LDS [ccap]: body_a | body_b << 8 | kind << 16
     float *rslot, *rstack;
     uint8_t* isl;
+    float* resid;
     // global scratch of this environment
     MRManifoldHeaderC* stage_h;
     MRManifoldPointC* stage_p;
@@ -1729,6 +1730,7 @@ mr_control_step_kernel(KernelArgs args) {
     e.rslot = lds_at<float>(lds_base, L.rslot);
     e.rstack = lds_at<float>(lds_base, L.rstack);
     e.isl = lds_at<uint8_t>(lds_base, L.isl);
+    e.resid = lds_at<float>(lds_base, L.resid);
     const uint32_t pcap = A.pair_capacity, mcap = A.manifold_capacity, ccap = A.constraint_capacity;
     e.stage_h = A.stage_h + (size_t)env * pcap;
     e.stage_p = A.stage_p + (size_t)env * pcap * MR_MANIFOLD_POINT_CAPACITY;
@@ -2492,34 +2494,52 @@ mr_control_step_kernel(KernelArgs args) {
         clk.mark(mr::kStageSweeps);
         if (s_fail != ~0ull) break;
         // ------------------------------------- residuals + impulse store
-        float max_normal = 0.0f, max_cone = 0.0f;
+        // Pass 1, DoF lanes: the normal-row relative velocity of every
+        // constraint (articulation fold plus dynamic scene endpoints).
+        float* res_n = e.resid;                 // [ccap]: rel0, then the normal residual
+        float* res_c = e.resid + ccap;          // [ccap]: cone violation
         float nj0 = ncons > 0u ? e.jrow[row_lane] : 0.0f;
         for (uint32_t c = 0; c < ncons; ++c) {
-            const HotConstraint hk = e.hot[c];
-            const float lamr[3] = {e.lam[3u * c], e.lam[3u * c + 1u], e.lam[3u * c + 2u]};
             const float j0 = nj0;
             if (c + 1u < ncons) nj0 = e.jrow[(size_t)(c + 1u) * 3u * mnv + row_lane];
-            const uint32_t kind = uniform_u32(hk.kind);
-            V3 va, vb;
-            scene_point_velocities(e, kind, uniform_u32(hk.body_a), uniform_u32(hk.body_b),
-                                   V3{hk.point[0], hk.point[1], hk.point[2]}, va, vb);
+            const uint32_t kind = uniform_u32(e.cpair[c] >> 16);
             const float a0 = (kind & mr::kHotArticulated) != 0u ? wave_fold(dof_lane ? j0 * vj : 0.0f) : 0.0f;
-            const float rel0 = rel_row(kind, hot_dir(hk, 0), va, vb, a0);
-            const float eq = (rel0 - hk.target[0]) + hk.reg[0] * lamr[0];
+            float rel0 = a0;
+            if ((kind & (mr::kHotDynamicA | mr::kHotDynamicB)) != 0u) {
+                const HotConstraint& hk = e.hot[c];
+                V3 va, vb;
+                scene_point_velocities(e, kind, uniform_u32(hk.body_a), uniform_u32(hk.body_b),
+                                       V3{hk.point[0], hk.point[1], hk.point[2]}, va, vb);
+                rel0 = rel_row(kind, hot_dir(hk, 0), va, vb, a0);
+            }
+            if (lane == 0u) res_n[c] = rel0;
+        }
+        __syncthreads();
+        // Pass 2, constraint lanes: residual terms and the impulse store.
+        for (uint32_t c = lane; c < ncons; c += kLanes) {
+            const HotConstraint hk = e.hot[c];
+            const float lamr[3] = {e.lam[3u * c], e.lam[3u * c + 1u], e.lam[3u * c + 2u]};
+            const float eq = (res_n[c] - hk.target[0]) + hk.reg[0] * lamr[0];
             const float nres = lamr[0] > kConeEpsilon ? fabs_(eq) : fmaxd(-eq, 0.0f);
-            max_normal = fmaxd(max_normal, nres);
             const float lu = hk.mu_u * lamr[0], lv = hk.mu_v * lamr[0];
             float viol;
             if (lu > 0.0f && lv > 0.0f)
                 viol = fmaxd(sqrt_rn((lamr[1] * lamr[1]) / (lu * lu) + (lamr[2] * lamr[2]) / (lv * lv)) - 1.0f, 0.0f);
             else
                 viol = sqrt_rn(lamr[1] * lamr[1] + lamr[2] * lamr[2]);
-            max_cone = fmaxd(max_cone, viol);
-            if (lane == (c & (kLanes - 1u))) {
-                const ColdConstraint& ck = e.cold[c];
-                new_p[(size_t)ck.manifold * MR_MANIFOLD_POINT_CAPACITY + ck.point].impulses =
-                    mr_float4{lamr[0], lamr[1], lamr[2], 0.0f};
-            }
+            res_n[c] = nres;
+            res_c[c] = viol;
+            const ColdConstraint& ck = e.cold[c];
+            new_p[(size_t)ck.manifold * MR_MANIFOLD_POINT_CAPACITY + ck.point].impulses =
+                mr_float4{lamr[0], lamr[1], lamr[2], 0.0f};
+        }
+        __syncthreads();
+        // Pass 3, uniform: maxima in constraint order (the oracle's order,
+        // including its handling of a non-finite term).
+        float max_normal = 0.0f, max_cone = 0.0f;
+        for (uint32_t c = 0; c < ncons; ++c) {
+            max_normal = fmaxd(max_normal, res_n[c]);
+            max_cone = fmaxd(max_cone, res_c[c]);
         }
         if (dof_lane) e.vcand[lane] = vj;
         __syncthreads();

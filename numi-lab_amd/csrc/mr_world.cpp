@@ -231,6 +231,7 @@ void World::compute_lds_layout() {
     union_end = std::max(union_end, off);
     off = union_start;
     L.isl = take(2u * (MR_MAX_BODIES + 1u), 4u);
+    L.resid = take(8u * cfg_.constraint_capacity, 4u);
     off = std::max(union_end, off);
     L.total = (off + 15u) / 16u * 16u;
     if (L.total > kMaxLdsBytes)
