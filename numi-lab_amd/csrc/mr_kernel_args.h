@@ -33,17 +33,26 @@ enum HotKind : uint32_t {
 };
 
 // Per-constraint data of the cone solve (global scratch, read-only during
-// the sweeps); the impulses themselves live in LDS.
+// the sweeps); the impulses themselves live in LDS. The first 64 bytes are
+// what every sweep reads (the tangent rows' targets and regularisation are
+// always zero); the geometry behind them is read only for constraints with a
+// dynamic scene endpoint.
 struct HotConstraint {
+    float winv[9];       // 3x3 block inverse (J M^-1 J^T while it is evaluated)
+    float target0, reg0; // normal-row target velocity and regularisation
+    float mu_u, mu_v;
+    float stage[3];      // relative velocity rows between the evaluation passes
     uint32_t body_a, body_b;
     uint32_t flags, kind; // kind: HotKind bits
     float dir[9];        // normal, tangent u, tangent v
     float point[3];      // contact midpoint
-    float target[3];
-    float reg[3];
-    float mu_u, mu_v;
+};
+
+// The leading 64 bytes of a HotConstraint.
+struct SolveBlock {
     float winv[9];
-    float pad[3];
+    float target0, reg0, mu_u, mu_v;
+    float stage[3];
 };
 
 // Per-constraint data produced by emission and consumed by the Jacobian and

@@ -35,6 +35,7 @@
 using namespace mrd;
 using mr::ColdConstraint;
 using mr::HotConstraint;
+using mr::SolveBlock;
 using mr::KernelArgs;
 
 namespace {
@@ -2291,29 +2292,23 @@ mr_control_step_kernel(KernelArgs args) {
             }
         };
         float nj[3] = {0.0f, 0.0f, 0.0f}, nr[3] = {0.0f, 0.0f, 0.0f};
-        HotConstraint hnext;
-        if (ncons > 0u) {
-            load_rows(0u, nj, nr);
-            hnext = e.hot[0];
-        }
+        if (ncons > 0u) load_rows(0u, nj, nr);
         for (uint32_t c = 0; c < ncons; ++c) {
             const float jr[3] = {nj[0], nj[1], nj[2]};
             const float rr[3] = {nr[0], nr[1], nr[2]};
-            const HotConstraint hk = hnext;
-            if (c + 1u < ncons) {
-                load_rows(c + 1u, nj, nr);
-                hnext = e.hot[c + 1u];
-            }
-            const uint32_t kind = uniform_u32(hk.kind);
+            if (c + 1u < ncons) load_rows(c + 1u, nj, nr);
+            const uint32_t kind = uniform_u32(e.cpair[c] >> 16);
             const bool art = (kind & mr::kHotArticulated) != 0u;
-            V3 va, vb;
-            scene_point_velocities(e, kind, uniform_u32(hk.body_a), uniform_u32(hk.body_b),
-                                   V3{hk.point[0], hk.point[1], hk.point[2]}, va, vb);
             float rel[3], jm[9];
 #pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                const float a = art ? wave_fold(dof_lane ? jr[r] * vj : 0.0f) : 0.0f;
-                rel[r] = rel_row(kind, hot_dir(hk, r), va, vb, a);
+            for (int r = 0; r < 3; ++r) rel[r] = art ? wave_fold(dof_lane ? jr[r] * vj : 0.0f) : 0.0f;
+            if ((kind & (mr::kHotDynamicA | mr::kHotDynamicB)) != 0u) {
+                const HotConstraint& hk = e.hot[c];
+                V3 va, vb;
+                scene_point_velocities(e, kind, uniform_u32(hk.body_a), uniform_u32(hk.body_b),
+                                       V3{hk.point[0], hk.point[1], hk.point[2]}, va, vb);
+#pragma unroll
+                for (int r = 0; r < 3; ++r) rel[r] = rel_row(kind, hot_dir(hk, r), va, vb, rel[r]);
             }
 #pragma unroll
             for (int r = 0; r < 3; ++r)
@@ -2322,9 +2317,9 @@ mr_control_step_kernel(KernelArgs args) {
                     jm[3 * r + q] = art ? wave_fold(dof_lane ? jr[r] * rr[q] : 0.0f) : 0.0f;
             if (lane == 0u) {
                 HotConstraint& hw = e.hot[c];
-                hw.target[0] = rel[0];
-                hw.target[1] = rel[1];
-                hw.target[2] = rel[2];
+                hw.stage[0] = rel[0];
+                hw.stage[1] = rel[1];
+                hw.stage[2] = rel[2];
 #pragma unroll
                 for (int k = 0; k < 9; ++k) hw.winv[k] = jm[k];
             }
@@ -2335,7 +2330,7 @@ mr_control_step_kernel(KernelArgs args) {
         for (uint32_t c = lane; c < ncons; c += kLanes) {
             HotConstraint hk = e.hot[c];
             const ColdConstraint ck = e.cold[c];
-            const float rel[3] = {hk.target[0], hk.target[1], hk.target[2]};
+            const float rel[3] = {hk.stage[0], hk.stage[1], hk.stage[2]};
             const float slip = sqrt_rn(rel[1] * rel[1] + rel[2] * rel[2]);
             const bool stat = slip <= 1.0e-3f;
             hk.mu_u = stat ? ck.friction[0] : ck.friction[2];
@@ -2351,14 +2346,10 @@ mr_control_step_kernel(KernelArgs args) {
                 const float incoming = rel[0] - src_target;
                 if (incoming < -ck.restitution_threshold) stab = fmaxd(stab, -ck.restitution * incoming);
             }
-            hk.target[0] = src_target + stab;
-            hk.target[1] = 0.0f;
-            hk.target[2] = 0.0f;
-            hk.reg[0] = fmaxd(ck.compliance / (h * h) + ck.dissipation / h, 0.0f);
-            hk.reg[1] = 0.0f;
-            hk.reg[2] = 0.0f;
+            hk.target0 = src_target + stab;
+            hk.reg0 = fmaxd(ck.compliance / (h * h) + ck.dissipation / h, 0.0f);
             bool ok = true;
-            if (!isfin(hk.target[0]) || !isfin(hk.reg[0])) {
+            if (!isfin(hk.target0) || !isfin(hk.reg0)) {
                 report_failure(&s_fail, c, MR_STEP_NONFINITE_RESULT, c);
                 ok = false;
             }
@@ -2371,7 +2362,7 @@ mr_control_step_kernel(KernelArgs args) {
                         float val = hk.winv[3 * r + q];
                         if (sa_dyn) val += dot(hot_dir(hk, r), scene_response(e.scene[hk.body_a - nl], x, hot_dir(hk, q)));
                         if (sb_dyn) val += dot(hot_dir(hk, r), scene_response(e.scene[hk.body_b - nl], x, hot_dir(hk, q)));
-                        if (r == q) val += hk.reg[r];
+                        if (r == q) val += r == 0 ? hk.reg0 : 0.0f;  // tangent rows carry none
                         W[r][q] = val;
                     }
                 if (!invert3x3(W, hk.winv)) report_failure(&s_fail, c, MR_STEP_FACTORIZATION_FAILED, c);
@@ -2425,40 +2416,54 @@ mr_control_step_kernel(KernelArgs args) {
         float sweep_delta = 0.0f;
         uint32_t iters = 0;
         bool solve_failed = false;
+        const SolveBlock* solve = reinterpret_cast<const SolveBlock*>(e.hot);
+        const uint32_t hot_words = (uint32_t)(sizeof(HotConstraint) / sizeof(SolveBlock));
+        SolveBlock snext;
         if (ncons > 0u) {
             load_rows(0u, nj, nr);
-            hnext = e.hot[0];
+            snext = solve[0];
         }
         for (uint32_t it = 0; it < max_iters && ncons > 0u; ++it) {
             sweep_delta = 0.0f;
             for (uint32_t c = 0; c < ncons; ++c) {
                 const float jr[3] = {nj[0], nj[1], nj[2]};
                 const float rr[3] = {nr[0], nr[1], nr[2]};
-                const HotConstraint hk = hnext;
+                const SolveBlock sb = snext;
                 const uint32_t cn = c + 1u < ncons ? c + 1u : 0u;
                 load_rows(cn, nj, nr);
-                hnext = e.hot[cn];
+                snext = solve[(size_t)cn * hot_words];
                 const float lam0 = e.lam[3u * c], lam1 = e.lam[3u * c + 1u], lam2 = e.lam[3u * c + 2u];
-                const uint32_t kind = uniform_u32(hk.kind);
-                const uint32_t ba = uniform_u32(hk.body_a), bb = uniform_u32(hk.body_b);
+                const uint32_t kind = uniform_u32(e.cpair[c] >> 16);
                 const bool art = (kind & mr::kHotArticulated) != 0u;
                 const bool sa_dyn = (kind & mr::kHotDynamicA) != 0u, sb_dyn = (kind & mr::kHotDynamicB) != 0u;
-                const V3 x{hk.point[0], hk.point[1], hk.point[2]};
-                V3 va, vb;
-                scene_point_velocities(e, kind, ba, bb, x, va, vb);
                 float rel[3];
 #pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    const float a = art ? wave_fold(dof_lane ? jr[r] * vj : 0.0f) : 0.0f;
-                    rel[r] = rel_row(kind, hot_dir(hk, r), va, vb, a);
+                for (int r = 0; r < 3; ++r) rel[r] = art ? wave_fold(dof_lane ? jr[r] * vj : 0.0f) : 0.0f;
+                V3 x{0.0f, 0.0f, 0.0f};
+                uint32_t ba = 0u, bb = 0u;
+                V3 dirs[3];
+                if (sa_dyn || sb_dyn) {
+                    const HotConstraint& hk = e.hot[c];
+                    ba = uniform_u32(hk.body_a);
+                    bb = uniform_u32(hk.body_b);
+                    x = V3{hk.point[0], hk.point[1], hk.point[2]};
+                    V3 va, vb;
+                    scene_point_velocities(e, kind, ba, bb, x, va, vb);
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        dirs[r] = hot_dir(hk, r);
+                        rel[r] = rel_row(kind, dirs[r], va, vb, rel[r]);
+                    }
                 }
-                float rhs[3];
                 const float lamc[3] = {lam0, lam1, lam2};
-                for (int r = 0; r < 3; ++r) rhs[r] = (hk.target[r] - rel[r]) - hk.reg[r] * lamc[r];
+                // the tangent rows' stored target and regularisation are +0
+                const float tgt[3] = {sb.target0, 0.0f, 0.0f}, reg[3] = {sb.reg0, 0.0f, 0.0f};
+                float rhs[3];
+                for (int r = 0; r < 3; ++r) rhs[r] = (tgt[r] - rel[r]) - reg[r] * lamc[r];
                 float cand[3] = {lam0, lam1, lam2};
                 for (int r = 0; r < 3; ++r)
-                    for (int q = 0; q < 3; ++q) cand[r] += hk.winv[3 * r + q] * rhs[q];
-                project_cone(cand, hk.mu_u, hk.mu_v);
+                    for (int q = 0; q < 3; ++q) cand[r] += sb.winv[3 * r + q] * rhs[q];
+                project_cone(cand, sb.mu_u, sb.mu_v);
                 const float d0 = cand[0] - lam0, d1 = cand[1] - lam1, d2 = cand[2] - lam2;
                 if (!isfin(cand[0]) || !isfin(cand[1]) || !isfin(cand[2]) || !isfin(d0) || !isfin(d1) || !isfin(d2)) {
                     if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_NONFINITE_RESULT << 24) | (c & 0xffffffu);
@@ -2475,7 +2480,7 @@ mr_control_step_kernel(KernelArgs args) {
                 __builtin_amdgcn_wave_barrier();
                 if (lane == 0u) {
                     if (sa_dyn || sb_dyn) {
-                        const V3 imp = add(add(mul(d0, hot_dir(hk, 0)), mul(d1, hot_dir(hk, 1))), mul(d2, hot_dir(hk, 2)));
+                        const V3 imp = add(add(mul(d0, dirs[0]), mul(d1, dirs[1])), mul(d2, dirs[2]));
                         if (sa_dyn) apply_scene_impulse(e.scene[ba - nl], x, neg(imp));
                         if (sb_dyn) apply_scene_impulse(e.scene[bb - nl], x, imp);
                     }
@@ -2519,7 +2524,7 @@ mr_control_step_kernel(KernelArgs args) {
         for (uint32_t c = lane; c < ncons; c += kLanes) {
             const HotConstraint hk = e.hot[c];
             const float lamr[3] = {e.lam[3u * c], e.lam[3u * c + 1u], e.lam[3u * c + 2u]};
-            const float eq = (res_n[c] - hk.target[0]) + hk.reg[0] * lamr[0];
+            const float eq = (res_n[c] - hk.target0) + hk.reg0 * lamr[0];
             const float nres = lamr[0] > kConeEpsilon ? fabs_(eq) : fmaxd(-eq, 0.0f);
             const float lu = hk.mu_u * lamr[0], lv = hk.mu_v * lamr[0];
             float viol;
@@ -2789,7 +2794,8 @@ hipError_t launch_advance_summary(const MRWorldStatusC* status, const MRContactS
 
 static_assert(sizeof(SpI) == 84, "SpI layout");
 static_assert(sizeof(Collider) == 124, "Collider layout");
-static_assert(sizeof(HotConstraint) == 144, "HotConstraint layout");
+static_assert(sizeof(HotConstraint) == 128, "HotConstraint layout");
+static_assert(sizeof(SolveBlock) == 64 && sizeof(HotConstraint) % sizeof(SolveBlock) == 0, "SolveBlock layout");
 
 hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream) {
     hipLaunchKernelGGL(mr_control_step_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
