@@ -278,6 +278,33 @@ MRD_INL float wave_fold(float x) {
     return __uint_as_float(r32[0]) + __uint_as_float(r32[1]);  // lower half + upper half
 }
 
+// wave_fold of N independent values, step by step across all of them so that
+// the N dependency chains interleave; each result has the bits wave_fold gives.
+template <int N> MRD_INL void wave_fold_n(float (&x)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = dpp_fold_step<0xB1>(x[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = dpp_fold_step<0x4E>(x[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = dpp_fold_step<0x141>(x[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = dpp_fold_step<0x140>(x[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const auto r16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(x[i]), __float_as_uint(x[i]), false, false);
+        x[i] = __uint_as_float(r16[0]) + __uint_as_float(r16[1]);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const auto r32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[i]), __float_as_uint(x[i]), false, false);
+        x[i] = __uint_as_float(r32[0]) + __uint_as_float(r32[1]);
+    }
+}
+
+// True on every lane when `bad` holds on any active lane: a uniform branch
+// condition for values every lane computes alike.
+MRD_INL bool wave_any(bool bad) { return __builtin_amdgcn_ballot_w64(bad) != 0ull; }
+
 MRD_INL float wave_max(float v) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v = fmaxd(v, __shfl_xor(v, d, 64));
@@ -2299,9 +2326,18 @@ mr_control_step_kernel(KernelArgs args) {
             if (c + 1u < ncons) load_rows(c + 1u, nj, nr);
             const uint32_t kind = uniform_u32(e.cpair[c] >> 16);
             const bool art = (kind & mr::kHotArticulated) != 0u;
-            float rel[3], jm[9];
+            // articulated part: rows 0-2 J v, rows 3-11 the 3x3 block J M^-1 J^T
+            float f[12] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            if (art) {
 #pragma unroll
-            for (int r = 0; r < 3; ++r) rel[r] = art ? wave_fold(dof_lane ? jr[r] * vj : 0.0f) : 0.0f;
+                for (int r = 0; r < 3; ++r) {
+                    f[r] = dof_lane ? jr[r] * vj : 0.0f;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) f[3 + 3 * r + q] = dof_lane ? jr[r] * rr[q] : 0.0f;
+                }
+                wave_fold_n(f);
+            }
+            float rel[3] = {f[0], f[1], f[2]};
             if ((kind & (mr::kHotDynamicA | mr::kHotDynamicB)) != 0u) {
                 const HotConstraint& hk = e.hot[c];
                 V3 va, vb;
@@ -2310,18 +2346,13 @@ mr_control_step_kernel(KernelArgs args) {
 #pragma unroll
                 for (int r = 0; r < 3; ++r) rel[r] = rel_row(kind, hot_dir(hk, r), va, vb, rel[r]);
             }
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int q = 0; q < 3; ++q)
-                    jm[3 * r + q] = art ? wave_fold(dof_lane ? jr[r] * rr[q] : 0.0f) : 0.0f;
             if (lane == 0u) {
                 HotConstraint& hw = e.hot[c];
                 hw.stage[0] = rel[0];
                 hw.stage[1] = rel[1];
                 hw.stage[2] = rel[2];
 #pragma unroll
-                for (int k = 0; k < 9; ++k) hw.winv[k] = jm[k];
+                for (int k = 0; k < 9; ++k) hw.winv[k] = f[3 + k];
             }
         }
         __syncthreads();
@@ -2418,7 +2449,7 @@ mr_control_step_kernel(KernelArgs args) {
         bool solve_failed = false;
         const SolveBlock* solve = reinterpret_cast<const SolveBlock*>(e.hot);
         const uint32_t hot_words = (uint32_t)(sizeof(HotConstraint) / sizeof(SolveBlock));
-        SolveBlock snext;
+        SolveBlock snext{};
         if (ncons > 0u) {
             load_rows(0u, nj, nr);
             snext = solve[0];
@@ -2436,12 +2467,15 @@ mr_control_step_kernel(KernelArgs args) {
                 const uint32_t kind = uniform_u32(e.cpair[c] >> 16);
                 const bool art = (kind & mr::kHotArticulated) != 0u;
                 const bool sa_dyn = (kind & mr::kHotDynamicA) != 0u, sb_dyn = (kind & mr::kHotDynamicB) != 0u;
-                float rel[3];
+                float rel[3] = {0.0f, 0.0f, 0.0f};
+                if (art) {
 #pragma unroll
-                for (int r = 0; r < 3; ++r) rel[r] = art ? wave_fold(dof_lane ? jr[r] * vj : 0.0f) : 0.0f;
+                    for (int r = 0; r < 3; ++r) rel[r] = dof_lane ? jr[r] * vj : 0.0f;
+                    wave_fold_n(rel);
+                }
                 V3 x{0.0f, 0.0f, 0.0f};
                 uint32_t ba = 0u, bb = 0u;
-                V3 dirs[3];
+                V3 dirs[3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
                 if (sa_dyn || sb_dyn) {
                     const HotConstraint& hk = e.hot[c];
                     ba = uniform_u32(hk.body_a);
@@ -2465,7 +2499,7 @@ mr_control_step_kernel(KernelArgs args) {
                     for (int q = 0; q < 3; ++q) cand[r] += sb.winv[3 * r + q] * rhs[q];
                 project_cone(cand, sb.mu_u, sb.mu_v);
                 const float d0 = cand[0] - lam0, d1 = cand[1] - lam1, d2 = cand[2] - lam2;
-                if (!isfin(cand[0]) || !isfin(cand[1]) || !isfin(cand[2]) || !isfin(d0) || !isfin(d1) || !isfin(d2)) {
+                if (wave_any(!isfin(cand[0]) | !isfin(cand[1]) | !isfin(cand[2]) | !isfin(d0) | !isfin(d1) | !isfin(d2))) {
                     if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_NONFINITE_RESULT << 24) | (c & 0xffffffu);
                     solve_failed = true;
                     break;
