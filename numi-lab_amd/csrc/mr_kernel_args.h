@@ -149,6 +149,7 @@ struct KernelArgs {
     // immutable model
     const MRModelHeaderC* models;
     const uint32_t* environment_model;
+    const uint8_t* response_slots;  // [model_count][MR_MAX_BODIES]: plan_response_slots per model
     const MRBodyC* bodies;
     const MRDofC* dofs;
     const MRShapeC* shapes;

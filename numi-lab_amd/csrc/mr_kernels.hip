@@ -1813,7 +1813,7 @@ mr_control_step_kernel(KernelArgs args) {
         }
         T.anc[lane] = anc;
         T.children[lane] = ch;
-        if (lane == 0u) mr::plan_response_slots([&](uint32_t i) { return (uint32_t)T.parent[i]; }, e.nlinks, T.slot);
+        T.slot[lane] = A.response_slots[(size_t)A.environment_model[env] * MR_MAX_BODIES + lane];
     }
 
     const uint32_t nv = e.nv, nq = e.nq, nl = e.nlinks, nsc = e.nscene;

@@ -68,6 +68,17 @@ World::World(const ModelData& model, const MRWorldStepConfigC& config) : model_(
     hip_check(hipEventCreate(&ev_end_), "hipEventCreate");
     upload(d_models_, model_.models);
     upload(d_env_model_, model_.environment_model);
+    {
+        // outward response slots per model (the plan depends on the tree only)
+        std::vector<uint8_t> plan(model_.models.size() * MR_MAX_BODIES, 0xffu);
+        for (size_t m = 0; m < model_.models.size(); ++m) {
+            const MRModelHeaderC& h = model_.models[m];
+            const MRBodyC* b = model_.bodies.data() + h.body_offset;
+            if (h.link_count > 0u)
+                plan_response_slots([&](uint32_t i) { return b[i].parent; }, h.link_count, &plan[m * MR_MAX_BODIES]);
+        }
+        upload(d_slot_plan_, plan);
+    }
     upload(d_bodies_, model_.bodies);
     upload(d_dofs_, model_.dofs);
     upload(d_shapes_, model_.shapes);
@@ -240,7 +251,7 @@ void World::compute_lds_layout() {
 }
 
 size_t World::retained_bytes() const {
-    const DeviceBuffer* all[] = {&d_models_, &d_env_model_, &d_bodies_, &d_dofs_, &d_shapes_, &d_materials_,
+    const DeviceBuffer* all[] = {&d_models_, &d_env_model_, &d_slot_plan_, &d_bodies_, &d_dofs_, &d_shapes_, &d_materials_,
                                  &d_pairs_, &d_heights_, &d_q_, &d_v_, &d_scene_, &d_mh_, &d_mp_, &d_mcount_,
                                  &d_mcurrent_, &d_reset_q_, &d_reset_v_, &d_reset_scene_, &d_commands_, &d_masks_,
                                  &d_status_, &d_cstatus_, &d_accel_, &d_ev_pairs_, &d_ev_counts_, &d_stage_h_,
@@ -323,6 +334,7 @@ KernelArgs World::base_args(uint32_t step) const {
     KernelArgs a;
     std::memset(&a, 0, sizeof(a));
     a.models = static_cast<const MRModelHeaderC*>(d_models_.ptr);
+    a.response_slots = static_cast<const uint8_t*>(d_slot_plan_.ptr);
     a.environment_model = static_cast<const uint32_t*>(d_env_model_.ptr);
     a.bodies = static_cast<const MRBodyC*>(d_bodies_.ptr);
     a.dofs = static_cast<const MRDofC*>(d_dofs_.ptr);

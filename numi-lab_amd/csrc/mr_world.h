@@ -86,7 +86,7 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_begin_ = nullptr, ev_end_ = nullptr;
     // model
-    DeviceBuffer d_models_, d_env_model_, d_bodies_, d_dofs_, d_shapes_, d_materials_, d_pairs_, d_heights_;
+    DeviceBuffer d_models_, d_env_model_, d_slot_plan_, d_bodies_, d_dofs_, d_shapes_, d_materials_, d_pairs_, d_heights_;
     // state
     DeviceBuffer d_q_, d_v_, d_scene_, d_mh_, d_mp_, d_mcount_, d_mcurrent_;
     DeviceBuffer d_reset_q_, d_reset_v_, d_reset_scene_;
