@@ -29,6 +29,7 @@ HOST_FLAGS = [
 ]
 SOURCES = [
     ("mr_kernels.hip", "hip"),
+    ("mr_mlp.hip", "hip"),
     ("mr_builder.cpp", "cxx"),
     ("mr_world.cpp", "cxx"),
     ("mr_c_api.cpp", "cxx"),

@@ -3,6 +3,7 @@
 #include "../../include/mr_c_api.h"
 
 #include "mr_builder.h"
+#include "mr_mlp_host.h"
 #include "mr_world.h"
 
 #include <algorithm>
@@ -545,6 +546,70 @@ int mr_task_rollout_copy_final_scene_states(const MRTaskRolloutHandle* handle, f
             o[7] = s.linear_velocity_inv_mass.x; o[8] = s.linear_velocity_inv_mass.y; o[9] = s.linear_velocity_inv_mass.z;
             o[10] = s.angular_velocity.x; o[11] = s.angular_velocity.y; o[12] = s.angular_velocity.z;
         }
+    });
+}
+
+// ------------------------------------------------------- warm-start MLP
+
+namespace {
+void require_weights(const MRMlpWeightsC* w) {
+    require(w != nullptr, "null MLP weights");
+    require(w->w1 && w->b1 && w->w2 && w->b2 && w->w3 && w->b3 && w->in_shift && w->in_scale && w->out_scale,
+            "MLP weights: every pointer must be set");
+}
+} // namespace
+
+int mr_mlp_forward(const MRMlpWeightsC* weights, const float* x, uint32_t rows, float* y_hat,
+                   const MRMlpTraceC* trace, void* stream) {
+    return translate([&] {
+        require_weights(weights);
+        require(x != nullptr && y_hat != nullptr, "mr_mlp_forward: null input or output");
+        require(rows > 0u, "mr_mlp_forward: rows must be positive");
+        if (trace != nullptr)
+            require(trace->xq && trace->z1 && trace->h1 && trace->z2 && trace->h2,
+                    "mr_mlp_forward: a trace needs every buffer");
+        mr::hip_check(mr::mlp_forward(*weights, x, rows, y_hat, trace, static_cast<hipStream_t>(stream)),
+                  "mr_mlp_forward launch");
+    });
+}
+
+uint32_t mr_mlp_loss_scratch_floats(uint32_t rows) { return mr::mlp_loss_scratch_floats(rows); }
+
+int mr_mlp_loss(const float* y_hat, const float* target, const float* out_scale, uint32_t rows, float* dy,
+                float* loss, float* scratch, void* stream) {
+    return translate([&] {
+        require(y_hat && target && out_scale && dy && loss && scratch, "mr_mlp_loss: null pointer");
+        require(rows > 0u && rows <= 0x7fffffffu / MR_MLP_OUTPUTS, "mr_mlp_loss: rows out of range");
+        mr::hip_check(mr::mlp_loss(y_hat, target, out_scale, rows, dy, loss, scratch, static_cast<hipStream_t>(stream)),
+                  "mr_mlp_loss launch");
+    });
+}
+
+int mr_mlp_backward(const MRMlpWeightsC* weights, const MRMlpTraceC* trace, const float* dy, uint32_t rows,
+                    float* dz2, float* dz1, const MRMlpGradsC* grads, void* stream) {
+    return translate([&] {
+        require_weights(weights);
+        require(trace != nullptr && trace->xq && trace->z1 && trace->h1 && trace->z2 && trace->h2,
+                "mr_mlp_backward: a trace needs every buffer");
+        require(grads != nullptr && grads->w1 && grads->b1 && grads->w2 && grads->b2 && grads->w3 && grads->b3,
+                "mr_mlp_backward: every gradient buffer must be set");
+        require(dy && dz2 && dz1, "mr_mlp_backward: null pointer");
+        require(rows > 0u && rows <= 65535u * 16u, "mr_mlp_backward: rows must be in 1..1048560");
+        mr::hip_check(mr::mlp_backward(*weights, *trace, dy, rows, dz2, dz1, *grads, static_cast<hipStream_t>(stream)),
+                  "mr_mlp_backward launch");
+    });
+}
+
+int mr_mlp_adam(float* param, uint16_t* param_bf16, const float* grad, float* m1, float* m2, uint32_t count,
+                float lr, float beta1, float beta2, float eps, uint32_t step, void* stream) {
+    return translate([&] {
+        require(param && grad && m1 && m2, "mr_mlp_adam: null pointer");
+        require(count > 0u && step >= 1u, "mr_mlp_adam: count and step must be positive");
+        require(beta1 >= 0.0f && beta1 < 1.0f && beta2 >= 0.0f && beta2 < 1.0f && eps > 0.0f && lr >= 0.0f,
+                "mr_mlp_adam: hyper-parameters out of range");
+        mr::hip_check(mr::mlp_adam(param, param_bf16, grad, m1, m2, count, lr, beta1, beta2, eps, step,
+                               static_cast<hipStream_t>(stream)),
+                  "mr_mlp_adam launch");
     });
 }
 

@@ -131,6 +131,25 @@ class TaskRolloutLayoutC(ct.Structure):
     ]
 
 
+class MlpWeightsC(ct.Structure):
+    """Mirror of MRMlpWeightsC (include/mr_mlp.h): device pointers."""
+    _fields_ = [("w1", ct.c_void_p), ("b1", ct.c_void_p), ("w2", ct.c_void_p), ("b2", ct.c_void_p),
+                ("w3", ct.c_void_p), ("b3", ct.c_void_p), ("in_shift", ct.c_void_p), ("in_scale", ct.c_void_p),
+                ("out_scale", ct.c_void_p)]
+
+
+class MlpTraceC(ct.Structure):
+    """Mirror of MRMlpTraceC."""
+    _fields_ = [("xq", ct.c_void_p), ("z1", ct.c_void_p), ("h1", ct.c_void_p), ("z2", ct.c_void_p),
+                ("h2", ct.c_void_p)]
+
+
+class MlpGradsC(ct.Structure):
+    """Mirror of MRMlpGradsC."""
+    _fields_ = [("w1", ct.c_void_p), ("b1", ct.c_void_p), ("w2", ct.c_void_p), ("b2", ct.c_void_p),
+                ("w3", ct.c_void_p), ("b3", ct.c_void_p)]
+
+
 class TaskRolloutAdvanceC(ct.Structure):
     _fields_ = [
         ("control_step_count", ct.c_uint32), ("successful_environment_steps", ct.c_uint32),
@@ -279,6 +298,15 @@ _SIGNATURES: Final = {
     "mr_task_rollout_active_contacts": ([ct.c_void_p], ct.POINTER(ct.c_uint32)),
     "mr_task_rollout_final_q": ([ct.c_void_p], ct.POINTER(ct.c_float)),
     "mr_task_rollout_copy_final_scene_states": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
+    "mr_mlp_forward": ([ct.POINTER(MlpWeightsC), ct.c_void_p, ct.c_uint32, ct.c_void_p, ct.POINTER(MlpTraceC),
+                        ct.c_void_p], ct.c_int),
+    "mr_mlp_loss_scratch_floats": ([ct.c_uint32], ct.c_uint32),
+    "mr_mlp_loss": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                     ct.c_void_p], ct.c_int),
+    "mr_mlp_backward": ([ct.POINTER(MlpWeightsC), ct.POINTER(MlpTraceC), ct.c_void_p, ct.c_uint32, ct.c_void_p,
+                         ct.c_void_p, ct.POINTER(MlpGradsC), ct.c_void_p], ct.c_int),
+    "mr_mlp_adam": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32, ct.c_float,
+                     ct.c_float, ct.c_float, ct.c_float, ct.c_uint32, ct.c_void_p], ct.c_int),
 }
 
 

@@ -1,5 +1,5 @@
 """The C-ABI library loads, exports every entry point declared in
-include/mr_c_api.h, agrees with the C compiler on record layouts, and reports
+include/*.h, agrees with the C compiler on record layouts, and reports
 errors through the thread-local mr_last_error (c_api.cpp:183, 321-332
 conventions) — all without launching GPU work."""
 
@@ -17,13 +17,16 @@ import metalrobo as mr
 from metalrobo import native as nat
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "mr_c_api.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if h.endswith(".h")]
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"\b(mr_[a-z0-9_]+)\s*\(", text)
+    names = []
+    for header in HEADERS:
+        text = open(header).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"\b(mr_[a-z0-9_]+)\s*\(", text)
     return sorted(set(names))
 
 
