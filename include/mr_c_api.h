@@ -17,6 +17,7 @@
 #ifndef MR_C_API_H
 #define MR_C_API_H
 
+#include "mr_mlp.h"
 #include "mr_world_types.h"
 
 #include <stddef.h>
@@ -147,6 +148,21 @@ int mr_world_read_step_milliseconds(const MRWorldHandle* world, double* out, siz
  * warm start, sweeps, residuals, integration). */
 int mr_world_set_stage_profiling(MRWorldHandle* world, uint32_t enabled);
 int mr_world_read_stage_cycles(const MRWorldHandle* world, uint64_t* out, size_t count);
+/* Learned warm start (no counterpart in the reference; see mr_mlp.h): with
+ * non-null device weights, later advances run the MLP instance of the
+ * control-step kernel, which predicts each contact's impulse from its
+ * features at the start of the cone solve and warm-starts from it; a
+ * microstep whose sweeps then miss residual_tolerance is solved again from
+ * the plain warm start (contact status diagnostics.z counts these). Needs
+ * the adaptive solve (residual_tolerance > 0). The weights must stay valid
+ * while set; null restores the plain path. */
+int mr_world_set_warm_start_mlp(MRWorldHandle* world, const MRMlpWeightsC* weights);
+/* Training-sample capture at microstep capture_substep of each control step:
+ * per constraint, the 8 features, the converged impulse (3) and a valid flag
+ * (1.0 for the step's constraints, 0.0 for unused rows). */
+int mr_world_set_mlp_capture(MRWorldHandle* world, uint32_t enabled);
+/* [E][constraint_capacity][12] floats of the last captured control step. */
+int mr_world_read_mlp_samples(const MRWorldHandle* world, float* out, size_t count);
 
 /* ------------------------------------------------- task rollout --- */
 

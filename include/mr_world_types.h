@@ -333,9 +333,11 @@ typedef struct MRContactStatusC {
     uint32_t first_failing_constraint;
     uint32_t max_active_pairs;      /* high water over the control step */
     uint32_t max_constraints;       /* high water over the control step */
-    /* final-sweep max |delta impulse|, normal residual, cone violation, unused */
+    /* final-sweep max |delta impulse|, normal residual, cone violation,
+     * cone sweeps run over the control step (all microsteps, fallbacks included) */
     mr_float4 residuals;
-    /* max penetration, retained fraction, unused, unused */
+    /* max penetration, retained fraction, microsteps whose learned warm start
+     * fell back to the plain one, unused */
     mr_float4 diagnostics;
 } MRContactStatusC;
 

@@ -380,6 +380,31 @@ int mr_world_read_stage_cycles(const MRWorldHandle* world, uint64_t* out, size_t
     });
 }
 
+int mr_world_set_warm_start_mlp(MRWorldHandle* world, const MRMlpWeightsC* weights) {
+    return translate([&]() {
+        require(world != nullptr, "null argument");
+        world->world->set_warm_start_mlp(weights);
+    });
+}
+
+int mr_world_set_mlp_capture(MRWorldHandle* world, uint32_t enabled) {
+    return translate([&]() {
+        require(world != nullptr, "null argument");
+        world->world->set_mlp_capture(enabled != 0u);
+    });
+}
+
+int mr_world_read_mlp_samples(const MRWorldHandle* world, float* out, size_t count) {
+    return translate([&]() {
+        require(world != nullptr && out != nullptr, "null argument");
+        require_count(count,
+                      (size_t)world->world->environment_count() * world->world->config().constraint_capacity *
+                          mr::kMlpSampleFloats,
+                      "MLP samples");
+        world->world->read_mlp_samples(out);
+    });
+}
+
 // ------------------------------------------------------------ task rollout
 
 MRTaskRolloutHandle* mr_create_task_rollout(const MRRunManifestC* manifest) {

@@ -2,6 +2,7 @@
 // executor and the device code.
 #pragma once
 
+#include "../../include/mr_mlp.h"
 #include "../../include/mr_world_types.h"
 
 #include <stdint.h>
@@ -142,6 +143,10 @@ struct LdsLayout {
     uint32_t rstack; // [stack][64] floats: joint terms of the inward path
     uint32_t isl;    // [MR_MAX_BODIES + 1] u8 island parents + used flags (after responses)
     uint32_t resid;  // [2][constraint_capacity] floats: per-constraint residual terms (after responses)
+    // MLP instance of the kernel only (past the plain layout's end):
+    uint32_t mlp;        // warm-start MLP tile scratch (kMlpTileLdsBytes)
+    uint32_t mlp_lam;    // [constraint_capacity][3] floats: the plain warm start, kept for the fallback
+    uint32_t mlp_scene;  // [max_scene_bodies] MRBodyStateC: scene bodies before the warm start
     uint32_t total;
 };
 
@@ -194,7 +199,17 @@ struct KernelArgs {
     HotConstraint* hot;           // [E][constraint_capacity]
     unsigned long long* stage_cycles; // [E][kStageCount] shader-clock cycles per stage, or null
     LdsLayout lds;
+    // warm-start MLP (read by the MLP instance of the kernel only)
+    MRMlpWeightsC mlp;     // device weights, used when mlp_enabled
+    uint32_t mlp_enabled;
+    float* mlp_samples;    // [E][constraint_capacity][kMlpSampleFloats] or null: capture at cfg.capture_substep
 };
+
+// LDS bytes of one warm-start MLP tile: 16 x 8 FP32 inputs and two
+// 16 x 264 BF16 hidden layers.
+constexpr uint32_t kMlpTileLdsBytes = 16u * 8u * 4u + 2u * 16u * 264u * 2u;
+// One captured sample: 8 features, the converged impulse (3), a valid flag.
+constexpr uint32_t kMlpSampleFloats = 12u;
 
 // Device-side summary of one advance (all steps x environments), reduced on
 // the GPU so that waiting for an advance reads back 24 bytes instead of

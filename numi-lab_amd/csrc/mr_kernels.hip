@@ -31,6 +31,9 @@
 
 #include "mr_device_math.h"
 #include "mr_kernel_args.h"
+#include "mr_mlp_tile.h"
+
+#include <mutex>
 
 using namespace mrd;
 using mr::ColdConstraint;
@@ -195,6 +198,10 @@ struct Env {
     float *rslot, *rstack;
     uint8_t* isl;
     float* resid;
+    // MLP instance only
+    unsigned char* mlp_lds;  // warm-start MLP tile scratch
+    float* mlp_lam;          // [ccap][3] plain warm start, kept for the fallback
+    MRBodyStateC* mlp_scene; // scene bodies before the warm start
     // global scratch of this environment
     MRManifoldHeaderC* stage_h;
     MRManifoldPointC* stage_p;
@@ -1678,13 +1685,116 @@ struct StageClock {
 
 } // namespace
 
+// ------------------------------------------------ learned warm start
+
+static_assert(mr::kMlpTileLdsBytes == mr::kMlpLdsBytes, "MLP tile LDS size");
+
+// Features of constraint c at the start of its cone solve: normal relative
+// velocity (J v_free, the evaluation's staged row), penetration depth, the
+// Euclidean norms of the normal and tangent Jacobian rows (articulated DoFs
+// plus the twist coordinates (dir, r x dir) of each dynamic scene endpoint)
+// and the projected previous-step impulse the plain path warm-starts from.
+MRD_INL void contact_features(const Env& e, uint32_t c, uint32_t mnv, float f[mr::kMlpIn]) {
+    const HotConstraint& hk = e.hot[c];
+    const ColdConstraint& ck = e.cold[c];
+    const uint32_t kind = e.cpair[c] >> 16;
+    f[0] = hk.stage[0];
+    f[1] = -ck.separation;
+    const V3 x{hk.point[0], hk.point[1], hk.point[2]};
+    for (int r = 0; r < 3; ++r) {
+        float s2 = 0.0f;
+        if ((kind & mr::kHotArticulated) != 0u) {
+            const float* row = e.jrow + ((size_t)c * 3u + (uint32_t)r) * mnv;
+            for (uint32_t j = 0; j < e.nv; ++j) s2 += row[j] * row[j];
+        }
+        const V3 d = hot_dir(hk, r);
+        if ((kind & mr::kHotDynamicA) != 0u) {
+            const V3 t = cross(sub(x, f3(e.scene[hk.body_a - e.nlinks].position)), d);
+            s2 += dot(d, d) + dot(t, t);
+        }
+        if ((kind & mr::kHotDynamicB) != 0u) {
+            const V3 t = cross(sub(x, f3(e.scene[hk.body_b - e.nlinks].position)), d);
+            s2 += dot(d, d) + dot(t, t);
+        }
+        f[2 + r] = sqrt_rn(s2);
+    }
+    f[5] = e.lam[3u * c];
+    f[6] = e.lam[3u * c + 1u];
+    f[7] = e.lam[3u * c + 2u];
+}
+
+MRD_INL mr::MlpView mlp_view(const MRMlpWeightsC& w) {
+    mr::MlpView m;
+    m.w1 = reinterpret_cast<const __bf16*>(w.w1);
+    m.b1 = w.b1;
+    m.w2 = reinterpret_cast<const __bf16*>(w.w2);
+    m.b2 = w.b2;
+    m.w3 = reinterpret_cast<const __bf16*>(w.w3);
+    m.b3 = w.b3;
+    m.in_shift = w.in_shift;
+    m.in_scale = w.in_scale;
+    m.out_scale = w.out_scale;
+    return m;
+}
+
+// Features of every constraint, 16 at a time: written to `samples` (the
+// environment's capture rows, or null) and, with `infer`, run through the
+// MLP; each prediction, projected onto its cone, replaces the plain warm
+// start (kept in e.mlp_lam, as are the scene bodies, for the fallback) unless
+// it is not finite. Uniform in `infer`; all 64 lanes must be active.
+MRD_INL void mlp_warm_start(const Env& e, uint32_t lane, uint32_t ncons, uint32_t mnv, bool infer, float* samples) {
+    const mr::MlpLds s = mr::mlp_lds(e.mlp_lds);
+    const mr::MlpView m = mlp_view(e.a->mlp);
+    if (infer) {
+        for (uint32_t i = lane; i < 3u * ncons; i += kLanes) e.mlp_lam[i] = e.lam[i];
+        for (uint32_t b = lane; b < e.nscene; b += kLanes) e.mlp_scene[b] = e.scene[b];
+    }
+    for (uint32_t t0 = 0; t0 < ncons; t0 += mr::kMlpRows) {
+        __syncthreads();
+        if (lane < mr::kMlpRows) {
+            const uint32_t c = t0 + lane;
+            float f[mr::kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            if (c < ncons) {
+                contact_features(e, c, mnv, f);
+                if (samples != nullptr)
+                    for (uint32_t k = 0; k < mr::kMlpIn; ++k) samples[(size_t)c * mr::kMlpSampleFloats + k] = f[k];
+            }
+            if (infer)
+                for (uint32_t k = 0; k < mr::kMlpIn; ++k)
+                    s.x[lane * mr::kMlpIn + k] = c < ncons ? (f[k] - m.in_shift[k]) * m.in_scale[k] : 0.0f;
+        }
+        __syncthreads();
+        if (!infer) continue;
+        float y[4];
+        mr::mlp_tile_forward(m, s, lane, y, nullptr, 0u, mr::kMlpRows);
+        __syncthreads();
+        const uint32_t col = lane & 15u;
+        if (col < mr::kMlpOut) {
+            const float scale = m.out_scale[col];
+            for (uint32_t r = 0; r < 4u; ++r) s.x[((lane >> 4) * 4u + r) * mr::kMlpIn + col] = y[r] * scale;
+        }
+        __syncthreads();
+        if (lane < mr::kMlpRows && t0 + lane < ncons) {
+            const uint32_t c = t0 + lane;
+            float lam[3] = {s.x[lane * mr::kMlpIn], s.x[lane * mr::kMlpIn + 1u], s.x[lane * mr::kMlpIn + 2u]};
+            const HotConstraint& hk = e.hot[c];
+            project_cone(lam, hk.mu_u, hk.mu_v);
+            if (isfin(lam[0]) && isfin(lam[1]) && isfin(lam[2])) {
+                e.lam[3u * c] = lam[0];
+                e.lam[3u * c + 1u] = lam[1];
+                e.lam[3u * c + 2u] = lam[2];
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // --------------------------------------------------------------- kernel
 
-// Two wavefronts per SIMD: the register budget is capped at 256 so that two
-// worlds share each SIMD and hide each other's memory and LDS latency (the
-// per-world stages are latency-bound; one wave per SIMD left it idle).
-extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
-mr_control_step_kernel(KernelArgs args) {
+// One control step of one environment (one wavefront per workgroup). kMlp
+// selects the instance with the learned warm start and its sample capture;
+// the plain instance carries none of that code.
+template <bool kMlp> __device__ __forceinline__ void control_step(const KernelArgs& args) {
     extern __shared__ __align__(16) unsigned char lds_base[];
     __shared__ unsigned long long s_fail;
     __shared__ uint32_t s_count[8];
@@ -1759,6 +1869,11 @@ mr_control_step_kernel(KernelArgs args) {
     e.rstack = lds_at<float>(lds_base, L.rstack);
     e.isl = lds_at<uint8_t>(lds_base, L.isl);
     e.resid = lds_at<float>(lds_base, L.resid);
+    if constexpr (kMlp) {
+        e.mlp_lds = lds_base + L.mlp;
+        e.mlp_lam = lds_at<float>(lds_base, L.mlp_lam);
+        e.mlp_scene = lds_at<MRBodyStateC>(lds_base, L.mlp_scene);
+    }
     const uint32_t pcap = A.pair_capacity, mcap = A.manifold_capacity, ccap = A.constraint_capacity;
     e.stage_h = A.stage_h + (size_t)env * pcap;
     e.stage_p = A.stage_p + (size_t)env * pcap * MR_MANIFOLD_POINT_CAPACITY;
@@ -2408,129 +2523,172 @@ mr_control_step_kernel(KernelArgs args) {
         __syncthreads();
         clk.mark(mr::kStageEvaluation);
         if (s_fail != ~0ull) break;
-        // ---------------------------------------------------- warm start
-        // DoF lanes accumulate the warm-start impulses in constraint order;
-        // scene lanes apply theirs in the same order.
-        if (dof_lane) {
-            for (uint32_t c = 0; c < ncons; ++c) {
-                if (((e.cpair[c] >> 16) & mr::kHotArticulated) == 0u) continue;
-                const float* rb = e.resp + (size_t)c * 3u * mnv + lane;
-                const float* lc = e.lam + 3u * c;
-                vj += (rb[0] * lc[0] + rb[mnv] * lc[1]) + rb[2u * mnv] * lc[2];
-            }
-        }
-        for (uint32_t s = lane; s < nsc; s += kLanes) {
-            const uint32_t b = nl + s;
-            if (!is_dynamic_scene(e, b)) continue;
-            MRBodyStateC st = e.scene[s];
-            for (uint32_t c = 0; c < ncons; ++c) {
-                const uint32_t pr = e.cpair[c];
-                if ((pr & 0xffu) != b && ((pr >> 8) & 0xffu) != b) continue;
-                const HotConstraint& hk = e.hot[c];
-                const float* lc = e.lam + 3u * c;
-                const V3 imp = add(add(mul(lc[0], hot_dir(hk, 0)), mul(lc[1], hot_dir(hk, 1))), mul(lc[2], hot_dir(hk, 2)));
-                const V3 x{hk.point[0], hk.point[1], hk.point[2]};
-                if (hk.body_a == b) apply_scene_impulse(st, x, neg(imp));
-                if (hk.body_b == b) apply_scene_impulse(st, x, imp);
-            }
-            e.scene[s] = st;
-        }
-        __syncthreads();
-        clk.mark(mr::kStageWarmStart);
-        // ------------------------------------------ ordered block sweeps
-        // Every lane runs the constraint loop; the block update is uniform,
-        // DoF lanes keep the candidate velocity in a register, and the next
-        // constraint's Jacobian / response rows are prefetched.
+        // -------------------------------------- learned warm start (MLP)
         const uint32_t base_iters = cfg.velocity_iterations + (final_sub ? cfg.final_velocity_iterations : 0u);
         const bool adaptive = cfg.residual_tolerance > 0.0f;
         const uint32_t max_iters = adaptive ? cfg.max_iterations : base_iters;
         float sweep_delta = 0.0f;
-        uint32_t iters = 0;
+        uint32_t iters = 0, sweeps_run = 0;
         bool solve_failed = false;
         const SolveBlock* solve = reinterpret_cast<const SolveBlock*>(e.hot);
         const uint32_t hot_words = (uint32_t)(sizeof(HotConstraint) / sizeof(SolveBlock));
-        SolveBlock snext{};
-        if (ncons > 0u) {
-            load_rows(0u, nj, nr);
-            snext = solve[0];
+        bool mlp_now = false;
+        float* capture_rows = nullptr;
+        const float vj_free = vj;
+        if constexpr (kMlp) {
+            mlp_now = A.mlp_enabled != 0u && ncons > 0u;
+            if (A.mlp_samples != nullptr && substep == cfg.capture_substep)
+                capture_rows = A.mlp_samples + (size_t)env * ccap * mr::kMlpSampleFloats;
+            if (mlp_now || capture_rows != nullptr) mlp_warm_start(e, lane, ncons, mnv, mlp_now, capture_rows);
         }
-        for (uint32_t it = 0; it < max_iters && ncons > 0u; ++it) {
+        // The plain path runs the attempt below once. With the MLP warm start, a
+        // microstep whose sweeps miss the residual tolerance is solved again
+        // from the plain warm start and the same free velocity.
+        for (uint32_t attempt = 0;; ++attempt) {
             sweep_delta = 0.0f;
-            for (uint32_t c = 0; c < ncons; ++c) {
-                const float jr[3] = {nj[0], nj[1], nj[2]};
-                const float rr[3] = {nr[0], nr[1], nr[2]};
-                const SolveBlock sb = snext;
-                const uint32_t cn = c + 1u < ncons ? c + 1u : 0u;
-                load_rows(cn, nj, nr);
-                snext = solve[(size_t)cn * hot_words];
-                const float lam0 = e.lam[3u * c], lam1 = e.lam[3u * c + 1u], lam2 = e.lam[3u * c + 2u];
-                const uint32_t kind = uniform_u32(e.cpair[c] >> 16);
-                const bool art = (kind & mr::kHotArticulated) != 0u;
-                const bool sa_dyn = (kind & mr::kHotDynamicA) != 0u, sb_dyn = (kind & mr::kHotDynamicB) != 0u;
-                float rel[3] = {0.0f, 0.0f, 0.0f};
-                if (art) {
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) rel[r] = dof_lane ? jr[r] * vj : 0.0f;
-                    wave_fold_n(rel);
+            iters = 0;
+            // ---------------------------------------------------- warm start
+            // DoF lanes accumulate the warm-start impulses in constraint order;
+            // scene lanes apply theirs in the same order.
+            if (dof_lane) {
+                for (uint32_t c = 0; c < ncons; ++c) {
+                    if (((e.cpair[c] >> 16) & mr::kHotArticulated) == 0u) continue;
+                    const float* rb = e.resp + (size_t)c * 3u * mnv + lane;
+                    const float* lc = e.lam + 3u * c;
+                    vj += (rb[0] * lc[0] + rb[mnv] * lc[1]) + rb[2u * mnv] * lc[2];
                 }
-                V3 x{0.0f, 0.0f, 0.0f};
-                uint32_t ba = 0u, bb = 0u;
-                V3 dirs[3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
-                if (sa_dyn || sb_dyn) {
-                    const HotConstraint& hk = e.hot[c];
-                    ba = uniform_u32(hk.body_a);
-                    bb = uniform_u32(hk.body_b);
-                    x = V3{hk.point[0], hk.point[1], hk.point[2]};
-                    V3 va, vb;
-                    scene_point_velocities(e, kind, ba, bb, x, va, vb);
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        dirs[r] = hot_dir(hk, r);
-                        rel[r] = rel_row(kind, dirs[r], va, vb, rel[r]);
-                    }
-                }
-                const float lamc[3] = {lam0, lam1, lam2};
-                // the tangent rows' stored target and regularisation are +0
-                const float tgt[3] = {sb.target0, 0.0f, 0.0f}, reg[3] = {sb.reg0, 0.0f, 0.0f};
-                float rhs[3];
-                for (int r = 0; r < 3; ++r) rhs[r] = (tgt[r] - rel[r]) - reg[r] * lamc[r];
-                float cand[3] = {lam0, lam1, lam2};
-                for (int r = 0; r < 3; ++r)
-                    for (int q = 0; q < 3; ++q) cand[r] += sb.winv[3 * r + q] * rhs[q];
-                project_cone(cand, sb.mu_u, sb.mu_v);
-                const float d0 = cand[0] - lam0, d1 = cand[1] - lam1, d2 = cand[2] - lam2;
-                if (wave_any(!isfin(cand[0]) | !isfin(cand[1]) | !isfin(cand[2]) | !isfin(d0) | !isfin(d1) | !isfin(d2))) {
-                    if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_NONFINITE_RESULT << 24) | (c & 0xffffffu);
-                    solve_failed = true;
-                    break;
-                }
-                sweep_delta = fmaxd(sweep_delta, fmaxd(fabs_(d0), fmaxd(fabs_(d1), fabs_(d2))));
-                if (art && dof_lane) vj += (rr[0] * d0 + rr[1] * d1) + rr[2] * d2;
-                // The workgroup is one wavefront and its LDS operations
-                // complete in issue order, so lane 0's writes below follow
-                // every lane's reads above and precede the next constraint's
-                // reads; the wave barrier only stops the compiler from
-                // moving memory operations across these points.
-                __builtin_amdgcn_wave_barrier();
-                if (lane == 0u) {
-                    if (sa_dyn || sb_dyn) {
-                        const V3 imp = add(add(mul(d0, dirs[0]), mul(d1, dirs[1])), mul(d2, dirs[2]));
-                        if (sa_dyn) apply_scene_impulse(e.scene[ba - nl], x, neg(imp));
-                        if (sb_dyn) apply_scene_impulse(e.scene[bb - nl], x, imp);
-                    }
-                    e.lam[3u * c] = cand[0];
-                    e.lam[3u * c + 1u] = cand[1];
-                    e.lam[3u * c + 2u] = cand[2];
-                }
-                __builtin_amdgcn_wave_barrier();
             }
-            if (solve_failed) break;
-            ++iters;
-            if (adaptive && sweep_delta <= cfg.residual_tolerance) break;
+            for (uint32_t s = lane; s < nsc; s += kLanes) {
+                const uint32_t b = nl + s;
+                if (!is_dynamic_scene(e, b)) continue;
+                MRBodyStateC st = e.scene[s];
+                for (uint32_t c = 0; c < ncons; ++c) {
+                    const uint32_t pr = e.cpair[c];
+                    if ((pr & 0xffu) != b && ((pr >> 8) & 0xffu) != b) continue;
+                    const HotConstraint& hk = e.hot[c];
+                    const float* lc = e.lam + 3u * c;
+                    const V3 imp = add(add(mul(lc[0], hot_dir(hk, 0)), mul(lc[1], hot_dir(hk, 1))), mul(lc[2], hot_dir(hk, 2)));
+                    const V3 x{hk.point[0], hk.point[1], hk.point[2]};
+                    if (hk.body_a == b) apply_scene_impulse(st, x, neg(imp));
+                    if (hk.body_b == b) apply_scene_impulse(st, x, imp);
+                }
+                e.scene[s] = st;
+            }
             __syncthreads();
+            clk.mark(mr::kStageWarmStart);
+            // ------------------------------------------ ordered block sweeps
+            // Every lane runs the constraint loop; the block update is uniform,
+            // DoF lanes keep the candidate velocity in a register, and the next
+            // constraint's Jacobian / response rows are prefetched.
+            SolveBlock snext{};
+            if (ncons > 0u) {
+                load_rows(0u, nj, nr);
+                snext = solve[0];
+            }
+            for (uint32_t it = 0; it < max_iters && ncons > 0u; ++it) {
+                sweep_delta = 0.0f;
+                for (uint32_t c = 0; c < ncons; ++c) {
+                    const float jr[3] = {nj[0], nj[1], nj[2]};
+                    const float rr[3] = {nr[0], nr[1], nr[2]};
+                    const SolveBlock sb = snext;
+                    const uint32_t cn = c + 1u < ncons ? c + 1u : 0u;
+                    load_rows(cn, nj, nr);
+                    snext = solve[(size_t)cn * hot_words];
+                    const float lam0 = e.lam[3u * c], lam1 = e.lam[3u * c + 1u], lam2 = e.lam[3u * c + 2u];
+                    const uint32_t kind = uniform_u32(e.cpair[c] >> 16);
+                    const bool art = (kind & mr::kHotArticulated) != 0u;
+                    const bool sa_dyn = (kind & mr::kHotDynamicA) != 0u, sb_dyn = (kind & mr::kHotDynamicB) != 0u;
+                    float rel[3] = {0.0f, 0.0f, 0.0f};
+                    if (art) {
+    #pragma unroll
+                        for (int r = 0; r < 3; ++r) rel[r] = dof_lane ? jr[r] * vj : 0.0f;
+                        wave_fold_n(rel);
+                    }
+                    V3 x{0.0f, 0.0f, 0.0f};
+                    uint32_t ba = 0u, bb = 0u;
+                    V3 dirs[3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+                    if (sa_dyn || sb_dyn) {
+                        const HotConstraint& hk = e.hot[c];
+                        ba = uniform_u32(hk.body_a);
+                        bb = uniform_u32(hk.body_b);
+                        x = V3{hk.point[0], hk.point[1], hk.point[2]};
+                        V3 va, vb;
+                        scene_point_velocities(e, kind, ba, bb, x, va, vb);
+    #pragma unroll
+                        for (int r = 0; r < 3; ++r) {
+                            dirs[r] = hot_dir(hk, r);
+                            rel[r] = rel_row(kind, dirs[r], va, vb, rel[r]);
+                        }
+                    }
+                    const float lamc[3] = {lam0, lam1, lam2};
+                    // the tangent rows' stored target and regularisation are +0
+                    const float tgt[3] = {sb.target0, 0.0f, 0.0f}, reg[3] = {sb.reg0, 0.0f, 0.0f};
+                    float rhs[3];
+                    for (int r = 0; r < 3; ++r) rhs[r] = (tgt[r] - rel[r]) - reg[r] * lamc[r];
+                    float cand[3] = {lam0, lam1, lam2};
+                    for (int r = 0; r < 3; ++r)
+                        for (int q = 0; q < 3; ++q) cand[r] += sb.winv[3 * r + q] * rhs[q];
+                    project_cone(cand, sb.mu_u, sb.mu_v);
+                    const float d0 = cand[0] - lam0, d1 = cand[1] - lam1, d2 = cand[2] - lam2;
+                    if (wave_any(!isfin(cand[0]) | !isfin(cand[1]) | !isfin(cand[2]) | !isfin(d0) | !isfin(d1) | !isfin(d2))) {
+                        if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_NONFINITE_RESULT << 24) | (c & 0xffffffu);
+                        solve_failed = true;
+                        break;
+                    }
+                    sweep_delta = fmaxd(sweep_delta, fmaxd(fabs_(d0), fmaxd(fabs_(d1), fabs_(d2))));
+                    if (art && dof_lane) vj += (rr[0] * d0 + rr[1] * d1) + rr[2] * d2;
+                    // The workgroup is one wavefront and its LDS operations
+                    // complete in issue order, so lane 0's writes below follow
+                    // every lane's reads above and precede the next constraint's
+                    // reads; the wave barrier only stops the compiler from
+                    // moving memory operations across these points.
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0u) {
+                        if (sa_dyn || sb_dyn) {
+                            const V3 imp = add(add(mul(d0, dirs[0]), mul(d1, dirs[1])), mul(d2, dirs[2]));
+                            if (sa_dyn) apply_scene_impulse(e.scene[ba - nl], x, neg(imp));
+                            if (sb_dyn) apply_scene_impulse(e.scene[bb - nl], x, imp);
+                        }
+                        e.lam[3u * c] = cand[0];
+                        e.lam[3u * c + 1u] = cand[1];
+                        e.lam[3u * c + 2u] = cand[2];
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+                if (solve_failed) break;
+                ++iters;
+                if (adaptive && sweep_delta <= cfg.residual_tolerance) break;
+                __syncthreads();
+            }
+            __syncthreads();
+            clk.mark(mr::kStageSweeps);
+            sweeps_run += iters;
+            if constexpr (kMlp) {
+                if (mlp_now && attempt == 0u && adaptive && !solve_failed && !(sweep_delta <= cfg.residual_tolerance)) {
+                    vj = vj_free;
+                    for (uint32_t i = lane; i < 3u * ncons; i += kLanes) e.lam[i] = e.mlp_lam[i];
+                    for (uint32_t b = lane; b < nsc; b += kLanes) e.scene[b] = e.mlp_scene[b];
+                    if (lane == 0u) cs.diagnostics.z += 1.0f;
+                    __syncthreads();
+                    continue;
+                }
+            }
+            break;
         }
-        __syncthreads();
-        clk.mark(mr::kStageSweeps);
+        if constexpr (kMlp) {
+            // capture targets: the impulse the solve converged to
+            if (capture_rows != nullptr && !solve_failed) {
+                for (uint32_t c = lane; c < ccap; c += kLanes) {
+                    float* row = capture_rows + (size_t)c * mr::kMlpSampleFloats;
+                    if (c < ncons) {
+                        row[8] = e.lam[3u * c];
+                        row[9] = e.lam[3u * c + 1u];
+                        row[10] = e.lam[3u * c + 2u];
+                    }
+                    row[11] = c < ncons ? 1.0f : 0.0f;
+                }
+            }
+        }
         if (s_fail != ~0ull) break;
         // ------------------------------------- residuals + impulse store
         // Pass 1, DoF lanes: the normal-row relative velocity of every
@@ -2588,7 +2746,8 @@ mr_control_step_kernel(KernelArgs args) {
             break;
         }
         cs.solver_iterations = ncons > 0u ? iters : 0u;
-        cs.residuals = mr_float4{ncons > 0u ? sweep_delta : 0.0f, max_normal, max_cone, 0.0f};
+        cs.residuals = mr_float4{ncons > 0u ? sweep_delta : 0.0f, max_normal, max_cone,
+                                 cs.residuals.w + (float)(ncons > 0u ? sweeps_run : 0u)};
         if (lane == 0u) {
             uint8_t* parent = e.isl;                      // LDS, dead response scratch
             uint8_t* used = e.isl + (MR_MAX_BODIES + 1u);
@@ -2765,6 +2924,22 @@ mr_control_step_kernel(KernelArgs args) {
     }
 }
 
+// Two wavefronts per SIMD: the register budget is capped at 256 so that two
+// worlds share each SIMD and hide each other's memory and LDS latency (the
+// per-world stages are latency-bound; one wave per SIMD left it idle).
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+mr_control_step_kernel(KernelArgs args) {
+    control_step<false>(args);
+}
+
+// The learned-warm-start instance: the MLP tile keeps its accumulators and
+// operands in registers next to the solver state, so it runs one wavefront
+// per SIMD with the full register file.
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+mr_control_step_mlp_kernel(KernelArgs args) {
+    control_step<true>(args);
+}
+
 // Advance summary: one thread clears it, then a grid-stride pass over the
 // [steps][environments] status records folds it with per-wave reductions and
 // one atomic per wave (kernels rather than a memset, so the sequence is also
@@ -2831,14 +3006,33 @@ static_assert(sizeof(Collider) == 124, "Collider layout");
 static_assert(sizeof(HotConstraint) == 128, "HotConstraint layout");
 static_assert(sizeof(SolveBlock) == 64 && sizeof(HotConstraint) % sizeof(SolveBlock) == 0, "SolveBlock layout");
 
-hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream) {
-    hipLaunchKernelGGL(mr_control_step_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
+hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream, bool mlp) {
+    if (mlp)
+        hipLaunchKernelGGL(mr_control_step_mlp_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
+    else
+        hipLaunchKernelGGL(mr_control_step_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
     return hipGetLastError();
 }
 
-hipError_t configure_control_step(uint32_t lds_bytes) {
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&mr_control_step_kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+// The dynamic-LDS limit is an attribute of the kernel function, shared by
+// every world in the process: it only ever grows to the largest world's need.
+hipError_t configure_control_step(uint32_t lds_bytes, uint32_t mlp_lds_bytes) {
+    static std::mutex guard;
+    static uint32_t plain_max = 0u, mlp_max = 0u;
+    std::lock_guard<std::mutex> lock(guard);
+    if (lds_bytes > plain_max) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mr_control_step_kernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (e != hipSuccess) return e;
+        plain_max = lds_bytes;
+    }
+    if (mlp_lds_bytes > mlp_max) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mr_control_step_mlp_kernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)mlp_lds_bytes);
+        if (e != hipSuccess) return e;
+        mlp_max = mlp_lds_bytes;
+    }
+    return hipSuccess;
 }
 
 } // namespace mr

@@ -9,8 +9,8 @@
 
 namespace mr {
 
-hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream);
-hipError_t configure_control_step(uint32_t lds_bytes);
+hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream, bool mlp);
+hipError_t configure_control_step(uint32_t lds_bytes, uint32_t mlp_lds_bytes);
 hipError_t launch_advance_summary(const MRWorldStatusC* status, const MRContactStatusC* cstatus, uint64_t count,
                                   uint32_t environments, AdvanceSummaryDev* summary, hipStream_t stream);
 
@@ -62,7 +62,7 @@ World::World(const ModelData& model, const MRWorldStepConfigC& config) : model_(
     finalize_model(model_, cfg_.control_timestep / (float)cfg_.physics_substeps);
     env_ = model_.environment_count();
     compute_lds_layout();
-    hip_check(configure_control_step(lds_.total), "hipFuncSetAttribute");
+    hip_check(configure_control_step(lds_.total, lds_mlp_.total), "hipFuncSetAttribute");
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreate(&ev_begin_), "hipEventCreate");
     hip_check(hipEventCreate(&ev_end_), "hipEventCreate");
@@ -248,6 +248,15 @@ void World::compute_lds_layout() {
     if (L.total > kMaxLdsBytes)
         throw std::runtime_error("world needs " + std::to_string(L.total) +
                                  " bytes of LDS per environment; lower pair/constraint capacity");
+    // the learned-warm-start instance appends its regions
+    LdsLayout& M = lds_mlp_;
+    M = L;
+    off = L.total;
+    M.mlp = take(kMlpTileLdsBytes, 16u);
+    M.mlp_lam = take(12u * cfg_.constraint_capacity, 16u);
+    M.mlp_scene = take((uint32_t)sizeof(MRBodyStateC) * nscene, 16u);
+    M.total = (off + 15u) / 16u * 16u;
+    if (M.total > kMaxLdsBytes) M.total = 0u;  // the MLP instance is unavailable for this world
 }
 
 size_t World::retained_bytes() const {
@@ -255,7 +264,7 @@ size_t World::retained_bytes() const {
                                  &d_pairs_, &d_heights_, &d_q_, &d_v_, &d_scene_, &d_mh_, &d_mp_, &d_mcount_,
                                  &d_mcurrent_, &d_reset_q_, &d_reset_v_, &d_reset_scene_, &d_commands_, &d_masks_,
                                  &d_status_, &d_cstatus_, &d_accel_, &d_ev_pairs_, &d_ev_counts_, &d_stage_h_,
-                                 &d_stage_p_, &d_jrow_, &d_resp_, &d_cold_, &d_hot_, &d_summary_};
+                                 &d_stage_p_, &d_jrow_, &d_resp_, &d_cold_, &d_hot_, &d_summary_, &d_samples_};
     size_t total = 0;
     for (const DeviceBuffer* b : all) total += b->bytes;
     return total;
@@ -377,8 +386,47 @@ KernelArgs World::base_args(uint32_t step) const {
     a.cold = static_cast<ColdConstraint*>(d_cold_.ptr);
     a.hot = static_cast<HotConstraint*>(d_hot_.ptr);
     a.stage_cycles = stage_profiling_ ? static_cast<unsigned long long*>(d_stage_cycles_.ptr) : nullptr;
-    a.lds = lds_;
+    a.lds = mlp_instance() ? lds_mlp_ : lds_;
+    a.mlp = mlp_;
+    a.mlp_enabled = mlp_enabled_ ? 1u : 0u;
+    a.mlp_samples = mlp_capture_ ? static_cast<float*>(d_samples_.ptr) : nullptr;
     return a;
+}
+
+void World::set_warm_start_mlp(const MRMlpWeightsC* weights) {
+    if (weights == nullptr) {
+        mlp_enabled_ = false;
+        mlp_ = MRMlpWeightsC{};
+        return;
+    }
+    if (!(weights->w1 && weights->b1 && weights->w2 && weights->b2 && weights->w3 && weights->b3 &&
+          weights->in_shift && weights->in_scale && weights->out_scale))
+        throw std::runtime_error("warm-start MLP: every weight pointer must be set");
+    if (lds_mlp_.total == 0u)
+        throw std::runtime_error("warm-start MLP: the world's LDS plus the MLP tile exceeds 160 KiB");
+    if (cfg_.residual_tolerance <= 0.0f)
+        throw std::runtime_error("warm-start MLP: needs the adaptive solve (residual_tolerance > 0) for its fallback");
+    mlp_ = *weights;
+    mlp_enabled_ = true;
+}
+
+void World::set_mlp_capture(bool enabled) {
+    if (enabled) {
+        if (lds_mlp_.total == 0u)
+            throw std::runtime_error("MLP sample capture: the world's LDS plus the MLP tile exceeds 160 KiB");
+        d_samples_.reserve((size_t)env_ * cfg_.constraint_capacity * kMlpSampleFloats * sizeof(float));
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        hip_check(hipMemset(d_samples_.ptr, 0, d_samples_.bytes), "hipMemset");
+    }
+    mlp_capture_ = enabled;
+}
+
+void World::read_mlp_samples(float* out) const {
+    if (d_samples_.ptr == nullptr) throw std::runtime_error("MLP sample capture was never enabled");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    hip_check(hipMemcpy(out, d_samples_.ptr, (size_t)env_ * cfg_.constraint_capacity * kMlpSampleFloats *
+                                                sizeof(float), hipMemcpyDeviceToHost),
+              "hipMemcpy samples");
 }
 
 void World::enqueue(const float* commands, const uint32_t* masks, uint32_t steps, bool device_inputs) {
@@ -422,7 +470,8 @@ void World::enqueue(const float* commands, const uint32_t* masks, uint32_t steps
         a.reset_mask = msk != nullptr ? msk + (size_t)s * E : nullptr;
         a.status = static_cast<MRWorldStatusC*>(d_status_.ptr) + (size_t)s * E;
         a.cstatus = static_cast<MRContactStatusC*>(d_cstatus_.ptr) + (size_t)s * E;
-        hip_check(launch_control_step(a, lds_.total, stream_), "control step launch");
+        hip_check(launch_control_step(a, mlp_instance() ? lds_mlp_.total : lds_.total, stream_, mlp_instance()),
+                  "control step launch");
         if (step_timing_) hip_check(hipEventRecord(step_events_[s + 1u], stream_), "event");
     }
     hip_check(hipEventRecord(ev_end_, stream_), "event");

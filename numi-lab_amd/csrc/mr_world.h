@@ -73,6 +73,14 @@ public:
     void set_stage_profiling(bool enabled);
     void read_stage_cycles(unsigned long long* out) const;
     void read_step_milliseconds(double* out, uint32_t steps) const;
+    // Learned warm start: device weights (the caller keeps them alive while
+    // set), or null for the plain warm start.
+    void set_warm_start_mlp(const MRMlpWeightsC* weights);
+    // Capture of (features, converged impulse) samples at microstep
+    // capture_substep of every control step of later advances.
+    void set_mlp_capture(bool enabled);
+    void read_mlp_samples(float* out) const;  // [E][constraint_capacity][kMlpSampleFloats] of the last step
+    const LdsLayout& mlp_lds() const { return lds_mlp_; }
 
 private:
     KernelArgs base_args(uint32_t step) const;
@@ -83,6 +91,10 @@ private:
     MRWorldStepConfigC cfg_;
     uint32_t env_ = 0;
     LdsLayout lds_{};
+    LdsLayout lds_mlp_{};  // the MLP instance's layout (plain layout plus its regions); total 0 if it does not fit
+    MRMlpWeightsC mlp_{};
+    bool mlp_enabled_ = false, mlp_capture_ = false;
+    bool mlp_instance() const { return mlp_enabled_ || mlp_capture_; }
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_begin_ = nullptr, ev_end_ = nullptr;
     // model
@@ -93,7 +105,7 @@ private:
     // per-advance inputs / outputs
     DeviceBuffer d_commands_, d_masks_, d_status_, d_cstatus_, d_accel_, d_ev_pairs_, d_ev_counts_;
     // scratch
-    DeviceBuffer d_stage_h_, d_stage_p_, d_jrow_, d_resp_, d_cold_, d_hot_, d_stage_cycles_, d_summary_;
+    DeviceBuffer d_stage_h_, d_stage_p_, d_jrow_, d_resp_, d_cold_, d_hot_, d_stage_cycles_, d_summary_, d_samples_;
     bool stage_profiling_ = false;
     uint32_t last_steps_ = 0;
     float last_kernel_ms_ = 0.0f;

@@ -298,6 +298,9 @@ _SIGNATURES: Final = {
     "mr_task_rollout_active_contacts": ([ct.c_void_p], ct.POINTER(ct.c_uint32)),
     "mr_task_rollout_final_q": ([ct.c_void_p], ct.POINTER(ct.c_float)),
     "mr_task_rollout_copy_final_scene_states": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
+    "mr_world_set_warm_start_mlp": ([ct.c_void_p, ct.c_void_p], ct.c_int),
+    "mr_world_set_mlp_capture": ([ct.c_void_p, ct.c_uint32], ct.c_int),
+    "mr_world_read_mlp_samples": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
     "mr_mlp_forward": ([ct.POINTER(MlpWeightsC), ct.c_void_p, ct.c_uint32, ct.c_void_p, ct.POINTER(MlpTraceC),
                         ct.c_void_p], ct.c_int),
     "mr_mlp_loss_scratch_floats": ([ct.c_uint32], ct.c_uint32),
@@ -584,6 +587,26 @@ class World:
 
     def set_step_timing(self, enabled: bool) -> None:
         self._b.check(self._b.lib.mr_world_set_step_timing(self._h, 1 if enabled else 0))
+
+    def set_warm_start_mlp(self, mlp) -> None:
+        """Warm-start later advances from `mlp` (a metalrobo.warmstart.WarmStartMLP, kept alive
+        here while set), or restore the plain warm start with None."""
+        self._mlp = mlp
+        if mlp is None:
+            self._b.check(self._b.lib.mr_world_set_warm_start_mlp(self._h, None))
+        else:
+            self._mlp_weights = mlp.weights()
+            self._b.check(self._b.lib.mr_world_set_warm_start_mlp(self._h, ct.byref(self._mlp_weights)))
+
+    def set_mlp_capture(self, enabled: bool) -> None:
+        self._b.check(self._b.lib.mr_world_set_mlp_capture(self._h, 1 if enabled else 0))
+
+    def mlp_samples(self) -> np.ndarray:
+        """[E][constraint_capacity][12]: 8 features, converged impulse, valid flag (last captured step)."""
+        cap = self.config.constraint_capacity
+        out = np.zeros((self.environment_count, cap, 12), np.float32)
+        self._b.check(self._b.lib.mr_world_read_mlp_samples(self._h, _ptr(out), out.size))
+        return out
 
     STAGE_NAMES = ("kinematics", "aba", "projection", "broadphase", "narrowphase", "emission", "responses",
                    "evaluation", "warm_start", "sweeps", "residuals", "integration")

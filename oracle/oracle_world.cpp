@@ -2145,7 +2145,8 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
         return false;
     }
     cs.solver_iterations = ncons > 0u ? iters : 0u;
-    cs.residuals = mr_float4{ncons > 0u ? sweep_delta : 0.0f, max_normal, max_cone, 0.0f};
+    cs.residuals = mr_float4{ncons > 0u ? sweep_delta : 0.0f, max_normal, max_cone,
+                             cs.residuals.w + (float)(ncons > 0u ? iters : 0u)};
     // islands over dynamic nodes: node 0 = articulation, 1 + s = scene body s
     {
         uint32_t parent[MR_MAX_BODIES + 1];

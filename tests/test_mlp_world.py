@@ -1,0 +1,151 @@
+"""The learned warm start inside the control-step kernel (the MLP instance).
+
+Bit-exact plumbing checks against the plain kernel on the chain-and-boxes
+workload:
+  * sample capture with the plain warm start changes nothing (states,
+    manifolds, statuses), and its targets are the impulses the step stored;
+  * an MLP that predicts zero impulses equals the plain path started from
+    zero impulses (one microstep per control step, one sweep each);
+each from a settled state with live contacts (40 plain control steps), and
+comparing manifolds up to each environment's count.
+  * when every MLP-warm-started solve misses the tolerance, the fallback
+    re-solves from the plain warm start and reproduces the plain path bit for
+    bit, counting one fallback per microstep and twice the sweeps.
+Training on captured samples and the iterations saved at equal residual are
+measured by bench.py's warm-start leg.
+"""
+
+import numpy as np
+import pytest
+
+
+def zero_mlp():
+    from metalrobo import warmstart as ws
+
+    m = ws.WarmStartMLP()
+    state = {k: np.zeros(shape, np.float32) for k, shape in ws.SHAPES.items()}
+    state.update(in_shift=np.zeros(8, np.float32), in_scale=np.ones(8, np.float32),
+                 out_scale=np.ones(3, np.float32))
+    m.load_state(state)
+    return m
+
+
+def chain_world(envs=8, **overrides):
+    import metalrobo as mr
+
+    model = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_CHAIN_BOXES, envs, 7)
+    cfg = model.default_config()
+    for k, v in overrides.items():
+        setattr(cfg, k, v)
+    return model, cfg
+
+
+_SETTLED = {}
+
+
+def settled(model):
+    """State and manifolds after 40 plain control steps (boxes landed, contacts live)."""
+    import metalrobo as mr
+
+    key = id(model)
+    if key not in _SETTLED:
+        w = mr.World(model, model.default_config())
+        w.advance(model.commands(0, 40))
+        _SETTLED[key] = (*w.state(), *w.manifolds())
+    return _SETTLED[key]
+
+
+def run(model, cfg, steps, setup=None, before_step=None):
+    import metalrobo as mr
+
+    q0, v0, s0, h0, p0, c0 = settled(model)
+    assert c0.sum() > 0
+    w = mr.World(model, cfg)
+    w.set_state(q0, v0, s0)
+    w.write_manifolds(h0, p0, c0)
+    if setup:
+        setup(w)
+    cmds = model.commands(40, steps)
+    for t in range(steps):
+        if before_step:
+            before_step(w)
+        w.advance(cmds[t:t + 1])
+    q, v, s = w.state()
+    h, p, c = w.manifolds()
+    # entries past each environment's manifold count are not state: clear them
+    for e in range(len(c)):
+        h[e, c[e]:] = np.zeros(1, h.dtype)
+        p[e, c[e]:] = np.zeros(1, p.dtype)
+    return w, (q, v, s, h, p, c, w.statuses(), w.contact_statuses())
+
+
+NAMES = ("q", "v", "scene", "manifold headers", "manifold points", "manifold counts", "statuses",
+         "contact statuses")
+
+
+def assert_same(a, b):
+    for i, (x, y) in enumerate(zip(a, b)):
+        same = np.array_equal(np.ascontiguousarray(x).view(np.uint8), np.ascontiguousarray(y).view(np.uint8))
+        assert same, (NAMES[i], a[5], b[5], a[6]["code"], b[6]["code"])
+
+
+@pytest.mark.gpu
+def test_capture_changes_nothing_and_records_the_stored_impulses(gpu):
+    model, cfg = chain_world(capture_substep=3)
+    _, plain = run(model, cfg, 6)
+    w, cap = run(model, cfg, 6, setup=lambda w: w.set_mlp_capture(True))
+    assert cap[7][-1]["constraints"].sum() > 0
+    assert_same(plain, cap)
+    samples = w.mlp_samples()
+    cs = cap[7][-1]
+    _, p, c = cap[3], cap[4], cap[5]
+    for env in range(model.environment_count):
+        rows = samples[env]
+        valid = rows[:, 11] == 1.0
+        assert valid.sum() == cs["constraints"][env]
+        assert np.all(rows[valid.sum():, 11] == 0.0)
+        hdr = cap[3][env]
+        stored = []
+        for m in range(int(c[env])):
+            n = int(hdr[m]["pair_and_count"][3])
+            stored += [tuple(p[env, m, k]["impulses"][:3]) for k in range(n)]
+        got = sorted(tuple(r) for r in rows[valid, 8:11])
+        assert got == sorted(stored)
+        assert np.all(np.isfinite(rows[valid, :8]))
+        assert np.all(rows[valid, 2] >= 0.0) and np.all(rows[valid, 1] >= -0.05)
+
+
+@pytest.mark.gpu
+def test_zero_prediction_equals_a_cold_start(gpu):
+    model, cfg = chain_world(physics_substeps=1, control_timestep=0.005, residual_tolerance=1.0e3,
+                             max_iterations=1)
+    mlp = zero_mlp()
+
+    def zero_impulses(w):
+        h, p, c = w.manifolds()
+        p["impulses"][:] = 0.0
+        w.write_manifolds(h, p, c)
+
+    _, cold = run(model, cfg, 10, before_step=zero_impulses)
+    _, learned = run(model, cfg, 10, setup=lambda w: w.set_warm_start_mlp(mlp))
+    assert learned[7]["constraints"].sum() > 0
+    assert_same(cold[:6], learned[:6])
+    assert_same(cold[6:7], learned[6:7])
+    assert np.all(learned[7]["diagnostics"][..., 2] == 0.0)
+
+
+@pytest.mark.gpu
+def test_forced_fallback_reproduces_the_plain_path(gpu):
+    model, cfg = chain_world(residual_tolerance=1.0e-30, max_iterations=3)
+    mlp = zero_mlp()
+    _, plain = run(model, cfg, 6)
+    _, fb = run(model, cfg, 6, setup=lambda w: w.set_warm_start_mlp(mlp))
+    assert fb[7]["constraints"].sum() > 0
+    assert_same(plain[:7], fb[:7])
+    pc, fc = plain[7], fb[7]
+    for k in ("code", "constraints", "solver_iterations", "manifolds"):
+        assert np.array_equal(pc[k], fc[k]), k
+    np.testing.assert_array_equal(pc["residuals"][..., :3], fc["residuals"][..., :3])
+    np.testing.assert_array_equal(fc["residuals"][..., 3], 2.0 * pc["residuals"][..., 3])
+    # every microstep with constraints fell back once
+    assert np.all(fc["diagnostics"][..., 2] == pc["residuals"][..., 3] / 3.0)
