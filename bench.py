@@ -115,6 +115,45 @@ def residual_leg(mr, model, commands, args):
     }
 
 
+def warm_start_leg(mr, args):
+    """Learned warm start on the chain-and-boxes workload: capture (features, converged impulse)
+    samples from the plain adaptive solve, train the MLP with the library's own kernels, then run
+    the same settled worlds at the residual tolerance with the plain and the learned warm start
+    (plain fallback on) and report sweeps per control step."""
+    import torch
+    from metalrobo import warmstart as ws
+
+    tol = args.residual_tolerance if args.residual_tolerance > 0.0 else 1.0e-5
+    model = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_CHAIN_BOXES, args.warm_start_collect_worlds, args.seed)
+    cfg = model.default_config()
+    cfg.residual_tolerance = tol
+    cfg.max_iterations = args.residual_max_iterations
+    t0 = time.perf_counter()
+    f, t = ws.collect_samples(model, cfg, control_steps=args.warm_start_collect_steps, settle_steps=40,
+                              max_samples=args.warm_start_samples)
+    collect_s = time.perf_counter() - t0
+    mlp = ws.WarmStartMLP(seed=args.seed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep = mlp.fit(f, t, steps=args.warm_start_train_steps, batch=8192, lr=1.0e-3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del f, t
+    ev_model = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_CHAIN_BOXES, args.warm_start_eval_worlds, args.seed + 1)
+    ev_cfg = ev_model.default_config()
+    ev_cfg.residual_tolerance = tol
+    ev_cfg.max_iterations = args.residual_max_iterations
+    ev = ws.evaluate(ev_model, ev_cfg, mlp, control_steps=args.warm_start_eval_steps, settle_steps=40)
+    plain, learned = ev["plain"]["sweeps_per_step"], ev["mlp"]["sweeps_per_step"]
+    return {
+        "workload": "chain7_boxes20_plane (config 1)", "residual_tolerance": tol,
+        "samples": rep.kept + rep.dropped, "samples_kept": rep.kept, "collect_s": round(collect_s, 2),
+        "train_steps": rep.steps, "train_batch": 8192, "train_s": round(train_s, 2),
+        "loss_first": round(rep.first_loss, 5), "loss_final": round(rep.final_loss, 5),
+        "eval": ev, "sweeps_saved_fraction": round(1.0 - learned / plain, 4) if plain > 0 else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +173,14 @@ def main():
     ap.add_argument("--stage-profile", type=int, default=0, metavar="STEPS",
                     help="after the timed region, run STEPS more control steps with the per-stage shader-clock "
                          "counters on and report each stage's share of the cycles")
+    ap.add_argument("--skip-warm-start", action="store_true",
+                    help="skip the learned-warm-start leg (chain-and-boxes workload)")
+    ap.add_argument("--warm-start-samples", type=int, default=10_000_000)
+    ap.add_argument("--warm-start-collect-worlds", type=int, default=4096)
+    ap.add_argument("--warm-start-collect-steps", type=int, default=40)
+    ap.add_argument("--warm-start-train-steps", type=int, default=1500)
+    ap.add_argument("--warm-start-eval-worlds", type=int, default=16384)
+    ap.add_argument("--warm-start-eval-steps", type=int, default=10)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the timed region, run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic that scales FETCH_SIZE / WRITE_SIZE counter readings)")
@@ -252,6 +299,9 @@ def main():
         del src, dst
     if args.residual_tolerance > 0.0 and args.residual_steps > 0:
         out["at_residual"] = residual_leg(mr, model, commands, args)
+    if not args.skip_warm_start:
+        del world
+        out["warm_start"] = warm_start_leg(mr, args)
     if not args.skip_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(kind, args.seed, args.cpu_envs, args.cpu_steps, threads)

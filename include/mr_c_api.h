@@ -150,10 +150,12 @@ int mr_world_set_stage_profiling(MRWorldHandle* world, uint32_t enabled);
 int mr_world_read_stage_cycles(const MRWorldHandle* world, uint64_t* out, size_t count);
 /* Learned warm start (no counterpart in the reference; see mr_mlp.h): with
  * non-null device weights, later advances run the MLP instance of the
- * control-step kernel, which predicts each contact's impulse from its
- * features at the start of the cone solve and warm-starts from it; a
- * microstep whose sweeps then miss residual_tolerance is solved again from
- * the plain warm start (contact status diagnostics.z counts these). Needs
+ * control-step kernel, which predicts each contact's initial impulse from
+ * its features at the start of the cone solve and warm-starts from it
+ * (contacts whose normalized features lie beyond 8 training standard
+ * deviations keep the plain warm start); a microstep whose sweeps then miss
+ * residual_tolerance is solved again from the plain warm start (contact
+ * status diagnostics.z counts these). Needs
  * the adaptive solve (residual_tolerance > 0). The weights must stay valid
  * while set; null restores the plain path. */
 int mr_world_set_warm_start_mlp(MRWorldHandle* world, const MRMlpWeightsC* weights);

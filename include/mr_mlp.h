@@ -2,9 +2,12 @@
  * types). The network maps per-contact features
  *   [normal velocity, penetration depth, |J_n|, |J_u|, |J_v|, lambda_n, lambda_u, lambda_v]
  * (all at the start of a microstep's cone solve; lambda is the previous-step
- * warm start) to the impulse the plain solve converges to:
+ * warm start) to the initial impulse of that solve, as a correction of the
+ * previous-step impulse:
  *   8 -> 256 (GELU) -> 256 (GELU) -> 3, BF16 weights on the matrix cores,
- *   FP32 accumulation; prediction = out_scale * y_hat.
+ *   FP32 accumulation; initial impulse = lambda + out_scale * y_hat, projected
+ *   onto the contact's cone. Trained on the impulses the plain solve converges
+ *   to (targets: converged impulse minus lambda).
  *
  * This subsystem has no counterpart in the reference: it is the learned
  * warm start that stands beside the plain temporalCone solve, which stays
@@ -41,6 +44,10 @@ typedef struct MRMlpWeightsC {
     const float* in_shift;   /* [8] feature mean */
     const float* in_scale;   /* [8] 1 / feature std */
     const float* out_scale;  /* [3] target std */
+    /* In-solver use only: a contact's correction is applied when its largest
+     * component exceeds this (impulse units); smaller ones keep the plain warm
+     * start, which is already as close as the network's precision. */
+    float min_correction;
 } MRMlpWeightsC;
 
 /* Activations a training forward pass records for the backward pass. */
@@ -68,10 +75,12 @@ int mr_mlp_forward(const MRMlpWeightsC* weights, const float* x, uint32_t rows, 
 /* Floats of device scratch mr_mlp_loss needs for `rows` rows. */
 uint32_t mr_mlp_loss_scratch_floats(uint32_t rows);
 
-/* Mean squared error in normalized target units, t_hat = target / out_scale:
- * loss = mean((y_hat - t_hat)^2) (device scalar), dy = d loss / d y_hat. */
-int mr_mlp_loss(const float* y_hat, const float* target, const float* out_scale, uint32_t rows, float* dy,
-                float* loss, float* scratch, void* stream);
+/* Loss in normalized target units, d = y_hat - target / out_scale: with
+ * huber_delta = 0 the mean squared error mean(d^2); with huber_delta > 0 the
+ * Huber loss (d^2 / 2 within delta, linear beyond). loss is a device scalar,
+ * dy = d loss / d y_hat. */
+int mr_mlp_loss(const float* y_hat, const float* target, const float* out_scale, uint32_t rows, float huber_delta,
+                float* dy, float* loss, float* scratch, void* stream);
 
 /* Gradients of the loss with respect to every weight and bias, from dy and a
  * trace of the same rows; dz2 and dz1 are [rows][256] FP32 scratch. */

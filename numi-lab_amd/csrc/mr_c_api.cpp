@@ -600,12 +600,14 @@ int mr_mlp_forward(const MRMlpWeightsC* weights, const float* x, uint32_t rows, 
 
 uint32_t mr_mlp_loss_scratch_floats(uint32_t rows) { return mr::mlp_loss_scratch_floats(rows); }
 
-int mr_mlp_loss(const float* y_hat, const float* target, const float* out_scale, uint32_t rows, float* dy,
-                float* loss, float* scratch, void* stream) {
+int mr_mlp_loss(const float* y_hat, const float* target, const float* out_scale, uint32_t rows, float huber_delta,
+                float* dy, float* loss, float* scratch, void* stream) {
     return translate([&] {
         require(y_hat && target && out_scale && dy && loss && scratch, "mr_mlp_loss: null pointer");
         require(rows > 0u && rows <= 0x7fffffffu / MR_MLP_OUTPUTS, "mr_mlp_loss: rows out of range");
-        mr::hip_check(mr::mlp_loss(y_hat, target, out_scale, rows, dy, loss, scratch, static_cast<hipStream_t>(stream)),
+        require(huber_delta >= 0.0f && std::isfinite(huber_delta), "mr_mlp_loss: huber_delta must be >= 0");
+        mr::hip_check(mr::mlp_loss(y_hat, target, out_scale, rows, huber_delta, dy, loss, scratch,
+                                   static_cast<hipStream_t>(stream)),
                   "mr_mlp_loss launch");
     });
 }

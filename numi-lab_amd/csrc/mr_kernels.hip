@@ -1688,6 +1688,9 @@ struct StageClock {
 // ------------------------------------------------ learned warm start
 
 static_assert(mr::kMlpTileLdsBytes == mr::kMlpLdsBytes, "MLP tile LDS size");
+// Normalized features beyond this many training standard deviations keep the
+// plain warm start.
+constexpr float kMlpFeatureBound = 8.0f;
 
 // Features of constraint c at the start of its cone solve: normal relative
 // velocity (J v_free, the evaluation's staged row), penetration depth, the
@@ -1739,17 +1742,25 @@ MRD_INL mr::MlpView mlp_view(const MRMlpWeightsC& w) {
 
 // Features of every constraint, 16 at a time: written to `samples` (the
 // environment's capture rows, or null) and, with `infer`, run through the
-// MLP; each prediction, projected onto its cone, replaces the plain warm
-// start (kept in e.mlp_lam, as are the scene bodies, for the fallback) unless
-// it is not finite. Uniform in `infer`; all 64 lanes must be active.
-MRD_INL void mlp_warm_start(const Env& e, uint32_t lane, uint32_t ncons, uint32_t mnv, bool infer, float* samples) {
+// MLP, which predicts the correction to the previous-step impulse:
+// lambda = lambda_prev + out_scale * y_hat, projected onto the cone. The
+// prediction replaces the plain warm start (kept in e.mlp_lam, as are the
+// scene bodies, for the fallback) when some component of the correction
+// exceeds the weights' min_correction, unless a normalized feature lies
+// beyond kMlpFeatureBound (outside what the network was trained on) or the
+// result is not finite; otherwise the warm start stays untouched, bit for bit.
+// Returns whether any warm start was replaced. Uniform in `infer`; all 64
+// lanes must be active.
+MRD_INL bool mlp_warm_start(const Env& e, uint32_t lane, uint32_t ncons, uint32_t mnv, bool infer, float* samples) {
     const mr::MlpLds s = mr::mlp_lds(e.mlp_lds);
     const mr::MlpView m = mlp_view(e.a->mlp);
     if (infer) {
         for (uint32_t i = lane; i < 3u * ncons; i += kLanes) e.mlp_lam[i] = e.lam[i];
         for (uint32_t b = lane; b < e.nscene; b += kLanes) e.mlp_scene[b] = e.scene[b];
     }
+    bool replaced = false;  // this lane replaced a warm start
     for (uint32_t t0 = 0; t0 < ncons; t0 += mr::kMlpRows) {
+        bool in_range = false;  // lanes 0-15: their row's features are all within bounds
         __syncthreads();
         if (lane < mr::kMlpRows) {
             const uint32_t c = t0 + lane;
@@ -1759,9 +1770,14 @@ MRD_INL void mlp_warm_start(const Env& e, uint32_t lane, uint32_t ncons, uint32_
                 if (samples != nullptr)
                     for (uint32_t k = 0; k < mr::kMlpIn; ++k) samples[(size_t)c * mr::kMlpSampleFloats + k] = f[k];
             }
-            if (infer)
-                for (uint32_t k = 0; k < mr::kMlpIn; ++k)
-                    s.x[lane * mr::kMlpIn + k] = c < ncons ? (f[k] - m.in_shift[k]) * m.in_scale[k] : 0.0f;
+            if (infer) {
+                in_range = true;
+                for (uint32_t k = 0; k < mr::kMlpIn; ++k) {
+                    const float xn = c < ncons ? (f[k] - m.in_shift[k]) * m.in_scale[k] : 0.0f;
+                    in_range = in_range && fabs_(xn) <= kMlpFeatureBound;
+                    s.x[lane * mr::kMlpIn + k] = in_range ? xn : 0.0f;
+                }
+            }
         }
         __syncthreads();
         if (!infer) continue;
@@ -1774,19 +1790,28 @@ MRD_INL void mlp_warm_start(const Env& e, uint32_t lane, uint32_t ncons, uint32_
             for (uint32_t r = 0; r < 4u; ++r) s.x[((lane >> 4) * 4u + r) * mr::kMlpIn + col] = y[r] * scale;
         }
         __syncthreads();
-        if (lane < mr::kMlpRows && t0 + lane < ncons) {
+        if (lane < mr::kMlpRows && t0 + lane < ncons && in_range) {
             const uint32_t c = t0 + lane;
-            float lam[3] = {s.x[lane * mr::kMlpIn], s.x[lane * mr::kMlpIn + 1u], s.x[lane * mr::kMlpIn + 2u]};
+            float lam[3];
+            bool changed = false;
+            const float gate = e.a->mlp.min_correction;
+            for (uint32_t k = 0; k < 3u; ++k) changed = changed || fabs_(s.x[lane * mr::kMlpIn + k]) > gate;
+            for (uint32_t k = 0; k < 3u; ++k) {
+                const float prev = e.lam[3u * c + k], d = s.x[lane * mr::kMlpIn + k];
+                lam[k] = changed ? prev + d : prev;
+            }
             const HotConstraint& hk = e.hot[c];
-            project_cone(lam, hk.mu_u, hk.mu_v);
-            if (isfin(lam[0]) && isfin(lam[1]) && isfin(lam[2])) {
+            if (changed) project_cone(lam, hk.mu_u, hk.mu_v);
+            if (changed && isfin(lam[0]) && isfin(lam[1]) && isfin(lam[2])) {
                 e.lam[3u * c] = lam[0];
                 e.lam[3u * c + 1u] = lam[1];
                 e.lam[3u * c + 2u] = lam[2];
+                replaced = true;
             }
         }
     }
     __syncthreads();
+    return __ballot(replaced) != 0ull;
 }
 
 // --------------------------------------------------------------- kernel
@@ -2539,11 +2564,12 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
             mlp_now = A.mlp_enabled != 0u && ncons > 0u;
             if (A.mlp_samples != nullptr && substep == cfg.capture_substep)
                 capture_rows = A.mlp_samples + (size_t)env * ccap * mr::kMlpSampleFloats;
-            if (mlp_now || capture_rows != nullptr) mlp_warm_start(e, lane, ncons, mnv, mlp_now, capture_rows);
+            if (mlp_now || capture_rows != nullptr)
+                mlp_now = mlp_warm_start(e, lane, ncons, mnv, mlp_now, capture_rows) && mlp_now;
         }
-        // The plain path runs the attempt below once. With the MLP warm start, a
-        // microstep whose sweeps miss the residual tolerance is solved again
-        // from the plain warm start and the same free velocity.
+        // The plain path runs the attempt below once. When the MLP replaced any
+        // warm start, a microstep whose sweeps miss the residual tolerance is
+        // solved again from the plain warm start and the same free velocity.
         for (uint32_t attempt = 0;; ++attempt) {
             sweep_delta = 0.0f;
             iters = 0;

@@ -45,19 +45,29 @@ extern "C" __global__ void __launch_bounds__(64) mr_mlp_forward_kernel(MlpView m
     }
 }
 
-// dy = 2 (y_hat - t / out_scale) / (3 n); per-block partial sums of the
-// squared error, summed by mr_mlp_sum_kernel in block order.
+// d = y_hat - t / out_scale. With delta = 0 the loss is mean(d^2) and
+// dy = 2 d / (3 n); with delta > 0 it is the Huber loss, mean(h(d)) with
+// h(d) = d^2 / 2 for |d| <= delta, delta (|d| - delta / 2) beyond, and
+// dy = clamp(d, -delta, delta) / (3 n). Per-block partial sums of the loss
+// terms are summed by mr_mlp_sum_kernel in block order.
 extern "C" __global__ void __launch_bounds__(256) mr_mlp_mse_kernel(const float* y_hat, const float* t,
-                                                                    const float* out_scale, uint32_t n, float* dy,
-                                                                    float* partial) {
+                                                                    const float* out_scale, uint32_t n, float delta,
+                                                                    float* dy, float* partial) {
     __shared__ float red[256];
     const uint32_t total = n * kMlpOut;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     float se = 0.0f;
     if (i < total) {
         const float d = y_hat[i] - t[i] / out_scale[i % kMlpOut];
-        dy[i] = 2.0f * d / (float)total;
-        se = d * d;
+        if (delta > 0.0f) {
+            const float ad = fabsf(d);
+            const float g = ad <= delta ? d : (d > 0.0f ? delta : -delta);
+            dy[i] = g / (float)total;
+            se = ad <= delta ? 0.5f * d * d : delta * (ad - 0.5f * delta);
+        } else {
+            dy[i] = 2.0f * d / (float)total;
+            se = d * d;
+        }
     }
     red[threadIdx.x] = se;
     __syncthreads();
@@ -208,10 +218,11 @@ hipError_t mlp_forward(const MRMlpWeightsC& w, const float* x, uint32_t n, float
 
 uint32_t mlp_loss_scratch_floats(uint32_t n) { return (n * kMlpOut + 255u) / 256u; }
 
-hipError_t mlp_loss(const float* y_hat, const float* t, const float* out_scale, uint32_t n, float* dy, float* loss,
-                    float* scratch, hipStream_t st) {
+hipError_t mlp_loss(const float* y_hat, const float* t, const float* out_scale, uint32_t n, float delta, float* dy,
+                    float* loss, float* scratch, hipStream_t st) {
     const uint32_t blocks = mlp_loss_scratch_floats(n);
-    hipLaunchKernelGGL(mr_mlp_mse_kernel, dim3(blocks), dim3(256), 0, st, y_hat, t, out_scale, n, dy, scratch);
+    hipLaunchKernelGGL(mr_mlp_mse_kernel, dim3(blocks), dim3(256), 0, st, y_hat, t, out_scale, n, delta, dy,
+                       scratch);
     hipLaunchKernelGGL(mr_mlp_sum_kernel, dim3(1), dim3(256), 0, st, scratch, blocks, 1.0f / (float)(n * kMlpOut),
                        loss);
     return hipGetLastError();

@@ -135,7 +135,7 @@ class MlpWeightsC(ct.Structure):
     """Mirror of MRMlpWeightsC (include/mr_mlp.h): device pointers."""
     _fields_ = [("w1", ct.c_void_p), ("b1", ct.c_void_p), ("w2", ct.c_void_p), ("b2", ct.c_void_p),
                 ("w3", ct.c_void_p), ("b3", ct.c_void_p), ("in_shift", ct.c_void_p), ("in_scale", ct.c_void_p),
-                ("out_scale", ct.c_void_p)]
+                ("out_scale", ct.c_void_p), ("min_correction", ct.c_float)]
 
 
 class MlpTraceC(ct.Structure):
@@ -304,8 +304,8 @@ _SIGNATURES: Final = {
     "mr_mlp_forward": ([ct.POINTER(MlpWeightsC), ct.c_void_p, ct.c_uint32, ct.c_void_p, ct.POINTER(MlpTraceC),
                         ct.c_void_p], ct.c_int),
     "mr_mlp_loss_scratch_floats": ([ct.c_uint32], ct.c_uint32),
-    "mr_mlp_loss": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32, ct.c_void_p, ct.c_void_p, ct.c_void_p,
-                     ct.c_void_p], ct.c_int),
+    "mr_mlp_loss": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32, ct.c_float, ct.c_void_p, ct.c_void_p,
+                     ct.c_void_p, ct.c_void_p], ct.c_int),
     "mr_mlp_backward": ([ct.POINTER(MlpWeightsC), ct.POINTER(MlpTraceC), ct.c_void_p, ct.c_uint32, ct.c_void_p,
                          ct.c_void_p, ct.POINTER(MlpGradsC), ct.c_void_p], ct.c_int),
     "mr_mlp_adam": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32, ct.c_float,

@@ -8,7 +8,9 @@ Features per contact, at the start of a microstep's cone solve:
     normal relative velocity, penetration depth, the Euclidean norms of the
     normal and two tangent Jacobian rows, and the previous-step impulse
     (normal, tangent u, tangent v) the plain path warm-starts from.
-Target: the impulse the plain solve converges to in that microstep.
+The network predicts the correction to the previous-step impulse; the
+initial impulse is lambda_prev + out_scale * y_hat (the solver projects it onto
+the cone), trained on the impulse the plain solve converges to.
 """
 
 from __future__ import annotations
@@ -49,6 +51,8 @@ class TrainReport:
     first_loss: float
     final_loss: float
     losses: list
+    kept: int = 0
+    dropped: int = 0
 
 
 class WarmStartMLP:
@@ -74,13 +78,15 @@ class WarmStartMLP:
         self.in_scale = torch.ones(IN, dtype=torch.float32, device=device)
         self.out_scale = torch.ones(OUT, dtype=torch.float32, device=device)
         self.step = 0
+        self.min_correction = 2.0  # impulse units; see MRMlpWeightsC.min_correction
         self._bufs = {}
 
     # -------------------------------------------------------------- views
     def weights(self) -> nat.MlpWeightsC:
         return nat.MlpWeightsC(_p(self.bf16["w1"]), _p(self.master["b1"]), _p(self.bf16["w2"]),
                                _p(self.master["b2"]), _p(self.bf16["w3"]), _p(self.master["b3"]),
-                               _p(self.in_shift), _p(self.in_scale), _p(self.out_scale))
+                               _p(self.in_shift), _p(self.in_scale), _p(self.out_scale),
+                               float(self.min_correction))
 
     def set_normalization(self, features, targets) -> None:
         """Feature mean / inverse std and target std from training data (rows x 8, rows x 3)."""
@@ -130,17 +136,36 @@ class WarmStartMLP:
         return y
 
     def predict(self, x):
-        """Impulses (physical units) predicted for features x."""
-        return self.forward(x) * self.out_scale
+        """Initial impulses (physical units, before the cone projection) for features x."""
+        return x[:, 5:8] + self.forward(x) * self.out_scale
 
-    def loss_and_grads(self, x, t):
-        """Forward with trace, MSE gradient and backward; gradients land in the grads buffers."""
+    def fit(self, features, targets, steps: int = 1500, batch: int = 8192, lr: float = 1.0e-3,
+            bound: float = 1.0e3, seed: int = 0, huber_delta: float = 0.1) -> TrainReport:
+        """Trains on captured samples (features, converged impulses): rows with a non-finite or
+        larger-than-`bound` value are dropped, the normalization is fitted to the rest, and the
+        network learns the correction converged - previous impulse with a Huber loss (the
+        corrections are heavy-tailed: most contacts need almost none)."""
+        f = np.asarray(features, np.float32)
+        t = np.asarray(targets, np.float32)
+        keep = np.all(np.isfinite(f), 1) & np.all(np.isfinite(t), 1)
+        keep &= np.all(np.abs(f) <= bound, 1) & np.all(np.abs(t) <= bound, 1)
+        f, t = f[keep], t[keep]
+        r = t - f[:, 5:8]
+        self.set_normalization(f, r)
+        rep = self.train(f, r, steps=steps, batch=batch, lr=lr, seed=seed, huber_delta=huber_delta)
+        rep.kept = int(keep.sum())
+        rep.dropped = int((~keep).sum())
+        return rep
+
+    def loss_and_grads(self, x, t, huber_delta: float = 0.0):
+        """Forward with trace, loss gradient (MSE, or Huber with huber_delta > 0) and backward;
+        gradients land in the grads buffers."""
         n = x.shape[0]
         b = self._buffers(n)
         y = self.forward(x, trace=True)
         t = t.contiguous()
-        self._b.check(self._b.lib.mr_mlp_loss(_p(y), _p(t), _p(self.out_scale), n, _p(b["dy"]), _p(b["loss"]),
-                                              _p(b["scratch"]), _stream()))
+        self._b.check(self._b.lib.mr_mlp_loss(_p(y), _p(t), _p(self.out_scale), n, huber_delta, _p(b["dy"]),
+                                              _p(b["loss"]), _p(b["scratch"]), _stream()))
         w = self.weights()
         tr = self._trace(b)
         g = b["grads"]
@@ -157,7 +182,7 @@ class WarmStartMLP:
                                                   w.numel(), lr, beta1, beta2, eps, self.step, _stream()))
 
     def train(self, features, targets, steps: int = 400, batch: int = 8192, lr: float = 1.0e-3,
-              seed: int = 0) -> TrainReport:
+              seed: int = 0, huber_delta: float = 0.0) -> TrainReport:
         """Minibatch Adam on (features, targets) resident on the GPU."""
         torch = _torch()
         f = torch.as_tensor(features, dtype=torch.float32, device=self.device)
@@ -168,7 +193,7 @@ class WarmStartMLP:
         losses = []
         for s in range(steps):
             idx = torch.randint(0, n, (batch,), generator=gen).to(self.device)
-            loss, grads = self.loss_and_grads(f[idx], t[idx])
+            loss, grads = self.loss_and_grads(f[idx], t[idx], huber_delta)
             self.adam_step(grads, lr=lr)
             if s == 0 or (s + 1) % 50 == 0 or s + 1 == steps:
                 losses.append(float(loss.item()))
@@ -200,3 +225,89 @@ class WarmStartMLP:
         with np.load(path, allow_pickle=False) as z:
             m.load_state({k: z[k] for k in z.files})
         return m
+
+
+# ------------------------------------------------------------------ workflows
+
+def _settled_world(model, cfg, settle_steps: int):
+    """A world advanced `settle_steps` control steps with `cfg` from the model's initial state."""
+    from .native import World
+
+    w = World(model, cfg)
+    if settle_steps > 0:
+        w.advance(model.commands(0, settle_steps))
+    return w
+
+
+def collect_samples(model, cfg, control_steps: int, settle_steps: int = 40, max_samples: int = 10_000_000,
+                    capture_substep: int = 1):
+    """(features [n][8], targets [n][3]) from the plain solve: one microstep per control step is
+    captured (its features at the start of the solve and the impulse it converged to)."""
+    cfg = cfg.copy()
+    cfg.capture_substep = capture_substep
+    w = _settled_world(model, cfg, settle_steps)
+    w.set_mlp_capture(True)
+    cmds = model.commands(settle_steps, control_steps)
+    feats, targs, total = [], [], 0
+    for t in range(control_steps):
+        w.advance(cmds[t:t + 1])
+        rows = w.mlp_samples().reshape(-1, 12)
+        rows = rows[rows[:, 11] == 1.0]
+        feats.append(rows[:, :8].copy())
+        targs.append(rows[:, 8:11].copy())
+        total += len(rows)
+        if total >= max_samples:
+            break
+    f = np.concatenate(feats)[:max_samples]
+    t = np.concatenate(targs)[:max_samples]
+    w.close()
+    return f, t
+
+
+def evaluate(model, cfg, mlp, control_steps: int, settle_steps: int = 40, healthy_residual: float = 1.0) -> dict:
+    """Cone sweeps per control step at the config's residual tolerance, from the same settled
+    state, with the plain warm start and with `mlp`. Worlds whose plain run ever shows a normal
+    residual above `healthy_residual` (diverging worlds) are reported apart."""
+    torch = _torch()
+    import time
+    from .native import World
+
+    cmds = model.commands(settle_steps, control_steps)
+    base = _settled_world(model, cfg, settle_steps)
+    q0, v0, s0 = base.state()
+    h0, p0, c0 = base.manifolds()
+    base.close()
+    runs = {}
+    for name, net in (("plain", None), ("mlp", mlp)):
+        w = World(model, cfg)
+        w.set_state(q0, v0, s0)
+        w.write_manifolds(h0, p0, c0)
+        if net is not None:
+            w.set_warm_start_mlp(net)
+        sweeps = np.zeros(model.environment_count)
+        fallbacks = np.zeros(model.environment_count)
+        worst = np.zeros(model.environment_count)
+        ms = []
+        for t in range(control_steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            w.advance(cmds[t:t + 1])
+            ms.append((time.perf_counter() - t0) * 1e3)
+            cs = w.contact_statuses()[-1]
+            sweeps += cs["residuals"][:, 3]
+            fallbacks += cs["diagnostics"][:, 2]
+            worst = np.maximum(worst, np.nan_to_num(cs["residuals"][:, 1], nan=np.inf))
+        runs[name] = (sweeps, fallbacks, worst, float(np.median(ms)))
+        w.close()
+    healthy = runs["plain"][2] <= healthy_residual
+    out = {"worlds": int(model.environment_count), "healthy_worlds": int(healthy.sum()),
+           "control_steps": int(control_steps)}
+    for name, (sweeps, fallbacks, worst, ms) in runs.items():
+        out[name] = {
+            "sweeps_per_step": float(sweeps[healthy].mean() / control_steps),
+            "sweeps_per_step_all": float(sweeps.mean() / control_steps),
+            "fallbacks_per_step": float(fallbacks[healthy].mean() / control_steps),
+            "normal_residual_max_healthy": float(worst[healthy].max()) if healthy.any() else 0.0,
+            "ms_per_step": ms,
+        }
+    return out

@@ -79,9 +79,15 @@ def forward(p, x, quantize=True):
     return {"xq": xq, "z1": z1, "h1": h1, "z2": z2, "h2": h2, "y_hat": y}
 
 
-def mse(y_hat, target, out_scale):
+def mse(y_hat, target, out_scale, huber_delta=0.0):
+    """Loss and d loss / d y_hat: mean squared error, or the Huber loss for huber_delta > 0."""
     t = np.asarray(target, np.float64) / np.asarray(out_scale, np.float64)
     d = np.asarray(y_hat, np.float64) - t
+    if huber_delta > 0.0:
+        ad = np.abs(d)
+        inner = ad <= huber_delta
+        loss = np.where(inner, 0.5 * d * d, huber_delta * (ad - 0.5 * huber_delta))
+        return float(np.mean(loss)), np.clip(d, -huber_delta, huber_delta) / d.size
     return float(np.mean(d * d)), 2.0 * d / d.size
 
 

@@ -2,13 +2,14 @@
 oracle/mlp_oracle.py.
 
 CPU tests pin the restatement itself: BF16 rounding known answers, closed-form
-GELU values, backward() against central finite differences of the float64
-loss, and the first Adam step's known magnitude. GPU tests compare each
+GELU values, backward() and the Huber gradient against central finite
+differences of the float64 loss, the Huber limits, and the first Adam step's
+known magnitude. GPU tests compare each
 kernel with the restatement on the same inputs:
   * forward: h1/h2 BF16 bits agree except where the FP32 and float64 GELU
     values straddle a BF16 rounding boundary (<= 0.5 % of elements, one BF16
     ulp), z and y_hat within 2e-3 relative of their scale;
-  * loss/dy within 1e-5 relative;
+  * loss/dy (squared error and Huber) within 1e-5 relative;
   * backward from the GPU's own trace: every gradient within 2e-3 of its
     scale (BF16 operands, FP32 accumulation in a different order);
   * Adam: 1e-6 relative;
@@ -91,6 +92,25 @@ def test_backward_matches_finite_differences():
         assert abs(g[name][idx] - fd) <= 1e-6 * max(1.0, abs(fd)), (name, g[name][idx], fd)
 
 
+def test_huber_loss_gradient_and_limits():
+    rng = np.random.default_rng(12)
+    y = rng.standard_normal((6, 3)) * 2.0
+    t = rng.standard_normal((6, 3))
+    s = np.array([1.0, 2.0, 0.5])
+    loss, g = mo.mse(y, t, s, huber_delta=0.3)
+    for idx in ((0, 0), (3, 1), (5, 2)):
+        yp, ym = y.copy(), y.copy()
+        yp[idx] += 1e-7
+        ym[idx] -= 1e-7
+        fd = (mo.mse(yp, t, s, 0.3)[0] - mo.mse(ym, t, s, 0.3)[0]) / 2e-7
+        assert abs(g[idx] - fd) <= 1e-6, (idx, g[idx], fd)
+    # tiny residuals: Huber is half the squared error; huge ones grow linearly
+    l_small, _ = mo.mse(np.full((1, 3), 1e-3), np.zeros((1, 3)), np.ones(3), huber_delta=1.0)
+    assert abs(l_small - 0.5e-6) < 1e-15
+    l_big, g_big = mo.mse(np.full((1, 3), 100.0), np.zeros((1, 3)), np.ones(3), huber_delta=1.0)
+    assert abs(l_big - 99.5) < 1e-12 and np.allclose(g_big, 1.0 / 3.0)
+
+
 def test_adam_first_step_moves_by_the_learning_rate():
     g = np.array([3.0, -0.02, 1e-3])
     w, m1, m2 = mo.adam(np.zeros(3), g, np.zeros(3), np.zeros(3), 0.01, 0.9, 0.999, 1e-12, 1)
@@ -167,6 +187,23 @@ def test_gpu_loss_and_backward_match_oracle(gpu):
     _close(b["dz1"].cpu().numpy(), ref_dz1, 2e-3)
     for k in ("w1", "b1", "w2", "b2", "w3", "b3"):
         _close(g[k].cpu().numpy(), ref_g[k], 2e-3)
+
+
+@pytest.mark.gpu
+def test_gpu_huber_loss_matches_oracle(gpu):
+    import torch
+
+    p = random_params(13)
+    rng = np.random.default_rng(14)
+    x = rng.standard_normal((200, 8)).astype(np.float32)
+    t = (rng.standard_normal((200, 3)) * 3.0).astype(np.float32)
+    m = _upload(p, torch)
+    loss, _ = m.loss_and_grads(torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda(), huber_delta=0.5)
+    torch.cuda.synchronize()
+    b = m._bufs
+    ref_loss, ref_dy = mo.mse(b["y"].cpu().numpy(), t, p["out_scale"], huber_delta=0.5)
+    assert abs(float(loss.item()) - ref_loss) <= 1e-5 * ref_loss
+    _close(b["dy"].cpu().numpy(), ref_dy, 1e-5)
 
 
 @pytest.mark.gpu

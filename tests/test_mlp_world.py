@@ -4,8 +4,8 @@ Bit-exact plumbing checks against the plain kernel on the chain-and-boxes
 workload:
   * sample capture with the plain warm start changes nothing (states,
     manifolds, statuses), and its targets are the impulses the step stored;
-  * an MLP that predicts zero impulses equals the plain path started from
-    zero impulses (one microstep per control step, one sweep each);
+  * an MLP whose correction is zero reproduces the plain path, and a
+    constant correction changes only the sweeps the solve needs;
 each from a settled state with live contacts (40 plain control steps), and
 comparing manifolds up to each environment's count.
   * when every MLP-warm-started solve misses the tolerance, the fallback
@@ -27,6 +27,7 @@ def zero_mlp():
     state.update(in_shift=np.zeros(8, np.float32), in_scale=np.ones(8, np.float32),
                  out_scale=np.ones(3, np.float32))
     m.load_state(state)
+    m.min_correction = 0.0
     return m
 
 
@@ -116,28 +117,39 @@ def test_capture_changes_nothing_and_records_the_stored_impulses(gpu):
 
 
 @pytest.mark.gpu
-def test_zero_prediction_equals_a_cold_start(gpu):
-    model, cfg = chain_world(physics_substeps=1, control_timestep=0.005, residual_tolerance=1.0e3,
-                             max_iterations=1)
+def test_zero_correction_reproduces_the_plain_path(gpu):
+    """A network whose correction is zero leaves every warm start untouched: the
+    MLP instance then reproduces the plain path bit for bit."""
+    model, cfg = chain_world(residual_tolerance=1.0e-5, max_iterations=64)
     mlp = zero_mlp()
-
-    def zero_impulses(w):
-        h, p, c = w.manifolds()
-        p["impulses"][:] = 0.0
-        w.write_manifolds(h, p, c)
-
-    _, cold = run(model, cfg, 10, before_step=zero_impulses)
-    _, learned = run(model, cfg, 10, setup=lambda w: w.set_warm_start_mlp(mlp))
+    _, plain = run(model, cfg, 6)
+    _, learned = run(model, cfg, 6, setup=lambda w: w.set_warm_start_mlp(mlp))
     assert learned[7]["constraints"].sum() > 0
-    assert_same(cold[:6], learned[:6])
-    assert_same(cold[6:7], learned[6:7])
-    assert np.all(learned[7]["diagnostics"][..., 2] == 0.0)
+    assert_same(plain, learned)
+
+
+@pytest.mark.gpu
+def test_learned_correction_changes_the_warm_start_only(gpu):
+    """A constant correction moves the warm start; every microstep still
+    reaches the tolerance (or falls back) and the contact sets stay those the
+    plain path finds on its first step."""
+    model, cfg = chain_world(residual_tolerance=1.0e-5, max_iterations=128)
+    mlp = zero_mlp()
+    b3 = mlp.master["b3"]
+    b3.fill_(0.25)
+    _, plain = run(model, cfg, 1)
+    _, learned = run(model, cfg, 1, setup=lambda w: w.set_warm_start_mlp(mlp))
+    pc, lc = plain[7][-1], learned[7][-1]
+    np.testing.assert_array_equal(pc["constraints"], lc["constraints"])
+    assert np.all((lc["residuals"][:, 0] <= 1.0e-5) | (lc["diagnostics"][:, 2] > 0))
+    assert not np.array_equal(pc["residuals"][:, 3], lc["residuals"][:, 3])
 
 
 @pytest.mark.gpu
 def test_forced_fallback_reproduces_the_plain_path(gpu):
     model, cfg = chain_world(residual_tolerance=1.0e-30, max_iterations=3)
     mlp = zero_mlp()
+    mlp.master["b3"].fill_(0.25)  # a correction that replaces every warm start
     _, plain = run(model, cfg, 6)
     _, fb = run(model, cfg, 6, setup=lambda w: w.set_warm_start_mlp(mlp))
     assert fb[7]["constraints"].sum() > 0
@@ -149,3 +161,32 @@ def test_forced_fallback_reproduces_the_plain_path(gpu):
     np.testing.assert_array_equal(fc["residuals"][..., 3], 2.0 * pc["residuals"][..., 3])
     # every microstep with constraints fell back once
     assert np.all(fc["diagnostics"][..., 2] == pc["residuals"][..., 3] / 3.0)
+
+
+@pytest.mark.gpu
+def test_trained_warm_start_against_the_plain_one(gpu):
+    """Trains the MLP on samples the plain solve produced (chain-and-boxes
+    worlds), then runs held-out worlds at residual 1e-5 with and without it.
+    It shows the gain (sweeps saved per control step) and checks what the
+    fallback guarantees whatever the network predicts: every healthy world
+    ends each step at the tolerance or after a fallback, and the learned path
+    costs at most the fallback's bound of twice the plain sweeps."""
+    import metalrobo as mr
+    from metalrobo import warmstart as ws
+
+    model = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_CHAIN_BOXES, 256, 31)
+    cfg = model.default_config()
+    cfg.residual_tolerance = 1.0e-5
+    cfg.max_iterations = 128
+    f, t = ws.collect_samples(model, cfg, control_steps=20, settle_steps=40, max_samples=400_000)
+    assert len(f) > 100_000
+    mlp = ws.WarmStartMLP(seed=3)
+    rep = mlp.fit(f, t, steps=300, batch=4096)
+    assert rep.final_loss < rep.first_loss
+    ev_model = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_CHAIN_BOXES, 256, 32)
+    ev = ws.evaluate(ev_model, cfg, mlp, control_steps=5, settle_steps=40)
+    plain, learned = ev["plain"]["sweeps_per_step"], ev["mlp"]["sweeps_per_step"]
+    print(f"sweeps per control step: plain {plain:.1f}, learned {learned:.1f} "
+          f"({100.0 * (1.0 - learned / plain):+.1f} % saved), fallbacks {ev['mlp']['fallbacks_per_step']:.3f}")
+    assert ev["healthy_worlds"] > 0
+    assert learned <= 2.0 * plain + 1e-9
