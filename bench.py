@@ -154,6 +154,71 @@ def warm_start_leg(mr, args):
     }
 
 
+def heterogeneous_leg(mr, args):
+    """Heterogeneous batch: per-world random trees (6-40 DoF) with 0-63 free bodies, settled
+    so the free bodies have landed; one horizon timed on the plain per-world path (eager
+    launches, one workgroup per world) and on the autotuned schedule replayed from a captured
+    graph, from the same state. The two end states must agree bit for bit."""
+    import torch
+    from metalrobo import autotune as at
+
+    E = args.hetero_envs
+    model = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_RANDOM_TREES, E, args.seed)
+    cfg = model.default_config()
+    world = mr.World(model, cfg)
+    world.advance(model.commands(0, args.hetero_settle))
+    horizon = args.hetero_horizon
+    cmds = torch.from_numpy(model.commands(args.hetero_settle, horizon)).to("cuda:0")
+    start = at.snapshot(world)
+    # plain per-world path
+    world.set_schedule(None, 0)
+    times = []
+    for _ in range(2):
+        at.restore(world, start)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        world.enqueue_device(cmds)
+        world.wait()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    plain_s = min(times)
+    plain_end = at.snapshot(world)
+    cstat = world.contact_statuses()
+    # autotuned schedule on graph-captured horizons
+    at.restore(world, start)
+    t0 = time.perf_counter()
+    tuned = at.autotune(world, cmds, trial_horizons=2)
+    tune_s = time.perf_counter() - t0
+    times = []
+    for _ in range(2):
+        at.restore(world, start)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        world.enqueue_graph(cmds)
+        world.wait()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    tuned_s = min(times)
+    tuned_end = at.snapshot(world)
+    same = all(np.array_equal(np.ascontiguousarray(a).view(np.uint8), np.ascontiguousarray(b).view(np.uint8))
+               for a, b in zip(tuned_end[:3], plain_end[:3]))
+    h = model.models()
+    return {
+        "workload": f"random_trees_{E}", "worlds": E, "dof_range": [int(h["nv"].min()), int(h["nv"].max())],
+        "bodies_range": [int(h["body_count"].min()), int(h["body_count"].max())],
+        "constraints_per_world_range": [int(cstat["constraints"][-1].min()), int(cstat["constraints"][-1].max())],
+        "horizon_steps": horizon,
+        "plain_env_steps_per_s": round(E * horizon / plain_s, 1),
+        "tuned_env_steps_per_s": round(E * horizon / tuned_s, 1),
+        "speedup": round(plain_s / tuned_s, 4),
+        "schedule": {"name": tuned.best.name, "grid": tuned.best.grid,
+                     "resident_workgroups": tuned.resident_workgroups, "trials": tuned.trials,
+                     "autotune_s": round(tune_s, 2)},
+        "bit_exact_vs_plain": bool(same),
+        "failed_env_steps": int((world.statuses()["code"] != 0).sum()),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +238,11 @@ def main():
     ap.add_argument("--stage-profile", type=int, default=0, metavar="STEPS",
                     help="after the timed region, run STEPS more control steps with the per-stage shader-clock "
                          "counters on and report each stage's share of the cycles")
+    ap.add_argument("--skip-heterogeneous", action="store_true",
+                    help="skip the heterogeneous random-tree leg (work queue, graphs, autotuner)")
+    ap.add_argument("--hetero-envs", type=int, default=8192)
+    ap.add_argument("--hetero-settle", type=int, default=30)
+    ap.add_argument("--hetero-horizon", type=int, default=50)
     ap.add_argument("--skip-warm-start", action="store_true",
                     help="skip the learned-warm-start leg (chain-and-boxes workload)")
     ap.add_argument("--warm-start-samples", type=int, default=10_000_000)
@@ -299,6 +369,8 @@ def main():
         del src, dst
     if args.residual_tolerance > 0.0 and args.residual_steps > 0:
         out["at_residual"] = residual_leg(mr, model, commands, args)
+    if not args.skip_heterogeneous:
+        out["heterogeneous"] = heterogeneous_leg(mr, args)
     if not args.skip_warm_start:
         del world
         out["warm_start"] = warm_start_leg(mr, args)

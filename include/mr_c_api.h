@@ -42,8 +42,13 @@ typedef struct MRWorldModelHandle MRWorldModelHandle;
 typedef struct MRWorldHandle MRWorldHandle;
 
 enum MRSyntheticWorldKindC {
-    MR_SYNTHETIC_CHAIN_BOXES = 1u,     /* 7-DoF chain + 20 boxes on a plane */
-    MR_SYNTHETIC_HUMANOID_TERRAIN = 2u /* floating 29-DoF humanoid on a heightfield */
+    MR_SYNTHETIC_CHAIN_BOXES = 1u,      /* 7-DoF chain + 20 boxes on a plane */
+    MR_SYNTHETIC_HUMANOID_TERRAIN = 2u, /* floating 29-DoF humanoid on a heightfield */
+    /* heterogeneous batch: every world its own random tree of 6-40 revolute
+     * DoF (branch probability 0.3, link masses log-uniform 0.1-10 kg) on a
+     * fixed root, with 0 to (63 - links) free boxes or spheres (a world holds
+     * at most 64 bodies) over a ground plane */
+    MR_SYNTHETIC_RANDOM_TREES = 3u
 };
 
 typedef struct MRSyntheticWorldConfigC {
@@ -123,6 +128,20 @@ int mr_world_advance(MRWorldHandle* world, const float* commands, size_t command
 int mr_world_enqueue_device(MRWorldHandle* world, const float* device_commands, const uint32_t* device_reset_masks,
                             uint32_t control_step_count);
 int mr_world_wait(MRWorldHandle* world, MRWorldAdvanceC* out);
+/* Launch schedule (no counterpart in the reference, whose encoder dispatches
+ * one threadgroup per environment; for heterogeneous batches and the step
+ * server). grid = 0 restores one workgroup per environment. grid > 0 runs the
+ * persistent work-queue kernel with `grid` workgroups taking the environments
+ * of `order` (distinct indices; a subset is allowed) in that order; unlisted
+ * environments are not stepped and keep their previous status records. */
+int mr_world_set_schedule(MRWorldHandle* world, const uint32_t* order, size_t count, uint32_t grid);
+/* Workgroups of the control-step kernel the GPU keeps resident for this world. */
+uint32_t mr_world_resident_workgroups(const MRWorldHandle* world);
+/* Graph-captured horizon: control_step_count steps on device commands
+ * [steps][E][max_nv] at a fixed address (no reset masks), captured into a HIP
+ * graph on first use and replayed while the commands pointer, the step count,
+ * the schedule and the world's buffers are unchanged; wait with mr_world_wait. */
+int mr_world_enqueue_graph(MRWorldHandle* world, const float* device_commands, uint32_t control_step_count);
 int mr_world_read_state(const MRWorldHandle* world, float* q, size_t q_count, float* v, size_t v_count,
                         MRBodyStateC* scene, size_t scene_count);
 int mr_world_read_manifolds(const MRWorldHandle* world, MRManifoldHeaderC* headers, MRManifoldPointC* points,

@@ -317,6 +317,149 @@ void build_chain_boxes(ModelData& m, uint32_t E, uint64_t seed) {
     c.constraint_capacity = 256u;
 }
 
+// ------------------------------------------------------- random trees
+//
+// The heterogeneous batch: world e gets its own tree of n ~ U{6..40}
+// revolute DoF hanging from a fixed root 1.2 m above a plane (trees longer
+// than that rest part of their links on it). Link i attaches to link
+// i - 1 or, with probability 0.3, to a uniformly drawn earlier link; masses
+// are log-uniform in [0.1, 10] kg, lengths U[0.06, 0.2] m, one capsule per
+// link, joint axes drawn among x / y / z, viscous drag on every link, seeded
+// joint torques N(0, 1 N m). Free bodies (boxes or spheres, size
+// U[0.05, 0.2] m, density 500 kg/m^3) number U{0 .. 63 - links}, dropped from
+// 0.3-1.5 m, so contact counts differ widely between worlds. A branching
+// link leaves its parent's end tilted by +-0.8 rad so that branches fan out.
+// The links of one tree do not collide with each other (siblings meet at a
+// shared joint, where their relative contact Jacobian vanishes); they collide
+// with the free bodies and the plane.
+void build_random_trees(ModelData& m, uint32_t E, uint64_t seed) {
+    const double radius = 0.025;
+    std::vector<uint32_t> dofs(E), frees(E);
+    uint32_t max_dof = 0, max_free = 0;
+    for (uint32_t e = 0; e < E; ++e) {
+        dofs[e] = 6u + (uint32_t)std::min(34.0, std::floor(35.0 * random_unit(seed, e, 0u, 1u)));
+        const uint32_t room = 63u - (dofs[e] + 1u);
+        frees[e] = (uint32_t)std::min((double)room, std::floor((room + 1.0) * random_unit(seed, e, 0u, 2u)));
+        max_dof = std::max(max_dof, dofs[e]);
+        max_free = std::max(max_free, frees[e]);
+    }
+    m.max_nq = max_dof;
+    m.max_nv = max_dof;
+    m.max_scene_bodies = max_free + 1u;
+    // slightly compliant contacts (1e-7 m/N): bounds the impulse a contact can take where its
+    // Delassus block is nearly singular (a point near a fixed joint)
+    m.materials.push_back(MRMaterialC{f4(0.8, 0.8, 0.8, 0.8), f4(0.0, 0.5, 1.0e-7, 0.0)});
+    m.initial_q.assign((size_t)E * m.max_nq, 0.0f);
+    m.initial_v.assign((size_t)E * m.max_nv, 0.0f);
+    m.initial_scene.resize((size_t)E * m.max_scene_bodies);
+    const D3 axes[3] = {D3{1, 0, 0}, D3{0, 1, 0}, D3{0, 0, 1}};
+    for (uint32_t e = 0; e < E; ++e) {
+        const uint32_t n = dofs[e], links = n + 1u, free_count = frees[e];
+        MRModelHeaderC h;
+        std::memset(&h, 0, sizeof(h));
+        h.body_offset = (uint32_t)m.bodies.size();
+        h.body_count = links + free_count + 1u;
+        h.link_count = links;
+        h.root_type = MR_ROOT_FIXED;
+        h.dof_offset = (uint32_t)m.dofs.size();
+        h.nq = n;
+        h.nv = n;
+        h.shape_offset = (uint32_t)m.shapes.size();
+        h.shape_count = n + free_count + 1u;
+        h.scene_body_count = free_count + 1u;
+        h.material_offset = 0u;
+        h.material_count = 1u;
+        MRBodyC root = make_body(1.0, 0.01, 0.01, 0.01, MR_MOTION_DYNAMIC);
+        m.bodies.push_back(root);
+        std::vector<double> len(links, 0.0);
+        std::vector<uint32_t> depth(links, 0u);
+        for (uint32_t i = 1; i < links; ++i) {
+            uint32_t parent = i - 1u;
+            if (i > 1u && random_unit(seed, e, 0u, 3000u + i) < 0.3)
+                parent = (uint32_t)std::min((double)(i - 1u), std::floor(i * random_unit(seed, e, 0u, 3100u + i)));
+            const double mass = std::exp(std::log(0.1) + (std::log(10.0) - std::log(0.1)) *
+                                         random_unit(seed, e, 0u, 3200u + i));
+            len[i] = 0.06 + 0.14 * random_unit(seed, e, 0u, 3300u + i);
+            const double w = 2.0 * radius;
+            MRBodyC b = make_body(mass, mass * (len[i] * len[i] + w * w) / 12.0, mass * (w * w + w * w) / 12.0,
+                                  mass * (len[i] * len[i] + w * w) / 12.0, MR_MOTION_DYNAMIC);
+            b.damping = f4(0.1, 0.05, 0.0, 0.0);  // viscous drag on every link (N s/m, N m s)
+            b.parent = parent;
+            b.joint_type = MR_JOINT_REVOLUTE;
+            b.q_index = i - 1u;
+            b.v_index = i - 1u;
+            depth[i] = depth[parent] + 1u;
+            b.depth = depth[i];
+            const D3 ax = axes[std::min(2u, (uint32_t)std::floor(3.0 * random_unit(seed, e, 0u, 3400u + i)))];
+            b.axis = f4(ax.x, ax.y, ax.z);
+            if (parent == 0u) {
+                b.parent_anchor = f4(0.0, 0.0, 1.2);
+                b.parent_rotation = qf(DQ{-std::sqrt(0.5), 0.0, 0.0, std::sqrt(0.5)});  // local +y points down
+            } else {
+                b.parent_anchor = f4(0.0, 0.5 * len[parent], 0.0);
+                if (parent != i - 1u) {
+                    const double tilt = random_unit(seed, e, 0u, 3450u + i) < 0.5 ? 0.8 : -0.8;
+                    const D3 around = random_unit(seed, e, 0u, 3460u + i) < 0.5 ? D3{1, 0, 0} : D3{0, 0, 1};
+                    b.parent_rotation = qf(dq_axis(around, tilt));
+                }
+            }
+            b.child_anchor = f4(0.0, -0.5 * len[i], 0.0);
+            m.bodies.push_back(b);
+            MRDofC d = make_dof(i, i - 1u, MR_DOF_FLAG_ACTUATED | MR_DOF_FLAG_EFFORT_LIMIT);
+            d.limits = f4(0.0, 0.0, 0.0, 20.0);
+            d.drive = f4(0.0, 0.0, 0.1, 0.0);  // armature (reflected rotor inertia), kg m^2
+            m.dofs.push_back(d);
+            m.shapes.push_back(make_shape(i, MR_SHAPE_CAPSULE, 0u, D3{0, 0, 0}, kIdentity,
+                                          D3{radius, 0.5 * len[i], 0.0}, 0.01));
+        }
+        std::vector<std::array<double, 4>> placed;
+        for (uint32_t k = 0; k < free_count; ++k) {
+            const double size = 0.05 + 0.15 * random_unit(seed, e, 0u, 3500u + k);
+            const bool sphere = random_unit(seed, e, 0u, 3600u + k) < 0.5;
+            const double r = 0.5 * size;
+            const double mass = sphere ? 500.0 * 4.18879020478639 * r * r * r : 500.0 * size * size * size;
+            const double inertia = sphere ? 0.4 * mass * r * r : mass * size * size / 6.0;
+            const double br = sphere ? r : r * std::sqrt(3.0);
+            MRBodyC b = make_body(mass, inertia, inertia, inertia, MR_MOTION_DYNAMIC);
+            b.damping = f4(0.0, 0.0, 0.0, 0.0);
+            m.bodies.push_back(b);
+            D3 pos{0, 0, 0};
+            for (uint32_t attempt = 0; attempt < 10000u; ++attempt) {
+                pos.x = -1.2 + 2.4 * random_unit(seed, e, attempt + 1u, 3700u + k);
+                pos.y = -1.2 + 2.4 * random_unit(seed, e, attempt + 1u, 3800u + k);
+                pos.z = 0.3 + 1.2 * random_unit(seed, e, attempt + 1u, 3900u + k);
+                bool ok = std::sqrt(pos.x * pos.x + pos.y * pos.y) > br + 0.45;
+                for (const auto& p : placed) {
+                    const double dx = pos.x - p[0], dy = pos.y - p[1], dz = pos.z - p[2];
+                    if (std::sqrt(dx * dx + dy * dy + dz * dz) < br + p[3] + 0.01) ok = false;
+                }
+                if (ok) break;
+            }
+            placed.push_back({pos.x, pos.y, pos.z, br});
+            m.shapes.push_back(make_shape(links + k, sphere ? MR_SHAPE_SPHERE : MR_SHAPE_BOX, 0u, D3{0, 0, 0}, kIdentity,
+                                          sphere ? D3{r, 0, 0} : D3{r, r, r}, 0.01));
+            m.initial_scene[(size_t)e * m.max_scene_bodies + k] = scene_state(pos, kIdentity, b, links + k);
+        }
+        MRBodyC plane = make_body(0.0, 0.0, 0.0, 0.0, MR_MOTION_STATIC);
+        m.bodies.push_back(plane);
+        m.shapes.push_back(make_shape(links + free_count, MR_SHAPE_PLANE, 0u, D3{0, 0, 0}, kYtoZ, D3{0, 0, 0}, 0.01));
+        m.initial_scene[(size_t)e * m.max_scene_bodies + free_count] =
+            scene_state(D3{0, 0, 0}, kIdentity, plane, links + free_count);
+        h.pair_offset = (uint32_t)m.pairs.size();
+        std::vector<MRCompiledPairC> pairs;
+        compile_pairs(h, &m.bodies[h.body_offset], &m.shapes[h.shape_offset], MR_MAX_LINKS, pairs);
+        h.pair_count = (uint32_t)pairs.size();
+        m.pairs.insert(m.pairs.end(), pairs.begin(), pairs.end());
+        m.models.push_back(h);
+        m.environment_model.push_back(e);
+    }
+    MRWorldStepConfigC& c = m.default_config;
+    c.actuation_mode = MR_ACTUATION_EFFORT;
+    c.pair_capacity = 384u;
+    c.manifold_capacity = 128u;
+    c.constraint_capacity = 384u;
+}
+
 // ----------------------------------------------------- humanoid on terrain
 
 struct LinkSpec {
@@ -609,6 +752,8 @@ void ModelData::commands(uint32_t first_step, uint32_t step_count, float* out) c
             for (uint32_t j = 0; j < max_nv; ++j) c[j] = 0.0f;
             if (kind == kChainBoxes) {
                 for (uint32_t j = 0; j < h.nv; ++j) c[j] = (float)(5.0 * random_normal(seed, e, step + 1u, 1000u + j));
+            } else if (kind == kRandomTrees) {
+                for (uint32_t j = 0; j < h.nv; ++j) c[j] = (float)(1.0 * random_normal(seed, e, step + 1u, 1000u + j));
             } else if (kind == kHumanoidTerrain) {
                 for (uint32_t j = 6u; j < h.nv; ++j)
                     c[j] = standing_pose[j] + (float)(-0.3 + 0.6 * random_unit(seed, e, step + 1u, 2000u + j));
@@ -774,6 +919,7 @@ ModelData build_synthetic(uint32_t kind, uint32_t E, uint64_t seed) {
     c.max_iterations = 64u;
     if (kind == kChainBoxes) build_chain_boxes(m, E, seed);
     else if (kind == kHumanoidTerrain) build_humanoid(m, E, seed);
+    else if (kind == kRandomTrees) build_random_trees(m, E, seed);
     else throw std::runtime_error("unknown synthetic world kind");
     finalize_model(m, c.control_timestep / (float)c.physics_substeps);
     return m;

@@ -19,8 +19,9 @@ double random_unit(uint64_t seed, uint64_t env, uint64_t step, uint64_t channel)
 double random_normal(uint64_t seed, uint64_t env, uint64_t step, uint64_t channel);
 
 enum SyntheticKind : uint32_t {
-    kChainBoxes = 1u,     // 7-DoF chain + 20 boxes on a plane (effort mode)
-    kHumanoidTerrain = 2u // floating 29-DoF humanoid on a heightfield (PD mode)
+    kChainBoxes = 1u,       // 7-DoF chain + 20 boxes on a plane (effort mode)
+    kHumanoidTerrain = 2u,  // floating 29-DoF humanoid on a heightfield (PD mode)
+    kRandomTrees = 3u       // per-world random 6-40 DoF revolute tree + free boxes/spheres (effort mode)
 };
 
 // Owned model + initial state. All per-environment arrays use the strides

@@ -274,6 +274,30 @@ int mr_world_enqueue_device(MRWorldHandle* world, const float* device_commands, 
     });
 }
 
+int mr_world_set_schedule(MRWorldHandle* world, const uint32_t* order, size_t count, uint32_t grid) {
+    return translate([&]() {
+        require(world != nullptr, "null argument");
+        require(count <= world->world->environment_count(), "set_schedule: more entries than environments");
+        world->world->set_schedule(order, (uint32_t)count, grid);
+    });
+}
+
+uint32_t mr_world_resident_workgroups(const MRWorldHandle* world) {
+    uint32_t out = 0u;
+    (void)translate([&]() {
+        require(world != nullptr, "null argument");
+        out = world->world->resident_workgroups();
+    });
+    return out;
+}
+
+int mr_world_enqueue_graph(MRWorldHandle* world, const float* device_commands, uint32_t control_step_count) {
+    return translate([&]() {
+        require(world != nullptr && device_commands != nullptr, "null argument");
+        world->world->enqueue_graph(device_commands, control_step_count);
+    });
+}
+
 int mr_world_wait(MRWorldHandle* world, MRWorldAdvanceC* out) {
     return translate([&]() {
         require(world != nullptr, "null argument");

@@ -181,6 +181,11 @@ struct KernelArgs {
     // inputs of this control step
     const float* command;       // [E][max_nv]
     const uint32_t* reset_mask; // [E] or null
+    // work-queue instance only: environments in processing order, and the
+    // head every workgroup takes its next entry from (zeroed before a launch)
+    const uint32_t* queue;
+    uint32_t queue_count;
+    uint32_t* queue_head;
     const float* reset_q;
     const float* reset_v;
     const MRBodyStateC* reset_scene;

@@ -1819,7 +1819,7 @@ MRD_INL bool mlp_warm_start(const Env& e, uint32_t lane, uint32_t ncons, uint32_
 // One control step of one environment (one wavefront per workgroup). kMlp
 // selects the instance with the learned warm start and its sample capture;
 // the plain instance carries none of that code.
-template <bool kMlp> __device__ __forceinline__ void control_step(const KernelArgs& args) {
+template <bool kMlp> __device__ __forceinline__ void control_step(const KernelArgs& args, uint32_t env) {
     extern __shared__ __align__(16) unsigned char lds_base[];
     __shared__ unsigned long long s_fail;
     __shared__ uint32_t s_count[8];
@@ -1828,7 +1828,6 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
     __shared__ MRContactStatusC s_cs;
 
     const uint32_t lane = threadIdx.x;
-    const uint32_t env = blockIdx.x;
     if (env >= args.environment_count) return;
     const KernelArgs& A = args;
 
@@ -2955,7 +2954,30 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
 // per-world stages are latency-bound; one wave per SIMD left it idle).
 extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 mr_control_step_kernel(KernelArgs args) {
-    control_step<false>(args);
+    control_step<false>(args, blockIdx.x);
+}
+
+// Persistent work-queue instance: a fixed grid of workgroups, each taking the
+// next environment of args.queue (in the queue's order) until the queue is
+// drained, so the order (heaviest first, say) and the number of resident
+// workgroups are the caller's choice. The head is zeroed by
+// mr_queue_reset_kernel before every launch.
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+mr_control_step_queue_kernel(KernelArgs args) {
+    __shared__ uint32_t s_next;
+    for (;;) {
+        if (threadIdx.x == 0u) s_next = atomicAdd(args.queue_head, 1u);
+        __syncthreads();
+        const uint32_t idx = s_next;
+        __syncthreads();
+        if (idx >= args.queue_count) break;
+        control_step<false>(args, args.queue[idx]);
+        __syncthreads();
+    }
+}
+
+extern "C" __global__ void mr_queue_reset_kernel(uint32_t* head) {
+    if (threadIdx.x == 0u && blockIdx.x == 0u) *head = 0u;
 }
 
 // The learned-warm-start instance: the MLP tile keeps its accumulators and
@@ -2963,7 +2985,7 @@ mr_control_step_kernel(KernelArgs args) {
 // per SIMD with the full register file.
 extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 mr_control_step_mlp_kernel(KernelArgs args) {
-    control_step<true>(args);
+    control_step<true>(args, blockIdx.x);
 }
 
 // Advance summary: one thread clears it, then a grid-stride pass over the
@@ -3032,12 +3054,25 @@ static_assert(sizeof(Collider) == 124, "Collider layout");
 static_assert(sizeof(HotConstraint) == 128, "HotConstraint layout");
 static_assert(sizeof(SolveBlock) == 64 && sizeof(HotConstraint) % sizeof(SolveBlock) == 0, "SolveBlock layout");
 
-hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream, bool mlp) {
+hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream, bool mlp,
+                               uint32_t queue_grid) {
+    if (queue_grid > 0u) {
+        hipLaunchKernelGGL(mr_queue_reset_kernel, dim3(1), dim3(64), 0, stream, args.queue_head);
+        hipLaunchKernelGGL(mr_control_step_queue_kernel, dim3(queue_grid), dim3(64), lds_bytes, stream, args);
+        return hipGetLastError();
+    }
     if (mlp)
         hipLaunchKernelGGL(mr_control_step_mlp_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
     else
         hipLaunchKernelGGL(mr_control_step_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
     return hipGetLastError();
+}
+
+// Workgroups of the plain or work-queue control-step kernel one CU holds.
+hipError_t control_step_occupancy(uint32_t lds_bytes, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
+                                                        reinterpret_cast<const void*>(&mr_control_step_queue_kernel),
+                                                        64, lds_bytes);
 }
 
 // The dynamic-LDS limit is an attribute of the kernel function, shared by
@@ -3047,8 +3082,11 @@ hipError_t configure_control_step(uint32_t lds_bytes, uint32_t mlp_lds_bytes) {
     static uint32_t plain_max = 0u, mlp_max = 0u;
     std::lock_guard<std::mutex> lock(guard);
     if (lds_bytes > plain_max) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mr_control_step_kernel),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mr_control_step_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mr_control_step_queue_kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
         if (e != hipSuccess) return e;
         plain_max = lds_bytes;
     }

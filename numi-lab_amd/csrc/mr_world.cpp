@@ -9,7 +9,9 @@
 
 namespace mr {
 
-hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream, bool mlp);
+hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream, bool mlp,
+                               uint32_t queue_grid);
+hipError_t control_step_occupancy(uint32_t lds_bytes, int* blocks_per_cu);
 hipError_t configure_control_step(uint32_t lds_bytes, uint32_t mlp_lds_bytes);
 hipError_t launch_advance_summary(const MRWorldStatusC* status, const MRContactStatusC* cstatus, uint64_t count,
                                   uint32_t environments, AdvanceSummaryDev* summary, hipStream_t stream);
@@ -118,8 +120,9 @@ World::World(const ModelData& model, const MRWorldStepConfigC& config) : model_(
 }
 
 World::~World() {
+    if (stream_ != nullptr) (void)hipStreamSynchronize(stream_);
+    drop_graph();
     if (stream_ != nullptr) {
-        (void)hipStreamSynchronize(stream_);
         (void)hipStreamDestroy(stream_);
     }
     if (ev_begin_ != nullptr) (void)hipEventDestroy(ev_begin_);
@@ -390,6 +393,9 @@ KernelArgs World::base_args(uint32_t step) const {
     a.mlp = mlp_;
     a.mlp_enabled = mlp_enabled_ ? 1u : 0u;
     a.mlp_samples = mlp_capture_ ? static_cast<float*>(d_samples_.ptr) : nullptr;
+    a.queue = static_cast<const uint32_t*>(d_queue_.ptr);
+    a.queue_count = queue_count_;
+    a.queue_head = static_cast<uint32_t*>(d_queue_head_.ptr);
     return a;
 }
 
@@ -429,6 +435,113 @@ void World::read_mlp_samples(float* out) const {
               "hipMemcpy samples");
 }
 
+void World::launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, bool events) {
+    const size_t E = env_;
+    const size_t per_step = E * model_.max_nv;
+    d_status_.reserve((size_t)steps * E * sizeof(MRWorldStatusC));
+    d_cstatus_.reserve((size_t)steps * E * sizeof(MRContactStatusC));
+    if (events) hip_check(hipEventRecord(ev_begin_, stream_), "event");
+    if (events && step_timing_) hip_check(hipEventRecord(step_events_[0], stream_), "event");
+    for (uint32_t s = 0; s < steps; ++s) {
+        KernelArgs a = base_args(s);
+        a.command = cmd + (size_t)s * per_step;
+        a.reset_mask = msk != nullptr ? msk + (size_t)s * E : nullptr;
+        a.status = static_cast<MRWorldStatusC*>(d_status_.ptr) + (size_t)s * E;
+        a.cstatus = static_cast<MRContactStatusC*>(d_cstatus_.ptr) + (size_t)s * E;
+        hip_check(launch_control_step(a, mlp_instance() ? lds_mlp_.total : lds_.total, stream_, mlp_instance(),
+                                      mlp_instance() ? 0u : queue_grid_),
+                  "control step launch");
+        if (events && step_timing_) hip_check(hipEventRecord(step_events_[s + 1u], stream_), "event");
+    }
+    if (events) hip_check(hipEventRecord(ev_end_, stream_), "event");
+    d_summary_.reserve(sizeof(AdvanceSummaryDev));
+    hip_check(launch_advance_summary(static_cast<const MRWorldStatusC*>(d_status_.ptr),
+                                     static_cast<const MRContactStatusC*>(d_cstatus_.ptr), (uint64_t)steps * E,
+                                     (uint32_t)E, static_cast<AdvanceSummaryDev*>(d_summary_.ptr), stream_),
+              "advance summary");
+}
+
+void World::set_schedule(const uint32_t* order, uint32_t count, uint32_t grid) {
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    if (grid == 0u) {
+        if (order != nullptr) throw std::runtime_error("set_schedule: an order needs the work-queue schedule (grid > 0)");
+        queue_grid_ = 0u;
+        queue_count_ = 0u;
+        ++schedule_version_;
+        return;
+    }
+    if (order == nullptr || count == 0u) throw std::runtime_error("set_schedule: the work queue needs an order");
+    if (mlp_instance()) throw std::runtime_error("set_schedule: the learned warm start runs one workgroup per environment");
+    std::vector<uint8_t> seen(env_, 0u);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (order[i] >= env_) throw std::runtime_error("set_schedule: environment index out of range");
+        if (seen[order[i]]) throw std::runtime_error("set_schedule: an environment appears twice in the order");
+        seen[order[i]] = 1u;
+    }
+    d_queue_.reserve((size_t)count * sizeof(uint32_t));
+    hip_check(hipMemcpy(d_queue_.ptr, order, (size_t)count * sizeof(uint32_t), hipMemcpyHostToDevice), "queue");
+    d_queue_head_.reserve(sizeof(uint32_t));
+    queue_grid_ = grid;
+    queue_count_ = count;
+    ++schedule_version_;
+}
+
+uint32_t World::resident_workgroups() const {
+    int per_cu = 0;
+    hip_check(control_step_occupancy(lds_.total, &per_cu), "occupancy");
+    int device = 0, cus = 0;
+    hip_check(hipGetDevice(&device), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
+    return (uint32_t)(per_cu * cus);
+}
+
+void World::drop_graph() {
+    if (graph_exec_ != nullptr) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_ != nullptr) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    graph_commands_ = nullptr;
+    graph_steps_ = 0u;
+}
+
+void World::enqueue_graph(const float* device_commands, uint32_t steps) {
+    if (steps == 0u || device_commands == nullptr) throw std::runtime_error("enqueue_graph: commands and steps required");
+    if (stage_profiling_) throw std::runtime_error("enqueue_graph: stage profiling is not captured in graphs");
+    // the buffers the launches point at exist before the comparison and the capture
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    d_status_.reserve((size_t)steps * env_ * sizeof(MRWorldStatusC));
+    d_cstatus_.reserve((size_t)steps * env_ * sizeof(MRContactStatusC));
+    d_summary_.reserve(sizeof(AdvanceSummaryDev));
+    const KernelArgs now = base_args(0u);
+    const bool reuse = graph_exec_ != nullptr && graph_commands_ == device_commands && graph_steps_ == steps &&
+        std::memcmp(&now, &graph_args_, sizeof(KernelArgs)) == 0 && graph_status_ == d_status_.ptr &&
+        graph_cstatus_ == d_cstatus_.ptr && graph_summary_ == d_summary_.ptr;
+    if (!reuse) {
+        drop_graph();
+        hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+        try {
+            launch_steps(device_commands, nullptr, steps, false);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g != nullptr) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(stream_, &graph_), "end capture");
+        hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "graph instantiate");
+        graph_commands_ = device_commands;
+        graph_steps_ = steps;
+        graph_args_ = now;
+        graph_status_ = d_status_.ptr;
+        graph_cstatus_ = d_cstatus_.ptr;
+        graph_summary_ = d_summary_.ptr;
+    }
+    hip_check(hipEventRecord(ev_begin_, stream_), "event");
+    hip_check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
+    hip_check(hipEventRecord(ev_end_, stream_), "event");
+    last_steps_ = steps;
+}
+
 void World::enqueue(const float* commands, const uint32_t* masks, uint32_t steps, bool device_inputs) {
     if (steps == 0u) throw std::runtime_error("control_step_count must be positive");
     if (commands == nullptr) throw std::runtime_error("commands are required");
@@ -462,24 +575,7 @@ void World::enqueue(const float* commands, const uint32_t* masks, uint32_t steps
             step_events_.push_back(ev);
         }
     }
-    hip_check(hipEventRecord(ev_begin_, stream_), "event");
-    if (step_timing_) hip_check(hipEventRecord(step_events_[0], stream_), "event");
-    for (uint32_t s = 0; s < steps; ++s) {
-        KernelArgs a = base_args(s);
-        a.command = cmd + (size_t)s * per_step;
-        a.reset_mask = msk != nullptr ? msk + (size_t)s * E : nullptr;
-        a.status = static_cast<MRWorldStatusC*>(d_status_.ptr) + (size_t)s * E;
-        a.cstatus = static_cast<MRContactStatusC*>(d_cstatus_.ptr) + (size_t)s * E;
-        hip_check(launch_control_step(a, mlp_instance() ? lds_mlp_.total : lds_.total, stream_, mlp_instance()),
-                  "control step launch");
-        if (step_timing_) hip_check(hipEventRecord(step_events_[s + 1u], stream_), "event");
-    }
-    hip_check(hipEventRecord(ev_end_, stream_), "event");
-    d_summary_.reserve(sizeof(AdvanceSummaryDev));
-    hip_check(launch_advance_summary(static_cast<const MRWorldStatusC*>(d_status_.ptr),
-                                     static_cast<const MRContactStatusC*>(d_cstatus_.ptr), (uint64_t)steps * E,
-                                     (uint32_t)E, static_cast<AdvanceSummaryDev*>(d_summary_.ptr), stream_),
-              "advance summary");
+    launch_steps(cmd, msk, steps, true);
     last_steps_ = steps;
 }
 

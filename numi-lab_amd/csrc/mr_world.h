@@ -81,6 +81,19 @@ public:
     void set_mlp_capture(bool enabled);
     void read_mlp_samples(float* out) const;  // [E][constraint_capacity][kMlpSampleFloats] of the last step
     const LdsLayout& mlp_lds() const { return lds_mlp_; }
+    // Launch schedule. grid == 0: one workgroup per environment (the default).
+    // grid > 0: the persistent work-queue instance with `grid` workgroups taking
+    // the environments of `order` (distinct indices, a subset allowed) in that
+    // order; environments not listed are not stepped and their status records
+    // are left as they were.
+    void set_schedule(const uint32_t* order, uint32_t count, uint32_t grid);
+    uint32_t queue_grid() const { return queue_grid_; }
+    // Workgroups of the control-step kernel the GPU holds at once for this world.
+    uint32_t resident_workgroups() const;
+    // Graph-captured horizon: `steps` control steps on device commands
+    // [steps][E][max_nv] at a fixed address, captured once per (pointer, steps,
+    // schedule) and replayed afterwards; wait() as after enqueue.
+    void enqueue_graph(const float* device_commands, uint32_t steps);
 
 private:
     KernelArgs base_args(uint32_t step) const;
@@ -95,6 +108,18 @@ private:
     MRMlpWeightsC mlp_{};
     bool mlp_enabled_ = false, mlp_capture_ = false;
     bool mlp_instance() const { return mlp_enabled_ || mlp_capture_; }
+    uint32_t queue_grid_ = 0, queue_count_ = 0, schedule_version_ = 0;
+    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    const float* graph_commands_ = nullptr;
+    uint32_t graph_steps_ = 0;
+    // what the captured launches were built from: replayed only while unchanged
+    KernelArgs graph_args_{};
+    const void* graph_status_ = nullptr;
+    const void* graph_cstatus_ = nullptr;
+    const void* graph_summary_ = nullptr;
+    void launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, bool events);
+    void drop_graph();
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_begin_ = nullptr, ev_end_ = nullptr;
     // model
@@ -105,7 +130,7 @@ private:
     // per-advance inputs / outputs
     DeviceBuffer d_commands_, d_masks_, d_status_, d_cstatus_, d_accel_, d_ev_pairs_, d_ev_counts_;
     // scratch
-    DeviceBuffer d_stage_h_, d_stage_p_, d_jrow_, d_resp_, d_cold_, d_hot_, d_stage_cycles_, d_summary_, d_samples_;
+    DeviceBuffer d_stage_h_, d_stage_p_, d_jrow_, d_resp_, d_cold_, d_hot_, d_stage_cycles_, d_summary_, d_samples_, d_queue_, d_queue_head_;
     bool stage_profiling_ = false;
     uint32_t last_steps_ = 0;
     float last_kernel_ms_ = 0.0f;

@@ -11,6 +11,7 @@ from .native import (  # noqa: F401
     MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN,
     MR_SYNTHETIC_CHAIN_BOXES,
     MR_SYNTHETIC_HUMANOID_TERRAIN,
+    MR_SYNTHETIC_RANDOM_TREES,
     PAIR_RECORD_DTYPE,
     STATUS_NAMES,
     WORLD_STATUS_DTYPE,

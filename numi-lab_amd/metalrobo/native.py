@@ -25,6 +25,7 @@ MR_MANIFOLD_POINT_CAPACITY: Final = 4
 
 MR_SYNTHETIC_CHAIN_BOXES: Final = 1
 MR_SYNTHETIC_HUMANOID_TERRAIN: Final = 2
+MR_SYNTHETIC_RANDOM_TREES: Final = 3
 MR_RUN_SOURCE_SYNTHETIC_CHAIN_BOXES: Final = 100
 MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN: Final = 101
 
@@ -298,6 +299,9 @@ _SIGNATURES: Final = {
     "mr_task_rollout_active_contacts": ([ct.c_void_p], ct.POINTER(ct.c_uint32)),
     "mr_task_rollout_final_q": ([ct.c_void_p], ct.POINTER(ct.c_float)),
     "mr_task_rollout_copy_final_scene_states": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
+    "mr_world_set_schedule": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_uint32], ct.c_int),
+    "mr_world_resident_workgroups": ([ct.c_void_p], ct.c_uint32),
+    "mr_world_enqueue_graph": ([ct.c_void_p, ct.c_void_p, ct.c_uint32], ct.c_int),
     "mr_world_set_warm_start_mlp": ([ct.c_void_p, ct.c_void_p], ct.c_int),
     "mr_world_set_mlp_capture": ([ct.c_void_p, ct.c_uint32], ct.c_int),
     "mr_world_read_mlp_samples": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
@@ -587,6 +591,28 @@ class World:
 
     def set_step_timing(self, enabled: bool) -> None:
         self._b.check(self._b.lib.mr_world_set_step_timing(self._h, 1 if enabled else 0))
+
+    def set_schedule(self, order=None, grid: int = 0) -> None:
+        """grid = 0: one workgroup per environment. grid > 0: the persistent work queue with
+        `grid` workgroups over `order` (environment indices, a subset allowed)."""
+        if grid == 0:
+            self._b.check(self._b.lib.mr_world_set_schedule(self._h, None, 0, 0))
+            return
+        o = np.ascontiguousarray(order, dtype=np.uint32)
+        self._b.check(self._b.lib.mr_world_set_schedule(self._h, _ptr(o), o.size, int(grid)))
+
+    def resident_workgroups(self) -> int:
+        n = int(self._b.lib.mr_world_resident_workgroups(self._h))
+        if n == 0:
+            raise MetalRoboError(self._b.last_error())
+        return n
+
+    def enqueue_graph(self, commands) -> None:
+        """Graph-captured horizon over device commands [steps][E][max_nv] (a torch tensor kept at
+        one address between calls); wait() afterwards."""
+        steps = int(commands.shape[0])
+        self.last_steps = steps
+        self._b.check(self._b.lib.mr_world_enqueue_graph(self._h, commands.data_ptr(), steps))
 
     def set_warm_start_mlp(self, mlp) -> None:
         """Warm-start later advances from `mlp` (a metalrobo.warmstart.WarmStartMLP, kept alive

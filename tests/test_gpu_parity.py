@@ -84,6 +84,7 @@ def compare_step(world, ost, k, o_status, o_cs, o_pairs, o_counts, log):
 @pytest.mark.parametrize("kind,envs,steps,seed", [
     (mr.MR_SYNTHETIC_CHAIN_BOXES, 16, 48, 3),
     (mr.MR_SYNTHETIC_HUMANOID_TERRAIN, 8, 24, 5),
+    (mr.MR_SYNTHETIC_RANDOM_TREES, 12, 30, 9),
 ])
 def test_teacher_forced_parity(gpu, kind, envs, steps, seed):
     model = mr.WorldModel.synthetic(kind, envs, seed)

@@ -1,0 +1,105 @@
+"""Launch schedules on the heterogeneous batch (per-world random trees with
+uneven contact counts): the persistent work-queue kernel in any order, a
+subset queue, and graph-captured horizons all give the per-world path's
+results bit for bit (the same per-environment kernel body runs in each)."""
+
+import numpy as np
+import pytest
+
+import metalrobo as mr
+
+pytestmark = pytest.mark.gpu
+
+
+def snapshot(world):
+    q, v, s = world.state()
+    h, p, c = world.manifolds()
+    for e in range(len(c)):
+        h[e, c[e]:] = np.zeros(1, h.dtype)
+        p[e, c[e]:] = np.zeros(1, p.dtype)
+    return q, v, s, h, p, c
+
+
+def same(a, b):
+    return all(np.array_equal(np.ascontiguousarray(x).view(np.uint8), np.ascontiguousarray(y).view(np.uint8))
+               for x, y in zip(a, b))
+
+
+def trees(envs=96, seed=4):
+    model = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_RANDOM_TREES, envs, seed)
+    return model, model.default_config()
+
+
+def test_work_queue_matches_the_per_world_grid(gpu):
+    model, cfg = trees()
+    cmds = model.commands(0, 12)
+    ref = mr.World(model, cfg)
+    ref.advance(cmds)
+    ref_status = ref.statuses()
+    queue = mr.World(model, cfg)
+    order = np.random.default_rng(0).permutation(model.environment_count)
+    queue.set_schedule(order, grid=17)  # fewer workgroups than worlds: each takes several
+    queue.advance(cmds)
+    assert same(snapshot(ref), snapshot(queue))
+    np.testing.assert_array_equal(ref_status["code"], queue.statuses()["code"])
+    assert ref.contact_statuses()["constraints"].sum() > 0
+
+
+def test_subset_queue_steps_only_the_listed_worlds(gpu):
+    model, cfg = trees(envs=48)
+    cmds = model.commands(0, 6)
+    full = mr.World(model, cfg)
+    full.advance(cmds)
+    part = mr.World(model, cfg)
+    listed = np.arange(0, model.environment_count, 3)
+    part.set_schedule(listed[::-1], grid=8)
+    before = snapshot(part)
+    part.advance(cmds)
+    after, stepped = snapshot(part), snapshot(full)
+    others = np.setdiff1d(np.arange(model.environment_count), listed)
+    for k in range(3):  # q, v, scene
+        np.testing.assert_array_equal(after[k][listed], stepped[k][listed])
+        np.testing.assert_array_equal(after[k][others], before[k][others])
+
+
+def test_graph_horizon_replays_match_eager(gpu):
+    import torch
+
+    model, cfg = trees(envs=64)
+    steps = 5
+    eager = mr.World(model, cfg)
+    graph = mr.World(model, cfg)
+    graph.set_schedule(np.arange(model.environment_count)[::-1], grid=32)
+    cmd_dev = torch.zeros((steps, model.environment_count, model.max_nv), dtype=torch.float32, device="cuda")
+    for rep in range(4):
+        host = model.commands(rep * steps, steps)
+        eager.advance(host)
+        cmd_dev.copy_(torch.from_numpy(host))
+        graph.enqueue_graph(cmd_dev)
+        graph.wait()
+        # eager work of the process between replays
+        _ = torch.ones(1 << 16, device="cuda").sum().item()
+        assert same(snapshot(eager), snapshot(graph)), f"replay {rep}"
+        np.testing.assert_array_equal(eager.statuses()["code"], graph.statuses()["code"])
+
+
+def test_autotuner_picks_a_schedule_that_reproduces_the_plain_path(gpu):
+    import torch
+
+    from metalrobo import autotune as at
+
+    model, cfg = trees(envs=256, seed=8)
+    world = mr.World(model, cfg)
+    world.advance(model.commands(0, 20))
+    cmds = torch.from_numpy(model.commands(20, 4)).cuda()
+    start = at.snapshot(world)
+    world.enqueue_device(cmds)
+    world.wait()
+    plain_end = snapshot(world)
+    at.restore(world, start)
+    res = at.autotune(world, cmds, trial_horizons=1)
+    assert len(res.trials) == 5 and res.resident_workgroups > 0
+    assert res.best.name == min(res.trials, key=lambda t: t["ms_per_step"])["name"]
+    world.enqueue_graph(cmds)
+    world.wait()
+    assert same(snapshot(world), plain_end)
