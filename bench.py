@@ -219,6 +219,36 @@ def heterogeneous_leg(mr, args):
     }
 
 
+def server_leg(mr, args):
+    """Step server on the 29-DoF humanoid: Poisson arrivals at 200 requests/s, N log-uniform in
+    [1, 1024] worlds, horizons 10-500 control steps, a resident pool admitted by a 200 GB HBM
+    budget; p50/p99 per-control-step latency and aggregate env-steps/s, next to the same first
+    requests served one after another with plain per-request launches."""
+    from metalrobo import server as sv
+
+    pool = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_HUMANOID_TERRAIN, args.serve_pool, args.seed)
+    srv = sv.StepServer(pool, hbm_budget_bytes=200e9)
+    reqs = sv.poisson_requests(200.0, args.serve_requests, seed=args.seed)
+    rep = srv.serve(reqs)
+    base_reqs = sv.poisson_requests(200.0, args.serve_requests, seed=args.seed)[:args.serve_serial_requests]
+    base = sv.serve_serially(lambda n: mr.WorldModel.synthetic(mr.MR_SYNTHETIC_HUMANOID_TERRAIN, n, args.seed),
+                             base_reqs)
+    return {
+        "workload": "humanoid29 requests, Poisson 200/s, N log-uniform 1-1024, horizon 10-500",
+        "requests": rep.requests, "pool_slots": int(pool.environment_count), "capacity_slots": rep.capacity_slots,
+        "bytes_per_slot": round(rep.bytes_per_slot, 1), "hbm_budget_bytes": rep.hbm_budget_bytes,
+        "env_steps": rep.env_steps, "wall_s": round(rep.wall_s, 3),
+        "env_steps_per_s": round(rep.env_steps_per_s, 1),
+        "latency_ms": {"p50": round(rep.latency_ms_p50, 4), "p99": round(rep.latency_ms_p99, 4)},
+        "ticks": rep.ticks, "mean_active_worlds": round(rep.mean_active_worlds, 1),
+        "max_active_worlds": rep.max_active_worlds,
+        "queue_wait_s": {"p50": round(rep.queue_wait_s_p50, 4), "p99": round(rep.queue_wait_s_p99, 4)},
+        "serial_plain": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in base.items()},
+        "speedup_vs_serial": round(rep.env_steps_per_s / base["env_steps_per_s"], 3)
+        if base["env_steps_per_s"] > 0 else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +268,10 @@ def main():
     ap.add_argument("--stage-profile", type=int, default=0, metavar="STEPS",
                     help="after the timed region, run STEPS more control steps with the per-stage shader-clock "
                          "counters on and report each stage's share of the cycles")
+    ap.add_argument("--skip-server", action="store_true", help="skip the step-server leg")
+    ap.add_argument("--serve-pool", type=int, default=16384)
+    ap.add_argument("--serve-requests", type=int, default=96)
+    ap.add_argument("--serve-serial-requests", type=int, default=16)
     ap.add_argument("--skip-heterogeneous", action="store_true",
                     help="skip the heterogeneous random-tree leg (work queue, graphs, autotuner)")
     ap.add_argument("--hetero-envs", type=int, default=8192)
@@ -371,6 +405,8 @@ def main():
         out["at_residual"] = residual_leg(mr, model, commands, args)
     if not args.skip_heterogeneous:
         out["heterogeneous"] = heterogeneous_leg(mr, args)
+    if not args.skip_server:
+        out["server"] = server_leg(mr, args)
     if not args.skip_warm_start:
         del world
         out["warm_start"] = warm_start_leg(mr, args)
