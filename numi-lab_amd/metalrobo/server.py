@@ -111,29 +111,40 @@ class StepServer:
         results = {}
         t0 = time.perf_counter()
         tick = 0
+        changed = True  # the active set differs from the one the schedule holds
         while pending or active:
             now = time.perf_counter() - t0
-            self.mask_host[:] = 0
+            admitted = []
             while pending and pending[0].arrival <= now and len(free) >= pending[0].worlds:
                 r = pending.popleft()
                 r.slots = np.array([free.popleft() for _ in range(r.worlds)], np.uint32)
                 r.admitted = now
-                self.mask_host[0, r.slots] = 1
-                active.append(r)
-            if not active:
+                admitted.append(r)
+            if not active and not admitted:
                 if pending and idle_sleep:
                     time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
                 continue
-            order = np.concatenate([r.slots for r in active])
-            self.world.set_schedule(order, min(self.grid, len(order)))
-            self.mask.copy_(torch.from_numpy(self.mask_host.view(np.int32)))
-            k = tick % self.commands.shape[0]
             ts = time.perf_counter()
-            self.world.enqueue_device(self.commands[k:k + 1], self.mask)
+            active.extend(admitted)
+            n_active = sum(r.worlds for r in active)
+            if admitted or changed:
+                # the schedule and reset mask travel only when the active set changes
+                order = np.concatenate([r.slots for r in active])
+                self.world.set_schedule(order, min(self.grid, len(order)))
+                changed = False
+            mask = None
+            if admitted:
+                self.mask_host[:] = 0
+                for r in admitted:
+                    self.mask_host[0, r.slots] = 1
+                self.mask.copy_(torch.from_numpy(self.mask_host.view(np.int32)))
+                mask = self.mask
+            k = tick % self.commands.shape[0]
+            self.world.enqueue_device(self.commands[k:k + 1], mask)
             self.world.wait()
             tick_ms.append((time.perf_counter() - ts) * 1e3)
-            active_counts.append(len(order))
-            env_steps += len(order)
+            active_counts.append(n_active)
+            env_steps += n_active
             tick += 1
             still = []
             end = time.perf_counter() - t0
@@ -146,6 +157,7 @@ class StepServer:
                         states = states if states is not None else self.world.state()
                         results[r.rid] = tuple(a[r.slots.astype(np.int64)].copy() for a in states)
                     free.extend(int(s) for s in r.slots)
+                    changed = True
                 else:
                     still.append(r)
             active = still
