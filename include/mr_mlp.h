@@ -11,9 +11,8 @@
  *
  * This subsystem has no counterpart in the reference: it is the learned
  * warm start that stands beside the plain temporalCone solve, which stays
- * its fallback (a microstep whose MLP-warm-started sweeps miss the residual
- * tolerance is solved again from the plain warm start) and the judge of its
- * residual. The stepping side is enabled per world through
+ * its fallback: a microstep whose MLP-warm-started sweeps miss the residual
+ * tolerance is solved again from the plain warm start. The stepping side is enabled per world through
  * mr_world_set_warm_start_mlp (mr_c_api.h).
  *
  * All pointers are device pointers (HIP); `stream` is a hipStream_t or

@@ -308,6 +308,9 @@ def evaluate(model, cfg, mlp, control_steps: int, settle_steps: int = 40, health
             "sweeps_per_step_all": float(sweeps.mean() / control_steps),
             "fallbacks_per_step": float(fallbacks[healthy].mean() / control_steps),
             "normal_residual_max_healthy": float(worst[healthy].max()) if healthy.any() else 0.0,
+            # worlds healthy under the plain warm start whose run here leaves that bound (a chaotic
+            # world can take another trajectory from a different, equally converged warm start)
+            "healthy_worlds_leaving_bound": int((worst[healthy] > healthy_residual).sum()),
             "ms_per_step": ms,
         }
     return out

@@ -2379,7 +2379,7 @@ int mr_oracle_control_step(const MRWorldModelDescC* desc, const MRWorldStepConfi
     return 0;
 }
 
-// Multi-threaded driver for the CPU baseline: splits [0, env_count) into
+// Multi-threaded runner for the CPU baseline: splits [0, env_count) into
 // contiguous blocks over `threads` std::threads.
 int mr_oracle_control_step_parallel(const MRWorldModelDescC* desc, const MRWorldStepConfigC* cfg,
                                     uint32_t env_count, uint32_t control_step, uint32_t threads,
