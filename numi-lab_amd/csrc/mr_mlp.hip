@@ -142,14 +142,59 @@ __global__ void __launch_bounds__(64) mlp_gemm_kernel(uint32_t M, uint32_t N, ui
     }
 }
 
-// out[j] = sum_m src[m][j], one thread per column, rows in order.
-extern "C" __global__ void __launch_bounds__(256) mr_mlp_colsum_kernel(const float* src, uint32_t n,
-                                                                       uint32_t cols, float* out) {
-    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
-    if (j >= cols) return;
-    float acc = 0.0f;
-    for (uint32_t m = 0; m < n; ++m) acc += src[(size_t)m * cols + j];
-    out[j] = acc;
+// Weight gradient C[i][j] = sum_p A[p][i] B[p][j] over a long row axis p
+// (the batch): A FP32 [K][a_ld], B [K][b_ld] (BF16 or FP32), both fed to the
+// matrix cores as BF16 with FP32 accumulation. One workgroup of 16
+// wavefronts per 16x16 output tile: wavefront w takes the 32-row chunks
+// w, w + 16, w + 32, ... of p, and the 16 partial tiles are summed through
+// LDS in wavefront order, so the result does not depend on scheduling. With
+// bias != null, the workgroups of the first column tile also return
+// bias[i] = sum_p A[p][i] from the FP32 values (the bias gradient), summed in
+// the same fixed order. Grid (ceil(N/16), ceil(M/16)), block 1024.
+constexpr uint32_t kWgradWaves = 16u;
+
+template <class TB>
+__global__ void __launch_bounds__(1024) mlp_wgrad_kernel(uint32_t M, uint32_t N, uint32_t K, const float* A,
+                                                         uint32_t a_ld, const TB* B, uint32_t b_ld, float* C,
+                                                         uint32_t c_ld, float* bias) {
+    __shared__ float s_tile[kWgradWaves][16][17];
+    __shared__ float s_bias[kWgradWaves][4][16];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t ar = lane & 15u, kg = lane >> 4;
+    const uint32_t i0 = blockIdx.y * 16u, j0 = blockIdx.x * 16u;
+    const uint32_t ai = i0 + ar, bj = j0 + ar;
+    const bool with_bias = bias != nullptr && blockIdx.x == 0u;
+    mlp_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;
+    for (uint32_t p0 = wave * 32u; p0 < K; p0 += kWgradWaves * 32u) {
+        mlp_bf16x8 a, b;
+#pragma unroll
+        for (uint32_t e = 0; e < 8u; ++e) {
+            const uint32_t p = p0 + kg * 8u + e;
+            const float av = (ai < M && p < K) ? A[(size_t)p * a_ld + ai] : 0.0f;
+            bsum += av;
+            a[e] = (__bf16)av;
+            b[e] = (bj < N && p < K) ? as_bf16(B[(size_t)p * b_ld + bj]) : (__bf16)0.0f;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < 4u; ++r) s_tile[wave][kg * 4u + r][ar] = acc[r];
+    s_bias[wave][kg][ar] = bsum;
+    __syncthreads();
+    if (threadIdx.x < 256u) {
+        const uint32_t row = threadIdx.x >> 4, col = threadIdx.x & 15u;
+        float v = 0.0f;
+        for (uint32_t w = 0; w < kWgradWaves; ++w) v += s_tile[w][row][col];
+        const uint32_t i = i0 + row, j = j0 + col;
+        if (i < M && j < N) C[(size_t)i * c_ld + j] = v;
+    } else if (with_bias && threadIdx.x < 272u) {
+        const uint32_t row = threadIdx.x - 256u;
+        float v = 0.0f;
+        for (uint32_t w = 0; w < kWgradWaves; ++w)
+            for (uint32_t g = 0; g < 4u; ++g) v += s_bias[w][g][row];
+        if (i0 + row < M) bias[i0 + row] = v;
+    }
 }
 
 extern "C" __global__ void __launch_bounds__(256) mr_mlp_adam_kernel(float* w, __bf16* w_bf16, const float* g,
@@ -175,6 +220,15 @@ hipError_t gemm(hipStream_t st, uint32_t M, uint32_t N, uint32_t K, const TA* A,
     const dim3 grid((N + 15u) / 16u, (M + 15u) / 16u);
     hipLaunchKernelGGL((mlp_gemm_kernel<TA, TB>), grid, dim3(64), 0, st, M, N, K, A, a_si, a_sp, B, b_sp, b_sj, C,
                        c_ld, gate);
+    return hipGetLastError();
+}
+
+template <class TB>
+hipError_t wgrad(hipStream_t st, uint32_t M, uint32_t N, uint32_t K, const float* A, uint32_t a_ld, const TB* B,
+                 uint32_t b_ld, float* C, uint32_t c_ld, float* bias) {
+    const dim3 grid((N + 15u) / 16u, (M + 15u) / 16u);
+    hipLaunchKernelGGL((mlp_wgrad_kernel<TB>), grid, dim3(64u * kWgradWaves), 0, st, M, N, K, A, a_ld, B, b_ld, C,
+                       c_ld, bias);
     return hipGetLastError();
 }
 
@@ -233,29 +287,23 @@ hipError_t mlp_backward(const MRMlpWeightsC& w, const MRMlpTraceC& trc, const fl
     const MlpView m = view_of(w);
     const MlpTrace tr = trace_of(trc);
     hipError_t err;
-    // layer 3
-    if ((err = gemm<float, __bf16>(st, kMlpOut, kMlpHidden, n, dy, 1u, kMlpOut, tr.h2, kMlpHidden, 1u, g.w3,
-                                   kMlpHidden, nullptr)) != hipSuccess)
+    // layer 3: dW3 = dy^T h2, db3 = column sums of dy
+    if ((err = wgrad<__bf16>(st, kMlpOut, kMlpHidden, n, dy, kMlpOut, tr.h2, kMlpHidden, g.w3, kMlpHidden, g.b3)) !=
+        hipSuccess)
         return err;
-    hipLaunchKernelGGL(mr_mlp_colsum_kernel, dim3(1), dim3(256), 0, st, dy, n, kMlpOut, g.b3);
     const size_t elems = (size_t)n * kMlpHidden;
     hipLaunchKernelGGL(mr_mlp_out_backward_kernel, dim3((uint32_t)((elems + 255u) / 256u)), dim3(256), 0, st, dy,
                        m.w3, tr.z2, n, dz2);
-    // layer 2
-    if ((err = gemm<float, __bf16>(st, kMlpHidden, kMlpHidden, n, dz2, 1u, kMlpHidden, tr.h1, kMlpHidden, 1u, g.w2,
-                                   kMlpHidden, nullptr)) != hipSuccess)
+    // layer 2: dW2 = dz2^T h1, db2 = column sums of dz2
+    if ((err = wgrad<__bf16>(st, kMlpHidden, kMlpHidden, n, dz2, kMlpHidden, tr.h1, kMlpHidden, g.w2, kMlpHidden,
+                             g.b2)) != hipSuccess)
         return err;
-    hipLaunchKernelGGL(mr_mlp_colsum_kernel, dim3(1), dim3(256), 0, st, dz2, n, kMlpHidden, g.b2);
     // dz1 = (dz2 W2) * gelu'(z1)
     if ((err = gemm<float, __bf16>(st, n, kMlpHidden, kMlpHidden, dz2, kMlpHidden, 1u, m.w2, kMlpHidden, 1u, dz1,
                                    kMlpHidden, tr.z1)) != hipSuccess)
         return err;
-    // layer 1
-    if ((err = gemm<float, __bf16>(st, kMlpHidden, kMlpIn, n, dz1, 1u, kMlpHidden, tr.xq, kMlpIn, 1u, g.w1, kMlpIn,
-                                   nullptr)) != hipSuccess)
-        return err;
-    hipLaunchKernelGGL(mr_mlp_colsum_kernel, dim3(1), dim3(256), 0, st, dz1, n, kMlpHidden, g.b1);
-    return hipGetLastError();
+    // layer 1: dW1 = dz1^T xq, db1 = column sums of dz1
+    return wgrad<__bf16>(st, kMlpHidden, kMlpIn, n, dz1, kMlpHidden, tr.xq, kMlpIn, g.w1, kMlpIn, g.b1);
 }
 
 hipError_t mlp_adam(float* w, uint16_t* w_bf16, const float* g, float* m1, float* m2, uint32_t count, float lr,

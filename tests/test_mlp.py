@@ -162,12 +162,14 @@ def test_gpu_forward_matches_oracle(gpu):
 
 
 @pytest.mark.gpu
-def test_gpu_loss_and_backward_match_oracle(gpu):
+@pytest.mark.parametrize("n", [300, 1000, 8192])
+def test_gpu_loss_and_backward_match_oracle(gpu, n):
+    """Rows not a multiple of the 32-row chunks (300, 1000) and the training batch (8192),
+    where the weight gradients split the rows over 16 wavefronts per tile."""
     import torch
 
     p = random_params(5)
     rng = np.random.default_rng(6)
-    n = 300
     x = rng.standard_normal((n, 8)).astype(np.float32)
     t = rng.standard_normal((n, 3)).astype(np.float32)
     m = _upload(p, torch)
