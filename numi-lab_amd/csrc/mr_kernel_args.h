@@ -130,12 +130,27 @@ struct LinkConst {
 };
 
 // Byte offsets of the per-workgroup dynamic LDS arrays.
+// One heightfield query result (32 bytes): unit normal, signed surface
+// distance (NaN when the point is off the grid), surface point, triangle
+// feature key.
+struct HfHit {
+    float n[3];
+    float d;
+    float p[3];
+    uint32_t feature;
+};
+static_assert(sizeof(HfHit) == 32, "HfHit layout");
+// Heightfield query points staged per control-step microstep: a humanoid on
+// terrain needs at most 64 (18 spheres, 10 capsules x 3, 2 boxes x 8).
+constexpr uint32_t kHfSlots = 128u;
+
 struct LdsLayout {
     uint32_t topo, lconst, arm;
     uint32_t q, v, vcand, effort, accel, cmd;
     uint32_t pos, rot, w, vl, sa, sl, ca, cl, r;
     uint32_t I, pt, pb, Ut, Ub, D, u, alpha, acc, L;
     uint32_t scene, col, active, npts, moff, okey;
+    uint32_t hfitem, hfq;  // heightfield query table (kHfSlots entries each), collision phase only
     uint32_t lam;    // [constraint_capacity][3] floats: cone impulses
     uint32_t cpair;  // [constraint_capacity]: body_a | body_b << 8 | kind << 16
     // response scratch, overlaying the collision arrays (dead after emission)

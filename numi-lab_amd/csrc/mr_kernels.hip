@@ -192,6 +192,8 @@ struct Env {
     MRBodyStateC* scene;
     Collider* col;
     uint32_t *active, *npts, *moff, *okey;
+    uint32_t* hfitem;  // [kHfSlots]: active pair | query << 16
+    mr::HfHit* hfq;    // [kHfSlots]: heightfield query results
     HotConstraint* hot;  // global scratch
     float* lam;          // LDS [ccap][3]
     uint32_t* cpair;     // LDS [ccap]: body_a | body_b << 8 | kind << 16
@@ -1161,13 +1163,18 @@ MRD_INL void append_surface_contact(RawBatch& out, bool surface_is_a, V3 n, V3 f
     append(out, c);
 }
 
-MRD_INL void heightfield_point(const Env& e, RawBatch& out, bool hf_is_a, V3 x, float rho, float accepted,
-                               uint32_t finite_feature) {
+// One query point of a finite shape against the heightfield: the triangle
+// under x, its unit normal n, the signed distance d of the shape surface
+// (x minus rho along n) and that surface point. d is NaN when x lies outside
+// the representable grid, so no threshold accepts it.
+MRD_INL mr::HfHit heightfield_query(const Env& e, V3 x, float rho) {
+    mr::HfHit h;
+    h.d = __builtin_nanf("");
     const MRHeightfieldC& hf = e.a->heightfield;
     const float s = hf.origin_spacing.z;
     const float fx = (x.x - hf.origin_spacing.x) / s;
     const float fy = (x.y - hf.origin_spacing.y) / s;
-    if (!isfin(fx) || !isfin(fy)) return;
+    if (!isfin(fx) || !isfin(fy)) return h;
     const int ix = (int)__builtin_floorf(clampd(fx, 0.0f, (float)(hf.nx - 2u)));
     const int iy = (int)__builtin_floorf(clampd(fy, 0.0f, (float)(hf.ny - 2u)));
     const float tx = fx - (float)ix;
@@ -1190,15 +1197,34 @@ MRD_INL void heightfield_point(const Env& e, RawBatch& out, bool hf_is_a, V3 x, 
     }
     const V3 n = normalize(cross(e1, e2));
     const float d = dot(n, sub(x, v00)) - rho;
-    if (d <= accepted) {
-        const uint32_t cell = (uint32_t)iy * (nx - 1u) + (uint32_t)ix;
-        append_surface_contact(out, hf_is_a, n, sub(x, mul(rho, n)), d, finite_feature,
-                               feature_key(MR_SHAPE_HEIGHTFIELD, 2u * cell + tri));
-    }
+    const V3 p = sub(x, mul(rho, n));
+    const uint32_t cell = (uint32_t)iy * (nx - 1u) + (uint32_t)ix;
+    h.n[0] = n.x; h.n[1] = n.y; h.n[2] = n.z;
+    h.d = d;
+    h.p[0] = p.x; h.p[1] = p.y; h.p[2] = p.z;
+    h.feature = feature_key(MR_SHAPE_HEIGHTFIELD, 2u * cell + tri);
+    return h;
 }
 
+// Query points of a finite shape against the heightfield (Collision.metal
+// meshContacts on a heightfield): a sphere's center; a capsule's two segment
+// ends and its center; a box's eight vertices.
+MRD_INL uint32_t hf_query_count(uint32_t type) {
+    return type == MR_SHAPE_SPHERE ? 1u : type == MR_SHAPE_CAPSULE ? 3u : type == MR_SHAPE_BOX ? 8u : 0u;
+}
+
+MRD_INL V3 hf_query_point(const Collider& fs, uint32_t q) {
+    if (fs.type == MR_SHAPE_CAPSULE) return q == 0u ? fs.p0 : q == 1u ? fs.p1 : fs.center;
+    if (fs.type == MR_SHAPE_BOX) return box_vertex(fs, q);
+    return fs.center;
+}
+
+MRD_INL float hf_query_radius(const Collider& fs) { return fs.type == MR_SHAPE_BOX ? 0.0f : fs.radius; }
+
+// hf_first: index of this pair's first precomputed heightfield query in e.hfq,
+// or ~0 to evaluate the queries here.
 MRD_INL void generate_contacts(const Env& e, uint32_t ia, uint32_t ib, uint32_t cls, const Collider& A,
-                               const Collider& B, RawBatch& out) {
+                               const Collider& B, RawBatch& out, uint32_t hf_first) {
     out.count = 0;
     const float accepted = (A.contact_offset + B.contact_offset) + pair_query_padding(A, B);
     if (cls == MR_PAIR_SPHERE_SPHERE) {
@@ -1297,15 +1323,13 @@ MRD_INL void generate_contacts(const Env& e, uint32_t ia, uint32_t ib, uint32_t 
     } else if (cls == MR_PAIR_HEIGHTFIELD) {
         const bool hf_a = A.type == MR_SHAPE_HEIGHTFIELD;
         const Collider& fs = hf_a ? B : A;
-        if (fs.type == MR_SHAPE_SPHERE) {
-            heightfield_point(e, out, hf_a, fs.center, fs.radius, accepted, feature_key(MR_SHAPE_SPHERE, 0u));
-        } else if (fs.type == MR_SHAPE_CAPSULE) {
-            heightfield_point(e, out, hf_a, fs.p0, fs.radius, accepted, feature_key(MR_SHAPE_CAPSULE, 0u));
-            heightfield_point(e, out, hf_a, fs.p1, fs.radius, accepted, feature_key(MR_SHAPE_CAPSULE, 1u));
-            heightfield_point(e, out, hf_a, fs.center, fs.radius, accepted, feature_key(MR_SHAPE_CAPSULE, 2u));
-        } else if (fs.type == MR_SHAPE_BOX) {
-            for (uint32_t vi = 0; vi < 8u; ++vi)
-                heightfield_point(e, out, hf_a, box_vertex(fs, vi), 0.0f, accepted, feature_key(MR_SHAPE_BOX, vi));
+        const uint32_t nq = hf_query_count(fs.type);
+        for (uint32_t q = 0; q < nq; ++q) {
+            const mr::HfHit h = hf_first != 0xffffffffu ? e.hfq[hf_first + q]
+                                                        : heightfield_query(e, hf_query_point(fs, q), hf_query_radius(fs));
+            if (h.d <= accepted)
+                append_surface_contact(out, hf_a, V3{h.n[0], h.n[1], h.n[2]}, V3{h.p[0], h.p[1], h.p[2]}, h.d,
+                                       feature_key(fs.type, q), h.feature);
         }
     }
 }
@@ -1887,6 +1911,8 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
     e.npts = lds_at<uint32_t>(lds_base, L.npts);
     e.moff = lds_at<uint32_t>(lds_base, L.moff);
     e.okey = lds_at<uint32_t>(lds_base, L.okey);
+    e.hfitem = lds_at<uint32_t>(lds_base, L.hfitem);
+    e.hfq = lds_at<mr::HfHit>(lds_base, L.hfq);
     e.lam = lds_at<float>(lds_base, L.lam);
     e.cpair = lds_at<uint32_t>(lds_base, L.cpair);
     e.rslot = lds_at<float>(lds_base, L.rslot);
@@ -2258,13 +2284,51 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
             const uint32_t ka = old_h[m].pair_and_count[1], kb = old_h[m].pair_and_count[2];
             e.okey[m] = (ka < 0x10000u && kb < 0x10000u) ? (ka | (kb << 16)) : 0xffffffffu;
         }
+        // Heightfield queries of all active pairs, one lane per query point
+        // (a sphere has one, a capsule three, a box eight), into e.hfq; moff
+        // holds each pair's first query until the manifold prefix below. A pair
+        // whose queries do not fit the table evaluates them in its own lane.
+        {
+            uint32_t nhf = 0;
+            for (uint32_t base = 0; base < nact; base += kLanes) {
+                const uint32_t k = base + lane;
+                uint32_t nq = 0;
+                if (k < nact) {
+                    const MRCompiledPairC pr = e.pairs[e.active[k]];
+                    if (pr.pair_class == MR_PAIR_HEIGHTFIELD) {
+                        const uint32_t fsi = e.col[pr.collider_a].type == MR_SHAPE_HEIGHTFIELD ? pr.collider_b
+                                                                                              : pr.collider_a;
+                        nq = hf_query_count(e.col[fsi].type);
+                    }
+                }
+                uint32_t tot;
+                const uint32_t before = wave_prefix_sum(nq, lane, tot);
+                if (k < nact) {
+                    const uint32_t first = nhf + before;
+                    const bool fits = nq > 0u && first + nq <= mr::kHfSlots;
+                    e.moff[k] = fits ? first : 0xffffffffu;
+                    if (fits)
+                        for (uint32_t q = 0; q < nq; ++q) e.hfitem[first + q] = k | (q << 16);
+                }
+                nhf += tot;
+            }
+            __syncthreads();
+            const uint32_t items = nhf < mr::kHfSlots ? nhf : mr::kHfSlots;
+            for (uint32_t i = lane; i < items; i += kLanes) {
+                const uint32_t k = e.hfitem[i] & 0xffffu, q = e.hfitem[i] >> 16;
+                const MRCompiledPairC pr = e.pairs[e.active[k]];
+                const uint32_t fsi = e.col[pr.collider_a].type == MR_SHAPE_HEIGHTFIELD ? pr.collider_b : pr.collider_a;
+                const Collider& fs = e.col[fsi];
+                e.hfq[i] = heightfield_query(e, hf_query_point(fs, q), hf_query_radius(fs));
+            }
+        }
         __syncthreads();
         for (uint32_t k = lane; k < nact; k += kLanes) {
             const MRCompiledPairC pr = e.pairs[e.active[k]];
             const Collider ca_ = e.col[pr.collider_a];
             const Collider cb_ = e.col[pr.collider_b];
             RawBatch raw;
-            generate_contacts(e, pr.collider_a, pr.collider_b, pr.pair_class, ca_, cb_, raw);
+            generate_contacts(e, pr.collider_a, pr.collider_b, pr.pair_class, ca_, cb_, raw, e.moff[k]);
             bool bad = false;
             for (uint32_t c = 0; c < raw.count; ++c)
                 if (!finite3(raw.c[c].normal) || !isfin(raw.c[c].separation) || !finite3(raw.c[c].point_a) ||

@@ -236,6 +236,8 @@ void World::compute_lds_layout() {
     L.npts = take(4u * cfg_.pair_capacity, 4u);
     L.moff = take(4u * cfg_.pair_capacity, 4u);
     L.okey = take(4u * cfg_.manifold_capacity, 4u);
+    L.hfitem = take(4u * kHfSlots, 4u);
+    L.hfq = take((uint32_t)sizeof(HfHit) * kHfSlots, 16u);
     union_end = std::max(union_end, off);
     off = union_start;
     uint32_t slots = 1u, stack = 1u;
