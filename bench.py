@@ -216,7 +216,15 @@ def heterogeneous_leg(mr, args):
                      "autotune_s": round(tune_s, 2)},
         "bit_exact_vs_plain": bool(same),
         "failed_env_steps": int((world.statuses()["code"] != 0).sum()),
+        "failed_by_status": {STATUS_NAMES.get(int(c), str(int(c))): int(n) for c, n in
+                             zip(*np.unique(world.statuses()["code"], return_counts=True)) if int(c) != 0},
     }
+
+
+STATUS_NAMES = {1: "fixed_budget_complete", 2: "nonfinite_input", 3: "nonfinite_result", 4: "pair_capacity_overflow",
+                5: "contact_capacity_overflow", 6: "manifold_capacity_overflow", 7: "constraint_capacity_overflow",
+                8: "island_capacity_overflow", 9: "ccd_event_budget_exhausted", 10: "factorization_failed",
+                11: "did_not_converge", 12: "unsupported", 13: "ccd_capacity_overflow"}
 
 
 def server_leg(mr, args):

@@ -373,6 +373,8 @@ void build_random_trees(ModelData& m, uint32_t E, uint64_t seed) {
         m.bodies.push_back(root);
         std::vector<double> len(links, 0.0);
         std::vector<uint32_t> depth(links, 0u);
+        std::vector<uint32_t> parent_of(links, 0u);
+        std::vector<DQ> in_parent(links, kIdentity);  // joint frame of each link in its parent
         for (uint32_t i = 1; i < links; ++i) {
             uint32_t parent = i - 1u;
             if (i > 1u && random_unit(seed, e, 0u, 3000u + i) < 0.3)
@@ -392,15 +394,18 @@ void build_random_trees(ModelData& m, uint32_t E, uint64_t seed) {
             b.depth = depth[i];
             const D3 ax = axes[std::min(2u, (uint32_t)std::floor(3.0 * random_unit(seed, e, 0u, 3400u + i)))];
             b.axis = f4(ax.x, ax.y, ax.z);
+            parent_of[i] = parent;
             if (parent == 0u) {
-                b.parent_anchor = f4(0.0, 0.0, 1.2);
-                b.parent_rotation = qf(DQ{-std::sqrt(0.5), 0.0, 0.0, std::sqrt(0.5)});  // local +y points down
+                b.parent_anchor = f4(0.0, 0.0, 1.2);  // raised below once the hanging extent is known
+                in_parent[i] = DQ{-std::sqrt(0.5), 0.0, 0.0, std::sqrt(0.5)};  // local +y points down
+                b.parent_rotation = qf(in_parent[i]);
             } else {
                 b.parent_anchor = f4(0.0, 0.5 * len[parent], 0.0);
                 if (parent != i - 1u) {
                     const double tilt = random_unit(seed, e, 0u, 3450u + i) < 0.5 ? 0.8 : -0.8;
                     const D3 around = random_unit(seed, e, 0u, 3460u + i) < 0.5 ? D3{1, 0, 0} : D3{0, 0, 1};
-                    b.parent_rotation = qf(dq_axis(around, tilt));
+                    in_parent[i] = dq_axis(around, tilt);
+                    b.parent_rotation = qf(in_parent[i]);
                 }
             }
             b.child_anchor = f4(0.0, -0.5 * len[i], 0.0);
@@ -412,6 +417,43 @@ void build_random_trees(ModelData& m, uint32_t E, uint64_t seed) {
             m.shapes.push_back(make_shape(i, MR_SHAPE_CAPSULE, 0u, D3{0, 0, 0}, kIdentity,
                                           D3{radius, 0.5 * len[i], 0.0}, 0.01));
         }
+        // Capsule segments of the links at the zero pose, with the tree hung
+        // from 1.2 m; the root is then raised until the lowest capsule clears
+        // the ground by 5 cm, so that no world starts with links below it.
+        std::vector<D3> seg0(links, D3{0, 0, 0}), seg1(links, D3{0, 0, 0});
+        {
+            std::vector<DQ> rot(links, kIdentity);
+            std::vector<D3> com(links, D3{0, 0, 0});
+            double lowest = 1.0e30;
+            for (uint32_t i = 1; i < links; ++i) {
+                const uint32_t p = parent_of[i];
+                rot[i] = dq_mul(rot[p], in_parent[i]);
+                const D3 anchor = p == 0u ? D3{0.0, 0.0, 1.2} : D3{0.0, 0.5 * len[p], 0.0};
+                const D3 ja = dq_rot(rot[p], anchor);
+                const D3 half = dq_rot(rot[i], D3{0.0, 0.5 * len[i], 0.0});
+                com[i] = D3{com[p].x + ja.x + half.x, com[p].y + ja.y + half.y, com[p].z + ja.z + half.z};
+                seg0[i] = D3{com[i].x - half.x, com[i].y - half.y, com[i].z - half.z};
+                seg1[i] = D3{com[i].x + half.x, com[i].y + half.y, com[i].z + half.z};
+                lowest = std::min(lowest, std::min(seg0[i].z, seg1[i].z) - radius);
+            }
+            const double lift = links > 1u ? std::max(0.0, 0.05 - lowest) : 0.0;
+            for (uint32_t i = 1; i < links; ++i) {
+                seg0[i].z += lift;
+                seg1[i].z += lift;
+                if (parent_of[i] == 0u) m.bodies[h.body_offset + i].parent_anchor = f4(0.0, 0.0, 1.2 + lift);
+            }
+        }
+        auto clear_of_links = [&](D3 x, double reach) {
+            for (uint32_t i = 1; i < links; ++i) {
+                const D3 d{seg1[i].x - seg0[i].x, seg1[i].y - seg0[i].y, seg1[i].z - seg0[i].z};
+                const D3 w{x.x - seg0[i].x, x.y - seg0[i].y, x.z - seg0[i].z};
+                const double dd = d.x * d.x + d.y * d.y + d.z * d.z;
+                const double t = dd > 0.0 ? std::clamp((w.x * d.x + w.y * d.y + w.z * d.z) / dd, 0.0, 1.0) : 0.0;
+                const D3 r{w.x - t * d.x, w.y - t * d.y, w.z - t * d.z};
+                if (std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z) < reach + radius) return false;
+            }
+            return true;
+        };
         std::vector<std::array<double, 4>> placed;
         for (uint32_t k = 0; k < free_count; ++k) {
             const double size = 0.05 + 0.15 * random_unit(seed, e, 0u, 3500u + k);
@@ -425,10 +467,10 @@ void build_random_trees(ModelData& m, uint32_t E, uint64_t seed) {
             m.bodies.push_back(b);
             D3 pos{0, 0, 0};
             for (uint32_t attempt = 0; attempt < 10000u; ++attempt) {
-                pos.x = -1.2 + 2.4 * random_unit(seed, e, attempt + 1u, 3700u + k);
-                pos.y = -1.2 + 2.4 * random_unit(seed, e, attempt + 1u, 3800u + k);
+                pos.x = -1.5 + 3.0 * random_unit(seed, e, attempt + 1u, 3700u + k);
+                pos.y = -1.5 + 3.0 * random_unit(seed, e, attempt + 1u, 3800u + k);
                 pos.z = 0.3 + 1.2 * random_unit(seed, e, attempt + 1u, 3900u + k);
-                bool ok = std::sqrt(pos.x * pos.x + pos.y * pos.y) > br + 0.45;
+                bool ok = clear_of_links(pos, br + 0.02);
                 for (const auto& p : placed) {
                     const double dx = pos.x - p[0], dy = pos.y - p[1], dz = pos.z - p[2];
                     if (std::sqrt(dx * dx + dy * dy + dz * dz) < br + p[3] + 0.01) ok = false;
