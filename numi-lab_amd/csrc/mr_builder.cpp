@@ -332,6 +332,37 @@ void build_chain_boxes(ModelData& m, uint32_t E, uint64_t seed) {
 // The links of one tree do not collide with each other (siblings meet at a
 // shared joint, where their relative contact Jacobian vanishes); they collide
 // with the free bodies and the plane.
+// Depth-first numbering of a tree whose parents precede their children
+// (parent[i] < i for i >= 1). Each node's children are visited in increasing
+// order of the outward-pass slots their subtrees need, the most demanding
+// last: the response stage keeps a link's acceleration in a slot until its
+// last (highest-numbered) child has read it, so with this numbering a parent's
+// slot is free again during its largest subtree and the tree needs about as
+// few slots as its branching allows. Returns order[new index] = old index.
+std::vector<uint32_t> depth_first_order(const std::vector<uint32_t>& parent) {
+    const uint32_t n = (uint32_t)parent.size();
+    std::vector<std::vector<uint32_t>> kids(n);
+    for (uint32_t i = 1; i < n; ++i) kids[parent[i]].push_back(i);
+    std::vector<uint32_t> need(n, 0u);
+    for (uint32_t v = n; v-- > 0u;) {
+        std::vector<uint32_t>& ch = kids[v];
+        std::stable_sort(ch.begin(), ch.end(), [&](uint32_t l, uint32_t r) { return need[l] < need[r]; });
+        if (ch.empty()) continue;
+        uint32_t peak = 1u;
+        for (size_t j = 0; j < ch.size(); ++j)
+            peak = std::max(peak, need[ch[j]] + (j + 1u < ch.size() ? 1u : 0u));
+        need[v] = peak;
+    }
+    std::vector<uint32_t> order, stack{0u};
+    while (!stack.empty()) {
+        const uint32_t v = stack.back();
+        stack.pop_back();
+        order.push_back(v);
+        for (size_t j = kids[v].size(); j-- > 0u;) stack.push_back(kids[v][j]);
+    }
+    return order;
+}
+
 void build_random_trees(ModelData& m, uint32_t E, uint64_t seed) {
     const double radius = 0.025;
     std::vector<uint32_t> dofs(E), frees(E);
@@ -375,13 +406,26 @@ void build_random_trees(ModelData& m, uint32_t E, uint64_t seed) {
         std::vector<uint32_t> depth(links, 0u);
         std::vector<uint32_t> parent_of(links, 0u);
         std::vector<DQ> in_parent(links, kIdentity);  // joint frame of each link in its parent
+        // The tree is drawn with generation indices o (link o hangs from o - 1, or with
+        // probability 0.3 branches from a random earlier link) and every per-link draw is
+        // keyed by o; the links are then numbered depth-first (depth_first_order).
+        std::vector<uint32_t> drawn_parent(links, 0u);
+        for (uint32_t o = 1; o < links; ++o) {
+            drawn_parent[o] = o - 1u;
+            if (o > 1u && random_unit(seed, e, 0u, 3000u + o) < 0.3)
+                drawn_parent[o] =
+                    (uint32_t)std::min((double)(o - 1u), std::floor(o * random_unit(seed, e, 0u, 3100u + o)));
+        }
+        const std::vector<uint32_t> order = depth_first_order(drawn_parent);
+        std::vector<uint32_t> number(links, 0u);
+        for (uint32_t i = 0; i < links; ++i) number[order[i]] = i;
         for (uint32_t i = 1; i < links; ++i) {
-            uint32_t parent = i - 1u;
-            if (i > 1u && random_unit(seed, e, 0u, 3000u + i) < 0.3)
-                parent = (uint32_t)std::min((double)(i - 1u), std::floor(i * random_unit(seed, e, 0u, 3100u + i)));
+            const uint32_t o = order[i];
+            const uint32_t parent = number[drawn_parent[o]];
+            const bool branch = drawn_parent[o] != o - 1u;
             const double mass = std::exp(std::log(0.1) + (std::log(10.0) - std::log(0.1)) *
-                                         random_unit(seed, e, 0u, 3200u + i));
-            len[i] = 0.06 + 0.14 * random_unit(seed, e, 0u, 3300u + i);
+                                         random_unit(seed, e, 0u, 3200u + o));
+            len[i] = 0.06 + 0.14 * random_unit(seed, e, 0u, 3300u + o);
             const double w = 2.0 * radius;
             MRBodyC b = make_body(mass, mass * (len[i] * len[i] + w * w) / 12.0, mass * (w * w + w * w) / 12.0,
                                   mass * (len[i] * len[i] + w * w) / 12.0, MR_MOTION_DYNAMIC);
@@ -392,7 +436,7 @@ void build_random_trees(ModelData& m, uint32_t E, uint64_t seed) {
             b.v_index = i - 1u;
             depth[i] = depth[parent] + 1u;
             b.depth = depth[i];
-            const D3 ax = axes[std::min(2u, (uint32_t)std::floor(3.0 * random_unit(seed, e, 0u, 3400u + i)))];
+            const D3 ax = axes[std::min(2u, (uint32_t)std::floor(3.0 * random_unit(seed, e, 0u, 3400u + o)))];
             b.axis = f4(ax.x, ax.y, ax.z);
             parent_of[i] = parent;
             if (parent == 0u) {
@@ -401,9 +445,9 @@ void build_random_trees(ModelData& m, uint32_t E, uint64_t seed) {
                 b.parent_rotation = qf(in_parent[i]);
             } else {
                 b.parent_anchor = f4(0.0, 0.5 * len[parent], 0.0);
-                if (parent != i - 1u) {
-                    const double tilt = random_unit(seed, e, 0u, 3450u + i) < 0.5 ? 0.8 : -0.8;
-                    const D3 around = random_unit(seed, e, 0u, 3460u + i) < 0.5 ? D3{1, 0, 0} : D3{0, 0, 1};
+                if (branch) {
+                    const double tilt = random_unit(seed, e, 0u, 3450u + o) < 0.5 ? 0.8 : -0.8;
+                    const D3 around = random_unit(seed, e, 0u, 3460u + o) < 0.5 ? D3{1, 0, 0} : D3{0, 0, 1};
                     in_parent[i] = dq_axis(around, tilt);
                     b.parent_rotation = qf(in_parent[i]);
                 }

@@ -236,8 +236,12 @@ void World::compute_lds_layout() {
     L.npts = take(4u * cfg_.pair_capacity, 4u);
     L.moff = take(4u * cfg_.pair_capacity, 4u);
     L.okey = take(4u * cfg_.manifold_capacity, 4u);
-    L.hfitem = take(4u * kHfSlots, 4u);
-    L.hfq = take((uint32_t)sizeof(HfHit) * kHfSlots, 16u);
+    // the heightfield query table exists only where some model has a heightfield
+    bool heightfield = false;
+    for (const MRShapeC& sh : model_.shapes) heightfield = heightfield || sh.shape_type == MR_SHAPE_HEIGHTFIELD;
+    const uint32_t hf_slots = heightfield ? kHfSlots : 0u;
+    L.hfitem = take(4u * hf_slots, 4u);
+    L.hfq = take((uint32_t)sizeof(HfHit) * hf_slots, 16u);
     union_end = std::max(union_end, off);
     off = union_start;
     uint32_t slots = 1u, stack = 1u;
