@@ -43,7 +43,7 @@ def read_traffic_profile(path):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(kind, seed, envs, steps, threads):
+def cpu_baseline(kind, seed, envs, steps, threads, solver=0):
     """Times the oracle (FP32 CPU restatement of the same microstep) on a
     bounded sample of the same workload: `envs` worlds over `steps` control
     steps, one contiguous block of worlds per std::thread."""
@@ -55,6 +55,7 @@ def cpu_baseline(kind, seed, envs, steps, threads):
 
     model = mr.WorldModel.synthetic(kind, envs, seed)
     cfg = model.default_config()
+    cfg.solver = solver
     st = ol.OracleState(model, cfg)
     cmds = model.commands(0, steps)
     lib = ol.load()
@@ -84,6 +85,7 @@ def residual_leg(mr, model, commands, args):
     import torch
 
     cfg = model.default_config()
+    cfg.solver = SOLVERS[args.solver]
     cfg.residual_tolerance = args.residual_tolerance
     cfg.max_iterations = args.residual_max_iterations
     world = mr.World(model, cfg)
@@ -221,6 +223,7 @@ def heterogeneous_leg(mr, args):
     }
 
 
+SOLVERS = {"sweeps": 0, "jacobi": 1}  # MRSolverMode
 STATUS_NAMES = {1: "fixed_budget_complete", 2: "nonfinite_input", 3: "nonfinite_result", 4: "pair_capacity_overflow",
                 5: "contact_capacity_overflow", 6: "manifold_capacity_overflow", 7: "constraint_capacity_overflow",
                 8: "island_capacity_overflow", 9: "ccd_event_budget_exhausted", 10: "factorization_failed",
@@ -276,6 +279,8 @@ def main():
     ap.add_argument("--stage-profile", type=int, default=0, metavar="STEPS",
                     help="after the timed region, run STEPS more control steps with the per-stage shader-clock "
                          "counters on and report each stage's share of the cycles")
+    ap.add_argument("--solver", choices=list(SOLVERS), default="sweeps",
+                    help="cone-solve iteration: ordered block sweeps or the relaxed block Jacobi")
     ap.add_argument("--skip-server", action="store_true", help="skip the step-server leg")
     ap.add_argument("--serve-pool", type=int, default=16384)
     ap.add_argument("--serve-requests", type=int, default=96)
@@ -315,6 +320,7 @@ def main():
     kind = mr.MR_SYNTHETIC_HUMANOID_TERRAIN if args.config == "humanoid" else mr.MR_SYNTHETIC_CHAIN_BOXES
     model = mr.WorldModel.synthetic(kind, args.envs, args.seed)
     cfg = model.default_config()
+    cfg.solver = SOLVERS[args.solver]
     world = mr.World(model, cfg)
     world.set_step_timing(True)
     total = args.warmup + args.steps
@@ -369,6 +375,7 @@ def main():
             "worlds": args.envs, "control_steps": args.steps, "microsteps_per_step": int(cfg.physics_substeps),
             "control_dt_s": float(cfg.control_timestep),
             "cone_sweeps": f"{cfg.velocity_iterations}+{cfg.final_velocity_iterations}",
+            "solver": args.solver,
             "eligible_pairs_per_world": int(h["pair_count"]), "plain_path": True,
         },
         "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
@@ -420,7 +427,7 @@ def main():
         out["warm_start"] = warm_start_leg(mr, args)
     if not args.skip_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(kind, args.seed, args.cpu_envs, args.cpu_steps, threads)
+        out["cpu_baseline"] = cpu_baseline(kind, args.seed, args.cpu_envs, args.cpu_steps, threads, SOLVERS[args.solver])
     print(json.dumps(out), flush=True)
     del adv
 

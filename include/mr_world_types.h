@@ -354,6 +354,19 @@ typedef struct MRPairRecordC {
 
 /* Mirrors the MetalWorldStepConfig fields of the temporalCone path
  * (include/metalrobo/MetalWorld.hpp:380-452). */
+/* Iteration of the cone solve (MRWorldStepConfigC.solver). Both start from
+ * the projected warm start and use the same per-contact 3x3 block inverse.
+ *   ORDERED_SWEEPS: ordered block Gauss-Seidel, each contact update applied
+ *     before the next (mr_world_solve_contact_islands,
+ *     MetalWorldContact.metal:7856-8316);
+ *   RELAXED_JACOBI: the Wave32 production iteration (mrWorldWave32SolvePacket,
+ *     MetalWorldContact.metal:6142-7213): every contact proposes its update
+ *     from the velocity at the start of the iteration, damped by
+ *     diag / sum_j |W_nn(c, j)| clamped to [1/32, 1], and the velocity then
+ *     takes all deltas at once; residuals.x is the largest |delta| over all
+ *     iterations of the microstep. */
+enum MRSolverMode { MR_SOLVER_ORDERED_SWEEPS = 0u, MR_SOLVER_RELAXED_JACOBI = 1u };
+
 typedef struct MRWorldStepConfigC {
     float control_timestep;           /* seconds per control step */
     uint32_t physics_substeps;        /* microsteps per control step */
@@ -375,6 +388,8 @@ typedef struct MRWorldStepConfigC {
     float max_depenetration_velocity; /* clamp of the stabilization velocity */
     float residual_tolerance;         /* >0: stop sweeps once final |dlambda| <= tol */
     uint32_t max_iterations;          /* sweep cap when residual_tolerance > 0 */
+    uint32_t solver;                  /* MRSolverMode */
+    uint32_t reserved;
 } MRWorldStepConfigC;
 
 #ifdef __cplusplus
@@ -394,6 +409,6 @@ MR_STATIC_ASSERT(sizeof(MRManifoldPointC) == 64, "MRManifoldPointC size");
 MR_STATIC_ASSERT(sizeof(MRWorldStatusC) == 48, "MRWorldStatusC size");
 MR_STATIC_ASSERT(sizeof(MRContactStatusC) == 96, "MRContactStatusC size");
 MR_STATIC_ASSERT(sizeof(MRPairRecordC) == 16, "MRPairRecordC size");
-MR_STATIC_ASSERT(sizeof(MRWorldStepConfigC) == 80, "MRWorldStepConfigC size");
+MR_STATIC_ASSERT(sizeof(MRWorldStepConfigC) == 88, "MRWorldStepConfigC size");
 
 #endif /* MR_WORLD_TYPES_H */

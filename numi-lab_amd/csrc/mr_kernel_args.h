@@ -158,6 +158,8 @@ struct LdsLayout {
     uint32_t rstack; // [stack][64] floats: joint terms of the inward path
     uint32_t isl;    // [MR_MAX_BODIES + 1] u8 island parents + used flags (after responses)
     uint32_t resid;  // [2][constraint_capacity] floats: per-constraint residual terms (after responses)
+    uint32_t jrelax; // [constraint_capacity] floats: relaxed-Jacobi damping per contact (after responses)
+    uint32_t jdelta; // [constraint_capacity][3] floats: relaxed-Jacobi impulse deltas (after responses)
     // MLP instance of the kernel only (past the plain layout's end):
     uint32_t mlp;        // warm-start MLP tile scratch (kMlpTileLdsBytes)
     uint32_t mlp_lam;    // [constraint_capacity][3] floats: the plain warm start, kept for the fallback

@@ -200,6 +200,7 @@ struct Env {
     float *rslot, *rstack;
     uint8_t* isl;
     float* resid;
+    float *jrelax, *jdelta;  // relaxed-Jacobi damping per contact and impulse deltas
     // MLP instance only
     unsigned char* mlp_lds;  // warm-start MLP tile scratch
     float* mlp_lam;          // [ccap][3] plain warm start, kept for the fallback
@@ -331,6 +332,15 @@ MRD_INL uint32_t wave_max_u32(uint32_t v) {
     for (int d = 32; d >= 1; d >>= 1) {
         const uint32_t o = __shfl_xor(v, d, 64);
         v = v > o ? v : o;
+    }
+    return v;
+}
+
+MRD_INL uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t o = __shfl_xor(v, d, 64);
+        v = v < o ? v : o;
     }
     return v;
 }
@@ -1692,6 +1702,99 @@ MRD_INL uint32_t uniform_u32(uint32_t v) { return (uint32_t)__builtin_amdgcn_rea
 // Optional per-stage shader-clock counters: lane 0 adds each stage's elapsed
 // cycles straight into the environment's global counters (no register array
 // held across the kernel when profiling is off).
+// ------------------------------------------------ relaxed block Jacobi
+// (mrWorldWave32SolvePacket, MetalWorldContact.metal:6142-7213). Articulated
+// dot products are serial in DoF order and sums over contacts run in
+// constraint order; contacts of other islands add exact zeros, so these are
+// the island sums.
+
+// Velocity change at point xc of scene body b from a unit impulse along d at
+// point xj (the scene part of typedNormalCrossContactResponse, :6740-6784).
+MRD_INL V3 scene_cross_response(const MRBodyStateC& b, V3 xc, V3 xj, V3 d) {
+    const V3 dw = inv_inertia_mul(b, cross(sub(xj, f3(b.position)), d));
+    return add(mul(b.linear_velocity_inv_mass.w, d), cross(dw, sub(xc, f3(b.position))));
+}
+
+// W_nn(c, j): normal velocity at contact c per unit normal impulse of contact j.
+MRD_INL float jacobi_normal_cross(const Env& e, uint32_t c, uint32_t j, uint32_t mnv) {
+    const uint32_t pc = e.cpair[c], pj = e.cpair[j];
+    float w = 0.0f;
+    if (((pc >> 16) & mr::kHotArticulated) != 0u && ((pj >> 16) & mr::kHotArticulated) != 0u) {
+        const float* x = e.jrow + (size_t)c * 3u * mnv;
+        const float* y = e.resp + (size_t)j * 3u * mnv;
+        for (uint32_t k = 0; k < e.nv; ++k) w = w + x[k] * y[k];
+    }
+    const uint32_t cb[2] = {pc & 0xffu, (pc >> 8) & 0xffu}, jb[2] = {pj & 0xffu, (pj >> 8) & 0xffu};
+    for (int x = 0; x < 2; ++x) {
+        if (!is_dynamic_scene(e, cb[x])) continue;
+        for (int y = 0; y < 2; ++y) {
+            if (jb[y] != cb[x]) continue;
+            const HotConstraint& hc = e.hot[c];
+            const HotConstraint& hj = e.hot[j];
+            const float t = dot(hot_dir(hc, 0), scene_cross_response(e.scene[cb[x] - e.nlinks],
+                                                                     V3{hc.point[0], hc.point[1], hc.point[2]},
+                                                                     V3{hj.point[0], hj.point[1], hj.point[2]},
+                                                                     hot_dir(hj, 0)));
+            w = (x == y) ? w + t : w - t;
+        }
+    }
+    return w;
+}
+
+// All impulse deltas of e.jdelta at once: a DoF lane sums its responses in
+// constraint order and applies the sum when any response touches it; a scene
+// lane accumulates linear and angular impulse over its contacts and applies
+// them once (:6858-6990).
+MRD_INL void jacobi_apply(Env& e, uint32_t lane, uint32_t ncons, uint32_t mnv, bool dof_lane, float& vj) {
+    const float* jd = e.jdelta;
+    if (dof_lane) {
+        float vd = 0.0f;
+        bool part = false;
+        for (uint32_t c = 0; c < ncons; ++c) {
+            if (((e.cpair[c] >> 16) & mr::kHotArticulated) == 0u) continue;
+            const float* rb = e.resp + (size_t)c * 3u * mnv + lane;
+            const float r0 = rb[0], r1 = rb[mnv], r2 = rb[2u * mnv];
+            part = part || r0 != 0.0f || r1 != 0.0f || r2 != 0.0f;
+            vd = vd + ((r0 * jd[3u * c] + r1 * jd[3u * c + 1u]) + r2 * jd[3u * c + 2u]);
+        }
+        if (part) vj = vj + vd;
+    }
+    for (uint32_t s = lane; s < e.nscene; s += kLanes) {
+        const uint32_t b = e.nlinks + s;
+        if (!is_dynamic_scene(e, b)) continue;
+        MRBodyStateC st = e.scene[s];
+        V3 lin{0.0f, 0.0f, 0.0f}, ang{0.0f, 0.0f, 0.0f};
+        bool touched = false;
+        for (uint32_t c = 0; c < ncons; ++c) {
+            const uint32_t pr = e.cpair[c];
+            if ((pr & 0xffu) != b && ((pr >> 8) & 0xffu) != b) continue;
+            touched = true;
+            const HotConstraint& hk = e.hot[c];
+            const float* d = jd + 3u * c;
+            const V3 imp = add(add(mul(d[0], hot_dir(hk, 0)), mul(d[1], hot_dir(hk, 1))), mul(d[2], hot_dir(hk, 2)));
+            const V3 lever = sub(V3{hk.point[0], hk.point[1], hk.point[2]}, f3(st.position));
+            if (hk.body_a == b) {
+                lin = sub(lin, imp);
+                ang = add(ang, cross(lever, neg(imp)));
+            }
+            if (hk.body_b == b) {
+                lin = add(lin, imp);
+                ang = add(ang, cross(lever, imp));
+            }
+        }
+        if (!touched) continue;
+        const V3 dv = mul(st.linear_velocity_inv_mass.w, lin);
+        st.linear_velocity_inv_mass.x += dv.x;
+        st.linear_velocity_inv_mass.y += dv.y;
+        st.linear_velocity_inv_mass.z += dv.z;
+        const V3 dw = inv_inertia_mul(st, ang);
+        st.angular_velocity.x += dw.x;
+        st.angular_velocity.y += dw.y;
+        st.angular_velocity.z += dw.z;
+        e.scene[s] = st;
+    }
+}
+
 struct StageClock {
     unsigned long long* out;
     unsigned long long last;
@@ -1919,6 +2022,8 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
     e.rstack = lds_at<float>(lds_base, L.rstack);
     e.isl = lds_at<uint8_t>(lds_base, L.isl);
     e.resid = lds_at<float>(lds_base, L.resid);
+    e.jrelax = lds_at<float>(lds_base, L.jrelax);
+    e.jdelta = lds_at<float>(lds_base, L.jdelta);
     if constexpr (kMlp) {
         e.mlp_lds = lds_base + L.mlp;
         e.mlp_lam = lds_at<float>(lds_base, L.mlp_lam);
@@ -2636,6 +2741,92 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
         for (uint32_t attempt = 0;; ++attempt) {
             sweep_delta = 0.0f;
             iters = 0;
+            if (cfg.solver == MR_SOLVER_RELAXED_JACOBI) {
+                // ---------------------------------- relaxed block Jacobi
+                // Lanes own contacts for the proposals, DoFs and scene bodies for
+                // the velocity update (jacobi_apply).
+                // damping per contact: diag / sum_j |W_nn(c, j)| clamped to [1/32, 1] (:6722-6798)
+                for (uint32_t c = lane; c < ncons; c += kLanes) {
+                    float sum = 0.0f, diag = 0.0f;
+                    for (uint32_t j = 0; j < ncons; ++j) {
+                        const float w = fabs_(jacobi_normal_cross(e, c, j, mnv));
+                        sum = sum + w;
+                        if (j == c) diag = w + e.hot[c].reg0;
+                    }
+                    e.jrelax[c] = sum > fmaxd(diag, kMatrixFloor) ? clampd(diag / sum, 1.0f / 32.0f, 1.0f) : 1.0f;
+                }
+                // warm start: the projected warm impulses are the first deltas (:6402-6470)
+                for (uint32_t i = lane; i < 3u * ncons; i += kLanes) e.jdelta[i] = e.lam[i];
+                __syncthreads();
+                jacobi_apply(e, lane, ncons, mnv, dof_lane, vj);
+                __syncthreads();
+                clk.mark(mr::kStageWarmStart);
+                float largest = 0.0f;
+                for (uint32_t it = 0; it < max_iters && ncons > 0u; ++it) {
+                    if (dof_lane) e.vcand[lane] = vj;
+                    __syncthreads();
+                    float my_delta = 0.0f;
+                    uint32_t my_fail = MR_INVALID_INDEX;
+                    for (uint32_t c = lane; c < ncons; c += kLanes) {
+                        const SolveBlock sb = solve[(size_t)c * hot_words];
+                        const uint32_t kind = e.cpair[c] >> 16;
+                        const HotConstraint& hk = e.hot[c];
+                        float rel[3] = {0.0f, 0.0f, 0.0f};
+                        if ((kind & mr::kHotArticulated) != 0u) {
+                            const float* jb = e.jrow + (size_t)c * 3u * mnv;
+                            for (int r = 0; r < 3; ++r) {
+                                float a = 0.0f;
+                                for (uint32_t k = 0; k < nv; ++k) a = a + jb[(size_t)r * mnv + k] * e.vcand[k];
+                                rel[r] = a;
+                            }
+                        }
+                        if ((kind & (mr::kHotDynamicA | mr::kHotDynamicB)) != 0u) {
+                            V3 va, vb;
+                            scene_point_velocities(e, kind, hk.body_a, hk.body_b, V3{hk.point[0], hk.point[1], hk.point[2]},
+                                                   va, vb);
+                            for (int r = 0; r < 3; ++r) rel[r] = rel_row(kind, hot_dir(hk, r), va, vb, rel[r]);
+                        }
+                        const float lam[3] = {e.lam[3u * c], e.lam[3u * c + 1u], e.lam[3u * c + 2u]};
+                        const float tgt[3] = {sb.target0, 0.0f, 0.0f}, reg[3] = {sb.reg0, 0.0f, 0.0f};
+                        float rhs[3];
+                        for (int r = 0; r < 3; ++r) rhs[r] = (tgt[r] - rel[r]) - reg[r] * lam[r];
+                        float prop[3] = {lam[0], lam[1], lam[2]};
+                        for (int r = 0; r < 3; ++r)
+                            for (int q = 0; q < 3; ++q) prop[r] += sb.winv[3 * r + q] * rhs[q];
+                        project_cone(prop, sb.mu_u, sb.mu_v);
+                        const float relax = e.jrelax[c];
+                        float d[3], cand[3];
+                        for (int r = 0; r < 3; ++r) {
+                            d[r] = relax * (prop[r] - lam[r]);
+                            cand[r] = lam[r] + d[r];
+                        }
+                        if (!isfin(cand[0]) || !isfin(cand[1]) || !isfin(cand[2]) || !isfin(d[0]) || !isfin(d[1]) ||
+                            !isfin(d[2])) {
+                            my_fail = c < my_fail ? c : my_fail;
+                            continue;
+                        }
+                        for (int r = 0; r < 3; ++r) {
+                            e.lam[3u * c + r] = cand[r];
+                            e.jdelta[3u * c + r] = d[r];
+                        }
+                        my_delta = fmaxd(my_delta, fmaxd(fabs_(d[0]), fmaxd(fabs_(d[1]), fabs_(d[2]))));
+                    }
+                    const uint32_t first_fail = wave_min_u32(my_fail);
+                    if (first_fail != MR_INVALID_INDEX) {
+                        if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_NONFINITE_RESULT << 24) | (first_fail & 0xffffffu);
+                        solve_failed = true;
+                        break;
+                    }
+                    __syncthreads();
+                    jacobi_apply(e, lane, ncons, mnv, dof_lane, vj);
+                    __syncthreads();
+                    const float it_delta = wave_max(my_delta);
+                    largest = fmaxd(largest, it_delta);
+                    ++iters;
+                    if (adaptive && it_delta <= cfg.residual_tolerance) break;
+                }
+                sweep_delta = largest;
+            } else {
             // ---------------------------------------------------- warm start
             // DoF lanes accumulate the warm-start impulses in constraint order;
             // scene lanes apply theirs in the same order.
@@ -2748,6 +2939,7 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
                 ++iters;
                 if (adaptive && sweep_delta <= cfg.residual_tolerance) break;
                 __syncthreads();
+            }
             }
             __syncthreads();
             clk.mark(mr::kStageSweeps);

@@ -54,6 +54,8 @@ World::World(const ModelData& model, const MRWorldStepConfigC& config) : model_(
         throw std::runtime_error("velocity_iterations must be positive");
     if (cfg_.residual_tolerance > 0.0f && cfg_.max_iterations == 0u)
         throw std::runtime_error("max_iterations must be positive with a residual tolerance");
+    if (cfg_.solver != MR_SOLVER_ORDERED_SWEEPS && cfg_.solver != MR_SOLVER_RELAXED_JACOBI)
+        throw std::runtime_error("unknown solver mode");
     if (cfg_.pair_capacity == 0u || cfg_.pair_capacity > 4096u) throw std::runtime_error("pair_capacity in [1, 4096]");
     if (cfg_.manifold_capacity == 0u || cfg_.manifold_capacity > 4096u)
         throw std::runtime_error("manifold_capacity in [1, 4096]");
@@ -252,6 +254,9 @@ void World::compute_lds_layout() {
     off = union_start;
     L.isl = take(2u * (MR_MAX_BODIES + 1u), 4u);
     L.resid = take(8u * cfg_.constraint_capacity, 4u);
+    const bool jacobi = cfg_.solver == MR_SOLVER_RELAXED_JACOBI;
+    L.jrelax = take(jacobi ? 4u * cfg_.constraint_capacity : 0u, 4u);
+    L.jdelta = take(jacobi ? 12u * cfg_.constraint_capacity : 0u, 4u);
     off = std::max(union_end, off);
     L.total = (off + 15u) / 16u * 16u;
     if (L.total > kMaxLdsBytes)
@@ -414,6 +419,8 @@ void World::set_warm_start_mlp(const MRMlpWeightsC* weights) {
     if (!(weights->w1 && weights->b1 && weights->w2 && weights->b2 && weights->w3 && weights->b3 &&
           weights->in_shift && weights->in_scale && weights->out_scale))
         throw std::runtime_error("warm-start MLP: every weight pointer must be set");
+    if (cfg_.solver != MR_SOLVER_ORDERED_SWEEPS)
+        throw std::runtime_error("warm-start MLP: the learned warm start runs with the ordered sweeps solver");
     if (lds_mlp_.total == 0u)
         throw std::runtime_error("warm-start MLP: the world's LDS plus the MLP tile exceeds 160 KiB");
     if (cfg_.residual_tolerance <= 0.0f)

@@ -11,6 +11,8 @@ from .native import (  # noqa: F401
     MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN,
     MR_SYNTHETIC_CHAIN_BOXES,
     MR_SYNTHETIC_HUMANOID_TERRAIN,
+    MR_SOLVER_ORDERED_SWEEPS,
+    MR_SOLVER_RELAXED_JACOBI,
     MR_SYNTHETIC_RANDOM_TREES,
     PAIR_RECORD_DTYPE,
     STATUS_NAMES,

@@ -31,6 +31,8 @@ MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN: Final = 101
 
 MR_ACTUATION_EFFORT: Final = 0
 MR_ACTUATION_IMPLICIT_POSITION_DRIVE: Final = 1
+MR_SOLVER_ORDERED_SWEEPS: Final = 0
+MR_SOLVER_RELAXED_JACOBI: Final = 1
 
 STATUS_NAMES: Final = {
     0: "success", 1: "fixed_budget_complete", 2: "nonfinite_input", 3: "nonfinite_result",
@@ -79,6 +81,7 @@ class WorldStepConfigC(ct.Structure):
         ("manifold_merge_distance", ct.c_float), ("manifold_normal_cosine", ct.c_float),
         ("contact_slop", ct.c_float), ("max_depenetration_velocity", ct.c_float),
         ("residual_tolerance", ct.c_float), ("max_iterations", ct.c_uint32),
+        ("solver", ct.c_uint32), ("reserved", ct.c_uint32),
     ]
 
     def copy(self) -> "WorldStepConfigC":
@@ -208,7 +211,7 @@ _EXPECTED_SIZES: Final = {
 }
 for _dt, _size in _EXPECTED_SIZES.items():
     assert _dt.itemsize == _size, (_dt, _dt.itemsize, _size)
-assert ct.sizeof(WorldStepConfigC) == 80
+assert ct.sizeof(WorldStepConfigC) == 88
 assert ct.sizeof(WorldModelDescC) == 200
 assert ct.sizeof(TaskRolloutLayoutC) == 64
 assert ct.sizeof(WorldAdvanceC) == 40

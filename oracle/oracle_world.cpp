@@ -1616,6 +1616,14 @@ V3 scene_response(const MRBodyStateC& b, V3 x, V3 d) {
     return add(mul(b.linear_velocity_inv_mass.w, d), cross(dw, lever));
 }
 
+// Velocity change at point xc of body b from a unit impulse along d at point xj
+// (the scene part of typedNormalCrossContactResponse,
+// MetalWorldContact.metal:6740-6784).
+V3 scene_cross_response(const MRBodyStateC& b, V3 xc, V3 xj, V3 d) {
+    const V3 dw = inv_inertia_mul(b, cross(sub(xj, f3(b.position)), d));
+    return add(mul(b.linear_velocity_inv_mass.w, d), cross(dw, sub(xc, f3(b.position))));
+}
+
 V3 scene_point_velocity(const MRBodyStateC& b, V3 x) {
     return add(f3(b.linear_velocity_inv_mass), cross(f3(b.angular_velocity), sub(x, f3(b.position))));
 }
@@ -2065,16 +2073,158 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
         if (is_dynamic_scene(ctx, k.body_a)) apply_scene_impulse(wk.bodies[k.body_a], k.point_world, neg(imp));
         if (is_dynamic_scene(ctx, k.body_b)) apply_scene_impulse(wk.bodies[k.body_b], k.point_world, imp);
     };
-    for (uint32_t c = 0; c < ncons; ++c) {
-        Constraint& k = cons[c];
-        project_cone(k.lambda, k.mu_u, k.mu_v);
-        apply_delta(k, c, k.lambda);
-    }
     const uint32_t base_iters = ctx.cfg->velocity_iterations + (final_sub ? ctx.cfg->final_velocity_iterations : 0u);
     const bool adaptive = ctx.cfg->residual_tolerance > 0.0f;
     const uint32_t max_iters = adaptive ? ctx.cfg->max_iterations : base_iters;
     float sweep_delta = 0.0f;
     uint32_t iters = 0;
+    const bool jacobi = ctx.cfg->solver == MR_SOLVER_RELAXED_JACOBI;
+    if (jacobi) {
+        // Relaxed block Jacobi (mrWorldWave32SolvePacket, MetalWorldContact.metal:6142-7213).
+        // Articulated dot products here are serial in DoF order; sums over
+        // contacts run in constraint order (contacts of other islands add
+        // exact zeros, so these are the island sums).
+        auto is_art = [&](const Constraint& k) { return is_link(ctx, k.body_a) || is_link(ctx, k.body_b); };
+        auto serial_dot = [&](const float* x, const float* y) {
+            float acc = 0.0f;
+            for (uint32_t j = 0; j < nv; ++j) acc = acc + x[j] * y[j];
+            return acc;
+        };
+        // W_nn(c, j): normal velocity at contact c per unit normal impulse of contact j
+        auto normal_cross = [&](uint32_t c, uint32_t j) {
+            const Constraint& kc = cons[c];
+            const Constraint& kj = cons[j];
+            float w = (is_art(kc) && is_art(kj)) ? serial_dot(&jrow[(size_t)c * 3u * nv], &resp[(size_t)j * 3u * nv])
+                                                 : 0.0f;
+            const uint32_t cb[2] = {kc.body_a, kc.body_b}, jb[2] = {kj.body_a, kj.body_b};
+            for (int x = 0; x < 2; ++x) {
+                if (!is_dynamic_scene(ctx, cb[x])) continue;
+                for (int y = 0; y < 2; ++y) {
+                    if (jb[y] != cb[x]) continue;
+                    const float t = dot(kc.dir[0], scene_cross_response(wk.bodies[cb[x]], kc.point_world,
+                                                                        kj.point_world, kj.dir[0]));
+                    w = (x == y) ? w + t : w - t;
+                }
+            }
+            return w;
+        };
+        // relaxation = diag / sum_j |W_nn(c, j)| clamped to [1/32, 1] (:6722-6798)
+        std::vector<float> relax(ncons, 1.0f);
+        for (uint32_t c = 0; c < ncons; ++c) {
+            float sum = 0.0f, diag = 0.0f;
+            for (uint32_t j = 0; j < ncons; ++j) {
+                const float w = std::fabs(normal_cross(c, j));
+                sum = sum + w;
+                if (j == c) diag = w + cons[c].reg[0];
+            }
+            relax[c] = sum > fmaxd(diag, kMatrixFloor) ? clampd(diag / sum, 1.0f / 32.0f, 1.0f) : 1.0f;
+        }
+        // all deltas at once: DoF sums in constraint order, applied to a DoF that any
+        // response touches; scene bodies accumulate linear and angular impulse (:6858-6990)
+        std::vector<float> delta((size_t)ncons * 3u, 0.0f);
+        auto apply_all = [&]() {
+            for (uint32_t j = 0; j < nv; ++j) {
+                float vd = 0.0f;
+                bool part = false;
+                for (uint32_t c = 0; c < ncons; ++c) {
+                    if (!is_art(cons[c])) continue;
+                    const float r0 = resp[((size_t)c * 3u + 0u) * nv + j];
+                    const float r1 = resp[((size_t)c * 3u + 1u) * nv + j];
+                    const float r2 = resp[((size_t)c * 3u + 2u) * nv + j];
+                    part = part || r0 != 0.0f || r1 != 0.0f || r2 != 0.0f;
+                    vd = vd + ((r0 * delta[3u * c] + r1 * delta[3u * c + 1u]) + r2 * delta[3u * c + 2u]);
+                }
+                if (part) wk.vcand[j] = wk.vcand[j] + vd;
+            }
+            for (uint32_t b = nlinks; b < hd.body_count; ++b) {
+                if (!is_dynamic_scene(ctx, b)) continue;
+                MRBodyStateC& st = wk.bodies[b];
+                V3 lin{0.0f, 0.0f, 0.0f}, ang{0.0f, 0.0f, 0.0f};
+                bool touched = false;
+                for (uint32_t c = 0; c < ncons; ++c) {
+                    const Constraint& k = cons[c];
+                    if (k.body_a != b && k.body_b != b) continue;
+                    touched = true;
+                    const float* d = &delta[3u * c];
+                    const V3 imp = add(add(mul(d[0], k.dir[0]), mul(d[1], k.dir[1])), mul(d[2], k.dir[2]));
+                    const V3 lever = sub(k.point_world, f3(st.position));
+                    if (k.body_a == b) {
+                        lin = sub(lin, imp);
+                        ang = add(ang, cross(lever, neg(imp)));
+                    }
+                    if (k.body_b == b) {
+                        lin = add(lin, imp);
+                        ang = add(ang, cross(lever, imp));
+                    }
+                }
+                if (!touched) continue;
+                const V3 dv = mul(st.linear_velocity_inv_mass.w, lin);
+                st.linear_velocity_inv_mass.x += dv.x;
+                st.linear_velocity_inv_mass.y += dv.y;
+                st.linear_velocity_inv_mass.z += dv.z;
+                const V3 dw = inv_inertia_mul(st, ang);
+                st.angular_velocity.x += dw.x;
+                st.angular_velocity.y += dw.y;
+                st.angular_velocity.z += dw.z;
+            }
+        };
+        // warm start: project, then apply every warm impulse once (:6402-6470)
+        for (uint32_t c = 0; c < ncons; ++c) {
+            Constraint& k = cons[c];
+            project_cone(k.lambda, k.mu_u, k.mu_v);
+            for (int r = 0; r < 3; ++r) delta[3u * c + r] = k.lambda[r];
+        }
+        apply_all();
+        float largest = 0.0f;
+        for (uint32_t it = 0; it < max_iters; ++it) {
+            float it_delta = 0.0f;
+            for (uint32_t c = 0; c < ncons; ++c) {
+                Constraint& k = cons[c];
+                float rel[3] = {0.0f, 0.0f, 0.0f};
+                const bool sa_dyn = is_dynamic_scene(ctx, k.body_a), sb_dyn = is_dynamic_scene(ctx, k.body_b);
+                const V3 va = sa_dyn ? scene_point_velocity(wk.bodies[k.body_a], k.point_world) : V3{0.0f, 0.0f, 0.0f};
+                const V3 vb = sb_dyn ? scene_point_velocity(wk.bodies[k.body_b], k.point_world) : V3{0.0f, 0.0f, 0.0f};
+                for (int r = 0; r < 3; ++r) {
+                    float a = is_art(k) ? serial_dot(&jrow[((size_t)c * 3u + (size_t)r) * nv], wk.vcand) : 0.0f;
+                    if (sb_dyn) a = a + dot(k.dir[r], vb);
+                    if (sa_dyn) a = a - dot(k.dir[r], va);
+                    rel[r] = a;
+                }
+                float rhs[3];
+                for (int r = 0; r < 3; ++r) rhs[r] = (k.target[r] - rel[r]) - k.reg[r] * k.lambda[r];
+                float prop[3] = {k.lambda[0], k.lambda[1], k.lambda[2]};
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q) prop[r] += k.winv[r][q] * rhs[q];
+                project_cone(prop, k.mu_u, k.mu_v);
+                float d[3], cand[3];
+                for (int r = 0; r < 3; ++r) {
+                    d[r] = relax[c] * (prop[r] - k.lambda[r]);
+                    cand[r] = k.lambda[r] + d[r];
+                }
+                if (!std::isfinite(cand[0]) || !std::isfinite(cand[1]) || !std::isfinite(cand[2]) ||
+                    !std::isfinite(d[0]) || !std::isfinite(d[1]) || !std::isfinite(d[2])) {
+                    f.code = MR_STEP_NONFINITE_RESULT;
+                    f.index = c;
+                    return false;
+                }
+                for (int r = 0; r < 3; ++r) {
+                    k.lambda[r] = cand[r];
+                    delta[3u * c + r] = d[r];
+                }
+                it_delta = fmaxd(it_delta, fmaxd(std::fabs(d[0]), fmaxd(std::fabs(d[1]), std::fabs(d[2]))));
+            }
+            apply_all();
+            largest = fmaxd(largest, it_delta);
+            ++iters;
+            if (adaptive && it_delta <= ctx.cfg->residual_tolerance) break;
+        }
+        sweep_delta = largest;
+    } else {
+    for (uint32_t c = 0; c < ncons; ++c) {
+        Constraint& k = cons[c];
+        project_cone(k.lambda, k.mu_u, k.mu_v);
+        apply_delta(k, c, k.lambda);
+    }
     for (uint32_t it = 0; it < max_iters; ++it) {
         sweep_delta = 0.0f;
         for (uint32_t c = 0; c < ncons; ++c) {
@@ -2102,6 +2252,7 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
         }
         ++iters;
         if (adaptive && sweep_delta <= ctx.cfg->residual_tolerance) break;
+    }
     }
     if (capture_problem && g_capture.used != 0u) {
         float* o = g_capture.out;

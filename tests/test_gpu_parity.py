@@ -81,14 +81,19 @@ def compare_step(world, ost, k, o_status, o_cs, o_pairs, o_counts, log):
     return errs
 
 
+SOLVERS = [mr.native.MR_SOLVER_ORDERED_SWEEPS, mr.native.MR_SOLVER_RELAXED_JACOBI]
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
 @pytest.mark.parametrize("kind,envs,steps,seed", [
     (mr.MR_SYNTHETIC_CHAIN_BOXES, 16, 48, 3),
     (mr.MR_SYNTHETIC_HUMANOID_TERRAIN, 8, 24, 5),
     (mr.MR_SYNTHETIC_RANDOM_TREES, 12, 30, 9),
 ])
-def test_teacher_forced_parity(gpu, kind, envs, steps, seed):
+def test_teacher_forced_parity(gpu, kind, envs, steps, seed, solver):
     model = mr.WorldModel.synthetic(kind, envs, seed)
     cfg = model.default_config()
+    cfg.solver = solver
     cfg.capture_evidence = 1
     cfg.capture_substep = 0
     world = mr.World(model, cfg)
@@ -108,16 +113,18 @@ def test_teacher_forced_parity(gpu, kind, envs, steps, seed):
         total_cons += int(o_cs["constraints"].sum())
         assert np.all(o_status["code"] == 0), f"step {k}: oracle failures {o_status['code']}"
     worst = {n: max(d[n] for d in log) for n in log[0]}
-    print(f"kind={kind} envs={envs} steps={steps} pairs={total_pairs} constraints={total_cons} "
+    print(f"kind={kind} solver={solver} envs={envs} steps={steps} pairs={total_pairs} constraints={total_cons} "
           f"bit_exact={exact} worst={worst}", flush=True)
     assert total_pairs > 0
     assert total_cons > 0, "the run never reached a contact"
 
 
-def test_free_running_horizon_matches_oracle(gpu):
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_free_running_horizon_matches_oracle(gpu, solver):
     """Whole-horizon run in one advance: the GPU trajectory equals the oracle's."""
     model = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_CHAIN_BOXES, 8, 11)
     cfg = model.default_config()
+    cfg.solver = solver
     world = mr.World(model, cfg)
     ost = ol.OracleState(model, cfg)
     steps = 40
@@ -133,11 +140,13 @@ def test_free_running_horizon_matches_oracle(gpu):
     assert rel_err(v, ost.v) <= REL_TOL
 
 
-def test_adaptive_residual_mode_matches_oracle(gpu):
-    """Adaptive cone solve (sweeps until the final-sweep max |dlambda| reaches
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_adaptive_residual_mode_matches_oracle(gpu, solver):
+    """Adaptive cone solve (iterations until an iteration's max |dlambda| reaches
     the tolerance, capped): same iteration counts, residuals and trajectory."""
     model = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_CHAIN_BOXES, 8, 13)
     cfg = model.default_config()
+    cfg.solver = solver
     cfg.residual_tolerance = 1.0e-4
     cfg.max_iterations = 40
     world = mr.World(model, cfg)
