@@ -19,9 +19,10 @@ import oracle_lib as ol
 
 
 def horizon_errors(kind, envs, steps, seed, env, tolerance=0.0, max_iterations=0, exact_tolerance=1.0e-8,
-                   first=0):
+                   first=0, solver=mr.MR_SOLVER_ORDERED_SWEEPS):
     model = mr.WorldModel.synthetic(kind, envs, seed)
     cfg = model.default_config()
+    cfg.solver = solver
     if tolerance > 0.0:
         cfg.residual_tolerance = tolerance
         cfg.max_iterations = max_iterations
