@@ -77,11 +77,14 @@ def cpu_baseline(kind, seed, envs, steps, threads, solver=0):
                       f"{threads} std::threads)", "seconds": round(dt, 2)}
 
 
-def residual_leg(mr, model, commands, args):
+def residual_leg(mr, model, commands, args, reorder=False):
     """The same workload with the adaptive cone solve: sweeps continue until
     the final-sweep max |dlambda| of a microstep is at or below the tolerance
     (capped at --residual-max-iterations sweeps). Reports throughput, control
-    step latency and how many microsteps reached the tolerance."""
+    step latency and how many microsteps reached the tolerance. With `reorder`,
+    the worlds run as a work queue over the resident workgroups, re-sorted on
+    the device after every control step by the sweeps each world took, most
+    first (the results are the same as the plain launch's)."""
     import torch
 
     cfg = model.default_config()
@@ -89,6 +92,9 @@ def residual_leg(mr, model, commands, args):
     cfg.residual_tolerance = args.residual_tolerance
     cfg.max_iterations = args.residual_max_iterations
     world = mr.World(model, cfg)
+    if reorder:
+        world.set_schedule(np.arange(model.environment_count), world.resident_workgroups())
+        world.set_schedule_reorder(True)
     world.set_step_timing(True)
     warm = min(args.warmup, commands.shape[0])
     if warm > 0:
@@ -418,6 +424,7 @@ def main():
         del src, dst
     if args.residual_tolerance > 0.0 and args.residual_steps > 0:
         out["at_residual"] = residual_leg(mr, model, commands, args)
+        out["at_residual_reordered"] = residual_leg(mr, model, commands, args, reorder=True)
     if not args.skip_heterogeneous:
         out["heterogeneous"] = heterogeneous_leg(mr, args)
     if not args.skip_server:

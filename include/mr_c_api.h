@@ -135,6 +135,14 @@ int mr_world_wait(MRWorldHandle* world, MRWorldAdvanceC* out);
  * of `order` (distinct indices; a subset is allowed) in that order; unlisted
  * environments are not stepped and keep their previous status records. */
 int mr_world_set_schedule(MRWorldHandle* world, const uint32_t* order, size_t count, uint32_t grid);
+/* With a work-queue schedule over every environment (mr_world_set_schedule
+ * with all E indices): after each control step, a one-workgroup kernel
+ * re-sorts the queue by the cone sweeps each world took in that step, most
+ * first, for the next step (no counterpart in the reference; for adaptive
+ * residual solves, whose per-world cost varies from world to world). Results
+ * do not depend on the order. enabled = 0 restores the fixed order; a new
+ * mr_world_set_schedule also clears it. */
+int mr_world_set_schedule_reorder(MRWorldHandle* world, uint32_t enabled);
 /* Workgroups of the control-step kernel the GPU keeps resident for this world. */
 uint32_t mr_world_resident_workgroups(const MRWorldHandle* world);
 /* Graph-captured horizon: control_step_count steps on device commands

@@ -282,6 +282,13 @@ int mr_world_set_schedule(MRWorldHandle* world, const uint32_t* order, size_t co
     });
 }
 
+int mr_world_set_schedule_reorder(MRWorldHandle* world, uint32_t enabled) {
+    return translate([&]() {
+        require(world != nullptr, "null argument");
+        world->world->set_schedule_reorder(enabled != 0u);
+    });
+}
+
 uint32_t mr_world_resident_workgroups(const MRWorldHandle* world) {
     uint32_t out = 0u;
     (void)translate([&]() {

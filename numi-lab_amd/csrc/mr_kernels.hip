@@ -3236,6 +3236,49 @@ extern "C" __global__ void mr_queue_reset_kernel(uint32_t* head) {
     if (threadIdx.x == 0u && blockIdx.x == 0u) *head = 0u;
 }
 
+// Work-queue order for the next control step: the environments of `status`
+// (one control step's contact statuses) sorted by the cone sweeps that step
+// took (residuals.w), most first, by a counting sort over 1024 sweep bins in
+// one workgroup. Only the schedule depends on it: which workgroup takes which
+// world never changes a world's results. Environments with equal counts land
+// in any order (atomic placement).
+extern "C" __global__ void __launch_bounds__(1024)
+mr_queue_order_kernel(const MRContactStatusC* status, uint32_t environments, uint32_t* order) {
+    __shared__ uint32_t bins[1024];
+    __shared__ uint32_t wave_base[16];
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    bins[t] = 0u;
+    __syncthreads();
+    auto key_of = [&](uint32_t e) {
+        const float w = fminf(fmaxf(status[e].residuals.w, 0.0f), 1023.0f);  // NaN -> 0
+        return 1023u - (uint32_t)w;
+    };
+    for (uint32_t e = t; e < environments; e += 1024u) atomicAdd(&bins[key_of(e)], 1u);
+    __syncthreads();
+    // exclusive prefix over the bins: wave scans, then the wave totals
+    const uint32_t v = bins[t];
+    uint32_t x = v;
+#pragma unroll
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63u) wave_base[wave] = x;
+    __syncthreads();
+    if (t == 0u) {
+        uint32_t run = 0;
+        for (uint32_t w = 0; w < 16u; ++w) {
+            const uint32_t c = wave_base[w];
+            wave_base[w] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    bins[t] = (x - v) + wave_base[wave];
+    __syncthreads();
+    for (uint32_t e = t; e < environments; e += 1024u) order[atomicAdd(&bins[key_of(e)], 1u)] = e;
+}
+
 // The learned-warm-start instance: the MLP tile keeps its accumulators and
 // operands in registers next to the solver state, so it runs one wavefront
 // per SIMD with the full register file.
@@ -3321,6 +3364,12 @@ hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipSt
         hipLaunchKernelGGL(mr_control_step_mlp_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
     else
         hipLaunchKernelGGL(mr_control_step_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launch_queue_order(const MRContactStatusC* status, uint32_t environments, uint32_t* order,
+                              hipStream_t stream) {
+    hipLaunchKernelGGL(mr_queue_order_kernel, dim3(1), dim3(1024), 0, stream, status, environments, order);
     return hipGetLastError();
 }
 

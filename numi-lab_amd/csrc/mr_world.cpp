@@ -12,6 +12,8 @@ namespace mr {
 hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream, bool mlp,
                                uint32_t queue_grid);
 hipError_t control_step_occupancy(uint32_t lds_bytes, int* blocks_per_cu);
+hipError_t launch_queue_order(const MRContactStatusC* status, uint32_t environments, uint32_t* order,
+                              hipStream_t stream);
 hipError_t configure_control_step(uint32_t lds_bytes, uint32_t mlp_lds_bytes);
 hipError_t launch_advance_summary(const MRWorldStatusC* status, const MRContactStatusC* cstatus, uint64_t count,
                                   uint32_t environments, AdvanceSummaryDev* summary, hipStream_t stream);
@@ -464,6 +466,9 @@ void World::launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, 
         hip_check(launch_control_step(a, mlp_instance() ? lds_mlp_.total : lds_.total, stream_, mlp_instance(),
                                       mlp_instance() ? 0u : queue_grid_),
                   "control step launch");
+        if (reorder_ && queue_grid_ > 0u)
+            hip_check(launch_queue_order(a.cstatus, (uint32_t)E, static_cast<uint32_t*>(d_queue_.ptr), stream_),
+                      "queue order launch");
         if (events && step_timing_) hip_check(hipEventRecord(step_events_[s + 1u], stream_), "event");
     }
     if (events) hip_check(hipEventRecord(ev_end_, stream_), "event");
@@ -476,6 +481,7 @@ void World::launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, 
 
 void World::set_schedule(const uint32_t* order, uint32_t count, uint32_t grid) {
     hip_check(hipStreamSynchronize(stream_), "sync");
+    reorder_ = false;
     if (grid == 0u) {
         if (order != nullptr) throw std::runtime_error("set_schedule: an order needs the work-queue schedule (grid > 0)");
         queue_grid_ = 0u;
@@ -496,6 +502,14 @@ void World::set_schedule(const uint32_t* order, uint32_t count, uint32_t grid) {
     d_queue_head_.reserve(sizeof(uint32_t));
     queue_grid_ = grid;
     queue_count_ = count;
+    ++schedule_version_;
+}
+
+void World::set_schedule_reorder(bool enabled) {
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    if (enabled && (queue_grid_ == 0u || queue_count_ != env_))
+        throw std::runtime_error("set_schedule_reorder: needs a work-queue schedule over every environment");
+    reorder_ = enabled;
     ++schedule_version_;
 }
 
@@ -528,7 +542,8 @@ void World::enqueue_graph(const float* device_commands, uint32_t steps) {
     const KernelArgs now = base_args(0u);
     const bool reuse = graph_exec_ != nullptr && graph_commands_ == device_commands && graph_steps_ == steps &&
         std::memcmp(&now, &graph_args_, sizeof(KernelArgs)) == 0 && graph_status_ == d_status_.ptr &&
-        graph_cstatus_ == d_cstatus_.ptr && graph_summary_ == d_summary_.ptr;
+        graph_cstatus_ == d_cstatus_.ptr && graph_summary_ == d_summary_.ptr &&
+        graph_schedule_ == schedule_version_;  // the grid and the reordering are launch shape, not arguments
     if (!reuse) {
         drop_graph();
         hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
@@ -548,6 +563,7 @@ void World::enqueue_graph(const float* device_commands, uint32_t steps) {
         graph_status_ = d_status_.ptr;
         graph_cstatus_ = d_cstatus_.ptr;
         graph_summary_ = d_summary_.ptr;
+        graph_schedule_ = schedule_version_;
     }
     hip_check(hipEventRecord(ev_begin_, stream_), "event");
     hip_check(hipGraphLaunch(graph_exec_, stream_), "graph launch");

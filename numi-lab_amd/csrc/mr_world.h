@@ -87,6 +87,9 @@ public:
     // order; environments not listed are not stepped and their status records
     // are left as they were.
     void set_schedule(const uint32_t* order, uint32_t count, uint32_t grid);
+    // After every control step, re-sort the work queue by that step's cone sweeps, heaviest
+    // first (needs a work-queue schedule over every environment).
+    void set_schedule_reorder(bool enabled);
     uint32_t queue_grid() const { return queue_grid_; }
     // Workgroups of the control-step kernel the GPU holds at once for this world.
     uint32_t resident_workgroups() const;
@@ -109,6 +112,7 @@ private:
     bool mlp_enabled_ = false, mlp_capture_ = false;
     bool mlp_instance() const { return mlp_enabled_ || mlp_capture_; }
     uint32_t queue_grid_ = 0, queue_count_ = 0, schedule_version_ = 0;
+    bool reorder_ = false;  // re-sort the work queue by each step's cone sweeps, heaviest first
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
     const float* graph_commands_ = nullptr;
@@ -118,6 +122,7 @@ private:
     const void* graph_status_ = nullptr;
     const void* graph_cstatus_ = nullptr;
     const void* graph_summary_ = nullptr;
+    uint32_t graph_schedule_ = 0;  // schedule_version_ the graph was captured with
     void launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, bool events);
     void drop_graph();
     hipStream_t stream_ = nullptr;

@@ -303,6 +303,7 @@ _SIGNATURES: Final = {
     "mr_task_rollout_final_q": ([ct.c_void_p], ct.POINTER(ct.c_float)),
     "mr_task_rollout_copy_final_scene_states": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
     "mr_world_set_schedule": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_uint32], ct.c_int),
+    "mr_world_set_schedule_reorder": ([ct.c_void_p, ct.c_uint32], ct.c_int),
     "mr_world_resident_workgroups": ([ct.c_void_p], ct.c_uint32),
     "mr_world_enqueue_graph": ([ct.c_void_p, ct.c_void_p, ct.c_uint32], ct.c_int),
     "mr_world_set_warm_start_mlp": ([ct.c_void_p, ct.c_void_p], ct.c_int),
@@ -603,6 +604,11 @@ class World:
             return
         o = np.ascontiguousarray(order, dtype=np.uint32)
         self._b.check(self._b.lib.mr_world_set_schedule(self._h, _ptr(o), o.size, int(grid)))
+
+    def set_schedule_reorder(self, enabled: bool = True) -> None:
+        """Re-sort the work queue after every control step by that step's cone sweeps, heaviest
+        first (needs set_schedule over every environment)."""
+        self._b.check(self._b.lib.mr_world_set_schedule_reorder(self._h, 1 if enabled else 0))
 
     def resident_workgroups(self) -> int:
         n = int(self._b.lib.mr_world_resident_workgroups(self._h))

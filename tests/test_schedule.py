@@ -103,3 +103,46 @@ def test_autotuner_picks_a_schedule_that_reproduces_the_plain_path(gpu):
     world.enqueue_graph(cmds)
     world.wait()
     assert same(snapshot(world), plain_end)
+
+
+def test_sweep_reordered_queue_matches_the_per_world_grid(gpu):
+    """Adaptive residual solves with the queue re-sorted after every control step by the
+    sweeps each world took (mr_world_set_schedule_reorder), eager and graph-captured: the
+    per-world path's results bit for bit; the schedule must not leak into the physics."""
+    import torch
+
+    model, cfg = trees(envs=64, seed=6)
+    cfg.residual_tolerance = 1.0e-5
+    cfg.max_iterations = 48
+    cmds = model.commands(0, 10)
+    ref = mr.World(model, cfg)
+    ref.advance(cmds)
+    ref_snap = snapshot(ref)
+    eager = mr.World(model, cfg)
+    eager.set_schedule(np.arange(model.environment_count), grid=13)
+    eager.set_schedule_reorder(True)
+    eager.advance(cmds)
+    assert same(ref_snap, snapshot(eager))
+    np.testing.assert_array_equal(ref.contact_statuses()["residuals"], eager.contact_statuses()["residuals"])
+    sweeps = ref.contact_statuses()["residuals"][..., 3]
+    assert sweeps.max() > sweeps.min(), "every world took the same number of sweeps"
+    graphed = mr.World(model, cfg)
+    graphed.set_schedule(np.arange(model.environment_count), grid=13)
+    graphed.set_schedule_reorder(True)
+    dev = torch.from_numpy(cmds[:5]).cuda()
+    graphed.enqueue_graph(dev)
+    graphed.wait()
+    dev.copy_(torch.from_numpy(cmds[5:]))
+    graphed.enqueue_graph(dev)  # replay of the same capture
+    graphed.wait()
+    assert same(ref_snap, snapshot(graphed))
+
+
+def test_reorder_needs_a_full_work_queue(gpu):
+    model, cfg = trees(envs=16)
+    w = mr.World(model, cfg)
+    with pytest.raises(mr.MetalRoboError, match="every environment"):
+        w.set_schedule_reorder(True)
+    w.set_schedule(np.arange(8), grid=4)
+    with pytest.raises(mr.MetalRoboError, match="every environment"):
+        w.set_schedule_reorder(True)
