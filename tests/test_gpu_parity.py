@@ -163,3 +163,37 @@ def test_adaptive_residual_mode_matches_oracle(gpu, solver):
     np.testing.assert_array_equal(g_cs["residuals"], o_cs["residuals"])
     assert np.any(o_cs["solver_iterations"] > 6), "the tolerance never asked for more sweeps than the default"
     assert rel_err(q, ost.q) <= REL_TOL and rel_err(v, ost.v) <= REL_TOL
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_full_size_humanoid_batch_matches_oracle_on_sampled_worlds(gpu, solver):
+    """The benchmark's shape (4096 humanoid worlds, two rounds of resident workgroups):
+    16 worlds from both ends and the middle of the grid, stepped by the oracle alone,
+    end where the GPU's batch puts them, bit for bit (state, manifolds, statuses)."""
+    E = 4096
+    model = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_HUMANOID_TERRAIN, E, 2024)
+    cfg = model.default_config()
+    cfg.solver = solver
+    world = mr.World(model, cfg)
+    steps = 10
+    cmds = model.commands(0, steps)
+    world.advance(cmds)
+    q, v, s = world.state()
+    mh, mp, mc = world.manifolds()
+    g_status = world.statuses()[-1]
+    ost = ol.OracleState(model, cfg)
+    sample = list(range(0, 6)) + list(range(2045, 2050)) + list(range(E - 5, E))
+    for k in range(steps):
+        for e in sample:
+            o_status, *_ = ost.step(cmds[k], control_step=k, env_begin=e, env_end=e + 1)
+    for e in sample:
+        assert np.array_equal(q[e].view(np.uint32), ost.q[e].view(np.uint32)), e
+        assert np.array_equal(v[e].view(np.uint32), ost.v[e].view(np.uint32)), e
+        assert np.array_equal(s[e].view(np.uint8), ost.scene[e].view(np.uint8)), e
+        n = int(mc[e])
+        assert n == int(ost.mcount[e])
+        np.testing.assert_array_equal(mh[e, :n]["pair_and_count"], ost.mh[e, :n]["pair_and_count"])
+        for f in ("feature_and_life", "impulses", "local_anchor_a", "local_anchor_b"):
+            np.testing.assert_array_equal(mp[e, :n][f], ost.mp[e, :n][f], err_msg=f"env {e} {f}")
+        assert int(g_status["code"][e]) == 0
+    assert sum(int(mc[e]) for e in sample) > 0
