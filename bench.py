@@ -311,8 +311,8 @@ def main():
 
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world_size != 1:
-        print("bench.py measures one GPU: every GPU box here has one GPU and the worlds shard across GPUs with "
-              "no exchange, so --gpus must be 1.", file=sys.stderr)
+        print("bench.py measures one GPU only (--gpus 1): this build runs on a single GPU. Worlds are "
+              "independent, so a multi-GPU run would shard them with no exchange step.", file=sys.stderr)
         sys.exit(2)
 
     import torch
