@@ -190,3 +190,20 @@ def test_trained_warm_start_against_the_plain_one(gpu):
           f"({100.0 * (1.0 - learned / plain):+.1f} % saved), fallbacks {ev['mlp']['fallbacks_per_step']:.3f}")
     assert ev["healthy_worlds"] > 0
     assert learned <= 2.0 * plain + 1e-9
+
+
+def test_learned_warm_start_needs_the_ordered_sweeps(gpu):
+    """The learned warm start and its fallback are built around the ordered sweeps; a world
+    configured for the relaxed Jacobi iteration refuses it, and an unknown solver mode is
+    refused when the world is created."""
+    import metalrobo as mr
+
+    model, cfg = chain_world(envs=2, residual_tolerance=1.0e-5, max_iterations=16)
+    cfg.solver = mr.MR_SOLVER_RELAXED_JACOBI
+    w = mr.World(model, cfg)
+    with pytest.raises(mr.MetalRoboError, match="ordered sweeps"):
+        w.set_warm_start_mlp(zero_mlp())
+    w.close()
+    cfg.solver = 7
+    with pytest.raises(mr.MetalRoboError, match="unknown solver mode"):
+        mr.World(model, cfg)
