@@ -56,6 +56,15 @@ struct SolveBlock {
     float stage[3];
 };
 
+// The trailing 64 bytes of a HotConstraint: what the sweeps need for a
+// contact with a dynamic scene endpoint.
+struct GeoBlock {
+    uint32_t body_a, body_b;
+    uint32_t flags, kind;
+    float dir[9];
+    float point[3];
+};
+
 // Per-constraint data produced by emission and consumed by the Jacobian and
 // IR evaluation passes (global scratch).
 struct ColdConstraint {

@@ -39,6 +39,7 @@ using namespace mrd;
 using mr::ColdConstraint;
 using mr::HotConstraint;
 using mr::SolveBlock;
+using mr::GeoBlock;
 using mr::KernelArgs;
 
 namespace {
@@ -314,6 +315,9 @@ template <int N> MRD_INL void wave_fold_n(float (&x)[N]) {
 // True on every lane when `bad` holds on any active lane: a uniform branch
 // condition for values every lane computes alike.
 MRD_INL bool wave_any(bool bad) { return __builtin_amdgcn_ballot_w64(bad) != 0ull; }
+
+// A compile-time flag passed to a generic lambda.
+template <bool B> struct BoolTag { static constexpr bool value = B; };
 
 MRD_INL float wave_max(float v) {
 #pragma unroll
@@ -2626,39 +2630,31 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
                 r3[r] = rb[(size_t)r * mnv];
             }
         };
+        // Pass 1, every lane, articulated constraints only: the DoF-lane folds
+        // of J v (rows 0-2) and of the 3x3 block J M^-1 J^T, staged in the
+        // record. Constraints between scene bodies need no fold and are left
+        // to the constraint-lane pass below, which also adds every constraint's
+        // scene endpoint terms.
         float nj[3] = {0.0f, 0.0f, 0.0f}, nr[3] = {0.0f, 0.0f, 0.0f};
         if (ncons > 0u) load_rows(0u, nj, nr);
         for (uint32_t c = 0; c < ncons; ++c) {
             const float jr[3] = {nj[0], nj[1], nj[2]};
             const float rr[3] = {nr[0], nr[1], nr[2]};
             if (c + 1u < ncons) load_rows(c + 1u, nj, nr);
-            const uint32_t kind = uniform_u32(e.cpair[c] >> 16);
-            const bool art = (kind & mr::kHotArticulated) != 0u;
-            // articulated part: rows 0-2 J v, rows 3-11 the 3x3 block J M^-1 J^T
-            float f[12] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-            if (art) {
+            if ((uniform_u32(e.cpair[c] >> 16) & mr::kHotArticulated) == 0u) continue;
+            float f[12];
 #pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    f[r] = dof_lane ? jr[r] * vj : 0.0f;
+            for (int r = 0; r < 3; ++r) {
+                f[r] = dof_lane ? jr[r] * vj : 0.0f;
 #pragma unroll
-                    for (int q = 0; q < 3; ++q) f[3 + 3 * r + q] = dof_lane ? jr[r] * rr[q] : 0.0f;
-                }
-                wave_fold_n(f);
+                for (int q = 0; q < 3; ++q) f[3 + 3 * r + q] = dof_lane ? jr[r] * rr[q] : 0.0f;
             }
-            float rel[3] = {f[0], f[1], f[2]};
-            if ((kind & (mr::kHotDynamicA | mr::kHotDynamicB)) != 0u) {
-                const HotConstraint& hk = e.hot[c];
-                V3 va, vb;
-                scene_point_velocities(e, kind, uniform_u32(hk.body_a), uniform_u32(hk.body_b),
-                                       V3{hk.point[0], hk.point[1], hk.point[2]}, va, vb);
-#pragma unroll
-                for (int r = 0; r < 3; ++r) rel[r] = rel_row(kind, hot_dir(hk, r), va, vb, rel[r]);
-            }
+            wave_fold_n(f);
             if (lane == 0u) {
                 HotConstraint& hw = e.hot[c];
-                hw.stage[0] = rel[0];
-                hw.stage[1] = rel[1];
-                hw.stage[2] = rel[2];
+                hw.stage[0] = f[0];
+                hw.stage[1] = f[1];
+                hw.stage[2] = f[2];
 #pragma unroll
                 for (int k = 0; k < 9; ++k) hw.winv[k] = f[3 + k];
             }
@@ -2669,6 +2665,15 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
         for (uint32_t c = lane; c < ncons; c += kLanes) {
             HotConstraint hk = e.hot[c];
             const ColdConstraint ck = e.cold[c];
+            if ((hk.kind & mr::kHotArticulated) == 0u) {
+                for (int r = 0; r < 3; ++r) hk.stage[r] = 0.0f;
+                for (int k = 0; k < 9; ++k) hk.winv[k] = 0.0f;
+            }
+            if ((hk.kind & (mr::kHotDynamicA | mr::kHotDynamicB)) != 0u) {
+                V3 va, vb;
+                scene_point_velocities(e, hk.kind, hk.body_a, hk.body_b, V3{hk.point[0], hk.point[1], hk.point[2]}, va, vb);
+                for (int r = 0; r < 3; ++r) hk.stage[r] = rel_row(hk.kind, hot_dir(hk, r), va, vb, hk.stage[r]);
+            }
             const float rel[3] = {hk.stage[0], hk.stage[1], hk.stage[2]};
             const float slip = sqrt_rn(rel[1] * rel[1] + rel[2] * rel[2]);
             const bool stat = slip <= 1.0e-3f;
@@ -2859,87 +2864,113 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
             // ------------------------------------------ ordered block sweeps
             // Every lane runs the constraint loop; the block update is uniform,
             // DoF lanes keep the candidate velocity in a register, and the next
-            // constraint's Jacobian / response rows are prefetched.
-            SolveBlock snext{};
-            if (ncons > 0u) {
-                load_rows(0u, nj, nr);
-                snext = solve[0];
-            }
-            for (uint32_t it = 0; it < max_iters && ncons > 0u; ++it) {
-                sweep_delta = 0.0f;
-                for (uint32_t c = 0; c < ncons; ++c) {
-                    const float jr[3] = {nj[0], nj[1], nj[2]};
-                    const float rr[3] = {nr[0], nr[1], nr[2]};
-                    const SolveBlock sb = snext;
-                    const uint32_t cn = c + 1u < ncons ? c + 1u : 0u;
-                    load_rows(cn, nj, nr);
-                    snext = solve[(size_t)cn * hot_words];
-                    const float lam0 = e.lam[3u * c], lam1 = e.lam[3u * c + 1u], lam2 = e.lam[3u * c + 2u];
-                    const uint32_t kind = uniform_u32(e.cpair[c] >> 16);
-                    const bool art = (kind & mr::kHotArticulated) != 0u;
-                    const bool sa_dyn = (kind & mr::kHotDynamicA) != 0u, sb_dyn = (kind & mr::kHotDynamicB) != 0u;
-                    float rel[3] = {0.0f, 0.0f, 0.0f};
-                    if (art) {
-    #pragma unroll
-                        for (int r = 0; r < 3; ++r) rel[r] = dof_lane ? jr[r] * vj : 0.0f;
-                        wave_fold_n(rel);
-                    }
-                    V3 x{0.0f, 0.0f, 0.0f};
-                    uint32_t ba = 0u, bb = 0u;
-                    V3 dirs[3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
-                    if (sa_dyn || sb_dyn) {
-                        const HotConstraint& hk = e.hot[c];
-                        ba = uniform_u32(hk.body_a);
-                        bb = uniform_u32(hk.body_b);
-                        x = V3{hk.point[0], hk.point[1], hk.point[2]};
-                        V3 va, vb;
-                        scene_point_velocities(e, kind, ba, bb, x, va, vb);
-    #pragma unroll
-                        for (int r = 0; r < 3; ++r) {
-                            dirs[r] = hot_dir(hk, r);
-                            rel[r] = rel_row(kind, dirs[r], va, vb, rel[r]);
-                        }
-                    }
-                    const float lamc[3] = {lam0, lam1, lam2};
-                    // the tangent rows' stored target and regularisation are +0
-                    const float tgt[3] = {sb.target0, 0.0f, 0.0f}, reg[3] = {sb.reg0, 0.0f, 0.0f};
-                    float rhs[3];
-                    for (int r = 0; r < 3; ++r) rhs[r] = (tgt[r] - rel[r]) - reg[r] * lamc[r];
-                    float cand[3] = {lam0, lam1, lam2};
-                    for (int r = 0; r < 3; ++r)
-                        for (int q = 0; q < 3; ++q) cand[r] += sb.winv[3 * r + q] * rhs[q];
-                    project_cone(cand, sb.mu_u, sb.mu_v);
-                    const float d0 = cand[0] - lam0, d1 = cand[1] - lam1, d2 = cand[2] - lam2;
-                    if (wave_any(!isfin(cand[0]) | !isfin(cand[1]) | !isfin(cand[2]) | !isfin(d0) | !isfin(d1) | !isfin(d2))) {
-                        if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_NONFINITE_RESULT << 24) | (c & 0xffffffu);
-                        solve_failed = true;
-                        break;
-                    }
-                    sweep_delta = fmaxd(sweep_delta, fmaxd(fabs_(d0), fmaxd(fabs_(d1), fabs_(d2))));
-                    if (art && dof_lane) vj += (rr[0] * d0 + rr[1] * d1) + rr[2] * d2;
-                    // The workgroup is one wavefront and its LDS operations
-                    // complete in issue order, so lane 0's writes below follow
-                    // every lane's reads above and precede the next constraint's
-                    // reads; the wave barrier only stops the compiler from
-                    // moving memory operations across these points.
-                    __builtin_amdgcn_wave_barrier();
-                    if (lane == 0u) {
-                        if (sa_dyn || sb_dyn) {
-                            const V3 imp = add(add(mul(d0, dirs[0]), mul(d1, dirs[1])), mul(d2, dirs[2]));
-                            if (sa_dyn) apply_scene_impulse(e.scene[ba - nl], x, neg(imp));
-                            if (sb_dyn) apply_scene_impulse(e.scene[bb - nl], x, imp);
-                        }
-                        e.lam[3u * c] = cand[0];
-                        e.lam[3u * c + 1u] = cand[1];
-                        e.lam[3u * c + 2u] = cand[2];
-                    }
-                    __builtin_amdgcn_wave_barrier();
+            // constraint's Jacobian / response rows and solve block are
+            // prefetched. The loop is compiled twice: a substep whose contacts
+            // all lack a dynamic scene endpoint (an articulation on static
+            // geometry) runs it without any scene work; the others also prefetch
+            // the next contact's geometric half (bodies, directions, point),
+            // spread over lanes 0-15 of one register, word k in lane k, and read
+            // back as scalars when used.
+            bool scene_lane = false;
+            for (uint32_t c = lane; c < ncons; c += kLanes)
+                scene_lane |= ((e.cpair[c] >> 16) & (mr::kHotDynamicA | mr::kHotDynamicB)) != 0u;
+            const bool any_scene = wave_any(scene_lane);
+            auto ordered_sweeps = [&](auto scene_tag) {
+                constexpr bool kScene = decltype(scene_tag)::value;
+                const uint32_t* geo_words = reinterpret_cast<const uint32_t*>(e.hot) + sizeof(SolveBlock) / 4u;
+                const uint32_t hot_u32 = (uint32_t)(sizeof(HotConstraint) / 4u);
+                auto load_geo = [&](uint32_t c) -> uint32_t {
+                    return lane < sizeof(GeoBlock) / 4u ? geo_words[(size_t)c * hot_u32 + lane] : 0u;
+                };
+                SolveBlock snext{};
+                uint32_t gnext = 0u;
+                if (ncons > 0u) {
+                    load_rows(0u, nj, nr);
+                    snext = solve[0];
+                    if constexpr (kScene) gnext = load_geo(0u);
                 }
-                if (solve_failed) break;
-                ++iters;
-                if (adaptive && sweep_delta <= cfg.residual_tolerance) break;
-                __syncthreads();
-            }
+                for (uint32_t it = 0; it < max_iters && ncons > 0u; ++it) {
+                    sweep_delta = 0.0f;
+                    for (uint32_t c = 0; c < ncons; ++c) {
+                        const float jr[3] = {nj[0], nj[1], nj[2]};
+                        const float rr[3] = {nr[0], nr[1], nr[2]};
+                        const SolveBlock sb = snext;
+                        [[maybe_unused]] const uint32_t gb = gnext;
+                        const uint32_t cn = c + 1u < ncons ? c + 1u : 0u;
+                        load_rows(cn, nj, nr);
+                        snext = solve[(size_t)cn * hot_words];
+                        if constexpr (kScene) gnext = load_geo(cn);
+                        const float lam0 = e.lam[3u * c], lam1 = e.lam[3u * c + 1u], lam2 = e.lam[3u * c + 2u];
+                        const uint32_t kind = uniform_u32(e.cpair[c] >> 16);
+                        const bool art = (kind & mr::kHotArticulated) != 0u;
+                        const bool sa_dyn = (kind & mr::kHotDynamicA) != 0u, sb_dyn = (kind & mr::kHotDynamicB) != 0u;
+                        float rel[3] = {0.0f, 0.0f, 0.0f};
+                        if (art) {
+        #pragma unroll
+                            for (int r = 0; r < 3; ++r) rel[r] = dof_lane ? jr[r] * vj : 0.0f;
+                            wave_fold_n(rel);
+                        }
+                        V3 x{0.0f, 0.0f, 0.0f};
+                        uint32_t ba = 0u, bb = 0u;
+                        V3 dirs[3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+                        if (kScene && (sa_dyn || sb_dyn)) {
+                            auto word = [&](uint32_t k) { return (uint32_t)__builtin_amdgcn_readlane((int)gb, (int)k); };
+                            auto wordf = [&](uint32_t k) { return __uint_as_float(word(k)); };
+                            constexpr uint32_t kDir = offsetof(GeoBlock, dir) / 4u, kPoint = offsetof(GeoBlock, point) / 4u;
+                            ba = word(offsetof(GeoBlock, body_a) / 4u);
+                            bb = word(offsetof(GeoBlock, body_b) / 4u);
+                            x = V3{wordf(kPoint), wordf(kPoint + 1u), wordf(kPoint + 2u)};
+                            V3 va, vb;
+                            scene_point_velocities(e, kind, ba, bb, x, va, vb);
+        #pragma unroll
+                            for (int r = 0; r < 3; ++r) {
+                                dirs[r] = V3{wordf(kDir + 3u * r), wordf(kDir + 3u * r + 1u), wordf(kDir + 3u * r + 2u)};
+                                rel[r] = rel_row(kind, dirs[r], va, vb, rel[r]);
+                            }
+                        }
+                        const float lamc[3] = {lam0, lam1, lam2};
+                        // the tangent rows' stored target and regularisation are +0
+                        const float tgt[3] = {sb.target0, 0.0f, 0.0f}, reg[3] = {sb.reg0, 0.0f, 0.0f};
+                        float rhs[3];
+                        for (int r = 0; r < 3; ++r) rhs[r] = (tgt[r] - rel[r]) - reg[r] * lamc[r];
+                        float cand[3] = {lam0, lam1, lam2};
+                        for (int r = 0; r < 3; ++r)
+                            for (int q = 0; q < 3; ++q) cand[r] += sb.winv[3 * r + q] * rhs[q];
+                        project_cone(cand, sb.mu_u, sb.mu_v);
+                        const float d0 = cand[0] - lam0, d1 = cand[1] - lam1, d2 = cand[2] - lam2;
+                        if (wave_any(!isfin(cand[0]) | !isfin(cand[1]) | !isfin(cand[2]) | !isfin(d0) | !isfin(d1) | !isfin(d2))) {
+                            if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_NONFINITE_RESULT << 24) | (c & 0xffffffu);
+                            solve_failed = true;
+                            break;
+                        }
+                        sweep_delta = fmaxd(sweep_delta, fmaxd(fabs_(d0), fmaxd(fabs_(d1), fabs_(d2))));
+                        if (art && dof_lane) vj += (rr[0] * d0 + rr[1] * d1) + rr[2] * d2;
+                        // The workgroup is one wavefront and its LDS operations
+                        // complete in issue order, so lane 0's writes below follow
+                        // every lane's reads above and precede the next constraint's
+                        // reads; the wave barrier only stops the compiler from
+                        // moving memory operations across these points.
+                        __builtin_amdgcn_wave_barrier();
+                        if (lane == 0u) {
+                            if (kScene && (sa_dyn || sb_dyn)) {
+                                const V3 imp = add(add(mul(d0, dirs[0]), mul(d1, dirs[1])), mul(d2, dirs[2]));
+                                if (sa_dyn) apply_scene_impulse(e.scene[ba - nl], x, neg(imp));
+                                if (sb_dyn) apply_scene_impulse(e.scene[bb - nl], x, imp);
+                            }
+                            e.lam[3u * c] = cand[0];
+                            e.lam[3u * c + 1u] = cand[1];
+                            e.lam[3u * c + 2u] = cand[2];
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                    if (solve_failed) break;
+                    ++iters;
+                    if (adaptive && sweep_delta <= cfg.residual_tolerance) break;
+                    __syncthreads();
+                }
+            };
+            if (any_scene) ordered_sweeps(BoolTag<true>{});
+            else ordered_sweeps(BoolTag<false>{});
             }
             __syncthreads();
             clk.mark(mr::kStageSweeps);
@@ -2972,32 +3003,31 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
         }
         if (s_fail != ~0ull) break;
         // ------------------------------------- residuals + impulse store
-        // Pass 1, DoF lanes: the normal-row relative velocity of every
-        // constraint (articulation fold plus dynamic scene endpoints).
-        float* res_n = e.resid;                 // [ccap]: rel0, then the normal residual
+        // Pass 1, every lane, articulated constraints only: the DoF-lane fold
+        // of the normal row's J v.
+        float* res_n = e.resid;                 // [ccap]: articulated J v, then the normal residual
         float* res_c = e.resid + ccap;          // [ccap]: cone violation
         float nj0 = ncons > 0u ? e.jrow[row_lane] : 0.0f;
         for (uint32_t c = 0; c < ncons; ++c) {
             const float j0 = nj0;
             if (c + 1u < ncons) nj0 = e.jrow[(size_t)(c + 1u) * 3u * mnv + row_lane];
-            const uint32_t kind = uniform_u32(e.cpair[c] >> 16);
-            const float a0 = (kind & mr::kHotArticulated) != 0u ? wave_fold(dof_lane ? j0 * vj : 0.0f) : 0.0f;
-            float rel0 = a0;
-            if ((kind & (mr::kHotDynamicA | mr::kHotDynamicB)) != 0u) {
-                const HotConstraint& hk = e.hot[c];
-                V3 va, vb;
-                scene_point_velocities(e, kind, uniform_u32(hk.body_a), uniform_u32(hk.body_b),
-                                       V3{hk.point[0], hk.point[1], hk.point[2]}, va, vb);
-                rel0 = rel_row(kind, hot_dir(hk, 0), va, vb, a0);
-            }
-            if (lane == 0u) res_n[c] = rel0;
+            if ((uniform_u32(e.cpair[c] >> 16) & mr::kHotArticulated) == 0u) continue;
+            const float a0 = wave_fold(dof_lane ? j0 * vj : 0.0f);
+            if (lane == 0u) res_n[c] = a0;
         }
         __syncthreads();
-        // Pass 2, constraint lanes: residual terms and the impulse store.
+        // Pass 2, constraint lanes: the dynamic scene endpoints' terms of the
+        // normal-row relative velocity, residual terms and the impulse store.
         for (uint32_t c = lane; c < ncons; c += kLanes) {
             const HotConstraint hk = e.hot[c];
+            float rel0 = (hk.kind & mr::kHotArticulated) != 0u ? res_n[c] : 0.0f;
+            if ((hk.kind & (mr::kHotDynamicA | mr::kHotDynamicB)) != 0u) {
+                V3 va, vb;
+                scene_point_velocities(e, hk.kind, hk.body_a, hk.body_b, V3{hk.point[0], hk.point[1], hk.point[2]}, va, vb);
+                rel0 = rel_row(hk.kind, hot_dir(hk, 0), va, vb, rel0);
+            }
             const float lamr[3] = {e.lam[3u * c], e.lam[3u * c + 1u], e.lam[3u * c + 2u]};
-            const float eq = (res_n[c] - hk.target0) + hk.reg0 * lamr[0];
+            const float eq = (rel0 - hk.target0) + hk.reg0 * lamr[0];
             const float nres = lamr[0] > kConeEpsilon ? fabs_(eq) : fmaxd(-eq, 0.0f);
             const float lu = hk.mu_u * lamr[0], lv = hk.mu_v * lamr[0];
             float viol;
@@ -3352,6 +3382,7 @@ static_assert(sizeof(SpI) == 84, "SpI layout");
 static_assert(sizeof(Collider) == 124, "Collider layout");
 static_assert(sizeof(HotConstraint) == 128, "HotConstraint layout");
 static_assert(sizeof(SolveBlock) == 64 && sizeof(HotConstraint) % sizeof(SolveBlock) == 0, "SolveBlock layout");
+static_assert(sizeof(GeoBlock) == 64 && offsetof(HotConstraint, body_a) == sizeof(SolveBlock), "GeoBlock layout");
 
 hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream, bool mlp,
                                uint32_t queue_grid) {
