@@ -2864,13 +2864,12 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
             // ------------------------------------------ ordered block sweeps
             // Every lane runs the constraint loop; the block update is uniform,
             // DoF lanes keep the candidate velocity in a register, and the next
-            // constraint's Jacobian / response rows and solve block are
-            // prefetched. The loop is compiled twice: a substep whose contacts
-            // all lack a dynamic scene endpoint (an articulation on static
-            // geometry) runs it without any scene work; the others also prefetch
-            // the next contact's geometric half (bodies, directions, point),
-            // spread over lanes 0-15 of one register, word k in lane k, and read
-            // back as scalars when used.
+            // constraint's Jacobian / response rows are prefetched. The loop is
+            // compiled twice: a substep whose contacts all lack a dynamic scene
+            // endpoint (an articulation on static geometry) runs it without any
+            // scene work; the others also prefetch the next contact's geometric
+            // half (bodies, directions, point), spread over lanes 0-15 of one
+            // register, word k in lane k, and read back as scalars when used.
             bool scene_lane = false;
             for (uint32_t c = lane; c < ncons; c += kLanes)
                 scene_lane |= ((e.cpair[c] >> 16) & (mr::kHotDynamicA | mr::kHotDynamicB)) != 0u;
@@ -2882,11 +2881,9 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
                 auto load_geo = [&](uint32_t c) -> uint32_t {
                     return lane < sizeof(GeoBlock) / 4u ? geo_words[(size_t)c * hot_u32 + lane] : 0u;
                 };
-                SolveBlock snext{};
                 uint32_t gnext = 0u;
                 if (ncons > 0u) {
                     load_rows(0u, nj, nr);
-                    snext = solve[0];
                     if constexpr (kScene) gnext = load_geo(0u);
                 }
                 for (uint32_t it = 0; it < max_iters && ncons > 0u; ++it) {
@@ -2894,11 +2891,12 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
                     for (uint32_t c = 0; c < ncons; ++c) {
                         const float jr[3] = {nj[0], nj[1], nj[2]};
                         const float rr[3] = {nr[0], nr[1], nr[2]};
-                        const SolveBlock sb = snext;
+                        // loaded here, not a contact ahead: the fold below covers
+                        // its latency, and a loop-carried copy costs 16 registers
+                        const SolveBlock sb = solve[(size_t)c * hot_words];
                         [[maybe_unused]] const uint32_t gb = gnext;
                         const uint32_t cn = c + 1u < ncons ? c + 1u : 0u;
                         load_rows(cn, nj, nr);
-                        snext = solve[(size_t)cn * hot_words];
                         if constexpr (kScene) gnext = load_geo(cn);
                         const float lam0 = e.lam[3u * c], lam1 = e.lam[3u * c + 1u], lam2 = e.lam[3u * c + 2u];
                         const uint32_t kind = uniform_u32(e.cpair[c] >> 16);
