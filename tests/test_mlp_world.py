@@ -192,6 +192,7 @@ def test_trained_warm_start_against_the_plain_one(gpu):
     assert learned <= 2.0 * plain + 1e-9
 
 
+@pytest.mark.gpu
 def test_learned_warm_start_needs_the_ordered_sweeps(gpu):
     """The learned warm start and its fallback are built around the ordered sweeps; a world
     configured for the relaxed Jacobi iteration refuses it, and an unknown solver mode is
