@@ -195,47 +195,183 @@ int mr_world_read_mlp_samples(const MRWorldHandle* world, float* out, size_t cou
 
 /* ------------------------------------------------- task rollout --- */
 
+/* The task-rollout records below are laid out byte for byte as the
+ * reference's (include/metalrobo/c_api.h:55-235), so a caller compiled
+ * against that header passes and receives the same bytes here; sizes and
+ * offsets are asserted at the end of this section. Fields that have no
+ * meaning for this build's physics-only synthetic sources must be zero (or
+ * null) and are rejected otherwise; output fields with no counterpart here
+ * are written as zero. */
+
 typedef struct MRTaskRolloutHandle MRTaskRolloutHandle;
 
-/* c_api.h:62-88 (subset) */
+/* c_api.h:55-61 */
+typedef struct MRTaskRolloutDynamicSphereC {
+    float position[3];
+    float linear_velocity[3];
+    float radius;
+    float mass;
+    uint32_t launch_step;
+} MRTaskRolloutDynamicSphereC;
+
+/* c_api.h:62-88. Read here: environment_count, physics_substeps,
+ * velocity_iterations, final_velocity_iterations, control_timestep_seconds
+ * (0 keeps each default), seed. materialize_articulated_contact_responses
+ * and disable_task_terminations are accepted with any value (this build
+ * always materializes the response columns and has no task terminations);
+ * every other field must be zero. */
 typedef struct MRTaskRolloutConfigC {
     uint32_t environment_count;
     uint32_t physics_substeps;
     uint32_t velocity_iterations;
     uint32_t final_velocity_iterations;
     float control_timestep_seconds;
-    uint32_t reserved;
     uint64_t seed;
+    const MRTaskRolloutDynamicSphereC* dynamic_spheres;
+    uint32_t dynamic_sphere_count;
+    uint32_t disable_task_terminations;
+    uint32_t interaction_reference_mode;
+    float interaction_student_authority;
+    uint32_t override_interaction_student_authority;
+    float interaction_reset_phase_fraction;
+    uint32_t override_interaction_reset_phase_fraction;
+    uint32_t materialize_articulated_contact_responses;
+    uint32_t override_difficulty_band_range;
+    uint32_t minimum_difficulty_band;
+    uint32_t maximum_difficulty_band;
+    float interaction_reset_phase_probability;
+    uint32_t override_interaction_reset_phase_probability;
+    float interaction_reset_maximum_phase;
+    uint32_t override_interaction_reset_maximum_phase;
 } MRTaskRolloutConfigC;
 
-enum MRRunManifestSourceC {
-    MR_RUN_SOURCE_SYNTHETIC_CHAIN_BOXES = 100u,
-    MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN = 101u
-};
+/* c_api.h:110-127; referenced by pointer only (must be null here). */
+typedef struct MRTaskVisualObservationConfigC MRTaskVisualObservationConfigC;
 
-/* c_api.h:171-190 (subset): a profile plus the world source. */
-typedef struct MRRunManifestC {
-    MRTaskRolloutConfigC profile;
-    uint32_t source;
-    uint32_t reserved;
-} MRRunManifestC;
-
-/* c_api.h:128-158 (subset) */
+/* c_api.h:128-158. Written here: environment_count, nq, nv, action_count,
+ * scene_body_count, world_fingerprint (FNV-1a of the compiled model
+ * tables), the submission counters, retained_buffer_bytes with its split
+ * into immutable (model tables), persistent state (q, v, scene bodies,
+ * manifolds) and transient (per-step scratch, commands, statuses) bytes,
+ * and the GPU / submission milliseconds. Observation, motion-feature,
+ * episode and the other fingerprints are zero: this build runs no task
+ * program. */
 typedef struct MRTaskRolloutLayoutC {
     uint32_t environment_count;
     uint32_t nq;
     uint32_t nv;
     uint32_t action_count;
+    uint32_t actor_observation_count;
+    uint32_t critic_observation_count;
     uint32_t scene_body_count;
-    uint32_t reserved;
+    uint32_t motion_feature_count;
+    uint32_t maximum_episode_steps;
+    uint64_t world_fingerprint;
+    uint64_t task_fingerprint;
+    uint64_t observation_fingerprint;
+    uint64_t action_fingerprint;
+    uint64_t run_fingerprint;
+    uint64_t robot_fingerprint;
+    uint64_t sensor_fingerprint;
+    uint64_t reality_fingerprint;
+    uint64_t teacher_fingerprint;
     uint64_t submitted_control_steps;
     uint64_t completed_environment_steps;
     uint64_t submission_count;
     size_t retained_buffer_bytes;
+    size_t immutable_private_bytes;
+    size_t persistent_state_private_bytes;
+    size_t transient_private_bytes;
+    size_t shared_boundary_bytes;
+    size_t peak_aliased_bytes;
     double total_gpu_milliseconds;
+    double total_submission_milliseconds;
 } MRTaskRolloutLayoutC;
 
-/* c_api.h:222-235 (subset) */
+/* c_api.h:160-166 robot-factory sources (not built here: creating a
+ * rollout from them fails with an error), plus this build's synthetic
+ * physics worlds. */
+enum MRRunManifestSourceC {
+    MR_RUN_SOURCE_UNITREE_G1 = 0u,
+    MR_RUN_SOURCE_FRANKA_PICK_PLACE = 1u,
+    MR_RUN_SOURCE_IMPORTED_URDF = 2u,
+    MR_RUN_SOURCE_WORLD_PACK = 3u,
+    MR_RUN_SOURCE_PX4_X500 = 4u,
+    /* 7-DoF effort-driven chain + 20 boxes on a ground plane; actions are
+     * joint efforts of 5 N m per unit */
+    MR_RUN_SOURCE_SYNTHETIC_CHAIN_BOXES = 100u,
+    /* floating 29-DoF humanoid on a 256 x 256 heightfield; actions are PD
+     * targets of 0.3 rad per unit around the standing pose */
+    MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN = 101u,
+    /* per-environment random 6-40 DoF revolute trees with free bodies;
+     * actions are joint efforts of 1 N m per unit */
+    MR_RUN_SOURCE_SYNTHETIC_RANDOM_TREES = 102u
+};
+
+/* c_api.h:29-32 */
+enum MRLocomotionSurfaceC { MR_LOCOMOTION_SURFACE_GROUND = 0u, MR_LOCOMOTION_SURFACE_TERRAIN = 1u };
+
+/* c_api.h:171-190. Read here: profile, source, surface (must name the
+ * synthetic source's own ground: TERRAIN for the humanoid, GROUND for the
+ * others). task and every pack path / visual program must be zero or null;
+ * metallib_path names a Metal shader library and is ignored. */
+typedef struct MRRunManifestC {
+    MRTaskRolloutConfigC profile;
+    uint32_t source;
+    uint32_t surface;
+    uint32_t task;
+    const char* urdf_path;
+    const char* srdf_path;
+    const char* world_pack_path;
+    const char* task_pack_path;
+    const char* robot_actuator_pack_path;
+    const char* sensor_program_pack_path;
+    const char* reality_program_pack_path;
+    const char* teacher_pack_path;
+    const char* teacher_clip_id;
+    const MRTaskVisualObservationConfigC* visual_sensor_program;
+    const MRTaskVisualObservationConfigC* inspection_visual_program;
+    const char* metallib_path;
+} MRRunManifestC;
+
+/* c_api.h:192-220. Maxima over the advance's published per-environment,
+ * per-control-step records: candidate_pairs (active broadphase pairs),
+ * raw_contacts, manifolds, constraint_blocks (contacts), constraint_rows
+ * (3 per contact), islands, articulation_point_queries (2 per contact)
+ * and operator_velocity_elements (nv). Stages this build does not have
+ * (convex, mesh, CCD, rods, quality solver, tiles) and the island
+ * reference counts stay zero. */
+typedef struct MRTaskRolloutStageHighWaterC {
+    uint32_t candidate_pairs;
+    uint32_t raw_contacts;
+    uint32_t manifolds;
+    uint32_t constraint_blocks;
+    uint32_t constraint_rows;
+    uint32_t islands;
+    uint32_t hard_convex_pairs;
+    uint32_t mesh_triangle_candidates;
+    uint32_t solver_tiles;
+    uint32_t spill_rows;
+    uint32_t ccd_candidates;
+    uint32_t ccd_events;
+    uint32_t endpoint_runtime_records;
+    uint32_t articulation_point_queries;
+    uint32_t rod_candidate_pairs;
+    uint32_t rod_raw_contacts;
+    uint32_t rod_manifolds;
+    uint32_t rod_ccd_events;
+    uint32_t quality_generalized_velocities;
+    uint32_t quality_rows;
+    uint32_t quality_krylov_vectors;
+    uint32_t quality_direct_tiles;
+    uint32_t dynamic_nodes;
+    uint32_t island_node_references;
+    uint32_t island_constraint_references;
+    uint32_t rod_factor_blocks;
+    uint32_t operator_velocity_elements;
+} MRTaskRolloutStageHighWaterC;
+
+/* c_api.h:222-235 */
 typedef struct MRTaskRolloutAdvanceC {
     uint32_t control_step_count;
     uint32_t successful_environment_steps;
@@ -245,8 +381,36 @@ typedef struct MRTaskRolloutAdvanceC {
     uint32_t first_gpu_status_code;
     uint32_t host_requested_resets;
     uint32_t maximum_active_contacts;
+    uint32_t maximum_manifolds;
+    MRTaskRolloutStageHighWaterC high_water;
     double gpu_milliseconds;
+    double submission_milliseconds;
 } MRTaskRolloutAdvanceC;
+
+#ifdef __cplusplus
+#define MR_TR_ASSERT(c, m) static_assert(c, m)
+#else
+#define MR_TR_ASSERT(c, m) _Static_assert(c, m)
+#endif
+/* Sizes and offsets of the reference's records on LP64 (x86-64 / arm64). */
+MR_TR_ASSERT(sizeof(MRTaskRolloutDynamicSphereC) == 36, "MRTaskRolloutDynamicSphereC size");
+MR_TR_ASSERT(sizeof(MRTaskRolloutConfigC) == 104, "MRTaskRolloutConfigC size");
+MR_TR_ASSERT(offsetof(MRTaskRolloutConfigC, seed) == 24, "MRTaskRolloutConfigC.seed");
+MR_TR_ASSERT(offsetof(MRTaskRolloutConfigC, dynamic_spheres) == 32, "MRTaskRolloutConfigC.dynamic_spheres");
+MR_TR_ASSERT(offsetof(MRTaskRolloutConfigC, override_interaction_reset_maximum_phase) == 96,
+             "MRTaskRolloutConfigC tail");
+MR_TR_ASSERT(sizeof(MRTaskRolloutLayoutC) == 200, "MRTaskRolloutLayoutC size");
+MR_TR_ASSERT(offsetof(MRTaskRolloutLayoutC, world_fingerprint) == 40, "MRTaskRolloutLayoutC.world_fingerprint");
+MR_TR_ASSERT(offsetof(MRTaskRolloutLayoutC, retained_buffer_bytes) == 136, "MRTaskRolloutLayoutC.retained_buffer_bytes");
+MR_TR_ASSERT(offsetof(MRTaskRolloutLayoutC, total_gpu_milliseconds) == 184, "MRTaskRolloutLayoutC.total_gpu_milliseconds");
+MR_TR_ASSERT(sizeof(MRRunManifestC) == 216, "MRRunManifestC size");
+MR_TR_ASSERT(offsetof(MRRunManifestC, source) == 104, "MRRunManifestC.source");
+MR_TR_ASSERT(offsetof(MRRunManifestC, urdf_path) == 120, "MRRunManifestC.urdf_path");
+MR_TR_ASSERT(offsetof(MRRunManifestC, metallib_path) == 208, "MRRunManifestC.metallib_path");
+MR_TR_ASSERT(sizeof(MRTaskRolloutStageHighWaterC) == 108, "MRTaskRolloutStageHighWaterC size");
+MR_TR_ASSERT(sizeof(MRTaskRolloutAdvanceC) == 160, "MRTaskRolloutAdvanceC size");
+MR_TR_ASSERT(offsetof(MRTaskRolloutAdvanceC, high_water) == 36, "MRTaskRolloutAdvanceC.high_water");
+MR_TR_ASSERT(offsetof(MRTaskRolloutAdvanceC, gpu_milliseconds) == 144, "MRTaskRolloutAdvanceC.gpu_milliseconds");
 
 /* c_api.h:583-585 */
 MRTaskRolloutHandle* mr_create_task_rollout(const MRRunManifestC* manifest);

@@ -173,6 +173,28 @@ typedef struct MRDofC {
     mr_float4 drive;   /* stiffness, damping, armature, dry friction */
 } MRDofC;
 
+/* Optional motor / transmission record per generalized velocity
+ * (MRActuatorProfileGPU, engine_types.h:386-395, 48 bytes). An inactive
+ * record (flags without MR_ACTUATOR_PROFILE_ACTIVE) must be all zero; it
+ * executes as no-load speed FLT_MAX, efficiency 1 and stall torque equal to
+ * the DoF's effort limit (FLT_MAX without one), as MetalWorld.mm:1376-1410
+ * expands it. Every command is clamped to the torque-speed envelope
+ *   min(limits.w, stall * efficiency * (1 - clamp(|v| / no_load_speed, 0, 1)))
+ * (MetalWorld.metal:252-269, 345-365), so a DoF without an effort limit
+ * receives no effort. */
+enum MRActuatorProfileFlags {
+    MR_ACTUATOR_PROFILE_ACTIVE = 1u << 0u,
+    MR_ACTUATOR_PROFILE_CALIBRATED = 1u << 1u
+};
+
+typedef struct MRActuatorProfileC {
+    /* joint-side torque constant N m/A, current limit A, no-load speed rad/s, efficiency [0, 1] */
+    mr_float4 motor_and_speed;
+    /* backlash play, command delay s, stall torque N m (= torque constant * current limit), reserved */
+    mr_float4 transmission_and_envelope;
+    uint32_t identity[4]; /* model-local v index, MRActuatorProfileFlags, reserved, reserved */
+} MRActuatorProfileC;
+
 /* engine_types.h:1430-1455 */
 typedef struct MRShapeC {
     uint32_t body;
@@ -266,6 +288,10 @@ typedef struct MRWorldModelDescC {
     uint32_t heightfield_vertex_count;
     MRHeightfieldC heightfield;
     const float* heightfield_heights;  /* [ny][nx] */
+    /* [dof_total], parallel to dofs, or null: every DoF inactive */
+    const MRActuatorProfileC* actuators;
+    uint32_t actuator_total;           /* dof_total, or 0 with actuators null */
+    uint32_t reserved_actuators;
 } MRWorldModelDescC;
 
 /* ---------------------------------------------------------------- state -- */
@@ -399,6 +425,7 @@ typedef struct MRWorldStepConfigC {
 #endif
 MR_STATIC_ASSERT(sizeof(MRBodyC) == 256, "MRBodyC size");
 MR_STATIC_ASSERT(sizeof(MRDofC) == 48, "MRDofC size");
+MR_STATIC_ASSERT(sizeof(MRActuatorProfileC) == 48, "MRActuatorProfileC size");
 MR_STATIC_ASSERT(sizeof(MRShapeC) == 96, "MRShapeC size");
 MR_STATIC_ASSERT(sizeof(MRMaterialC) == 32, "MRMaterialC size");
 MR_STATIC_ASSERT(sizeof(MRCompiledPairC) == 16, "MRCompiledPairC size");

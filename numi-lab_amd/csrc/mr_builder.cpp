@@ -825,6 +825,8 @@ MRWorldModelDescC ModelData::describe() const {
     d.heightfield_vertex_count = (uint32_t)heights.size();
     d.heightfield = heightfield;
     d.heightfield_heights = heights.empty() ? nullptr : heights.data();
+    d.actuators = actuators.empty() ? nullptr : actuators.data();
+    d.actuator_total = (uint32_t)actuators.size();
     return d;
 }
 
@@ -848,8 +850,45 @@ void ModelData::commands(uint32_t first_step, uint32_t step_count, float* out) c
     }
 }
 
+namespace {
+// EngineModel.cpp validActuatorProfile: an inactive record is all zero; an
+// active one drives an actuated, effort-limited DoF with positive motor
+// constants, efficiency at most 1 and a stall torque equal to torque
+// constant x current limit.
+void validate_actuator(const MRActuatorProfileC& a, const MRDofC& d, uint32_t local_v) {
+    const auto finite4 = [](const mr_float4& v) {
+        return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z) && std::isfinite(v.w);
+    };
+    const uint32_t known = MR_ACTUATOR_PROFILE_ACTIVE | MR_ACTUATOR_PROFILE_CALIBRATED;
+    if (a.identity[0] != local_v || a.identity[2] != 0u || a.identity[3] != 0u || (a.identity[1] & ~known) != 0u ||
+        !finite4(a.motor_and_speed) || !finite4(a.transmission_and_envelope))
+        throw std::runtime_error("actuator profile " + std::to_string(local_v) +
+                                 ": identity must be (local v index, known flags, 0, 0) with finite values");
+    const mr_float4& mo = a.motor_and_speed;
+    const mr_float4& tr = a.transmission_and_envelope;
+    if ((a.identity[1] & MR_ACTUATOR_PROFILE_ACTIVE) == 0u) {
+        if ((a.identity[1] & MR_ACTUATOR_PROFILE_CALIBRATED) != 0u || mo.x != 0.0f || mo.y != 0.0f ||
+            mo.z != 0.0f || mo.w != 0.0f || tr.x != 0.0f || tr.y != 0.0f || tr.z != 0.0f || tr.w != 0.0f)
+            throw std::runtime_error("actuator profile " + std::to_string(local_v) + ": an inactive profile must be zero");
+        return;
+    }
+    if ((d.flags & MR_DOF_FLAG_ACTUATED) == 0u || (d.flags & MR_DOF_FLAG_EFFORT_LIMIT) == 0u ||
+        !(d.limits.w > 0.0f) || !(mo.x > 0.0f) || !(mo.y > 0.0f) || !(mo.z > 0.0f) || !(mo.w > 0.0f) ||
+        mo.w > 1.0f || tr.x < 0.0f || tr.y < 0.0f || !(tr.z > 0.0f) || tr.w != 0.0f)
+        throw std::runtime_error("actuator profile " + std::to_string(local_v) +
+                                 ": an active profile needs an actuated, effort-limited DoF, positive motor "
+                                 "constants and an efficiency in (0, 1]");
+    const double expected = (double)mo.x * (double)mo.y;
+    if (std::fabs((double)tr.z - expected) > 2.0e-5 * std::max(1.0, expected))
+        throw std::runtime_error("actuator profile " + std::to_string(local_v) +
+                                 ": stall torque must equal torque constant x current limit");
+}
+} // namespace
+
 void finalize_model(ModelData& m, float microstep) {
     if (m.models.empty() || m.environment_model.empty()) throw std::runtime_error("model has no environments");
+    if (!m.actuators.empty() && m.actuators.size() != m.dofs.size())
+        throw std::runtime_error("actuator profiles must be parallel to the DoF records (dof_total of them)");
     uint32_t max_shapes = 0, max_pairs = 0;
     for (size_t mi = 0; mi < m.models.size(); ++mi) {
         MRModelHeaderC& h = m.models[mi];
@@ -901,18 +940,34 @@ void finalize_model(ModelData& m, float microstep) {
             const MRDofC& d = m.dofs[h.dof_offset + j];
             if (d.body >= h.link_count) throw std::runtime_error("dof body outside the articulation");
         }
+        for (uint32_t j = 0; j < h.nv; ++j) {
+            const MRDofC& d = m.dofs[h.dof_offset + j];
+            if (d.q_index != MR_INVALID_INDEX && d.q_index >= h.nq)
+                throw std::runtime_error("dof q_index outside the model's coordinates");
+            if (!m.actuators.empty()) validate_actuator(m.actuators[h.dof_offset + j], d, j);
+        }
         for (uint32_t s = 0; s < h.shape_count; ++s) {
             const MRShapeC& sh = m.shapes[h.shape_offset + s];
             if (sh.body >= h.body_count || sh.material >= h.material_count)
                 throw std::runtime_error("shape references an invalid body or material");
+            if (sh.shape_type != MR_SHAPE_SPHERE && sh.shape_type != MR_SHAPE_CAPSULE && sh.shape_type != MR_SHAPE_BOX &&
+                sh.shape_type != MR_SHAPE_PLANE && sh.shape_type != MR_SHAPE_HEIGHTFIELD)
+                throw std::runtime_error("unsupported shape type");
             if (sh.shape_type == MR_SHAPE_HEIGHTFIELD &&
-                (!m.heightfield.present || m.heightfield.nx < 2u || m.heightfield.ny < 2u))
-                throw std::runtime_error("heightfield shape without heightfield data");
+                (!m.heightfield.present || m.heightfield.nx < 2u || m.heightfield.ny < 2u ||
+                 m.heights.size() < (size_t)m.heightfield.nx * m.heightfield.ny ||
+                 !(m.heightfield.origin_spacing.z > 0.0f)))
+                throw std::runtime_error("heightfield shape without heightfield data (nx, ny >= 2, "
+                                         "nx * ny heights, positive spacing)");
         }
         for (uint32_t p = 0; p < h.pair_count; ++p) {
             const MRCompiledPairC& pr = m.pairs[h.pair_offset + p];
             if (pr.collider_a >= pr.collider_b || pr.collider_b >= h.shape_count)
                 throw std::runtime_error("compiled pairs must be canonical (A<B)");
+            const uint32_t cls = pair_class(m.shapes[h.shape_offset + pr.collider_a].shape_type,
+                                            m.shapes[h.shape_offset + pr.collider_b].shape_type);
+            if (cls == MR_PAIR_UNSUPPORTED || pr.pair_class != cls)
+                throw std::runtime_error("compiled pair class does not match its shape types");
             if (p > 0u) {
                 const MRCompiledPairC& prev = m.pairs[h.pair_offset + p - 1u];
                 if (prev.collider_a > pr.collider_a ||
@@ -932,12 +987,27 @@ void finalize_model(ModelData& m, float microstep) {
     m.max_pairs = max_pairs;
 }
 
-ModelData copy_model(const MRWorldModelDescC& d) {
+ModelData copy_model(const MRWorldModelDescC& d, float microstep) {
+    auto need = [](const void* p, uint32_t n, const char* what) {
+        if (n > 0u && p == nullptr) throw std::runtime_error(std::string(what) + " is null with a nonzero count");
+    };
+    need(d.models, d.model_count, "models");
+    need(d.environment_model, d.environment_count, "environment_model");
+    need(d.bodies, d.body_total, "bodies");
+    need(d.dofs, d.dof_total, "dofs");
+    need(d.shapes, d.shape_total, "shapes");
+    need(d.materials, d.material_total, "materials");
+    need(d.pairs, d.pair_total, "pairs");
+    need(d.heightfield_heights, d.heightfield_vertex_count, "heightfield_heights");
+    need(d.actuators, d.actuator_total, "actuators");
+    if (d.actuators != nullptr && d.actuator_total != d.dof_total)
+        throw std::runtime_error("actuator_total must equal dof_total");
     ModelData m;
     m.models.assign(d.models, d.models + d.model_count);
     m.environment_model.assign(d.environment_model, d.environment_model + d.environment_count);
     m.bodies.assign(d.bodies, d.bodies + d.body_total);
     m.dofs.assign(d.dofs, d.dofs + d.dof_total);
+    if (d.actuators != nullptr) m.actuators.assign(d.actuators, d.actuators + d.actuator_total);
     m.shapes.assign(d.shapes, d.shapes + d.shape_total);
     m.materials.assign(d.materials, d.materials + d.material_total);
     m.pairs.assign(d.pairs, d.pairs + d.pair_total);
@@ -948,6 +1018,10 @@ ModelData copy_model(const MRWorldModelDescC& d) {
     m.max_nq = d.max_nq;
     m.max_nv = d.max_nv;
     m.max_scene_bodies = d.max_scene_bodies;
+    // Validate every index and count before anything below reads through
+    // them; finalize_model also raises the strides to the per-model maxima,
+    // so the state arrays are sized from the final strides.
+    finalize_model(m, microstep);
     // Default initial state: zero coordinates (identity root orientation),
     // zero velocities, scene bodies at the origin with their authored mass.
     const uint32_t E = d.environment_count;

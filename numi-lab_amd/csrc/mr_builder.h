@@ -31,6 +31,7 @@ struct ModelData {
     std::vector<uint32_t> environment_model;
     std::vector<MRBodyC> bodies;
     std::vector<MRDofC> dofs;
+    std::vector<MRActuatorProfileC> actuators;  // parallel to dofs, or empty: every DoF inactive
     std::vector<MRShapeC> shapes;
     std::vector<MRMaterialC> materials;
     std::vector<MRCompiledPairC> pairs;
@@ -57,7 +58,9 @@ struct ModelData {
 // derived fields (levels, damping scales for timestep h). Throws
 // std::runtime_error with a message on violation.
 void finalize_model(ModelData& model, float microstep);
-ModelData copy_model(const MRWorldModelDescC& desc);
+// Copies a caller-described model, validating it (finalize_model) before
+// the default initial state is sized and filled.
+ModelData copy_model(const MRWorldModelDescC& desc, float microstep);
 ModelData build_synthetic(uint32_t kind, uint32_t environment_count, uint64_t seed);
 
 // Canonical eligible pairs (MetalWorld.mm:14546-14615): every unordered

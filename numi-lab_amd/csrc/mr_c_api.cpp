@@ -7,6 +7,7 @@
 #include "mr_world.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -36,6 +37,7 @@ struct MRTaskRolloutHandle {
     uint64_t completed_environment_steps = 0;
     uint64_t submission_count = 0;
     double total_gpu_ms = 0.0;
+    double total_submission_ms = 0.0;
 };
 
 namespace {
@@ -132,8 +134,7 @@ MRWorldModelHandle* mr_world_model_create(const MRWorldModelDescC* desc, float m
         require(microstep > 0.0f, "microstep must be positive");
         auto* h = new MRWorldModelHandle();
         try {
-            h->model = mr::copy_model(*desc);
-            mr::finalize_model(h->model, microstep);
+            h->model = mr::copy_model(*desc, microstep);
         } catch (...) {
             delete h;
             throw;
@@ -441,11 +442,48 @@ int mr_world_read_mlp_samples(const MRWorldHandle* world, float* out, size_t cou
 MRTaskRolloutHandle* mr_create_task_rollout(const MRRunManifestC* manifest) {
     return construct<MRTaskRolloutHandle>([&]() {
         require(manifest != nullptr, "manifest is null");
-        uint32_t kind = 0;
-        if (manifest->source == MR_RUN_SOURCE_SYNTHETIC_CHAIN_BOXES) kind = mr::kChainBoxes;
-        else if (manifest->source == MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN) kind = mr::kHumanoidTerrain;
-        else throw std::runtime_error("unsupported run source");
         const MRTaskRolloutConfigC& p = manifest->profile;
+        uint32_t kind = 0, surface = MR_LOCOMOTION_SURFACE_GROUND;
+        switch (manifest->source) {
+        case MR_RUN_SOURCE_SYNTHETIC_CHAIN_BOXES: kind = mr::kChainBoxes; break;
+        case MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN:
+            kind = mr::kHumanoidTerrain;
+            surface = MR_LOCOMOTION_SURFACE_TERRAIN;
+            break;
+        case MR_RUN_SOURCE_SYNTHETIC_RANDOM_TREES: kind = mr::kRandomTrees; break;
+        case MR_RUN_SOURCE_UNITREE_G1:
+        case MR_RUN_SOURCE_FRANKA_PICK_PLACE:
+        case MR_RUN_SOURCE_IMPORTED_URDF:
+        case MR_RUN_SOURCE_WORLD_PACK:
+        case MR_RUN_SOURCE_PX4_X500:
+            throw std::runtime_error("unsupported run source " + std::to_string(manifest->source) +
+                                     ": robot-factory and pack sources are not built by this engine; use a "
+                                     "synthetic source (100-102)");
+        default: throw std::runtime_error("unsupported run source " + std::to_string(manifest->source));
+        }
+        require(p.environment_count > 0u, "profile.environment_count must be positive");
+        require(manifest->surface == surface,
+                "manifest.surface must name the synthetic source's own ground (TERRAIN for the humanoid, "
+                "GROUND otherwise)");
+        require(manifest->task == 0u, "manifest.task must be 0: this build runs no task program");
+        require(manifest->urdf_path == nullptr && manifest->srdf_path == nullptr &&
+                    manifest->world_pack_path == nullptr && manifest->task_pack_path == nullptr &&
+                    manifest->robot_actuator_pack_path == nullptr && manifest->sensor_program_pack_path == nullptr &&
+                    manifest->reality_program_pack_path == nullptr && manifest->teacher_pack_path == nullptr &&
+                    manifest->teacher_clip_id == nullptr,
+                "manifest pack paths must be null for a synthetic source");
+        require(manifest->visual_sensor_program == nullptr && manifest->inspection_visual_program == nullptr,
+                "visual programs are not supported by this build");
+        require(p.dynamic_spheres == nullptr && p.dynamic_sphere_count == 0u,
+                "profile.dynamic_spheres are not supported by this build");
+        require(p.interaction_reference_mode == 0u && p.interaction_student_authority == 0.0f &&
+                    p.override_interaction_student_authority == 0u && p.interaction_reset_phase_fraction == 0.0f &&
+                    p.override_interaction_reset_phase_fraction == 0u && p.override_difficulty_band_range == 0u &&
+                    p.minimum_difficulty_band == 0u && p.maximum_difficulty_band == 0u &&
+                    p.interaction_reset_phase_probability == 0.0f &&
+                    p.override_interaction_reset_phase_probability == 0u &&
+                    p.interaction_reset_maximum_phase == 0.0f && p.override_interaction_reset_maximum_phase == 0u,
+                "profile interaction and difficulty-band fields must be zero: this build runs no task program");
         auto* h = new MRTaskRolloutHandle();
         try {
             h->model = mr::build_synthetic(kind, p.environment_count, p.seed);
@@ -456,8 +494,8 @@ MRTaskRolloutHandle* mr_create_task_rollout(const MRRunManifestC* manifest) {
             if (p.control_timestep_seconds > 0.0f) cfg.control_timestep = p.control_timestep_seconds;
             h->world = std::make_unique<mr::World>(h->model, cfg);
             const MRModelHeaderC& m0 = h->model.models[0];
-            h->action_count = m0.root_type == MR_ROOT_FLOATING ? m0.nv - 6u : m0.nv;
-            h->action_scale = kind == mr::kChainBoxes ? 5.0f : 0.3f;
+            h->action_count = m0.root_type == MR_ROOT_FLOATING ? h->model.max_nv - 6u : h->model.max_nv;
+            h->action_scale = kind == mr::kChainBoxes ? 5.0f : (kind == mr::kHumanoidTerrain ? 0.3f : 1.0f);
         } catch (...) {
             delete h;
             throw;
@@ -480,6 +518,10 @@ int mr_task_rollout_reset(MRTaskRolloutHandle* handle, uint64_t seed) {
             handle->world->set_state(handle->model.initial_q.data(), handle->model.initial_v.data(),
                                      handle->model.initial_scene.data());
         }
+        handle->status_codes.clear();
+        handle->active_contacts.clear();
+        handle->final_q.clear();
+        handle->final_scene.clear();
     });
 }
 
@@ -493,31 +535,45 @@ int mr_task_rollout_set_state_readback(MRTaskRolloutHandle* handle, uint32_t ena
 int mr_task_rollout_advance(MRTaskRolloutHandle* handle, const float* normalized_actions, size_t action_count,
                             const uint32_t* reset_masks, size_t mask_count, uint32_t control_step_count,
                             uint64_t policy_revision, uint32_t evaluate_final_policy, MRTaskRolloutAdvanceC* out) {
-    (void)policy_revision;
+    (void)policy_revision;  // no policy program: the host action stream drives every step
+    if (out == nullptr) {
+        gLastError = "task-rollout advance result is null.";
+        return -1;
+    }
+    std::memset(out, 0, sizeof(*out));
+    out->first_failing_environment = MR_INVALID_INDEX;
+    out->first_failing_control_step = MR_INVALID_INDEX;
     return translate([&]() {
-        require(handle != nullptr && normalized_actions != nullptr, "null argument");
-        require(evaluate_final_policy == 0u, "this build has no policy program");
-        require(control_step_count > 0u, "control_step_count must be positive");
+        const auto t0 = std::chrono::steady_clock::now();
+        require(handle != nullptr, "null handle");
+        require(control_step_count > 0u, "task-rollout control_step_count must be positive.");
+        require(evaluate_final_policy == 0u, "this build has no policy program to evaluate");
         mr::World& w = *handle->world;
         const mr::ModelData& m = handle->model;
         const size_t E = w.environment_count(), A = handle->action_count, nv = m.max_nv;
-        require_count(action_count, (size_t)control_step_count * E * A, "actions");
-        if (reset_masks != nullptr) require_count(mask_count, (size_t)control_step_count * E, "reset masks");
+        require(normalized_actions != nullptr && action_count == (size_t)control_step_count * E * A,
+                "normalized task action count must equal step_count * environment_count * compiled action_count");
+        require((reset_masks == nullptr && mask_count == 0u) ||
+                    (reset_masks != nullptr && mask_count == (size_t)control_step_count * E),
+                "task reset mask count must be zero or step_count * environment_count");
         std::vector<float> commands((size_t)control_step_count * E * nv, 0.0f);
         const uint32_t first = m.models[0].root_type == MR_ROOT_FLOATING ? 6u : 0u;
         uint32_t resets = 0;
         for (size_t s = 0; s < control_step_count; ++s)
             for (size_t e = 0; e < E; ++e) {
                 if (reset_masks != nullptr && reset_masks[s * E + e] != 0u) ++resets;
+                const uint32_t env_nv = m.models[m.environment_model[e]].nv;
                 for (size_t a = 0; a < A; ++a) {
                     float x = normalized_actions[(s * E + e) * A + a];
                     if (!std::isfinite(x)) throw std::runtime_error("nonfinite action input");
-                    x = x < -1.0f ? -1.0f : (x > 1.0f ? 1.0f : x);
                     const size_t j = first + a;
+                    if (j >= env_nv) continue;  // random trees: beyond this world's DoFs
+                    x = x < -1.0f ? -1.0f : (x > 1.0f ? 1.0f : x);
                     const float base = m.kind == mr::kHumanoidTerrain ? m.standing_pose[j] : 0.0f;
                     commands[(s * E + e) * nv + j] = base + handle->action_scale * x;
                 }
             }
+        out->host_requested_resets = resets;
         w.enqueue(commands.data(), reset_masks, control_step_count, false);
         const mr::AdvanceSummary sum = w.wait();
         const size_t n = (size_t)control_step_count * E;
@@ -527,53 +583,96 @@ int mr_task_rollout_advance(MRTaskRolloutHandle* handle, const float* normalized
         w.read_contact_statuses(cs.data(), control_step_count);
         handle->status_codes.resize(n);
         handle->active_contacts.resize(n);
-        uint32_t max_contacts = 0;
+        MRTaskRolloutStageHighWaterC& hw = out->high_water;
         for (size_t i = 0; i < n; ++i) {
-            handle->status_codes[i] = st[i].code;
+            handle->status_codes[i] = st[i].code != MR_STEP_SUCCESS ? st[i].code : cs[i].code;
             handle->active_contacts[i] = cs[i].constraints;
-            max_contacts = std::max(max_contacts, cs[i].max_constraints);
+            out->maximum_active_contacts = std::max(out->maximum_active_contacts, cs[i].constraints);
+            out->maximum_manifolds = std::max(out->maximum_manifolds, cs[i].required_manifolds);
+            hw.candidate_pairs = std::max(hw.candidate_pairs, cs[i].max_active_pairs);
+            hw.raw_contacts = std::max(hw.raw_contacts, cs[i].raw_contacts);
+            hw.manifolds = std::max(hw.manifolds, cs[i].manifolds);
+            hw.constraint_blocks = std::max(hw.constraint_blocks, cs[i].max_constraints);
+            hw.islands = std::max(hw.islands, cs[i].islands);
         }
+        hw.constraint_rows = 3u * hw.constraint_blocks;
+        hw.articulation_point_queries = 2u * hw.constraint_blocks;
+        hw.operator_velocity_elements = m.max_nv;
         if (handle->state_readback) {
             handle->final_q.resize(E * m.max_nq);
             handle->final_scene.resize(E * m.max_scene_bodies);
             w.get_state(handle->final_q.data(), nullptr, handle->final_scene.data());
         }
+        const double wall_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         handle->submitted_control_steps += control_step_count;
         handle->completed_environment_steps += sum.successful_environment_steps;
         handle->submission_count += 1u;
         handle->total_gpu_ms += sum.gpu_milliseconds;
-        if (out != nullptr) {
-            out->control_step_count = control_step_count;
-            out->successful_environment_steps = sum.successful_environment_steps;
-            out->failed_environment_steps = sum.failed_environment_steps;
-            out->first_failing_environment = sum.first_failing_environment;
-            out->first_failing_control_step = sum.first_failing_control_step;
-            out->first_gpu_status_code = sum.first_gpu_status_code;
-            out->host_requested_resets = resets;
-            out->maximum_active_contacts = max_contacts;
-            out->gpu_milliseconds = sum.gpu_milliseconds;
-        }
+        handle->total_submission_ms += wall_ms;
+        out->control_step_count = control_step_count;
+        out->successful_environment_steps = sum.successful_environment_steps;
+        out->failed_environment_steps = sum.failed_environment_steps;
+        out->first_failing_environment = sum.first_failing_environment;
+        out->first_failing_control_step = sum.first_failing_control_step;
+        out->first_gpu_status_code = sum.first_gpu_status_code;
+        out->gpu_milliseconds = sum.gpu_milliseconds;
+        out->submission_milliseconds = wall_ms;
         if (sum.failed_environment_steps > 0u)
-            throw std::runtime_error("GPU environment step failed with status " +
+            throw std::runtime_error("task-rollout GPU step failed with status " +
                                      std::to_string(sum.first_gpu_status_code) + " in environment " +
-                                     std::to_string(sum.first_failing_environment));
+                                     std::to_string(sum.first_failing_environment) + " at control step " +
+                                     std::to_string(sum.first_failing_control_step));
     });
 }
+
+namespace {
+// FNV-1a over the compiled model tables: equal for equal worlds.
+uint64_t world_fingerprint(const mr::ModelData& m) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void* data, size_t bytes) {
+        const auto* b = static_cast<const unsigned char*>(data);
+        for (size_t i = 0; i < bytes; ++i) {
+            h ^= b[i];
+            h *= 1099511628211ull;
+        }
+    };
+    auto mix_vec = [&](const auto& v) { mix(v.data(), v.size() * sizeof(v[0])); };
+    mix_vec(m.models);
+    mix_vec(m.environment_model);
+    mix_vec(m.bodies);
+    mix_vec(m.dofs);
+    mix_vec(m.shapes);
+    mix_vec(m.materials);
+    mix_vec(m.pairs);
+    mix_vec(m.heights);
+    mix(&m.heightfield, sizeof(m.heightfield));
+    mix(&m.gravity, sizeof(m.gravity));
+    return h;
+}
+} // namespace
 
 MRTaskRolloutLayoutC mr_task_rollout_layout(const MRTaskRolloutHandle* handle) {
     MRTaskRolloutLayoutC out;
     std::memset(&out, 0, sizeof(out));
     if (handle == nullptr) return out;
-    out.environment_count = handle->world->environment_count();
+    const mr::World& w = *handle->world;
+    out.environment_count = w.environment_count();
     out.nq = handle->model.max_nq;
     out.nv = handle->model.max_nv;
     out.action_count = handle->action_count;
     out.scene_body_count = handle->model.max_scene_bodies;
+    out.world_fingerprint = world_fingerprint(handle->model);
     out.submitted_control_steps = handle->submitted_control_steps;
     out.completed_environment_steps = handle->completed_environment_steps;
     out.submission_count = handle->submission_count;
-    out.retained_buffer_bytes = handle->world->retained_bytes();
+    out.immutable_private_bytes = w.immutable_bytes();
+    out.persistent_state_private_bytes = w.persistent_bytes();
+    out.transient_private_bytes = w.transient_bytes();
+    out.retained_buffer_bytes = out.immutable_private_bytes + out.persistent_state_private_bytes +
+                                out.transient_private_bytes;
     out.total_gpu_milliseconds = handle->total_gpu_ms;
+    out.total_submission_milliseconds = handle->total_submission_ms;
     return out;
 }
 

@@ -183,6 +183,7 @@ struct KernelArgs {
     const uint8_t* response_slots;  // [model_count][MR_MAX_BODIES]: plan_response_slots per model
     const MRBodyC* bodies;
     const MRDofC* dofs;
+    const MRActuatorProfileC* actuators;  // parallel to dofs, or null: every DoF inactive
     const MRShapeC* shapes;
     const MRMaterialC* materials;
     const MRCompiledPairC* pairs;

@@ -174,6 +174,7 @@ struct Env {
     const MRModelHeaderC* hd;
     const MRBodyC* bodies;
     const MRDofC* dofs;
+    const MRActuatorProfileC* acts;  // [nv] or null: every DoF inactive
     const MRShapeC* shapes;
     const MRMaterialC* mats;
     const MRCompiledPairC* pairs;
@@ -359,10 +360,30 @@ MRD_INL float effective_armature(const Env& e, uint32_t j) {
     return value;
 }
 
+// Torque-speed envelope of DoF j at joint velocity vel (MetalWorld.metal:
+// 252-269): min(effort limit, stall x efficiency x (1 - speed fraction)). An
+// inactive profile executes as no-load speed FLT_MAX, efficiency 1 and stall
+// torque = the effort limit, or FLT_MAX without one (MetalWorld.mm:1376-1410).
+MRD_INL float actuator_envelope(const MRDofC& dof, const MRActuatorProfileC* act, float vel) {
+    constexpr float kFloatMax = 3.40282346638528859812e38f;
+    float no_load = kFloatMax, efficiency = 1.0f;
+    float stall = ((dof.flags & MR_DOF_FLAG_EFFORT_LIMIT) != 0u && dof.limits.w > 0.0f) ? dof.limits.w : kFloatMax;
+    if (act != nullptr && (act->identity[1] & MR_ACTUATOR_PROFILE_ACTIVE) != 0u) {
+        no_load = act->motor_and_speed.z;
+        efficiency = act->motor_and_speed.w;
+        stall = act->transmission_and_envelope.z;
+    }
+    const float speed_fraction = clampd(fabs_(vel) / fmaxd(no_load, 1.175494351e-38f), 0.0f, 1.0f);
+    return fmind(dof.limits.w, stall * efficiency * (1.0f - speed_fraction));
+}
+
+// Actuator command of DoF j from the current q / v, clamped to the envelope
+// (MetalWorld.metal:184-270 prepare, 280-366 refresh).
 MRD_INL float drive_effort(const Env& e, uint32_t j) {
     const MRDofC& dof = e.dofs[j];
     const float command = e.cmd[j];
-    if (e.a->cfg.actuation_mode == MR_ACTUATION_EFFORT) return command;
+    const float envelope = actuator_envelope(dof, e.acts != nullptr ? &e.acts[j] : nullptr, e.v[j]);
+    if (e.a->cfg.actuation_mode == MR_ACTUATION_EFFORT) return clampd(command, -envelope, envelope);
     float cmd = 0.0f;
     if ((dof.flags & MR_DOF_FLAG_DRIVE) != 0u && dof.q_index != MR_INVALID_INDEX) {
         float target = command;
@@ -377,7 +398,7 @@ MRD_INL float drive_effort(const Env& e, uint32_t j) {
         if ((dof.flags & MR_DOF_FLAG_EFFORT_LIMIT) != 0u && dof.limits.w > 0.0f)
             cmd = clampd(cmd, -dof.limits.w, dof.limits.w);
     }
-    return cmd;
+    return clampd(cmd, -envelope, envelope);
 }
 
 // ------------------------------------------------------------- kinematics
@@ -1969,6 +1990,7 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
     const MRModelHeaderC& hd = *e.hd;
     e.bodies = A.bodies + hd.body_offset;
     e.dofs = A.dofs + hd.dof_offset;
+    e.acts = A.actuators != nullptr ? A.actuators + hd.dof_offset : nullptr;
     e.shapes = A.shapes + hd.shape_offset;
     e.mats = A.materials + hd.material_offset;
     e.pairs = A.pairs + hd.pair_offset;
@@ -2150,7 +2172,11 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
         const bool final_sub = substep + 1u == cfg.physics_substeps;
         const bool capture = cfg.capture_evidence != 0u && substep == cfg.capture_substep && A.evidence_pairs != nullptr;
         // ---------------------------------------------------------- drives
-        for (uint32_t j = lane; j < nv; j += kLanes) e.effort[j] = drive_effort(e, j);
+        // Implicit drives refresh from the current state every microstep; an
+        // effort command is clamped once, at the control step's start velocity,
+        // and held (MetalWorld.metal:184-270 prepare, 280-366 refresh).
+        if (substep == 0u || cfg.actuation_mode == MR_ACTUATION_IMPLICIT_POSITION_DRIVE)
+            for (uint32_t j = lane; j < nv; j += kLanes) e.effort[j] = drive_effort(e, j);
         __syncthreads();
         // ------------------------------------------------------ kinematics
         if (lane == 0u) {
@@ -2397,8 +2423,11 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
         // (a sphere has one, a capsule three, a box eight), into e.hfq; moff
         // holds each pair's first query until the manifold prefix below. A pair
         // whose queries do not fit the table evaluates them in its own lane.
+        // Query offsets grow with the pair index, so the pairs that fit own
+        // exactly the slots [0, used); the slots of a pair that straddles the
+        // table end are never written and never read.
         {
-            uint32_t nhf = 0;
+            uint32_t nhf = 0, used = 0;
             for (uint32_t base = 0; base < nact; base += kLanes) {
                 const uint32_t k = base + lane;
                 uint32_t nq = 0;
@@ -2412,17 +2441,22 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
                 }
                 uint32_t tot;
                 const uint32_t before = wave_prefix_sum(nq, lane, tot);
+                uint32_t end = 0;
                 if (k < nact) {
                     const uint32_t first = nhf + before;
                     const bool fits = nq > 0u && first + nq <= mr::kHfSlots;
                     e.moff[k] = fits ? first : 0xffffffffu;
-                    if (fits)
+                    if (fits) {
                         for (uint32_t q = 0; q < nq; ++q) e.hfitem[first + q] = k | (q << 16);
+                        end = first + nq;
+                    }
                 }
+                const uint32_t chunk_end = wave_max_u32(end);
+                used = chunk_end > used ? chunk_end : used;
                 nhf += tot;
             }
             __syncthreads();
-            const uint32_t items = nhf < mr::kHfSlots ? nhf : mr::kHfSlots;
+            const uint32_t items = used;
             for (uint32_t i = lane; i < items; i += kLanes) {
                 const uint32_t k = e.hfitem[i] & 0xffffu, q = e.hfitem[i] >> 16;
                 const MRCompiledPairC pr = e.pairs[e.active[k]];
@@ -3318,7 +3352,9 @@ mr_control_step_mlp_kernel(KernelArgs args) {
 // Advance summary: one thread clears it, then a grid-stride pass over the
 // [steps][environments] status records folds it with per-wave reductions and
 // one atomic per wave (kernels rather than a memset, so the sequence is also
-// valid inside a captured graph).
+// valid inside a captured graph). With a subset work-queue schedule only the
+// listed environments were stepped, so only their records are folded:
+// `order` lists them (null for every environment).
 extern "C" __global__ void mr_summary_clear_kernel(mr::AdvanceSummaryDev* s) {
     if (threadIdx.x == 0u && blockIdx.x == 0u) {
         s->first_failure = ~0ull;
@@ -3330,20 +3366,25 @@ extern "C" __global__ void mr_summary_clear_kernel(mr::AdvanceSummaryDev* s) {
 }
 
 extern "C" __global__ void __launch_bounds__(256)
-mr_summary_reduce_kernel(const MRWorldStatusC* status, const MRContactStatusC* cstatus, uint64_t count,
-                         uint32_t environments, mr::AdvanceSummaryDev* s) {
+mr_summary_reduce_kernel(const MRWorldStatusC* status, const MRContactStatusC* cstatus, uint64_t steps,
+                         uint32_t environments, const uint32_t* order, uint32_t order_count,
+                         mr::AdvanceSummaryDev* s) {
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t failed = 0u, pairs = 0u, cons = 0u;
     unsigned long long first = ~0ull;
+    const uint32_t stepped = order != nullptr ? order_count : environments;
+    const uint64_t count = steps * stepped;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t code = status[i].code;
+        const uint64_t step = i / stepped;
+        const uint64_t env = order != nullptr ? order[i % stepped] : i % stepped;
+        const uint64_t rec = step * environments + env;
+        const uint32_t code = status[rec].code;
         if (code != MR_STEP_SUCCESS) {
             ++failed;
-            const unsigned long long step = i / environments, env = i % environments;
             const unsigned long long key = (step << 40) | (env << 8) | (unsigned long long)(code & 0xffu);
             first = key < first ? key : first;
         }
-        const MRContactStatusC& c = cstatus[i];
+        const MRContactStatusC& c = cstatus[rec];
         pairs = c.max_active_pairs > pairs ? c.max_active_pairs : pairs;
         cons = c.max_constraints > cons ? c.max_constraints : cons;
     }
@@ -3366,13 +3407,15 @@ mr_summary_reduce_kernel(const MRWorldStatusC* status, const MRContactStatusC* c
 
 namespace mr {
 
-hipError_t launch_advance_summary(const MRWorldStatusC* status, const MRContactStatusC* cstatus, uint64_t count,
-                                  uint32_t environments, AdvanceSummaryDev* summary, hipStream_t stream) {
+hipError_t launch_advance_summary(const MRWorldStatusC* status, const MRContactStatusC* cstatus, uint64_t steps,
+                                  uint32_t environments, const uint32_t* order, uint32_t order_count,
+                                  AdvanceSummaryDev* summary, hipStream_t stream) {
     hipLaunchKernelGGL(mr_summary_clear_kernel, dim3(1), dim3(64), 0, stream, summary);
+    const uint64_t count = steps * (order != nullptr ? order_count : environments);
     const uint64_t blocks = (count + 255u) / 256u;
     const uint32_t grid = (uint32_t)(blocks < 1024u ? (blocks > 0u ? blocks : 1u) : 1024u);
-    hipLaunchKernelGGL(mr_summary_reduce_kernel, dim3(grid), dim3(256), 0, stream, status, cstatus, count,
-                       environments, summary);
+    hipLaunchKernelGGL(mr_summary_reduce_kernel, dim3(grid), dim3(256), 0, stream, status, cstatus, steps,
+                       environments, order, order_count, summary);
     return hipGetLastError();
 }
 

@@ -15,8 +15,9 @@ hipError_t control_step_occupancy(uint32_t lds_bytes, int* blocks_per_cu);
 hipError_t launch_queue_order(const MRContactStatusC* status, uint32_t environments, uint32_t* order,
                               hipStream_t stream);
 hipError_t configure_control_step(uint32_t lds_bytes, uint32_t mlp_lds_bytes);
-hipError_t launch_advance_summary(const MRWorldStatusC* status, const MRContactStatusC* cstatus, uint64_t count,
-                                  uint32_t environments, AdvanceSummaryDev* summary, hipStream_t stream);
+hipError_t launch_advance_summary(const MRWorldStatusC* status, const MRContactStatusC* cstatus, uint64_t steps,
+                                  uint32_t environments, const uint32_t* order, uint32_t order_count,
+                                  AdvanceSummaryDev* summary, hipStream_t stream);
 
 namespace {
 constexpr uint32_t kSpIBytes = 84u;
@@ -87,6 +88,7 @@ World::World(const ModelData& model, const MRWorldStepConfigC& config) : model_(
     }
     upload(d_bodies_, model_.bodies);
     upload(d_dofs_, model_.dofs);
+    upload(d_actuators_, model_.actuators);
     upload(d_shapes_, model_.shapes);
     upload(d_materials_, model_.materials);
     upload(d_pairs_, model_.pairs);
@@ -275,16 +277,36 @@ void World::compute_lds_layout() {
     if (M.total > kMaxLdsBytes) M.total = 0u;  // the MLP instance is unavailable for this world
 }
 
-size_t World::retained_bytes() const {
-    const DeviceBuffer* all[] = {&d_models_, &d_env_model_, &d_slot_plan_, &d_bodies_, &d_dofs_, &d_shapes_, &d_materials_,
-                                 &d_pairs_, &d_heights_, &d_q_, &d_v_, &d_scene_, &d_mh_, &d_mp_, &d_mcount_,
-                                 &d_mcurrent_, &d_reset_q_, &d_reset_v_, &d_reset_scene_, &d_commands_, &d_masks_,
-                                 &d_status_, &d_cstatus_, &d_accel_, &d_ev_pairs_, &d_ev_counts_, &d_stage_h_,
-                                 &d_stage_p_, &d_jrow_, &d_resp_, &d_cold_, &d_hot_, &d_summary_, &d_samples_};
+namespace {
+template <size_t N>
+size_t sum_bytes(const DeviceBuffer* const (&bufs)[N]) {
     size_t total = 0;
-    for (const DeviceBuffer* b : all) total += b->bytes;
+    for (const DeviceBuffer* b : bufs) total += b->bytes;
     return total;
 }
+} // namespace
+
+size_t World::immutable_bytes() const {
+    const DeviceBuffer* const all[] = {&d_models_, &d_env_model_, &d_slot_plan_, &d_bodies_, &d_dofs_,
+                                       &d_actuators_, &d_shapes_, &d_materials_, &d_pairs_, &d_heights_};
+    return sum_bytes(all);
+}
+
+size_t World::persistent_bytes() const {
+    const DeviceBuffer* const all[] = {&d_q_, &d_v_, &d_scene_, &d_mh_, &d_mp_, &d_mcount_, &d_mcurrent_,
+                                       &d_reset_q_, &d_reset_v_, &d_reset_scene_};
+    return sum_bytes(all);
+}
+
+size_t World::transient_bytes() const {
+    const DeviceBuffer* const all[] = {&d_commands_, &d_masks_, &d_status_, &d_cstatus_, &d_accel_, &d_ev_pairs_,
+                                       &d_ev_counts_, &d_stage_h_, &d_stage_p_, &d_jrow_, &d_resp_, &d_cold_,
+                                       &d_hot_, &d_summary_, &d_samples_, &d_stage_cycles_, &d_queue_,
+                                       &d_queue_head_};
+    return sum_bytes(all);
+}
+
+size_t World::retained_bytes() const { return immutable_bytes() + persistent_bytes() + transient_bytes(); }
 
 void World::set_state(const float* q, const float* v, const MRBodyStateC* scene) {
     hip_check(hipStreamSynchronize(stream_), "sync");
@@ -363,6 +385,7 @@ KernelArgs World::base_args(uint32_t step) const {
     a.environment_model = static_cast<const uint32_t*>(d_env_model_.ptr);
     a.bodies = static_cast<const MRBodyC*>(d_bodies_.ptr);
     a.dofs = static_cast<const MRDofC*>(d_dofs_.ptr);
+    a.actuators = model_.actuators.empty() ? nullptr : static_cast<const MRActuatorProfileC*>(d_actuators_.ptr);
     a.shapes = static_cast<const MRShapeC*>(d_shapes_.ptr);
     a.materials = static_cast<const MRMaterialC*>(d_materials_.ptr);
     a.pairs = static_cast<const MRCompiledPairC*>(d_pairs_.ptr);
@@ -425,6 +448,9 @@ void World::set_warm_start_mlp(const MRMlpWeightsC* weights) {
         throw std::runtime_error("warm-start MLP: the learned warm start runs with the ordered sweeps solver");
     if (lds_mlp_.total == 0u)
         throw std::runtime_error("warm-start MLP: the world's LDS plus the MLP tile exceeds 160 KiB");
+    if (queue_grid_ > 0u)
+        throw std::runtime_error("warm-start MLP: runs one workgroup per environment; clear the work-queue "
+                                 "schedule first (mr_world_set_schedule with grid 0)");
     if (cfg_.residual_tolerance <= 0.0f)
         throw std::runtime_error("warm-start MLP: needs the adaptive solve (residual_tolerance > 0) for its fallback");
     mlp_ = *weights;
@@ -435,6 +461,9 @@ void World::set_mlp_capture(bool enabled) {
     if (enabled) {
         if (lds_mlp_.total == 0u)
             throw std::runtime_error("MLP sample capture: the world's LDS plus the MLP tile exceeds 160 KiB");
+        if (queue_grid_ > 0u)
+            throw std::runtime_error("MLP sample capture: runs one workgroup per environment; clear the work-queue "
+                                     "schedule first (mr_world_set_schedule with grid 0)");
         d_samples_.reserve((size_t)env_ * cfg_.constraint_capacity * kMlpSampleFloats * sizeof(float));
         hip_check(hipStreamSynchronize(stream_), "sync");
         hip_check(hipMemset(d_samples_.ptr, 0, d_samples_.bytes), "hipMemset");
@@ -473,10 +502,15 @@ void World::launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, 
     }
     if (events) hip_check(hipEventRecord(ev_end_, stream_), "event");
     d_summary_.reserve(sizeof(AdvanceSummaryDev));
+    // a subset work queue steps only the listed environments: fold only theirs
+    const bool subset = !mlp_instance() && queue_grid_ > 0u && queue_count_ < env_;
     hip_check(launch_advance_summary(static_cast<const MRWorldStatusC*>(d_status_.ptr),
-                                     static_cast<const MRContactStatusC*>(d_cstatus_.ptr), (uint64_t)steps * E,
-                                     (uint32_t)E, static_cast<AdvanceSummaryDev*>(d_summary_.ptr), stream_),
+                                     static_cast<const MRContactStatusC*>(d_cstatus_.ptr), (uint64_t)steps,
+                                     (uint32_t)E, subset ? static_cast<const uint32_t*>(d_queue_.ptr) : nullptr,
+                                     subset ? queue_count_ : 0u, static_cast<AdvanceSummaryDev*>(d_summary_.ptr),
+                                     stream_),
               "advance summary");
+    last_stepped_ = subset ? queue_count_ : env_;
 }
 
 void World::set_schedule(const uint32_t* order, uint32_t count, uint32_t grid) {
@@ -616,7 +650,7 @@ AdvanceSummary World::wait() {
     s.gpu_milliseconds = last_kernel_ms_;
     AdvanceSummaryDev d;
     hip_check(hipMemcpy(&d, d_summary_.ptr, sizeof(d), hipMemcpyDeviceToHost), "advance summary");
-    const uint64_t n = (uint64_t)last_steps_ * env_;
+    const uint64_t n = (uint64_t)last_steps_ * last_stepped_;
     s.failed_environment_steps = d.failed;
     s.successful_environment_steps = (uint32_t)(n - d.failed);
     if (d.first_failure != ~0ull) {

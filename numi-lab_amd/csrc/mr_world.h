@@ -44,6 +44,12 @@ public:
     const MRWorldStepConfigC& config() const { return cfg_; }
     const LdsLayout& lds() const { return lds_; }
     size_t retained_bytes() const;
+    // retained_bytes split by lifetime: immutable model tables, persistent
+    // state (q, v, scene bodies, manifolds, reset state), transient per-step
+    // scratch and boundary buffers (commands, masks, statuses).
+    size_t immutable_bytes() const;
+    size_t persistent_bytes() const;
+    size_t transient_bytes() const;
     hipStream_t stream() const { return stream_; }
 
     // Host-array state boundary (all environment-major with model strides).
@@ -128,7 +134,7 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_begin_ = nullptr, ev_end_ = nullptr;
     // model
-    DeviceBuffer d_models_, d_env_model_, d_slot_plan_, d_bodies_, d_dofs_, d_shapes_, d_materials_, d_pairs_, d_heights_;
+    DeviceBuffer d_models_, d_env_model_, d_slot_plan_, d_bodies_, d_dofs_, d_actuators_, d_shapes_, d_materials_, d_pairs_, d_heights_;
     // state
     DeviceBuffer d_q_, d_v_, d_scene_, d_mh_, d_mp_, d_mcount_, d_mcurrent_;
     DeviceBuffer d_reset_q_, d_reset_v_, d_reset_scene_;
@@ -138,6 +144,7 @@ private:
     DeviceBuffer d_stage_h_, d_stage_p_, d_jrow_, d_resp_, d_cold_, d_hot_, d_stage_cycles_, d_summary_, d_samples_, d_queue_, d_queue_head_;
     bool stage_profiling_ = false;
     uint32_t last_steps_ = 0;
+    uint32_t last_stepped_ = 0;  // environments each control step of the last advance stepped
     float last_kernel_ms_ = 0.0f;
     bool step_timing_ = false;
     std::vector<hipEvent_t> step_events_;

@@ -9,6 +9,7 @@ from .native import (  # noqa: F401
     MANIFOLD_POINT_DTYPE,
     MR_RUN_SOURCE_SYNTHETIC_CHAIN_BOXES,
     MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN,
+    MR_RUN_SOURCE_SYNTHETIC_RANDOM_TREES,
     MR_SYNTHETIC_CHAIN_BOXES,
     MR_SYNTHETIC_HUMANOID_TERRAIN,
     MR_SOLVER_ORDERED_SWEEPS,

@@ -28,6 +28,9 @@ MR_SYNTHETIC_HUMANOID_TERRAIN: Final = 2
 MR_SYNTHETIC_RANDOM_TREES: Final = 3
 MR_RUN_SOURCE_SYNTHETIC_CHAIN_BOXES: Final = 100
 MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN: Final = 101
+MR_RUN_SOURCE_SYNTHETIC_RANDOM_TREES: Final = 102
+MR_LOCOMOTION_SURFACE_GROUND: Final = 0
+MR_LOCOMOTION_SURFACE_TERRAIN: Final = 1
 
 MR_ACTUATION_EFFORT: Final = 0
 MR_ACTUATION_IMPLICIT_POSITION_DRIVE: Final = 1
@@ -67,6 +70,7 @@ class WorldModelDescC(ct.Structure):
         ("body_total", ct.c_uint32), ("dof_total", ct.c_uint32), ("shape_total", ct.c_uint32),
         ("material_total", ct.c_uint32), ("pair_total", ct.c_uint32), ("heightfield_vertex_count", ct.c_uint32),
         ("heightfield", HeightfieldC), ("heightfield_heights", ct.c_void_p),
+        ("actuators", ct.c_void_p), ("actuator_total", ct.c_uint32), ("reserved_actuators", ct.c_uint32),
     ]
 
 
@@ -113,26 +117,64 @@ class WorldLayoutC(ct.Structure):
     ]
 
 
+class TaskRolloutDynamicSphereC(ct.Structure):
+    """c_api.h:55-61 (the reference's layout; not supported by this build's sources)."""
+    _fields_ = [("position", ct.c_float * 3), ("linear_velocity", ct.c_float * 3), ("radius", ct.c_float),
+                ("mass", ct.c_float), ("launch_step", ct.c_uint32)]
+
+
 class TaskRolloutConfigC(ct.Structure):
+    """c_api.h:62-88, byte for byte."""
     _fields_ = [
         ("environment_count", ct.c_uint32), ("physics_substeps", ct.c_uint32),
         ("velocity_iterations", ct.c_uint32), ("final_velocity_iterations", ct.c_uint32),
-        ("control_timestep_seconds", ct.c_float), ("reserved", ct.c_uint32), ("seed", ct.c_uint64),
+        ("control_timestep_seconds", ct.c_float), ("seed", ct.c_uint64),
+        ("dynamic_spheres", ct.c_void_p), ("dynamic_sphere_count", ct.c_uint32),
+        ("disable_task_terminations", ct.c_uint32), ("interaction_reference_mode", ct.c_uint32),
+        ("interaction_student_authority", ct.c_float), ("override_interaction_student_authority", ct.c_uint32),
+        ("interaction_reset_phase_fraction", ct.c_float), ("override_interaction_reset_phase_fraction", ct.c_uint32),
+        ("materialize_articulated_contact_responses", ct.c_uint32), ("override_difficulty_band_range", ct.c_uint32),
+        ("minimum_difficulty_band", ct.c_uint32), ("maximum_difficulty_band", ct.c_uint32),
+        ("interaction_reset_phase_probability", ct.c_float),
+        ("override_interaction_reset_phase_probability", ct.c_uint32),
+        ("interaction_reset_maximum_phase", ct.c_float), ("override_interaction_reset_maximum_phase", ct.c_uint32),
     ]
 
 
 class RunManifestC(ct.Structure):
-    _fields_ = [("profile", TaskRolloutConfigC), ("source", ct.c_uint32), ("reserved", ct.c_uint32)]
+    """c_api.h:171-190, byte for byte."""
+    _fields_ = [("profile", TaskRolloutConfigC), ("source", ct.c_uint32), ("surface", ct.c_uint32),
+                ("task", ct.c_uint32)] + [(n, ct.c_char_p) for n in (
+                    "urdf_path", "srdf_path", "world_pack_path", "task_pack_path", "robot_actuator_pack_path",
+                    "sensor_program_pack_path", "reality_program_pack_path", "teacher_pack_path",
+                    "teacher_clip_id")] + [("visual_sensor_program", ct.c_void_p),
+                                           ("inspection_visual_program", ct.c_void_p),
+                                           ("metallib_path", ct.c_char_p)]
 
 
 class TaskRolloutLayoutC(ct.Structure):
-    _fields_ = [
-        ("environment_count", ct.c_uint32), ("nq", ct.c_uint32), ("nv", ct.c_uint32),
-        ("action_count", ct.c_uint32), ("scene_body_count", ct.c_uint32), ("reserved", ct.c_uint32),
-        ("submitted_control_steps", ct.c_uint64), ("completed_environment_steps", ct.c_uint64),
-        ("submission_count", ct.c_uint64), ("retained_buffer_bytes", ct.c_size_t),
-        ("total_gpu_milliseconds", ct.c_double),
-    ]
+    """c_api.h:128-158, byte for byte."""
+    _fields_ = [(n, ct.c_uint32) for n in (
+        "environment_count", "nq", "nv", "action_count", "actor_observation_count", "critic_observation_count",
+        "scene_body_count", "motion_feature_count", "maximum_episode_steps")] + [(n, ct.c_uint64) for n in (
+            "world_fingerprint", "task_fingerprint", "observation_fingerprint", "action_fingerprint",
+            "run_fingerprint", "robot_fingerprint", "sensor_fingerprint", "reality_fingerprint",
+            "teacher_fingerprint", "submitted_control_steps", "completed_environment_steps",
+            "submission_count")] + [(n, ct.c_size_t) for n in (
+                "retained_buffer_bytes", "immutable_private_bytes", "persistent_state_private_bytes",
+                "transient_private_bytes", "shared_boundary_bytes", "peak_aliased_bytes")] + [
+        ("total_gpu_milliseconds", ct.c_double), ("total_submission_milliseconds", ct.c_double)]
+
+
+class TaskRolloutStageHighWaterC(ct.Structure):
+    """c_api.h:192-220, byte for byte."""
+    _fields_ = [(n, ct.c_uint32) for n in (
+        "candidate_pairs", "raw_contacts", "manifolds", "constraint_blocks", "constraint_rows", "islands",
+        "hard_convex_pairs", "mesh_triangle_candidates", "solver_tiles", "spill_rows", "ccd_candidates",
+        "ccd_events", "endpoint_runtime_records", "articulation_point_queries", "rod_candidate_pairs",
+        "rod_raw_contacts", "rod_manifolds", "rod_ccd_events", "quality_generalized_velocities", "quality_rows",
+        "quality_krylov_vectors", "quality_direct_tiles", "dynamic_nodes", "island_node_references",
+        "island_constraint_references", "rod_factor_blocks", "operator_velocity_elements")]
 
 
 class MlpWeightsC(ct.Structure):
@@ -155,12 +197,14 @@ class MlpGradsC(ct.Structure):
 
 
 class TaskRolloutAdvanceC(ct.Structure):
+    """c_api.h:222-235, byte for byte."""
     _fields_ = [
         ("control_step_count", ct.c_uint32), ("successful_environment_steps", ct.c_uint32),
         ("failed_environment_steps", ct.c_uint32), ("first_failing_environment", ct.c_uint32),
         ("first_failing_control_step", ct.c_uint32), ("first_gpu_status_code", ct.c_uint32),
         ("host_requested_resets", ct.c_uint32), ("maximum_active_contacts", ct.c_uint32),
-        ("gpu_milliseconds", ct.c_double),
+        ("maximum_manifolds", ct.c_uint32), ("high_water", TaskRolloutStageHighWaterC),
+        ("gpu_milliseconds", ct.c_double), ("submission_milliseconds", ct.c_double),
     ]
 
 
@@ -203,20 +247,30 @@ CONTACT_STATUS_DTYPE: Final = np.dtype([(n, "<u4") for n in (
     [("residuals", *F4), ("diagnostics", *F4)])
 PAIR_RECORD_DTYPE: Final = np.dtype([("environment", "<u4"), ("collider_a", "<u4"), ("collider_b", "<u4"),
                                      ("pair_class", "<u4")])
+# MRActuatorProfileC (engine_types.h MRActuatorProfileGPU layout)
+ACTUATOR_DTYPE: Final = np.dtype([("motor_and_speed", *F4), ("transmission_and_envelope", *F4),
+                                  ("identity", "<u4", (4,))])
+MR_ACTUATOR_PROFILE_ACTIVE: Final = 1
+MR_ACTUATOR_PROFILE_CALIBRATED: Final = 2
 
 _EXPECTED_SIZES: Final = {
-    MODEL_HEADER_DTYPE: 64, BODY_DTYPE: 256, DOF_DTYPE: 48, SHAPE_DTYPE: 96, MATERIAL_DTYPE: 32, PAIR_DTYPE: 16,
+    MODEL_HEADER_DTYPE: 64, BODY_DTYPE: 256, DOF_DTYPE: 48, ACTUATOR_DTYPE: 48, SHAPE_DTYPE: 96, MATERIAL_DTYPE: 32, PAIR_DTYPE: 16,
     BODY_STATE_DTYPE: 128, MANIFOLD_HEADER_DTYPE: 64, MANIFOLD_POINT_DTYPE: 64, WORLD_STATUS_DTYPE: 48,
     CONTACT_STATUS_DTYPE: 96, PAIR_RECORD_DTYPE: 16,
 }
 for _dt, _size in _EXPECTED_SIZES.items():
     assert _dt.itemsize == _size, (_dt, _dt.itemsize, _size)
 assert ct.sizeof(WorldStepConfigC) == 88
-assert ct.sizeof(WorldModelDescC) == 200
-assert ct.sizeof(TaskRolloutLayoutC) == 64
+assert ct.sizeof(WorldModelDescC) == 216
 assert ct.sizeof(WorldAdvanceC) == 40
 assert ct.sizeof(WorldLayoutC) == 40
-assert ct.sizeof(RunManifestC) == 40
+# the reference's task-rollout records (c_api.h), LP64
+assert ct.sizeof(TaskRolloutDynamicSphereC) == 36
+assert ct.sizeof(TaskRolloutConfigC) == 104
+assert ct.sizeof(RunManifestC) == 216 and RunManifestC.source.offset == 104
+assert ct.sizeof(TaskRolloutLayoutC) == 200
+assert ct.sizeof(TaskRolloutStageHighWaterC) == 108
+assert ct.sizeof(TaskRolloutAdvanceC) == 160 and TaskRolloutAdvanceC.gpu_milliseconds.offset == 144
 
 
 def _ptr(arr: Optional[np.ndarray]) -> Optional[int]:
@@ -431,6 +485,10 @@ class WorldModel:
     def dofs(self) -> np.ndarray:
         return self._view(self.desc.dofs, DOF_DTYPE, self.desc.dof_total)
 
+    def actuators(self) -> np.ndarray:
+        """Per-DoF actuator profiles (empty when every DoF is inactive)."""
+        return self._view(self.desc.actuators, ACTUATOR_DTYPE, self.desc.actuator_total)
+
     def shapes(self) -> np.ndarray:
         return self._view(self.desc.shapes, SHAPE_DTYPE, self.desc.shape_total)
 
@@ -545,16 +603,29 @@ class World:
         self.last_steps = steps
         return out
 
+    def _device_commands(self, commands) -> int:
+        """Checks a device command tensor [steps][E][max_nv] (contiguous float32 on the GPU): the
+        kernel reads steps x E x max_nv floats from its address, so nothing smaller may pass."""
+        import torch
+
+        steps = int(commands.shape[0]) if commands.dim() > 0 else 0
+        expect = (steps, self.environment_count, self.model.max_nv)
+        if (steps == 0 or tuple(commands.shape) != expect or commands.dtype != torch.float32
+                or not commands.is_contiguous() or not commands.is_cuda):
+            raise ValueError(f"device commands must be a contiguous float32 GPU tensor of shape {expect}")
+        return steps
+
     def enqueue_device(self, commands, reset_masks=None) -> None:
         """Enqueue with inputs already in device memory (torch tensors on the GPU)."""
-        steps = int(commands.shape[0])
-        expect = (steps, self.environment_count, self.model.max_nv)
-        if tuple(commands.shape) != expect or not commands.is_contiguous() or not commands.is_cuda:
-            raise ValueError(f"device commands must be a contiguous GPU tensor of shape {expect}")
+        import torch
+
+        steps = self._device_commands(commands)
         mask_ptr = None
         if reset_masks is not None:
-            if tuple(reset_masks.shape) != (steps, self.environment_count) or not reset_masks.is_cuda:
-                raise ValueError("device reset masks must be a GPU tensor [steps][environments]")
+            if (tuple(reset_masks.shape) != (steps, self.environment_count) or not reset_masks.is_cuda
+                    or not reset_masks.is_contiguous() or reset_masks.dtype not in (torch.int32, torch.uint32)):
+                raise ValueError("device reset masks must be a contiguous int32/uint32 GPU tensor "
+                                 "[steps][environments]")
             mask_ptr = reset_masks.data_ptr()
         self._b.check(self._b.lib.mr_world_enqueue_device(self._h, commands.data_ptr(), mask_ptr, steps))
         self.last_steps = steps
@@ -619,9 +690,9 @@ class World:
     def enqueue_graph(self, commands) -> None:
         """Graph-captured horizon over device commands [steps][E][max_nv] (a torch tensor kept at
         one address between calls); wait() afterwards."""
-        steps = int(commands.shape[0])
-        self.last_steps = steps
+        steps = self._device_commands(commands)
         self._b.check(self._b.lib.mr_world_enqueue_graph(self._h, commands.data_ptr(), steps))
+        self.last_steps = steps
 
     def set_warm_start_mlp(self, mlp) -> None:
         """Warm-start later advances from `mlp` (a metalrobo.warmstart.WarmStartMLP, kept alive
@@ -668,9 +739,15 @@ class TaskRollout:
                  control_timestep_seconds: float = 0.0, library: str | os.PathLike[str] | None = None) -> None:
         self._b = load(library)
         manifest = RunManifestC()
-        manifest.profile = TaskRolloutConfigC(environment_count, physics_substeps, velocity_iterations,
-                                              final_velocity_iterations, control_timestep_seconds, 0, seed)
+        manifest.profile.environment_count = environment_count
+        manifest.profile.physics_substeps = physics_substeps
+        manifest.profile.velocity_iterations = velocity_iterations
+        manifest.profile.final_velocity_iterations = final_velocity_iterations
+        manifest.profile.control_timestep_seconds = control_timestep_seconds
+        manifest.profile.seed = seed
         manifest.source = source
+        manifest.surface = MR_LOCOMOTION_SURFACE_TERRAIN if source == MR_RUN_SOURCE_SYNTHETIC_HUMANOID_TERRAIN \
+            else MR_LOCOMOTION_SURFACE_GROUND
         h = self._b.lib.mr_create_task_rollout(ct.byref(manifest))
         if not h:
             raise MetalRoboError(self._b.last_error())

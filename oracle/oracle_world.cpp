@@ -36,6 +36,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <thread>
 #include <vector>
 
@@ -145,6 +146,7 @@ struct EnvModel {
     const MRModelHeaderC* header;
     const MRBodyC* bodies;
     const MRDofC* dofs;
+    const MRActuatorProfileC* actuators;  // [nv] or null
     const MRShapeC* shapes;
     const MRMaterialC* materials;
     const MRCompiledPairC* pairs;
@@ -158,6 +160,7 @@ EnvModel env_model(const MRWorldModelDescC* desc, uint32_t env) {
     m.header = &desc->models[desc->environment_model[env]];
     m.bodies = desc->bodies + m.header->body_offset;
     m.dofs = desc->dofs + m.header->dof_offset;
+    m.actuators = desc->actuators != nullptr ? desc->actuators + m.header->dof_offset : nullptr;
     m.shapes = desc->shapes + m.header->shape_offset;
     m.materials = desc->materials + m.header->material_offset;
     m.pairs = desc->pairs + m.header->pair_offset;
@@ -270,14 +273,35 @@ struct Failure {
 
 // ------------------------------------------------------------------ drives
 
-// MetalWorld.metal:280-366 (implicit position drive refreshed from current q/v).
+// MetalWorld.metal:252-269 / 345-365: the actuator torque-speed envelope
+//   min(limits.w, stall * efficiency * (1 - clamp(|v| / no_load, 0, 1))),
+// with an inactive profile executing as no-load FLT_MAX, efficiency 1 and
+// stall = the effort limit (FLT_MAX without one), MetalWorld.mm:1376-1410.
+float actuator_envelope(const MRDofC& dof, const MRActuatorProfileC* act, float vel) {
+    const float big = std::numeric_limits<float>::max();
+    float no_load = big, efficiency = 1.0f;
+    float stall = ((dof.flags & MR_DOF_FLAG_EFFORT_LIMIT) != 0u && dof.limits.w > 0.0f) ? dof.limits.w : big;
+    if (act != nullptr && (act->identity[1] & MR_ACTUATOR_PROFILE_ACTIVE) != 0u) {
+        no_load = act->motor_and_speed.z;
+        efficiency = act->motor_and_speed.w;
+        stall = act->transmission_and_envelope.z;
+    }
+    const float speed_fraction = clampd(std::fabs(vel) / fmaxd(no_load, std::numeric_limits<float>::min()), 0.0f, 1.0f);
+    return fmind(dof.limits.w, stall * efficiency * (1.0f - speed_fraction));
+}
+
+// MetalWorld.metal:184-270 (prepare: effort commands, clamped at the control
+// step's start velocity and held for its microsteps) and 280-366 (implicit
+// position drive refreshed from the current q/v every microstep); every
+// command is clamped to the actuator envelope.
 void refresh_drives(const StepContext& ctx, const float* q, const float* v,
                     const float* command, Work& wk) {
     const uint32_t nv = ctx.m.header->nv;
     for (uint32_t j = 0; j < nv; ++j) {
         const MRDofC& dof = ctx.m.dofs[j];
+        const float envelope = actuator_envelope(dof, ctx.m.actuators != nullptr ? &ctx.m.actuators[j] : nullptr, v[j]);
         if (ctx.cfg->actuation_mode == MR_ACTUATION_EFFORT) {
-            wk.effort[j] = command[j];
+            wk.effort[j] = clampd(command[j], -envelope, envelope);
             continue;
         }
         float cmd = 0.0f;
@@ -297,7 +321,7 @@ void refresh_drives(const StepContext& ctx, const float* q, const float* v,
             if ((dof.flags & MR_DOF_FLAG_EFFORT_LIMIT) != 0u && dof.limits.w > 0.0f)
                 cmd = clampd(cmd, -dof.limits.w, dof.limits.w);
         }
-        wk.effort[j] = cmd;
+        wk.effort[j] = clampd(cmd, -envelope, envelope);
     }
 }
 
@@ -1687,7 +1711,7 @@ struct MicroOutputs {
     uint32_t* pair_count;
 };
 
-bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool final_sub,
+bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool first_sub, bool final_sub,
                bool capture, MicroOutputs& out, Work& wk, Failure& f, std::vector<Constraint>& cons,
                std::vector<float>& jrow, std::vector<float>& resp) {
     const MRModelHeaderC& hd = *ctx.m.header;
@@ -1696,8 +1720,10 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
     const float h = ctx.h;
     MRContactStatusC& cs = *out.cstatus;
 
-    // 1. drives from the current state
-    refresh_drives(ctx, st.q, st.v, command, wk);
+    // 1. drives: implicit drives from the current state every microstep;
+    // effort commands once per control step (held in wk.effort)
+    if (first_sub || ctx.cfg->actuation_mode == MR_ACTUATION_IMPLICIT_POSITION_DRIVE)
+        refresh_drives(ctx, st.q, st.v, command, wk);
     // 2. kinematics + ABA candidate velocity
     if (!forward_kinematics(ctx, st.q, st.v, wk, f)) return false;
     if (!aba(ctx, st.v, wk, f)) return false;
@@ -2487,7 +2513,7 @@ int mr_oracle_control_step(const MRWorldModelDescC* desc, const MRWorldStepConfi
         Failure fail;
         for (uint32_t sub = 0; sub < cfg->physics_substeps; ++sub) {
             const bool capture = cfg->capture_evidence != 0u && sub == cfg->capture_substep;
-            if (!microstep(ctx, st, cmd, sub + 1u == cfg->physics_substeps, capture, mo, *wk, fail, cons,
+            if (!microstep(ctx, st, cmd, sub == 0u, sub + 1u == cfg->physics_substeps, capture, mo, *wk, fail, cons,
                            jrow, resp)) {
                 ws_out.failing_substep = sub;
                 ws_out.failing_index = fail.index;

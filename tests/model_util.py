@@ -61,6 +61,16 @@ def dof(body_index, q_index, flags=0, limits=(0, 0, 0, 0), drive=(0, 0, 0, 0)):
     return d
 
 
+def actuator(local_v, torque_constant=0.0, current_limit=0.0, no_load_speed=0.0, efficiency=0.0, active=True):
+    """MRActuatorProfileC: stall torque = torque constant x current limit."""
+    a = np.zeros(1, nat.ACTUATOR_DTYPE)[0]
+    if active:
+        a["motor_and_speed"] = [torque_constant, current_limit, no_load_speed, efficiency]
+        a["transmission_and_envelope"] = [0, 0, np.float32(torque_constant) * np.float32(current_limit), 0]
+    a["identity"] = [local_v, nat.MR_ACTUATOR_PROFILE_ACTIVE if active else 0, 0, 0]
+    return a
+
+
 def pair_class(a, b):
     lo, hi = min(a, b), max(a, b)
     table = {(0, 0): 1, (0, 3): 2, (1, 3): 3, (2, 3): 4, (0, 1): 6, (1, 1): 7, (0, 2): 8, (1, 2): 9, (2, 2): 10}
@@ -113,7 +123,7 @@ class CustomWorld:
 
     def __init__(self, bodies, dofs, shapes, link_count, root_type, nq, nv, materials=None, envs=1,
                  gravity=(0, 0, -9.81), microstep=0.005, exclusion_distance=1, heights=None, hf_origin=None,
-                 hf_spacing=None):
+                 hf_spacing=None, actuators=None):
         self.bodies = np.array(bodies, nat.BODY_DTYPE)
         self.dofs = np.array(dofs, nat.DOF_DTYPE) if len(dofs) else np.zeros(1, nat.DOF_DTYPE)
         self.shapes = np.array(shapes, nat.SHAPE_DTYPE) if len(shapes) else np.zeros(1, nat.SHAPE_DTYPE)
@@ -172,6 +182,11 @@ class CustomWorld:
             desc.heightfield.bounds_lower = nat._F4(*lo, 0)
             desc.heightfield.bounds_upper = nat._F4(*hi, 0)
             desc.heightfield_heights = self.heights.ctypes.data
+        if actuators is not None:
+            self.actuators = np.array(actuators, nat.ACTUATOR_DTYPE)
+            desc.actuators = self.actuators.ctypes.data
+            desc.actuator_total = len(self.actuators)
+        self._desc = desc  # kept (with the arrays it points into) for tests that alter a copy
         b = mr.load()
         h = b.lib.mr_world_model_create(ct.byref(desc), microstep)
         if not h:

@@ -118,6 +118,135 @@ def test_model_validation_rejects_malformed_trees():
 def test_task_rollout_rejects_unknown_source():
     with pytest.raises(mr.MetalRoboError, match="unsupported run source"):
         mr.TaskRollout(7, 4)
+    with pytest.raises(mr.MetalRoboError, match="robot-factory"):
+        mr.TaskRollout(0, 4)  # MR_RUN_SOURCE_UNITREE_G1: not built here
+
+
+def _golden_layout():
+    import json
+
+    return json.load(open(os.path.join(ROOT, "tests", "golden", "task_rollout_layout.json")))["records"]
+
+
+def test_task_rollout_records_have_the_reference_layout():
+    """Every field of the task-rollout records sits at the reference header's
+    offset (fixture written from include/metalrobo/c_api.h by
+    tests/golden/task_rollout_layout.py), in C and in the ctypes mirrors."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("trl", os.path.join(ROOT, "tests", "golden", "task_rollout_layout.py"))
+    trl = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(trl)
+    golden = _golden_layout()
+    fields = trl.record_fields(os.path.join(ROOT, "include", "mr_c_api.h"))
+    assert set(golden) == set(fields)
+    for name, fs in fields.items():
+        assert set(golden[name]) == set(fs) | {"size"}, name
+    ours = trl.layout(os.path.join(ROOT, "include"), "mr_c_api.h", fields)
+    assert ours == golden
+    mirrors = {"MRTaskRolloutDynamicSphereC": nat.TaskRolloutDynamicSphereC,
+               "MRTaskRolloutConfigC": nat.TaskRolloutConfigC, "MRTaskRolloutLayoutC": nat.TaskRolloutLayoutC,
+               "MRRunManifestC": nat.RunManifestC, "MRTaskRolloutStageHighWaterC": nat.TaskRolloutStageHighWaterC,
+               "MRTaskRolloutAdvanceC": nat.TaskRolloutAdvanceC}
+    for name, cls in mirrors.items():
+        assert ct.sizeof(cls) == golden[name]["size"], name
+        for f in fields[name]:
+            assert getattr(cls, f).offset == golden[name][f], (name, f)
+
+
+@pytest.mark.skipif(not os.path.isfile("/root/reference/include/metalrobo/c_api.h"),
+                    reason="the reference header is not present here")
+def test_task_rollout_layout_fixture_matches_the_reference_header():
+    """Where the reference is present: its header declares the same fields in
+    the same order, and the compiler lays them out as the fixture says."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("trl", os.path.join(ROOT, "tests", "golden", "task_rollout_layout.py"))
+    trl = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(trl)
+    ref_fields = trl.record_fields("/root/reference/include/metalrobo/c_api.h")
+    assert ref_fields == trl.record_fields(os.path.join(ROOT, "include", "mr_c_api.h"))
+    assert trl.layout("/root/reference/include", "metalrobo/c_api.h", ref_fields) == _golden_layout()
+
+
+def test_task_rollout_manifest_fields_without_meaning_are_rejected():
+    """Fields of the reference manifest that name task programs, packs or
+    visual sensors must be zero here; a non-zero value is an error, not
+    silently ignored."""
+    b = mr.load()
+
+    def manifest(**changes):
+        m = nat.RunManifestC()
+        m.profile.environment_count = 2
+        m.source = nat.MR_RUN_SOURCE_SYNTHETIC_CHAIN_BOXES
+        for k, v in changes.items():
+            if k.startswith("profile_"):
+                setattr(m.profile, k[len("profile_"):], v)
+            else:
+                setattr(m, k, v)
+        return m
+
+    for changes, msg in [({"task": 2}, "task program"), ({"urdf_path": b"robot.urdf"}, "pack paths"),
+                         ({"surface": 1}, "surface"), ({"profile_dynamic_sphere_count": 1}, "dynamic_spheres"),
+                         ({"profile_interaction_reference_mode": 1}, "interaction"),
+                         ({"profile_environment_count": 0}, "environment_count")]:
+        m = manifest(**changes)
+        assert not b.lib.mr_create_task_rollout(ct.byref(m)), changes
+        assert msg in b.last_error(), (changes, b.last_error())
+    out = nat.TaskRolloutAdvanceC()
+    assert b.lib.mr_task_rollout_advance(None, None, 0, None, 0, 1, 0, 0, None) == -1
+    assert "result is null" in b.last_error()
+    assert b.lib.mr_task_rollout_advance(None, None, 0, None, 0, 1, 0, 0, ct.byref(out)) == -1
+    assert out.first_failing_environment == 0xFFFFFFFF and out.first_failing_control_step == 0xFFFFFFFF
+
+
+def test_malformed_descriptors_are_rejected_before_use():
+    """mr_world_model_create validates the descriptor before reading through
+    it: out-of-range model indices, under-stated strides, short heightfields
+    and pair classes that contradict their shapes are errors."""
+    import model_util as mu
+
+    def world(**kw):
+        bodies = [mu.body(1.0, [0.01] * 3), mu.body(1.0, [0.01] * 3), mu.body(0.0, [1, 1, 1], motion=0)]
+        shapes = [mu.shape(1, 0, [0.05]), mu.shape(2, 3, [0, 0, 0], rot=mu.Y_TO_Z)]
+        return mu.CustomWorld(bodies, [], shapes, link_count=1, root_type=0, nq=0, nv=0, **kw)
+
+    w = world()
+    b = mr.load()
+
+    def try_create(mutate):
+        desc = nat.WorldModelDescC()
+        ct.memmove(ct.byref(desc), ct.byref(w._desc), ct.sizeof(desc))
+        keep = mutate(desc)
+        h = b.lib.mr_world_model_create(ct.byref(desc), 0.005)
+        del keep
+        return h
+
+    env_model = np.array([5], np.uint32)
+    assert not try_create(lambda d: (setattr(d, "environment_model", env_model.ctypes.data), env_model))
+    assert "environment model index" in b.last_error()
+    pairs = w.pairs.copy()
+    pairs["pair_class"] = 12  # heightfield class on a sphere-plane pair
+    assert not try_create(lambda d: (setattr(d, "pairs", pairs.ctypes.data), pairs))
+    assert "pair class" in b.last_error()
+    hdr = w.header.copy()
+    hdr["body_count"] = 40
+    hdr["scene_body_count"] = 39
+    assert not try_create(lambda d: (setattr(d, "models", hdr.ctypes.data), hdr))
+    assert "offsets outside" in b.last_error()
+    # an under-stated stride is raised to the model's need, never read past
+    h = try_create(lambda d: (setattr(d, "max_scene_bodies", 0), None))
+    assert h
+    b.lib.mr_world_model_destroy(h)
+    heights = np.zeros((4, 4), np.float32)
+    hf = mu.CustomWorld([mu.body(1.0, [0.01] * 3), mu.body(1.0, [0.01] * 3), mu.body(0.0, [1, 1, 1], motion=0)], [],
+                        [mu.shape(1, 0, [0.05]), mu.shape(2, 7, [0, 0, 0])], link_count=1, root_type=0, nq=0, nv=0,
+                        heights=heights, hf_origin=(0, 0), hf_spacing=0.1)
+    desc = nat.WorldModelDescC()
+    ct.memmove(ct.byref(desc), ct.byref(hf._desc), ct.sizeof(desc))
+    desc.heightfield_vertex_count = 8  # fewer heights than nx * ny
+    assert not b.lib.mr_world_model_create(ct.byref(desc), 0.005)
+    assert "heightfield" in b.last_error()
 
 
 def test_synthetic_models_have_the_documented_shape():

@@ -54,12 +54,21 @@ def test_subset_queue_steps_only_the_listed_worlds(gpu):
     listed = np.arange(0, model.environment_count, 3)
     part.set_schedule(listed[::-1], grid=8)
     before = snapshot(part)
-    part.advance(cmds)
+    out = part.advance(cmds)
     after, stepped = snapshot(part), snapshot(full)
     others = np.setdiff1d(np.arange(model.environment_count), listed)
     for k in range(3):  # q, v, scene
         np.testing.assert_array_equal(after[k][listed], stepped[k][listed])
         np.testing.assert_array_equal(after[k][others], before[k][others])
+    # the advance summary counts the listed worlds only
+    codes = part.statuses()["code"][:, listed]
+    cs = part.contact_statuses()[:, listed]
+    assert out.successful_environment_steps + out.failed_environment_steps == len(cmds) * len(listed)
+    assert out.failed_environment_steps == int(np.sum(codes != 0))
+    assert out.maximum_active_pairs == int(cs["max_active_pairs"].max())
+    assert out.maximum_constraints == int(cs["max_constraints"].max())
+    if out.failed_environment_steps:
+        assert out.first_failing_environment in set(listed.tolist())
 
 
 def test_graph_horizon_replays_match_eager(gpu):
@@ -146,3 +155,13 @@ def test_reorder_needs_a_full_work_queue(gpu):
     w.set_schedule(np.arange(8), grid=4)
     with pytest.raises(mr.MetalRoboError, match="every environment"):
         w.set_schedule_reorder(True)
+
+
+def test_learned_warm_start_capture_refuses_a_work_queue(gpu):
+    """The MLP instance runs one workgroup per environment; a world with a work-queue
+    schedule must refuse it rather than step the environments its order left out."""
+    model, cfg = trees(envs=16)
+    w = mr.World(model, cfg)
+    w.set_schedule(np.arange(8), grid=4)
+    with pytest.raises(mr.MetalRoboError, match="work-queue"):
+        w.set_mlp_capture(True)
