@@ -128,6 +128,26 @@ int mr_world_advance(MRWorldHandle* world, const float* commands, size_t command
 int mr_world_enqueue_device(MRWorldHandle* world, const float* device_commands, const uint32_t* device_reset_masks,
                             uint32_t control_step_count);
 int mr_world_wait(MRWorldHandle* world, MRWorldAdvanceC* out);
+/* Largest per-step scratch, in bytes, that mr_world_enqueue_device_static can serve. */
+#define MR_WORLD_STATIC_SCRATCH_BYTES ((size_t)64u << 20)
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mr_world_enqueue_device that calls no hipMalloc, hipFree
+ * or allocating library routine. The per-step scratch of the control-step
+ * kernel (staged manifolds, Jacobian and response rows, constraint records)
+ * and the advance summary live in one library-owned device array of
+ * MR_WORLD_STATIC_SCRATCH_BYTES per GPU, held until the process ends, instead
+ * of the world's own scratch; a world whose scratch exceeds it is refused
+ * with an error. The statuses go to the caller's device buffers:
+ * device_status [steps][E] MRWorldStatusC and device_contact_status [steps][E]
+ * MRContactStatusC, which mr_world_read_statuses / _contact_statuses then read.
+ * Call with the world's device current. Every call on one GPU shares that
+ * GPU's array, which keeps the last call's contents: calls that overlap on one
+ * GPU (another world's stream included) race, so order them (mr_world_wait).
+ * Per-launch step timing must be off. Wait with mr_world_wait. */
+int mr_world_enqueue_device_static(MRWorldHandle* world, const float* device_commands,
+                                   const uint32_t* device_reset_masks, uint32_t control_step_count,
+                                   MRWorldStatusC* device_status, MRContactStatusC* device_contact_status);
 /* Launch schedule (no counterpart in the reference, whose encoder dispatches
  * one threadgroup per environment; for heterogeneous batches and the step
  * server). grid = 0 restores one workgroup per environment. grid > 0 runs the

@@ -275,6 +275,16 @@ int mr_world_enqueue_device(MRWorldHandle* world, const float* device_commands, 
     });
 }
 
+int mr_world_enqueue_device_static(MRWorldHandle* world, const float* device_commands,
+                                   const uint32_t* device_reset_masks, uint32_t control_step_count,
+                                   MRWorldStatusC* device_status, MRContactStatusC* device_contact_status) {
+    return translate([&]() {
+        require(world != nullptr && device_commands != nullptr, "null argument");
+        world->world->enqueue_static(device_commands, device_reset_masks, control_step_count, device_status,
+                                     device_contact_status);
+    });
+}
+
 int mr_world_set_schedule(MRWorldHandle* world, const uint32_t* order, size_t count, uint32_t grid) {
     return translate([&]() {
         require(world != nullptr, "null argument");

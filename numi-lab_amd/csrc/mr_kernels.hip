@@ -31,6 +31,7 @@
 
 #include "mr_device_math.h"
 #include "mr_kernel_args.h"
+#include "../../include/mr_c_api.h"
 #include "mr_mlp_tile.h"
 
 #include <mutex>
@@ -3437,6 +3438,36 @@ hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipSt
     else
         hipLaunchKernelGGL(mr_control_step_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
     return hipGetLastError();
+}
+
+// Scratch of the allocation-free entry point (mr_world_enqueue_device_static):
+// one file-scope device array, which every GPU the library runs on holds at
+// its own address until the process ends. Host code never passes the array's
+// name to a kernel (on the host it is a stand-in, not a device address); it
+// passes the address static_scratch() looked up for the current device.
+static __device__ __align__(256) unsigned char scratch[MR_WORLD_STATIC_SCRATCH_BYTES];
+
+// Device address of `scratch` on the current device: looked up once per
+// device with hipGetSymbolAddress, then served from the cache.
+hipError_t static_scratch(void** out, size_t* bytes) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex guard;
+    static void* address[kMaxDevices] = {};
+    int device = -1;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(guard);
+    if (address[device] == nullptr) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (ptr == nullptr) return hipErrorInvalidSymbol;
+        address[device] = ptr;
+    }
+    *out = address[device];
+    *bytes = MR_WORLD_STATIC_SCRATCH_BYTES;
+    return hipSuccess;
 }
 
 hipError_t launch_queue_order(const MRContactStatusC* status, uint32_t environments, uint32_t* order,

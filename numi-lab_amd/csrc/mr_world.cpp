@@ -18,6 +18,7 @@ hipError_t configure_control_step(uint32_t lds_bytes, uint32_t mlp_lds_bytes);
 hipError_t launch_advance_summary(const MRWorldStatusC* status, const MRContactStatusC* cstatus, uint64_t steps,
                                   uint32_t environments, const uint32_t* order, uint32_t order_count,
                                   AdvanceSummaryDev* summary, hipStream_t stream);
+hipError_t static_scratch(void** out, size_t* bytes);
 
 namespace {
 constexpr uint32_t kSpIBytes = 84u;
@@ -68,6 +69,7 @@ World::World(const ModelData& model, const MRWorldStepConfigC& config) : model_(
         throw std::runtime_error("capture_substep must be below physics_substeps");
     finalize_model(model_, cfg_.control_timestep / (float)cfg_.physics_substeps);
     env_ = model_.environment_count();
+    hip_check(hipGetDevice(&device_), "hipGetDevice");
     compute_lds_layout();
     hip_check(configure_control_step(lds_.total, lds_mlp_.total), "hipFuncSetAttribute");
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
@@ -479,19 +481,37 @@ void World::read_mlp_samples(float* out) const {
               "hipMemcpy samples");
 }
 
-void World::launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, bool events) {
+World::StepBuffers World::own_buffers() {
+    StepBuffers b;
+    b.stage_h = static_cast<MRManifoldHeaderC*>(d_stage_h_.ptr);
+    b.stage_p = static_cast<MRManifoldPointC*>(d_stage_p_.ptr);
+    b.jrow = static_cast<float*>(d_jrow_.ptr);
+    b.resp = static_cast<float*>(d_resp_.ptr);
+    b.cold = static_cast<ColdConstraint*>(d_cold_.ptr);
+    b.hot = static_cast<HotConstraint*>(d_hot_.ptr);
+    b.status = static_cast<MRWorldStatusC*>(d_status_.ptr);
+    b.cstatus = static_cast<MRContactStatusC*>(d_cstatus_.ptr);
+    b.summary = static_cast<AdvanceSummaryDev*>(d_summary_.ptr);
+    return b;
+}
+
+void World::launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, bool events, const StepBuffers& buf) {
     const size_t E = env_;
     const size_t per_step = E * model_.max_nv;
-    d_status_.reserve((size_t)steps * E * sizeof(MRWorldStatusC));
-    d_cstatus_.reserve((size_t)steps * E * sizeof(MRContactStatusC));
     if (events) hip_check(hipEventRecord(ev_begin_, stream_), "event");
     if (events && step_timing_) hip_check(hipEventRecord(step_events_[0], stream_), "event");
     for (uint32_t s = 0; s < steps; ++s) {
         KernelArgs a = base_args(s);
+        a.stage_h = buf.stage_h;
+        a.stage_p = buf.stage_p;
+        a.jrow = buf.jrow;
+        a.resp = buf.resp;
+        a.cold = buf.cold;
+        a.hot = buf.hot;
         a.command = cmd + (size_t)s * per_step;
         a.reset_mask = msk != nullptr ? msk + (size_t)s * E : nullptr;
-        a.status = static_cast<MRWorldStatusC*>(d_status_.ptr) + (size_t)s * E;
-        a.cstatus = static_cast<MRContactStatusC*>(d_cstatus_.ptr) + (size_t)s * E;
+        a.status = buf.status + (size_t)s * E;
+        a.cstatus = buf.cstatus + (size_t)s * E;
         hip_check(launch_control_step(a, mlp_instance() ? lds_mlp_.total : lds_.total, stream_, mlp_instance(),
                                       mlp_instance() ? 0u : queue_grid_),
                   "control step launch");
@@ -501,16 +521,16 @@ void World::launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, 
         if (events && step_timing_) hip_check(hipEventRecord(step_events_[s + 1u], stream_), "event");
     }
     if (events) hip_check(hipEventRecord(ev_end_, stream_), "event");
-    d_summary_.reserve(sizeof(AdvanceSummaryDev));
     // a subset work queue steps only the listed environments: fold only theirs
     const bool subset = !mlp_instance() && queue_grid_ > 0u && queue_count_ < env_;
-    hip_check(launch_advance_summary(static_cast<const MRWorldStatusC*>(d_status_.ptr),
-                                     static_cast<const MRContactStatusC*>(d_cstatus_.ptr), (uint64_t)steps,
-                                     (uint32_t)E, subset ? static_cast<const uint32_t*>(d_queue_.ptr) : nullptr,
-                                     subset ? queue_count_ : 0u, static_cast<AdvanceSummaryDev*>(d_summary_.ptr),
-                                     stream_),
+    hip_check(launch_advance_summary(buf.status, buf.cstatus, (uint64_t)steps, (uint32_t)E,
+                                     subset ? static_cast<const uint32_t*>(d_queue_.ptr) : nullptr,
+                                     subset ? queue_count_ : 0u, buf.summary, stream_),
               "advance summary");
     last_stepped_ = subset ? queue_count_ : env_;
+    last_status_ = buf.status;
+    last_cstatus_ = buf.cstatus;
+    last_summary_ = buf.summary;
 }
 
 void World::set_schedule(const uint32_t* order, uint32_t count, uint32_t grid) {
@@ -582,7 +602,7 @@ void World::enqueue_graph(const float* device_commands, uint32_t steps) {
         drop_graph();
         hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
         try {
-            launch_steps(device_commands, nullptr, steps, false);
+            launch_steps(device_commands, nullptr, steps, false, own_buffers());
         } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(stream_, &g);
@@ -603,6 +623,10 @@ void World::enqueue_graph(const float* device_commands, uint32_t steps) {
     hip_check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
     hip_check(hipEventRecord(ev_end_, stream_), "event");
     last_steps_ = steps;
+    last_stepped_ = (!mlp_instance() && queue_grid_ > 0u && queue_count_ < env_) ? queue_count_ : env_;
+    last_status_ = static_cast<const MRWorldStatusC*>(d_status_.ptr);
+    last_cstatus_ = static_cast<const MRContactStatusC*>(d_cstatus_.ptr);
+    last_summary_ = static_cast<const AdvanceSummaryDev*>(d_summary_.ptr);
 }
 
 void World::enqueue(const float* commands, const uint32_t* masks, uint32_t steps, bool device_inputs) {
@@ -638,7 +662,65 @@ void World::enqueue(const float* commands, const uint32_t* masks, uint32_t steps
             step_events_.push_back(ev);
         }
     }
-    launch_steps(cmd, msk, steps, true);
+    d_summary_.reserve(sizeof(AdvanceSummaryDev));
+    launch_steps(cmd, msk, steps, true, own_buffers());
+    last_steps_ = steps;
+}
+
+namespace {
+constexpr size_t kStaticAlign = 256u;
+size_t align_up(size_t n) { return (n + kStaticAlign - 1u) / kStaticAlign * kStaticAlign; }
+} // namespace
+
+size_t World::static_scratch_bytes() const {
+    const size_t E = env_, pcap = cfg_.pair_capacity, ccap = cfg_.constraint_capacity;
+    const size_t row_stride = std::max(1u, model_.max_nv);
+    return align_up(E * pcap * sizeof(MRManifoldHeaderC)) +
+        align_up(E * pcap * MR_MANIFOLD_POINT_CAPACITY * sizeof(MRManifoldPointC)) +
+        2u * align_up(E * ccap * 3u * row_stride * sizeof(float)) + align_up(E * ccap * sizeof(ColdConstraint)) +
+        align_up(E * ccap * sizeof(HotConstraint)) + align_up(sizeof(AdvanceSummaryDev));
+}
+
+void World::enqueue_static(const float* device_commands, const uint32_t* device_masks, uint32_t steps,
+                           MRWorldStatusC* device_status, MRContactStatusC* device_cstatus) {
+    if (steps == 0u) throw std::runtime_error("control_step_count must be positive");
+    if (device_commands == nullptr) throw std::runtime_error("commands are required");
+    if (device_status == nullptr || device_cstatus == nullptr)
+        throw std::runtime_error("enqueue_device_static: the status buffers are required");
+    if (step_timing_) throw std::runtime_error("enqueue_device_static: per-launch step timing must be off");
+    int device = -1;
+    hip_check(hipGetDevice(&device), "hipGetDevice");
+    if (device != device_) throw std::runtime_error("enqueue_device_static: the world's device must be current");
+    void* base = nullptr;
+    size_t capacity = 0;
+    hip_check(static_scratch(&base, &capacity), "static scratch address");
+    const size_t need = static_scratch_bytes();
+    if (need > capacity)
+        throw std::runtime_error("enqueue_device_static: the world needs " + std::to_string(need) +
+                                 " bytes of scratch; the static array holds " + std::to_string(capacity));
+    const size_t E = env_, pcap = cfg_.pair_capacity, ccap = cfg_.constraint_capacity;
+    const size_t row_stride = std::max(1u, model_.max_nv);
+    unsigned char* at = static_cast<unsigned char*>(base);
+    auto carve = [&](size_t bytes) {
+        unsigned char* p = at;
+        at += align_up(bytes);
+        return p;
+    };
+    StepBuffers b;
+    b.stage_h = reinterpret_cast<MRManifoldHeaderC*>(carve(E * pcap * sizeof(MRManifoldHeaderC)));
+    b.stage_p = reinterpret_cast<MRManifoldPointC*>(
+        carve(E * pcap * MR_MANIFOLD_POINT_CAPACITY * sizeof(MRManifoldPointC)));
+    b.jrow = reinterpret_cast<float*>(carve(E * ccap * 3u * row_stride * sizeof(float)));
+    b.resp = reinterpret_cast<float*>(carve(E * ccap * 3u * row_stride * sizeof(float)));
+    b.cold = reinterpret_cast<ColdConstraint*>(carve(E * ccap * sizeof(ColdConstraint)));
+    b.hot = reinterpret_cast<HotConstraint*>(carve(E * ccap * sizeof(HotConstraint)));
+    b.summary = reinterpret_cast<AdvanceSummaryDev*>(carve(sizeof(AdvanceSummaryDev)));
+    b.status = device_status;
+    b.cstatus = device_cstatus;
+    if (stage_profiling_)
+        hip_check(hipMemsetAsync(d_stage_cycles_.ptr, 0, E * kStageCount * sizeof(unsigned long long), stream_),
+                  "clear stage cycles");
+    launch_steps(device_commands, device_masks, steps, true, b);
     last_steps_ = steps;
 }
 
@@ -649,7 +731,8 @@ AdvanceSummary World::wait() {
     s.control_step_count = last_steps_;
     s.gpu_milliseconds = last_kernel_ms_;
     AdvanceSummaryDev d;
-    hip_check(hipMemcpy(&d, d_summary_.ptr, sizeof(d), hipMemcpyDeviceToHost), "advance summary");
+    if (last_summary_ == nullptr) throw std::runtime_error("nothing was enqueued");
+    hip_check(hipMemcpy(&d, last_summary_, sizeof(d), hipMemcpyDeviceToHost), "advance summary");
     const uint64_t n = (uint64_t)last_steps_ * last_stepped_;
     s.failed_environment_steps = d.failed;
     s.successful_environment_steps = (uint32_t)(n - d.failed);
@@ -666,14 +749,14 @@ AdvanceSummary World::wait() {
 void World::read_statuses(MRWorldStatusC* out, uint32_t steps) const {
     if (steps > last_steps_) throw std::runtime_error("fewer control steps were executed");
     hip_check(hipStreamSynchronize(stream_), "sync");
-    hip_check(hipMemcpy(out, d_status_.ptr, (size_t)steps * env_ * sizeof(MRWorldStatusC), hipMemcpyDeviceToHost),
+    hip_check(hipMemcpy(out, last_status_, (size_t)steps * env_ * sizeof(MRWorldStatusC), hipMemcpyDeviceToHost),
               "read statuses");
 }
 
 void World::read_contact_statuses(MRContactStatusC* out, uint32_t steps) const {
     if (steps > last_steps_) throw std::runtime_error("fewer control steps were executed");
     hip_check(hipStreamSynchronize(stream_), "sync");
-    hip_check(hipMemcpy(out, d_cstatus_.ptr, (size_t)steps * env_ * sizeof(MRContactStatusC), hipMemcpyDeviceToHost),
+    hip_check(hipMemcpy(out, last_cstatus_, (size_t)steps * env_ * sizeof(MRContactStatusC), hipMemcpyDeviceToHost),
               "read contact statuses");
 }
 

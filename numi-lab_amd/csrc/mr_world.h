@@ -64,6 +64,14 @@ public:
     // the pointers are device memory already resident in HBM; otherwise they
     // are host arrays copied before the first launch.
     void enqueue(const float* commands, const uint32_t* masks, uint32_t steps, bool device_inputs);
+    // enqueue() on device inputs that allocates nothing: the per-step scratch
+    // and the advance summary come from the library's static device array
+    // (static_scratch), the statuses go to the caller's device buffers
+    // [steps][E]. Throws when the world's scratch exceeds the array.
+    void enqueue_static(const float* device_commands, const uint32_t* device_masks, uint32_t steps,
+                        MRWorldStatusC* device_status, MRContactStatusC* device_cstatus);
+    // Bytes of static scratch enqueue_static needs for this world.
+    size_t static_scratch_bytes() const;
     // Waits for the stream and summarises the statuses of the last enqueue.
     AdvanceSummary wait();
 
@@ -105,6 +113,19 @@ public:
     void enqueue_graph(const float* device_commands, uint32_t steps);
 
 private:
+    // Device buffers one advance writes besides the persistent state.
+    struct StepBuffers {
+        MRManifoldHeaderC* stage_h;
+        MRManifoldPointC* stage_p;
+        float* jrow;
+        float* resp;
+        ColdConstraint* cold;
+        HotConstraint* hot;
+        MRWorldStatusC* status;      // [steps][E]
+        MRContactStatusC* cstatus;   // [steps][E]
+        AdvanceSummaryDev* summary;
+    };
+    StepBuffers own_buffers();
     KernelArgs base_args(uint32_t step) const;
     void response_scratch_bounds(uint32_t& slots, uint32_t& stack) const;
     void compute_lds_layout();
@@ -129,7 +150,12 @@ private:
     const void* graph_cstatus_ = nullptr;
     const void* graph_summary_ = nullptr;
     uint32_t graph_schedule_ = 0;  // schedule_version_ the graph was captured with
-    void launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, bool events);
+    void launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, bool events, const StepBuffers& buf);
+    // where the last advance wrote its statuses and summary
+    const MRWorldStatusC* last_status_ = nullptr;
+    const MRContactStatusC* last_cstatus_ = nullptr;
+    const AdvanceSummaryDev* last_summary_ = nullptr;
+    int device_ = -1;  // the device current at creation
     void drop_graph();
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_begin_ = nullptr, ev_end_ = nullptr;

@@ -331,6 +331,8 @@ _SIGNATURES: Final = {
     "mr_world_advance": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_void_p, ct.c_size_t, ct.c_uint32,
                           ct.POINTER(WorldAdvanceC)], ct.c_int),
     "mr_world_enqueue_device": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32], ct.c_int),
+    "mr_world_enqueue_device_static": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32, ct.c_void_p,
+                                        ct.c_void_p], ct.c_int),
     "mr_world_wait": ([ct.c_void_p, ct.POINTER(WorldAdvanceC)], ct.c_int),
     "mr_world_read_state": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_void_p, ct.c_size_t, ct.c_void_p,
                              ct.c_size_t], ct.c_int),
@@ -615,19 +617,41 @@ class World:
             raise ValueError(f"device commands must be a contiguous float32 GPU tensor of shape {expect}")
         return steps
 
+    def _device_masks(self, reset_masks, steps: int) -> Optional[int]:
+        import torch
+
+        if reset_masks is None:
+            return None
+        if (tuple(reset_masks.shape) != (steps, self.environment_count) or not reset_masks.is_cuda
+                or not reset_masks.is_contiguous() or reset_masks.dtype not in (torch.int32, torch.uint32)):
+            raise ValueError("device reset masks must be a contiguous int32/uint32 GPU tensor "
+                             "[steps][environments]")
+        return reset_masks.data_ptr()
+
     def enqueue_device(self, commands, reset_masks=None) -> None:
         """Enqueue with inputs already in device memory (torch tensors on the GPU)."""
+        steps = self._device_commands(commands)
+        mask_ptr = self._device_masks(reset_masks, steps)
+        self._b.check(self._b.lib.mr_world_enqueue_device(self._h, commands.data_ptr(), mask_ptr, steps))
+        self.last_steps = steps
+
+    def enqueue_device_static(self, commands, status, contact_status, reset_masks=None) -> None:
+        """mr_world_enqueue_device_static: enqueue_device that allocates no device memory. The
+        per-step scratch comes from the library's static device array; `status` and
+        `contact_status` are the caller's contiguous uint8 GPU tensors of steps x E x 48 and
+        steps x E x 96 bytes that receive the statuses."""
         import torch
 
         steps = self._device_commands(commands)
-        mask_ptr = None
-        if reset_masks is not None:
-            if (tuple(reset_masks.shape) != (steps, self.environment_count) or not reset_masks.is_cuda
-                    or not reset_masks.is_contiguous() or reset_masks.dtype not in (torch.int32, torch.uint32)):
-                raise ValueError("device reset masks must be a contiguous int32/uint32 GPU tensor "
-                                 "[steps][environments]")
-            mask_ptr = reset_masks.data_ptr()
-        self._b.check(self._b.lib.mr_world_enqueue_device(self._h, commands.data_ptr(), mask_ptr, steps))
+        mask_ptr = self._device_masks(reset_masks, steps)
+        E = self.environment_count
+        for t, rec in ((status, WORLD_STATUS_DTYPE), (contact_status, CONTACT_STATUS_DTYPE)):
+            if (t.dtype != torch.uint8 or not t.is_cuda or not t.is_contiguous()
+                    or t.numel() != steps * E * rec.itemsize):
+                raise ValueError(f"status buffers must be contiguous uint8 GPU tensors of "
+                                 f"{steps} x {E} x {rec.itemsize} bytes")
+        self._b.check(self._b.lib.mr_world_enqueue_device_static(self._h, commands.data_ptr(), mask_ptr, steps,
+                                                                 status.data_ptr(), contact_status.data_ptr()))
         self.last_steps = steps
 
     def wait(self) -> WorldAdvanceC:
