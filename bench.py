@@ -3,12 +3,26 @@
 Workload (default): the floating 29-DoF humanoid on a 256x256 Perlin
 heightfield, 4096 worlds, fp32, 4 microsteps of 5 ms per 20 ms control step,
 implicit PD targets U[-0.3, 0.3] rad around a standing pose, elliptic cone
-(mu_u 1.0, mu_v 0.7), 4+2 cone sweeps. One "step" is one control step of all
-worlds (one launch of the fused kernel); the metric is env-steps/s.
+(mu_u 1.0, mu_v 0.7). The cone solve runs at the metric's residual: sweeps
+continue until a microstep's final-sweep max |dlambda| is at or below 1e-5
+(capped at 128 sweeps). One "step" is one control step of all worlds; the
+metric is env-steps/s.
 
-Prints one JSON line (see the keys below). Inputs are generated up front and
-resident in HBM before the timed region; the timed region is bracketed by
-device synchronisation on both sides.
+The worlds are first rolled into steady contact for a fixed number of control
+steps (--preroll, not controlled by --warmup), so that any --steps measures
+the same phase of the horizon. From that saved state three paths run the same
+commands:
+  * batched (the headline `value`): the persistent work-queue kernel over all
+    worlds, re-sorted on the device after every control step by the sweeps
+    each world took, heaviest first;
+  * plain at the same residual: one workgroup per world, hardware order;
+  * plain at the fixed 4+2 sweep budget.
+`vs_plain` is batched / plain at residual 1e-5. Extended legs (heterogeneous
+batch, step server at the same residual, learned warm start) follow.
+
+Prints one JSON line. Inputs are generated up front and resident in HBM
+before the timed region; the timed region is bracketed by device
+synchronisation on both sides.
 """
 
 import argparse
@@ -24,7 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "numi-lab_amd"))
 
 HBM_PEAK_GBPS = 8000.0
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "control_step_humanoid_4096_traffic.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "control_step_queue_humanoid_4096_traffic.md")
 
 
 def bytes_per_env_step(nq, nv, s_dyn, s_static, m):
@@ -36,17 +50,30 @@ def bytes_per_env_step(nq, nv, s_dyn, s_static, m):
 
 
 def read_traffic_profile(path):
+    """HBM bytes per launch of the headline kernel from the committed counter profile, and the
+    mean manifolds per world of the run it was measured on (the same command)."""
     if not os.path.isfile(path):
-        return None
+        return None, None
     text = open(path).read()
     m = re.search(r"hbm_bytes_per_launch\s*[=:]\s*([0-9.eE+]+)", text)
-    return float(m.group(1)) if m else None
+    k = re.search(r"mean_manifolds\s*[=:]\s*([0-9.eE+]+)", text)
+    return (float(m.group(1)) if m else None), (float(k.group(1)) if k else None)
 
 
-def cpu_baseline(kind, seed, envs, steps, threads, solver=0):
-    """Times the oracle (FP32 CPU restatement of the same microstep) on a
-    bounded sample of the same workload: `envs` worlds over `steps` control
-    steps, one contiguous block of worlds per std::thread."""
+def host_cores():
+    """Threads the CPU baseline uses and how they were chosen: the process affinity mask,
+    bounded by the host's share for one GPU (the OMP_NUM_THREADS the GPU host sets, 16 when
+    unset)."""
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(affinity, share)), {"hardware_concurrency": os.cpu_count(), "affinity_mask_cpus": affinity,
+                                          "cpu_share_per_gpu": share}
+
+
+def cpu_baseline(kind, seed, envs, preroll, steps, threads, cfg_of):
+    """Times the oracle (FP32 CPU restatement of the same microstep) on a bounded sample of the
+    same workload: `envs` worlds rolled `preroll` control steps into steady contact (untimed),
+    then `steps` control steps timed, one contiguous block of worlds per std::thread."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import ctypes as ct
 
@@ -54,73 +81,102 @@ def cpu_baseline(kind, seed, envs, steps, threads, solver=0):
     import oracle_lib as ol
 
     model = mr.WorldModel.synthetic(kind, envs, seed)
-    cfg = model.default_config()
-    cfg.solver = solver
+    cfg = cfg_of(model)
     st = ol.OracleState(model, cfg)
-    cmds = model.commands(0, steps)
+    cmds = model.commands(0, preroll + steps)
     lib = ol.load()
     E = envs
     status = np.zeros(E, mr.WORLD_STATUS_DTYPE)
     cstatus = np.zeros(E, mr.CONTACT_STATUS_DTYPE)
     acc = np.zeros((E, model.max_nv), np.float32)
     p = lambda a: a.ctypes.data  # noqa: E731
-    t0 = time.perf_counter()
-    for k in range(steps):
+
+    def run(k):
         rc = lib.mr_oracle_control_step_parallel(ct.byref(model.desc), ct.byref(cfg), E, k, threads, p(st.q),
                                                  p(st.v), p(st.scene), p(st.mh), p(st.mp), p(st.mcount),
                                                  p(np.ascontiguousarray(cmds[k])), p(status), p(cstatus), p(acc))
         if rc != 0:
             raise RuntimeError("oracle failed")
+
+    for k in range(preroll):
+        run(k)
+    t0 = time.perf_counter()
+    for k in range(preroll, preroll + steps):
+        run(k)
     dt = time.perf_counter() - t0
     return {"value": round(E * steps / dt, 1), "unit": "env-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{E} worlds x {steps} control steps of the same workload (oracle, FP32, "
-                      f"{threads} std::threads)", "seconds": round(dt, 2)}
+            "sample": f"{E} worlds x {steps} control steps of the same workload after a {preroll}-step pre-roll "
+                      f"(oracle, FP32, same cone-solve configuration, {threads} std::threads)",
+            "seconds": round(dt, 2)}
 
 
-def residual_leg(mr, model, commands, args, reorder=False):
-    """The same workload with the adaptive cone solve: sweeps continue until
-    the final-sweep max |dlambda| of a microstep is at or below the tolerance
-    (capped at --residual-max-iterations sweeps). Reports throughput, control
-    step latency and how many microsteps reached the tolerance. With `reorder`,
-    the worlds run as a work queue over the resident workgroups, re-sorted on
-    the device after every control step by the sweeps each world took, most
-    first (the results are the same as the plain launch's)."""
+def timed_run(world, commands, first, steps, envs):
+    """Runs commands[first:first+steps] eagerly (per-launch HIP events on the world's stream)
+    between two device synchronisations; returns throughput, per-step latencies and statuses."""
     import torch
 
-    cfg = model.default_config()
-    cfg.solver = SOLVERS[args.solver]
-    cfg.residual_tolerance = args.residual_tolerance
-    cfg.max_iterations = args.residual_max_iterations
-    world = mr.World(model, cfg)
-    if reorder:
-        world.set_schedule(np.arange(model.environment_count), world.resident_workgroups())
-        world.set_schedule_reorder(True)
-    world.set_step_timing(True)
-    warm = min(args.warmup, commands.shape[0])
-    if warm > 0:
-        world.enqueue_device(commands[:warm])
-        world.wait()
-    steps = min(args.residual_steps, commands.shape[0] - warm)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    world.enqueue_device(commands[warm:warm + steps])
+    world.enqueue_device(commands[first:first + steps])
     world.wait()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     step_ms = world.step_milliseconds()
-    cs = world.contact_statuses()
+    return {"seconds": t1 - t0, "env_steps_per_s": envs * steps / (t1 - t0), "step_ms": step_ms,
+            "world_step_us": world.world_step_microseconds(), "cstat": world.contact_statuses(),
+            "stat": world.statuses()}
+
+
+def timed_horizon(world, commands, first, steps, horizon, envs):
+    """Runs commands[first:first+steps] as persistent horizon launches of `horizon` control steps
+    (each world runs its steps back to back inside a launch); returns throughput, the per-world
+    per-step wall-clock times the kernel stamps, and the statuses of every step."""
+    import torch
+
+    step_us, cstat, stat, kernel_ms = [], [], [], 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c0 in range(first, first + steps, horizon):
+        n = min(horizon, first + steps - c0)
+        world.enqueue_device_horizon(commands[c0:c0 + n])
+        world.wait()
+        kernel_ms += world.last_kernel_milliseconds() * n  # the library reports the per-step average
+        step_us.append(world.world_step_microseconds())
+        cstat.append(world.contact_statuses())
+        stat.append(world.statuses())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return {"seconds": t1 - t0, "env_steps_per_s": envs * steps / (t1 - t0), "kernel_ms": kernel_ms,
+            "world_step_us": np.concatenate(step_us), "cstat": np.concatenate(cstat), "stat": np.concatenate(stat)}
+
+
+def summarize(run, steps, tol):
+    cs = run["cstat"]
     with_contacts = cs["constraints"] > 0
     res = cs["residuals"][..., 0][with_contacts]
     iters = cs["solver_iterations"][with_contacts]
-    return {
-        "tolerance": args.residual_tolerance, "max_iterations": args.residual_max_iterations, "steps": int(steps),
-        "env_steps_per_s": round(args.envs * steps / (t1 - t0), 1),
-        "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
-                       "p99": round(float(np.percentile(step_ms, 99)), 4)},
+    out = {
+        "env_steps_per_s": round(run["env_steps_per_s"], 1), "steps": int(steps),
+        "ms_per_step": round(run["seconds"] * 1e3 / steps, 4),
         "final_microstep_iterations_mean": round(float(iters.mean()), 2) if iters.size else 0.0,
-        "final_microstep_at_tolerance": round(float(np.mean(res <= args.residual_tolerance)), 4) if res.size else 1.0,
         "final_sweep_max_dlambda_p99": float(np.percentile(res, 99)) if res.size else 0.0,
+        "mean_manifolds": round(float(cs["manifolds"].mean()), 3),
+        "failed_env_steps": int(np.sum(run["stat"]["code"] != 0)),
     }
+    if tol > 0.0:
+        out["final_microstep_at_tolerance"] = round(float(np.mean(res <= tol)), 4) if res.size else 1.0
+    if "step_ms" in run:
+        # what a caller waiting on each control step of every world sees: the launch
+        out["launch_latency_ms"] = {"p50": round(float(np.percentile(run["step_ms"], 50)), 4),
+                                    "p99": round(float(np.percentile(run["step_ms"], 99)), 4),
+                                    "max": round(float(run["step_ms"].max()), 4)}
+    if "world_step_us" in run:
+        # each world's own control step, start to published statuses, on the GPU's constant clock
+        w = run["world_step_us"] * 1e-3
+        out["world_step_latency_ms"] = {"p50": round(float(np.percentile(w, 50)), 4),
+                                        "p99": round(float(np.percentile(w, 99)), 4),
+                                        "max": round(float(w.max()), 4)}
+    return out
 
 
 def warm_start_leg(mr, args):
@@ -153,7 +209,11 @@ def warm_start_leg(mr, args):
     ev_cfg.max_iterations = args.residual_max_iterations
     ev = ws.evaluate(ev_model, ev_cfg, mlp, control_steps=args.warm_start_eval_steps, settle_steps=40)
     plain, learned = ev["plain"]["sweeps_per_step"], ev["mlp"]["sweeps_per_step"]
+    saved = 1.0 - learned / plain if plain > 0 else 0.0
     return {
+        # the learned warm start is switched on only where it saves sweeps at equal residual; its
+        # kernel instance runs one wave per SIMD, so a network that saves none only costs time
+        "enabled": bool(saved > 0.0),
         "workload": "chain7_boxes20_plane (config 1)", "residual_tolerance": tol,
         "samples": rep.kept + rep.dropped, "samples_kept": rep.kept, "collect_s": round(collect_s, 2),
         "train_steps": rep.steps, "train_batch": 8192, "train_s": round(train_s, 2),
@@ -244,14 +304,15 @@ def server_leg(mr, args):
     from metalrobo import server as sv
 
     pool = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_HUMANOID_TERRAIN, args.serve_pool, args.seed)
-    srv = sv.StepServer(pool, hbm_budget_bytes=200e9)
+    srv = sv.StepServer(pool, headline_config(pool, args), hbm_budget_bytes=200e9)
     reqs = sv.poisson_requests(200.0, args.serve_requests, seed=args.seed)
     rep = srv.serve(reqs)
     base_reqs = sv.poisson_requests(200.0, args.serve_requests, seed=args.seed)[:args.serve_serial_requests]
     base = sv.serve_serially(lambda n: mr.WorldModel.synthetic(mr.MR_SYNTHETIC_HUMANOID_TERRAIN, n, args.seed),
-                             base_reqs)
+                             base_reqs, config_of=lambda m: headline_config(m, args))
     return {
         "workload": "humanoid29 requests, Poisson 200/s, N log-uniform 1-1024, horizon 10-500",
+        "residual_tolerance": args.residual_tolerance, "budget_blocked_ticks": rep.budget_blocked_ticks,
         "requests": rep.requests, "pool_slots": int(pool.environment_count), "capacity_slots": rep.capacity_slots,
         "bytes_per_slot": round(rep.bytes_per_slot, 1), "hbm_budget_bytes": rep.hbm_budget_bytes,
         "env_steps": rep.env_steps, "wall_s": round(rep.wall_s, 3),
@@ -266,27 +327,45 @@ def server_leg(mr, args):
     }
 
 
+def headline_config(model, args, fixed_budget=False):
+    """The metric's cone-solve configuration: sweeps until the final-sweep max |dlambda| is at or
+    below --residual-tolerance (capped at --residual-max-iterations), or the fixed 4+2 budget."""
+    cfg = model.default_config()
+    cfg.solver = SOLVERS[args.solver]
+    if not fixed_budget and args.residual_tolerance > 0.0:
+        cfg.residual_tolerance = args.residual_tolerance
+        cfg.max_iterations = args.residual_max_iterations
+    return cfg
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--preroll", type=int, default=200,
+                    help="control steps that bring the worlds into steady contact before --warmup and the "
+                         "timed steps (fixed, so that any --steps measures the same phase)")
+    ap.add_argument("--horizon", type=int, default=50,
+                    help="control steps per persistent launch of the headline path (0: one launch per step)")
     ap.add_argument("--envs", type=int, default=4096)
     ap.add_argument("--config", choices=["humanoid", "chain"], default="humanoid")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-envs", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=500)
+    ap.add_argument("--cpu-preroll", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=50)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--skip-cpu-baseline", action="store_true")
     ap.add_argument("--residual-tolerance", type=float, default=1.0e-5,
-                    help="final-sweep max |dlambda| target of the adaptive leg (0 disables the leg)")
+                    help="final-sweep max |dlambda| the cone solve runs to (0: the fixed 4+2 budget)")
     ap.add_argument("--residual-max-iterations", type=int, default=128)
-    ap.add_argument("--residual-steps", type=int, default=50)
     ap.add_argument("--stage-profile", type=int, default=0, metavar="STEPS",
                     help="after the timed region, run STEPS more control steps with the per-stage shader-clock "
                          "counters on and report each stage's share of the cycles")
     ap.add_argument("--solver", choices=list(SOLVERS), default="sweeps",
                     help="cone-solve iteration: ordered block sweeps or the relaxed block Jacobi")
+    ap.add_argument("--skip-comparisons", action="store_true",
+                    help="time only the headline path (no plain-at-residual or fixed-budget legs)")
     ap.add_argument("--skip-server", action="store_true", help="skip the step-server leg")
     ap.add_argument("--serve-pool", type=int, default=16384)
     ap.add_argument("--serve-requests", type=int, default=96)
@@ -318,38 +397,74 @@ def main():
     import torch
 
     import metalrobo as mr
+    from metalrobo import autotune as at
 
     if not torch.cuda.is_available():
         print("no GPU available", file=sys.stderr)
         sys.exit(1)
     torch.cuda.init()
     kind = mr.MR_SYNTHETIC_HUMANOID_TERRAIN if args.config == "humanoid" else mr.MR_SYNTHETIC_CHAIN_BOXES
-    model = mr.WorldModel.synthetic(kind, args.envs, args.seed)
-    cfg = model.default_config()
-    cfg.solver = SOLVERS[args.solver]
+    E, K, W, P = args.envs, args.steps, args.warmup, args.preroll
+    model = mr.WorldModel.synthetic(kind, E, args.seed)
+    cfg = headline_config(model, args)
+    tol = float(cfg.residual_tolerance)
+    commands = torch.from_numpy(model.commands(0, P + W + K)).to("cuda:0")
+    torch.cuda.synchronize()
+
+    # pre-roll into steady contact (same configuration; results do not depend on the schedule)
     world = mr.World(model, cfg)
-    world.set_step_timing(True)
-    total = args.warmup + args.steps
-    commands = torch.from_numpy(model.commands(0, total)).to("cuda:0")
-    torch.cuda.synchronize()
-
-    if args.warmup > 0:
-        world.enqueue_device(commands[:args.warmup])
+    if P > 0:
+        world.enqueue_device(commands[:P])
         world.wait()
+    start = at.snapshot(world)
 
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    world.enqueue_device(commands[args.warmup:])
-    adv = world.wait()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = args.envs * args.steps / (t1 - t0)
-
-    step_ms = world.step_milliseconds()
-    cstat = world.contact_statuses()
-    stat = world.statuses()
+    # headline: the batched path, a persistent work queue over every world (re-sorted by
+    # sweeps after each launch), each launch running every world through --horizon steps
+    H = args.horizon if args.horizon > 0 else 1
+    world.set_schedule(np.arange(E), world.resident_workgroups())
+    world.set_schedule_reorder(True)
+    world.set_step_timing(True)
+    if W > 0:
+        # warmup on the per-step queue kernel: the horizon kernel then launches only in the timed region
+        world.enqueue_device(commands[P:P + W])
+        world.wait()
+    if args.horizon > 0:
+        head = timed_horizon(world, commands, P + W, K, H, E)
+        launches = (K + H - 1) // H
+        avg_launch_s = head["kernel_ms"] * 1e-3 / launches
+        steps_per_launch = K / launches
+        kernel_name = "mr_control_horizon_queue_kernel"
+    else:
+        head = timed_run(world, commands, P + W, K, E)
+        avg_launch_s = float(head["step_ms"].mean()) * 1e-3
+        steps_per_launch = 1.0
+        kernel_name = "mr_control_step_queue_kernel"
+    head_sum = summarize(head, K, tol)
+    ms_per_step = head["seconds"] * 1e3 / K
+    value = head["env_steps_per_s"]
+    cstat, stat = head["cstat"], head["stat"]
     mean_manifolds = float(cstat["manifolds"].mean())
+
+    legs = {}
+    if not args.skip_comparisons:
+        variants = [("plain_at_residual", cfg, False), ("queue_per_step_at_residual", cfg, True),
+                    ("plain_fixed_budget", headline_config(model, args, True), False)]
+        for name, c, queue in variants:
+            if tol <= 0.0 and name != "plain_fixed_budget":
+                continue
+            w = mr.World(model, c)
+            at.restore(w, start)
+            if queue:
+                w.set_schedule(np.arange(E), w.resident_workgroups())
+                w.set_schedule_reorder(True)
+            w.set_step_timing(True)
+            if W > 0:
+                w.enqueue_device(commands[P:P + W])
+                w.wait()
+            legs[name] = summarize(timed_run(w, commands, P + W, K, E), K, float(c.residual_tolerance))
+            w.close()
+    plain = legs.get("plain_at_residual")
+
     h = model.models()[0]
     nq, nv = int(model.max_nq), int(model.max_nv)
     bodies = model.bodies()
@@ -357,9 +472,10 @@ def main():
     s_dyn = int(np.sum(scene["motion_type"] == 2))
     s_static = int(len(scene) - s_dyn)
     b_env = bytes_per_env_step(nq, nv, s_dyn, s_static, mean_manifolds)
-    avg_launch_s = float(step_ms.mean()) * 1e-3
-    achieved = args.envs * b_env / avg_launch_s / 1e9
-    traffic = read_traffic_profile(TRAFFIC_PROFILE)
+    achieved = E * steps_per_launch * b_env / avg_launch_s / 1e9
+    traffic, traffic_m = read_traffic_profile(TRAFFIC_PROFILE)
+    p99 = head_sum["world_step_latency_ms"]["p99"]
+    vs_plain = round(value / plain["env_steps_per_s"], 4) if plain else None
 
     out = {
         "metric": "env-steps/s (29-DoF, 4096 worlds) vs plain; p99 step latency at residual 1e-5"
@@ -367,27 +483,37 @@ def main():
         "value": round(value, 1),
         "unit": "env-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": K,
+        "warmup": W,
         "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
+        "vs_plain": vs_plain,
         "dtype": "fp32",
         "data": "synthetic (seeded worlds and commands; no datasets)",
         "config": {
             "workload": ("humanoid29_heightfield256_pd_elliptic" if args.config == "humanoid"
                          else "chain7_boxes20_plane_effort"),
-            "worlds": args.envs, "control_steps": args.steps, "microsteps_per_step": int(cfg.physics_substeps),
-            "control_dt_s": float(cfg.control_timestep),
-            "cone_sweeps": f"{cfg.velocity_iterations}+{cfg.final_velocity_iterations}",
-            "solver": args.solver,
-            "eligible_pairs_per_world": int(h["pair_count"]), "plain_path": True,
+            "worlds": E, "control_steps": K, "preroll_steps": P, "microsteps_per_step": int(cfg.physics_substeps),
+            "control_dt_s": float(cfg.control_timestep), "solver": args.solver,
+            "residual_tolerance": tol, "max_iterations": int(cfg.max_iterations) if tol > 0.0 else None,
+            "cone_sweeps": None if tol > 0.0 else f"{cfg.velocity_iterations}+{cfg.final_velocity_iterations}",
+            "path": (f"batched: persistent work queue over every world, each launch running every world "
+                     f"through {H} control steps back to back, re-sorted by sweeps after each launch"
+                     if args.horizon > 0 else
+                     "batched: persistent work queue over every world, re-sorted by sweeps after each step"),
+            "horizon_steps_per_launch": H if args.horizon > 0 else 1,
+            "eligible_pairs_per_world": int(h["pair_count"]),
         },
-        "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
-                       "p99": round(float(np.percentile(step_ms, 99)), 4),
-                       "max": round(float(step_ms.max()), 4)},
-        "residual": {"final_sweep_max_dlambda_p99": float(np.percentile(cstat["residuals"][..., 0], 99)),
+        # per-control-step latency of each world (start of its step to its published statuses)
+        "latency_ms": head_sum["world_step_latency_ms"],
+        "target": {"vs_plain": 3.0, "p99_ms": 2.0,
+                   "met": bool(vs_plain is not None and vs_plain >= 3.0 and p99 < 2.0)},
+        "batched": {k: v for k, v in head_sum.items() if k not in ("world_step_latency_ms",)},
+        "residual": {"final_sweep_max_dlambda_p99": head_sum["final_sweep_max_dlambda_p99"],
+                     "final_microstep_at_tolerance": head_sum.get("final_microstep_at_tolerance"),
+                     "final_microstep_iterations_mean": head_sum["final_microstep_iterations_mean"],
                      "normal_p99": float(np.percentile(cstat["residuals"][..., 1], 99)),
                      "cone_violation_max": float(cstat["residuals"][..., 2].max())},
         "contacts": {"mean_manifolds": round(mean_manifolds, 3),
@@ -396,24 +522,27 @@ def main():
                      "failed_env_steps": int(np.sum(stat["code"] != 0))},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 6), "traffic": traffic,
+                     "kernel": kernel_name,
+                     "traffic_mean_manifolds": traffic_m,
                      "algorithmic_bytes_per_env_step": round(b_env, 1),
+                     "algorithmic_bytes_per_launch": round(E * steps_per_launch * b_env, 1),
                      "kernel_ms_per_launch": round(avg_launch_s * 1e3, 4)},
         "cpu_baseline": None,
     }
+    out.update(legs)
     if args.stage_profile > 0:
         prof_world = mr.World(model, cfg)
-        prof_world.enqueue_device(commands[:args.warmup])
-        prof_world.wait()
+        at.restore(prof_world, start)
         prof_world.set_stage_profiling(True)
-        n = min(args.stage_profile, commands.shape[0] - args.warmup)
-        prof_world.enqueue_device(commands[args.warmup:args.warmup + n])
+        n = min(args.stage_profile, K + W)
+        prof_world.enqueue_device(commands[P:P + n])
         prof_world.wait()
         cyc = prof_world.stage_cycles().astype(np.float64)[:, :len(mr.World.STAGE_NAMES)]
         out["stage_cycles"] = {
-            "steps": int(n), "cycles_per_env_step": round(float(cyc.sum() / args.envs / n), 0),
+            "steps": int(n), "cycles_per_env_step": round(float(cyc.sum() / E / n), 0),
             "fraction": {name: round(float(f), 4) for name, f in zip(mr.World.STAGE_NAMES, cyc.sum(0) / cyc.sum())},
         }
-        del prof_world
+        prof_world.close()
     if args.copy_calibration_mib > 0:
         n = args.copy_calibration_mib * 1024 * 1024 // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda:0")
@@ -422,21 +551,22 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if args.residual_tolerance > 0.0 and args.residual_steps > 0:
-        out["at_residual"] = residual_leg(mr, model, commands, args)
-        out["at_residual_reordered"] = residual_leg(mr, model, commands, args, reorder=True)
+    world.close()
+    del commands
     if not args.skip_heterogeneous:
         out["heterogeneous"] = heterogeneous_leg(mr, args)
     if not args.skip_server:
         out["server"] = server_leg(mr, args)
     if not args.skip_warm_start:
-        del world
         out["warm_start"] = warm_start_leg(mr, args)
     if not args.skip_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(kind, args.seed, args.cpu_envs, args.cpu_steps, threads, SOLVERS[args.solver])
+        threads, how = host_cores()
+        if args.cpu_threads:
+            threads = args.cpu_threads
+        out["cpu_baseline"] = cpu_baseline(kind, args.seed, args.cpu_envs, args.cpu_preroll, args.cpu_steps,
+                                           threads, lambda m: headline_config(m, args))
+        out["cpu_baseline"].update(how)
     print(json.dumps(out), flush=True)
-    del adv
 
 
 if __name__ == "__main__":

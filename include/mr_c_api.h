@@ -128,6 +128,25 @@ int mr_world_advance(MRWorldHandle* world, const float* commands, size_t command
 int mr_world_enqueue_device(MRWorldHandle* world, const float* device_commands, const uint32_t* device_reset_masks,
                             uint32_t control_step_count);
 int mr_world_wait(MRWorldHandle* world, MRWorldAdvanceC* out);
+/* Persistent horizon (no counterpart in the reference, whose encoder submits
+ * every control step for all environments): needs a work-queue schedule
+ * (mr_world_set_schedule with grid > 0); every scheduled world runs all
+ * control_step_count steps inside one launch, its next step starting as soon
+ * as its previous one ends, so no step waits for the slowest world of the
+ * step before it. Inputs as mr_world_enqueue_device ([steps][E][max_nv],
+ * [steps][E]); each world's results are those of one launch per control step.
+ * With a reordered schedule the queue is re-sorted once, after the launch, by
+ * each world's mean sweeps per control step over the horizon, most first.
+ * Wait with mr_world_wait. */
+int mr_world_enqueue_device_horizon(MRWorldHandle* world, const float* device_commands,
+                                    const uint32_t* device_reset_masks, uint32_t control_step_count);
+/* With step timing on (mr_world_set_step_timing), the wall-clock duration of
+ * every control step of every world in the last eager advance, [steps][E] in
+ * microseconds, measured in the kernel on the GPU's constant-rate clock
+ * (s_memrealtime, whose rate is measured once per device against HIP events
+ * when step timing is enabled) from the start of the world's step to the
+ * store of its statuses. */
+int mr_world_read_world_step_microseconds(const MRWorldHandle* world, float* out, size_t count);
 /* Largest per-step scratch, in bytes, that mr_world_enqueue_device_static can serve. */
 #define MR_WORLD_STATIC_SCRATCH_BYTES ((size_t)64u << 20)
 /* No counterpart in the reference. For callers that cannot allocate device
@@ -207,10 +226,15 @@ int mr_world_read_stage_cycles(const MRWorldHandle* world, uint64_t* out, size_t
  * while set; null restores the plain path. */
 int mr_world_set_warm_start_mlp(MRWorldHandle* world, const MRMlpWeightsC* weights);
 /* Training-sample capture at microstep capture_substep of each control step:
- * per constraint, the 8 features, the converged impulse (3) and a valid flag
- * (1.0 for the step's constraints, 0.0 for unused rows). */
+ * per constraint, the 8 features, the converged impulse (3), a valid flag
+ * (1.0 for the step's constraints, 0.0 for unused rows), the warm start the
+ * cone solve began from (3) and whether the network replaced it. */
 int mr_world_set_mlp_capture(MRWorldHandle* world, uint32_t enabled);
-/* [E][constraint_capacity][12] floats of the last captured control step. */
+/* [E][constraint_capacity][20] floats of the last captured control step: the
+ * 8 features, the converged impulse (3), a valid flag, the warm start the
+ * solve began from (3), 1.0 where the network replaced the plain warm start
+ * (0.0 otherwise), and with the learned warm start on, the network's scaled
+ * correction (3) and 1.0 where every normalized feature was within bounds. */
 int mr_world_read_mlp_samples(const MRWorldHandle* world, float* out, size_t count);
 
 /* ------------------------------------------------- task rollout --- */

@@ -47,6 +47,11 @@ typedef struct MRMlpWeightsC {
      * component exceeds this (impulse units); smaller ones keep the plain warm
      * start, which is already as close as the network's precision. */
     float min_correction;
+    /* In-solver use only: a microstep whose learned solve reaches the residual
+     * tolerance but leaves some contact's normal residual above this is solved
+     * again from the plain warm start, like one that misses the tolerance
+     * (0 disables the check). */
+    float max_normal_residual;
 } MRMlpWeightsC;
 
 /* Activations a training forward pass records for the backward pass. */

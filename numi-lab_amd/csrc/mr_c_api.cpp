@@ -275,6 +275,23 @@ int mr_world_enqueue_device(MRWorldHandle* world, const float* device_commands, 
     });
 }
 
+int mr_world_enqueue_device_horizon(MRWorldHandle* world, const float* device_commands,
+                                    const uint32_t* device_reset_masks, uint32_t control_step_count) {
+    return translate([&]() {
+        require(world != nullptr && device_commands != nullptr, "null argument");
+        world->world->enqueue_horizon(device_commands, device_reset_masks, control_step_count);
+    });
+}
+
+int mr_world_read_world_step_microseconds(const MRWorldHandle* world, float* out, size_t count) {
+    return translate([&]() {
+        require(world != nullptr && out != nullptr, "null argument");
+        const mr::World& w = *world->world;
+        require_count(count, (size_t)w.last_steps() * w.environment_count(), "world step times");
+        w.read_world_step_microseconds(out, w.last_steps());
+    });
+}
+
 int mr_world_enqueue_device_static(MRWorldHandle* world, const float* device_commands,
                                    const uint32_t* device_reset_masks, uint32_t control_step_count,
                                    MRWorldStatusC* device_status, MRContactStatusC* device_contact_status) {

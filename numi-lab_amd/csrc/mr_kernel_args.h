@@ -206,8 +206,8 @@ struct KernelArgs {
     uint32_t* mcount;        // [3][E]
     uint32_t* mcurrent;      // [E]
     // inputs of this control step
-    const float* command;       // [E][max_nv]
-    const uint32_t* reset_mask; // [E] or null
+    const float* command;       // [E][max_nv] ([horizon][E][max_nv] for the horizon instance)
+    const uint32_t* reset_mask; // [E] or null ([horizon][E] for the horizon instance)
     // work-queue instance only: environments in processing order, and the
     // head every workgroup takes its next entry from (zeroed before a launch)
     const uint32_t* queue;
@@ -217,8 +217,11 @@ struct KernelArgs {
     const float* reset_v;
     const MRBodyStateC* reset_scene;
     // outputs of this control step
-    MRWorldStatusC* status;     // [E]
-    MRContactStatusC* cstatus;  // [E]
+    MRWorldStatusC* status;     // [E] ([horizon][E] for the horizon instance)
+    MRContactStatusC* cstatus;  // [E] ([horizon][E] for the horizon instance)
+    float* step_us;             // [horizon][E] wall-clock microseconds per control step of each world, or null
+    uint32_t horizon;           // control steps per launch of the horizon instance
+    float us_per_tick;          // microseconds per tick of the constant-rate clock (s_memrealtime)
     float* acceleration;        // [E][max_nv] or null
     MRPairRecordC* evidence_pairs; // [E][pair_capacity] or null
     uint32_t* evidence_counts;     // [E] or null
@@ -240,8 +243,12 @@ struct KernelArgs {
 // LDS bytes of one warm-start MLP tile: 16 x 8 FP32 inputs and two
 // 16 x 264 BF16 hidden layers.
 constexpr uint32_t kMlpTileLdsBytes = 16u * 8u * 4u + 2u * 16u * 264u * 2u;
-// One captured sample: 8 features, the converged impulse (3), a valid flag.
-constexpr uint32_t kMlpSampleFloats = 12u;
+// One captured sample: 8 features, the converged impulse (3), a valid flag,
+// the warm start the cone solve began from (3), 1.0 where the network
+// replaced the plain warm start (0.0 otherwise), and with inference on the
+// network's scaled correction (3) and 1.0 where every normalized feature was
+// within bounds (0.0 and a zero correction otherwise).
+constexpr uint32_t kMlpSampleFloats = 20u;
 
 // Device-side summary of one advance (all steps x environments), reduced on
 // the GPU so that waiting for an advance reads back 24 bytes instead of

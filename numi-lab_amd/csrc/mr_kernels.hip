@@ -1920,8 +1920,14 @@ MRD_INL bool mlp_warm_start(const Env& e, uint32_t lane, uint32_t ncons, uint32_
             float f[mr::kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             if (c < ncons) {
                 contact_features(e, c, mnv, f);
-                if (samples != nullptr)
-                    for (uint32_t k = 0; k < mr::kMlpIn; ++k) samples[(size_t)c * mr::kMlpSampleFloats + k] = f[k];
+                if (samples != nullptr) {
+                    float* row = samples + (size_t)c * mr::kMlpSampleFloats;
+                    for (uint32_t k = 0; k < mr::kMlpIn; ++k) row[k] = f[k];
+                    // the warm start the solve begins from, unless the network replaces it below
+                    for (uint32_t k = 0; k < 3u; ++k) row[12u + k] = e.lam[3u * c + k];
+                    row[15] = 0.0f;
+                    for (uint32_t k = 16u; k < 20u; ++k) row[k] = 0.0f;
+                }
             }
             if (infer) {
                 in_range = true;
@@ -1943,6 +1949,11 @@ MRD_INL bool mlp_warm_start(const Env& e, uint32_t lane, uint32_t ncons, uint32_
             for (uint32_t r = 0; r < 4u; ++r) s.x[((lane >> 4) * 4u + r) * mr::kMlpIn + col] = y[r] * scale;
         }
         __syncthreads();
+        if (samples != nullptr && lane < mr::kMlpRows && t0 + lane < ncons && in_range) {
+            float* row = samples + (size_t)(t0 + lane) * mr::kMlpSampleFloats;
+            for (uint32_t k = 0; k < 3u; ++k) row[16u + k] = s.x[lane * mr::kMlpIn + k];
+            row[19] = 1.0f;
+        }
         if (lane < mr::kMlpRows && t0 + lane < ncons && in_range) {
             const uint32_t c = t0 + lane;
             float lam[3];
@@ -1960,6 +1971,11 @@ MRD_INL bool mlp_warm_start(const Env& e, uint32_t lane, uint32_t ncons, uint32_
                 e.lam[3u * c + 1u] = lam[1];
                 e.lam[3u * c + 2u] = lam[2];
                 replaced = true;
+                if (samples != nullptr) {
+                    float* row = samples + (size_t)c * mr::kMlpSampleFloats;
+                    for (uint32_t k = 0; k < 3u; ++k) row[12u + k] = lam[k];
+                    row[15] = 1.0f;
+                }
             }
         }
     }
@@ -1972,13 +1988,17 @@ MRD_INL bool mlp_warm_start(const Env& e, uint32_t lane, uint32_t ncons, uint32_
 // One control step of one environment (one wavefront per workgroup). kMlp
 // selects the instance with the learned warm start and its sample capture;
 // the plain instance carries none of that code.
-template <bool kMlp> __device__ __forceinline__ void control_step(const KernelArgs& args, uint32_t env) {
+// `step` selects the control step of a multi-step launch: the command, reset
+// mask and status records of step `step` ([steps][E] each); 0 elsewhere.
+template <bool kMlp> __device__ __forceinline__ void control_step(const KernelArgs& args, uint32_t env,
+                                                                 uint32_t step = 0u) {
     extern __shared__ __align__(16) unsigned char lds_base[];
     __shared__ unsigned long long s_fail;
     __shared__ uint32_t s_count[8];
     __shared__ float s_fmax[2];
     __shared__ float s_root[7];
     __shared__ MRContactStatusC s_cs;
+    __shared__ unsigned long long s_t0;  // start of this control step on the constant-rate clock (step timing)
 
     const uint32_t lane = threadIdx.x;
     if (env >= args.environment_count) return;
@@ -2121,7 +2141,9 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
     const size_t E = A.environment_count;
 
     // ---- prepare: reset, load state (the HBM state is the checkpoint)
-    const bool reset = A.reset_mask != nullptr && A.reset_mask[env] != 0u;
+    const size_t step_rec = (size_t)step * E + env;  // this step's record of this environment
+    if (A.step_us != nullptr && lane == 0u) s_t0 = __builtin_amdgcn_s_memrealtime();
+    const bool reset = A.reset_mask != nullptr && A.reset_mask[step_rec] != 0u;
     for (uint32_t i = lane; i < nq; i += kLanes) {
         const float val = reset ? A.reset_q[(size_t)env * A.max_nq + i] : gq[i];
         e.q[i] = val;
@@ -2131,7 +2153,7 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
         const float val = reset ? A.reset_v[(size_t)env * A.max_nv + i] : gv[i];
         e.v[i] = val;
         if (reset) gv[i] = val;
-        e.cmd[i] = A.command[(size_t)env * A.max_nv + i];
+        e.cmd[i] = A.command[step_rec * A.max_nv + i];
     }
     for (uint32_t s = lane; s < nsc; s += kLanes) {
         const MRBodyStateC val = reset ? A.reset_scene[(size_t)env * A.max_scene + s] : gs[s];
@@ -3009,13 +3031,50 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
             clk.mark(mr::kStageSweeps);
             sweeps_run += iters;
             if constexpr (kMlp) {
-                if (mlp_now && attempt == 0u && adaptive && !solve_failed && !(sweep_delta <= cfg.residual_tolerance)) {
-                    vj = vj_free;
-                    for (uint32_t i = lane; i < 3u * ncons; i += kLanes) e.lam[i] = e.mlp_lam[i];
-                    for (uint32_t b = lane; b < nsc; b += kLanes) e.scene[b] = e.mlp_scene[b];
-                    if (lane == 0u) cs.diagnostics.z += 1.0f;
-                    __syncthreads();
-                    continue;
+                if (mlp_now && attempt == 0u && adaptive && !solve_failed) {
+                    const bool missed = !(sweep_delta <= cfg.residual_tolerance);
+                    bool diverged = false;
+                    if (!missed && A.mlp.max_normal_residual > 0.0f) {
+                        // The learned solve reached the tolerance: its normal
+                        // residuals, computed as the residual pass below does,
+                        // must also stay within max_normal_residual.
+                        float* res_n = e.resid;
+                        for (uint32_t c = 0; c < ncons; ++c) {
+                            if ((uniform_u32(e.cpair[c] >> 16) & mr::kHotArticulated) == 0u) continue;
+                            const float j0 = e.jrow[(size_t)c * 3u * mnv + row_lane];
+                            const float a0 = wave_fold(dof_lane ? j0 * vj : 0.0f);
+                            if (lane == 0u) res_n[c] = a0;
+                        }
+                        __syncthreads();
+                        bool over = false;
+                        for (uint32_t c = lane; c < ncons; c += kLanes) {
+                            const HotConstraint hk = e.hot[c];
+                            float rel0 = (hk.kind & mr::kHotArticulated) != 0u ? res_n[c] : 0.0f;
+                            if ((hk.kind & (mr::kHotDynamicA | mr::kHotDynamicB)) != 0u) {
+                                V3 va, vb;
+                                scene_point_velocities(e, hk.kind, hk.body_a, hk.body_b,
+                                                       V3{hk.point[0], hk.point[1], hk.point[2]}, va, vb);
+                                rel0 = rel_row(hk.kind, hot_dir(hk, 0), va, vb, rel0);
+                            }
+                            const float l0 = e.lam[3u * c];
+                            const float eq = (rel0 - hk.target0) + hk.reg0 * l0;
+                            const float nres = l0 > kConeEpsilon ? fabs_(eq) : fmaxd(-eq, 0.0f);
+                            over = over || !(nres <= A.mlp.max_normal_residual);
+                        }
+                        diverged = __ballot(over) != 0ull;
+                        __syncthreads();
+                    }
+                    if (missed || diverged) {
+                        vj = vj_free;
+                        for (uint32_t i = lane; i < 3u * ncons; i += kLanes) e.lam[i] = e.mlp_lam[i];
+                        for (uint32_t b = lane; b < nsc; b += kLanes) e.scene[b] = e.mlp_scene[b];
+                        if (lane == 0u) {
+                            cs.diagnostics.z += 1.0f;
+                            if (diverged) cs.diagnostics.w += 1.0f;
+                        }
+                        __syncthreads();
+                        continue;
+                    }
                 }
             }
             break;
@@ -3242,7 +3301,7 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
         MRWorldStatusC ws;
         ws.code = code;
         ws.environment = env;
-        ws.control_step = A.control_step;
+        ws.control_step = A.control_step + step;
         ws.successful_substeps = successful;
         ws.aba_code = 0u;
         ws.failing_substep = failed ? failing_substep : MR_INVALID_INDEX;
@@ -3258,13 +3317,16 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
         } else {
             ws.diagnostics = mr_float4{0.0f, 0.0f, 0.0f, 0.0f};
         }
-        A.status[env] = ws;
+        A.status[step_rec] = ws;
         cs.code = code;
         if (failed) {
             if (code == MR_STEP_PAIR_CAPACITY_OVERFLOW) cs.first_failing_pair = findex;
             else cs.first_failing_constraint = findex;
         }
-        A.cstatus[env] = cs;
+        A.cstatus[step_rec] = cs;
+        // wall-clock duration of this control step of this world (the GPU's constant-rate clock)
+        if (A.step_us != nullptr)
+            A.step_us[step_rec] = A.us_per_tick * (float)(__builtin_amdgcn_s_memrealtime() - s_t0);
     }
 }
 
@@ -3295,6 +3357,30 @@ mr_control_step_queue_kernel(KernelArgs args) {
     }
 }
 
+// Persistent horizon instance: the work queue of mr_control_step_queue_kernel,
+// where each entry is a world advanced through all args.horizon control steps
+// back to back (step t's command, reset mask and status records at [t][E]).
+// A world's next control step starts as soon as its previous one ends, so
+// the launch is not held to the slowest world of every step; each world's
+// steps run in order on one workgroup, so its results are those of one
+// launch per control step.
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+mr_control_horizon_queue_kernel(KernelArgs args) {
+    __shared__ uint32_t s_next;
+    for (;;) {
+        if (threadIdx.x == 0u) s_next = atomicAdd(args.queue_head, 1u);
+        __syncthreads();
+        const uint32_t idx = s_next;
+        __syncthreads();
+        if (idx >= args.queue_count) break;
+        const uint32_t env = args.queue[idx];
+        for (uint32_t t = 0; t < args.horizon; ++t) {
+            control_step<false>(args, env, t);
+            __syncthreads();
+        }
+    }
+}
+
 extern "C" __global__ void mr_queue_reset_kernel(uint32_t* head) {
     if (threadIdx.x == 0u && blockIdx.x == 0u) *head = 0u;
 }
@@ -3305,15 +3391,22 @@ extern "C" __global__ void mr_queue_reset_kernel(uint32_t* head) {
 // one workgroup. Only the schedule depends on it: which workgroup takes which
 // world never changes a world's results. Environments with equal counts land
 // in any order (atomic placement).
+//
+// With steps > 1 (after a persistent horizon launch, status holding [steps][E]
+// records) the key is each world's mean sweeps per control step over the
+// horizon, so the worlds that cost most over the whole horizon start first
+// (longest-processing-time-first order for the next horizon).
 extern "C" __global__ void __launch_bounds__(1024)
-mr_queue_order_kernel(const MRContactStatusC* status, uint32_t environments, uint32_t* order) {
+mr_queue_order_kernel(const MRContactStatusC* status, uint32_t environments, uint32_t steps, uint32_t* order) {
     __shared__ uint32_t bins[1024];
     __shared__ uint32_t wave_base[16];
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     bins[t] = 0u;
     __syncthreads();
     auto key_of = [&](uint32_t e) {
-        const float w = fminf(fmaxf(status[e].residuals.w, 0.0f), 1023.0f);  // NaN -> 0
+        float sum = 0.0f;
+        for (uint32_t s = 0; s < steps; ++s) sum += status[(size_t)s * environments + e].residuals.w;
+        const float w = fminf(fmaxf(sum / (float)steps, 0.0f), 1023.0f);  // NaN -> 0
         return 1023u - (uint32_t)w;
     };
     for (uint32_t e = t; e < environments; e += 1024u) atomicAdd(&bins[key_of(e)], 1u);
@@ -3428,6 +3521,11 @@ static_assert(sizeof(GeoBlock) == 64 && offsetof(HotConstraint, body_a) == sizeo
 
 hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream, bool mlp,
                                uint32_t queue_grid) {
+    if (queue_grid > 0u && args.horizon > 0u) {
+        hipLaunchKernelGGL(mr_queue_reset_kernel, dim3(1), dim3(64), 0, stream, args.queue_head);
+        hipLaunchKernelGGL(mr_control_horizon_queue_kernel, dim3(queue_grid), dim3(64), lds_bytes, stream, args);
+        return hipGetLastError();
+    }
     if (queue_grid > 0u) {
         hipLaunchKernelGGL(mr_queue_reset_kernel, dim3(1), dim3(64), 0, stream, args.queue_head);
         hipLaunchKernelGGL(mr_control_step_queue_kernel, dim3(queue_grid), dim3(64), lds_bytes, stream, args);
@@ -3470,9 +3568,60 @@ hipError_t static_scratch(void** out, size_t* bytes) {
     return hipSuccess;
 }
 
-hipError_t launch_queue_order(const MRContactStatusC* status, uint32_t environments, uint32_t* order,
+// Spins one thread until the constant-rate clock (s_memrealtime) has advanced
+// `ticks`, or `max_iterations` reads have passed, and reports the ticks it saw;
+// timed with events on the host, it gives the clock's rate.
+extern "C" __global__ void mr_clock_spin_kernel(unsigned long long ticks, unsigned long long max_iterations,
+                                                unsigned long long* seen) {
+    if (threadIdx.x != 0u || blockIdx.x != 0u) return;
+    const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+    unsigned long long t = t0;
+    for (unsigned long long i = 0; i < max_iterations && t - t0 < ticks; ++i) t = __builtin_amdgcn_s_memrealtime();
+    *seen = t - t0;
+}
+
+// Microseconds per tick of s_memrealtime on the current device, measured once
+// per device: a spin of 2^20 ticks (a few milliseconds at the rates GPUs use)
+// timed with events.
+hipError_t clock_us_per_tick(float* out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex guard;
+    static float cached[kMaxDevices] = {};
+    int device = -1;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(guard);
+    if (cached[device] == 0.0f) {
+        unsigned long long* seen = nullptr;
+        hipEvent_t a = nullptr, b = nullptr;
+        e = hipMalloc(&seen, sizeof(unsigned long long));
+        if (e == hipSuccess) e = hipEventCreate(&a);
+        if (e == hipSuccess) e = hipEventCreate(&b);
+        float ms = 0.0f;
+        unsigned long long ticks = 0ull;
+        if (e == hipSuccess) {
+            (void)hipEventRecord(a, nullptr);
+            hipLaunchKernelGGL(mr_clock_spin_kernel, dim3(1), dim3(64), 0, nullptr, 1ull << 20, 1ull << 26, seen);
+            (void)hipEventRecord(b, nullptr);
+            e = hipEventSynchronize(b);
+        }
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        if (e == hipSuccess) e = hipMemcpy(&ticks, seen, sizeof(ticks), hipMemcpyDeviceToHost);
+        if (a != nullptr) (void)hipEventDestroy(a);
+        if (b != nullptr) (void)hipEventDestroy(b);
+        if (seen != nullptr) (void)hipFree(seen);
+        if (e != hipSuccess) return e;
+        if (ticks == 0ull || !(ms > 0.0f)) return hipErrorUnknown;
+        cached[device] = 1000.0f * ms / (float)ticks;
+    }
+    *out = cached[device];
+    return hipSuccess;
+}
+
+hipError_t launch_queue_order(const MRContactStatusC* status, uint32_t environments, uint32_t steps, uint32_t* order,
                               hipStream_t stream) {
-    hipLaunchKernelGGL(mr_queue_order_kernel, dim3(1), dim3(1024), 0, stream, status, environments, order);
+    hipLaunchKernelGGL(mr_queue_order_kernel, dim3(1), dim3(1024), 0, stream, status, environments, steps, order);
     return hipGetLastError();
 }
 
@@ -3494,6 +3643,9 @@ hipError_t configure_control_step(uint32_t lds_bytes, uint32_t mlp_lds_bytes) {
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
         if (e != hipSuccess) return e;
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mr_control_step_queue_kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mr_control_horizon_queue_kernel),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
         if (e != hipSuccess) return e;
         plain_max = lds_bytes;

@@ -12,13 +12,14 @@ namespace mr {
 hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipStream_t stream, bool mlp,
                                uint32_t queue_grid);
 hipError_t control_step_occupancy(uint32_t lds_bytes, int* blocks_per_cu);
-hipError_t launch_queue_order(const MRContactStatusC* status, uint32_t environments, uint32_t* order,
+hipError_t launch_queue_order(const MRContactStatusC* status, uint32_t environments, uint32_t steps, uint32_t* order,
                               hipStream_t stream);
 hipError_t configure_control_step(uint32_t lds_bytes, uint32_t mlp_lds_bytes);
 hipError_t launch_advance_summary(const MRWorldStatusC* status, const MRContactStatusC* cstatus, uint64_t steps,
                                   uint32_t environments, const uint32_t* order, uint32_t order_count,
                                   AdvanceSummaryDev* summary, hipStream_t stream);
 hipError_t static_scratch(void** out, size_t* bytes);
+hipError_t clock_us_per_tick(float* out);
 
 namespace {
 constexpr uint32_t kSpIBytes = 84u;
@@ -70,6 +71,7 @@ World::World(const ModelData& model, const MRWorldStepConfigC& config) : model_(
     finalize_model(model_, cfg_.control_timestep / (float)cfg_.physics_substeps);
     env_ = model_.environment_count();
     hip_check(hipGetDevice(&device_), "hipGetDevice");
+
     compute_lds_layout();
     hip_check(configure_control_step(lds_.total, lds_mlp_.total), "hipFuncSetAttribute");
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
@@ -138,7 +140,11 @@ World::~World() {
     for (hipEvent_t ev : step_events_) (void)hipEventDestroy(ev);
 }
 
-void World::set_step_timing(bool enabled) { step_timing_ = enabled; }
+void World::set_step_timing(bool enabled) {
+    // per-world step times are stamped on the constant-rate clock, whose rate is measured once per device
+    if (enabled && us_per_tick_ == 0.0f) hip_check(clock_us_per_tick(&us_per_tick_), "clock rate");
+    step_timing_ = enabled;
+}
 
 void World::set_stage_profiling(bool enabled) {
     stage_profiling_ = enabled;
@@ -154,6 +160,7 @@ void World::read_stage_cycles(unsigned long long* out) const {
 }
 
 void World::read_step_milliseconds(double* out, uint32_t steps) const {
+    if (last_horizon_) throw std::runtime_error("the last advance ran as one horizon launch: no per-launch times");
     if (!step_timing_ || steps > last_steps_ || step_events_.size() < (size_t)steps + 1u)
         throw std::runtime_error("step timing was not enabled for the last advance");
     hip_check(hipStreamSynchronize(stream_), "sync");
@@ -434,6 +441,7 @@ KernelArgs World::base_args(uint32_t step) const {
     a.queue = static_cast<const uint32_t*>(d_queue_.ptr);
     a.queue_count = queue_count_;
     a.queue_head = static_cast<uint32_t*>(d_queue_head_.ptr);
+    a.us_per_tick = us_per_tick_;
     return a;
 }
 
@@ -495,30 +503,56 @@ World::StepBuffers World::own_buffers() {
     return b;
 }
 
-void World::launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, bool events, const StepBuffers& buf) {
+void World::launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, bool events, const StepBuffers& buf,
+                         bool horizon) {
     const size_t E = env_;
     const size_t per_step = E * model_.max_nv;
+    // per-world step times only on eager launches (a captured graph keeps no pointer to this buffer)
+    float* step_us = events && step_timing_ && d_step_us_.bytes >= (size_t)steps * E * sizeof(float)
+        ? static_cast<float*>(d_step_us_.ptr) : nullptr;
     if (events) hip_check(hipEventRecord(ev_begin_, stream_), "event");
-    if (events && step_timing_) hip_check(hipEventRecord(step_events_[0], stream_), "event");
-    for (uint32_t s = 0; s < steps; ++s) {
-        KernelArgs a = base_args(s);
+    if (horizon) {
+        // one launch: every world runs all steps back to back
+        KernelArgs a = base_args(0u);
         a.stage_h = buf.stage_h;
         a.stage_p = buf.stage_p;
         a.jrow = buf.jrow;
         a.resp = buf.resp;
         a.cold = buf.cold;
         a.hot = buf.hot;
-        a.command = cmd + (size_t)s * per_step;
-        a.reset_mask = msk != nullptr ? msk + (size_t)s * E : nullptr;
-        a.status = buf.status + (size_t)s * E;
-        a.cstatus = buf.cstatus + (size_t)s * E;
-        hip_check(launch_control_step(a, mlp_instance() ? lds_mlp_.total : lds_.total, stream_, mlp_instance(),
-                                      mlp_instance() ? 0u : queue_grid_),
-                  "control step launch");
-        if (reorder_ && queue_grid_ > 0u)
-            hip_check(launch_queue_order(a.cstatus, (uint32_t)E, static_cast<uint32_t*>(d_queue_.ptr), stream_),
+        a.command = cmd;
+        a.reset_mask = msk;
+        a.status = buf.status;
+        a.cstatus = buf.cstatus;
+        a.step_us = step_us;
+        a.horizon = steps;
+        hip_check(launch_control_step(a, lds_.total, stream_, false, queue_grid_), "control horizon launch");
+        if (reorder_)
+            hip_check(launch_queue_order(buf.cstatus, (uint32_t)E, steps, static_cast<uint32_t*>(d_queue_.ptr), stream_),
                       "queue order launch");
-        if (events && step_timing_) hip_check(hipEventRecord(step_events_[s + 1u], stream_), "event");
+    } else {
+        if (events && step_timing_) hip_check(hipEventRecord(step_events_[0], stream_), "event");
+        for (uint32_t s = 0; s < steps; ++s) {
+            KernelArgs a = base_args(s);
+            a.stage_h = buf.stage_h;
+            a.stage_p = buf.stage_p;
+            a.jrow = buf.jrow;
+            a.resp = buf.resp;
+            a.cold = buf.cold;
+            a.hot = buf.hot;
+            a.command = cmd + (size_t)s * per_step;
+            a.reset_mask = msk != nullptr ? msk + (size_t)s * E : nullptr;
+            a.status = buf.status + (size_t)s * E;
+            a.cstatus = buf.cstatus + (size_t)s * E;
+            a.step_us = step_us != nullptr ? step_us + (size_t)s * E : nullptr;
+            hip_check(launch_control_step(a, mlp_instance() ? lds_mlp_.total : lds_.total, stream_, mlp_instance(),
+                                          mlp_instance() ? 0u : queue_grid_),
+                      "control step launch");
+            if (reorder_ && queue_grid_ > 0u)
+                hip_check(launch_queue_order(a.cstatus, (uint32_t)E, 1u, static_cast<uint32_t*>(d_queue_.ptr), stream_),
+                          "queue order launch");
+            if (events && step_timing_) hip_check(hipEventRecord(step_events_[s + 1u], stream_), "event");
+        }
     }
     if (events) hip_check(hipEventRecord(ev_end_, stream_), "event");
     // a subset work queue steps only the listed environments: fold only theirs
@@ -531,6 +565,7 @@ void World::launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, 
     last_status_ = buf.status;
     last_cstatus_ = buf.cstatus;
     last_summary_ = buf.summary;
+    last_horizon_ = horizon;
 }
 
 void World::set_schedule(const uint32_t* order, uint32_t count, uint32_t grid) {
@@ -661,10 +696,38 @@ void World::enqueue(const float* commands, const uint32_t* masks, uint32_t steps
             hip_check(hipEventCreate(&ev), "hipEventCreate");
             step_events_.push_back(ev);
         }
+        d_step_us_.reserve((size_t)steps * E * sizeof(float));
     }
     d_summary_.reserve(sizeof(AdvanceSummaryDev));
     launch_steps(cmd, msk, steps, true, own_buffers());
     last_steps_ = steps;
+}
+
+void World::enqueue_horizon(const float* device_commands, const uint32_t* device_masks, uint32_t steps) {
+    if (steps == 0u) throw std::runtime_error("control_step_count must be positive");
+    if (device_commands == nullptr) throw std::runtime_error("commands are required");
+    if (queue_grid_ == 0u)
+        throw std::runtime_error("enqueue_horizon: needs a work-queue schedule (mr_world_set_schedule with grid > 0)");
+    if (mlp_instance())
+        throw std::runtime_error("enqueue_horizon: the learned warm start runs one workgroup per environment");
+    const size_t E = env_;
+    if (stage_profiling_)
+        hip_check(hipMemsetAsync(d_stage_cycles_.ptr, 0, E * kStageCount * sizeof(unsigned long long), stream_),
+                  "clear stage cycles");
+    d_status_.reserve((size_t)steps * E * sizeof(MRWorldStatusC));
+    d_cstatus_.reserve((size_t)steps * E * sizeof(MRContactStatusC));
+    if (step_timing_) d_step_us_.reserve((size_t)steps * E * sizeof(float));
+    d_summary_.reserve(sizeof(AdvanceSummaryDev));
+    launch_steps(device_commands, device_masks, steps, true, own_buffers(), true);
+    last_steps_ = steps;
+}
+
+void World::read_world_step_microseconds(float* out, uint32_t steps) const {
+    if (!step_timing_ || steps > last_steps_ || d_step_us_.bytes < (size_t)steps * env_ * sizeof(float))
+        throw std::runtime_error("step timing was not enabled for the last advance");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    hip_check(hipMemcpy(out, d_step_us_.ptr, (size_t)steps * env_ * sizeof(float), hipMemcpyDeviceToHost),
+              "read world step times");
 }
 
 namespace {

@@ -70,6 +70,13 @@ public:
     // [steps][E]. Throws when the world's scratch exceeds the array.
     void enqueue_static(const float* device_commands, const uint32_t* device_masks, uint32_t steps,
                         MRWorldStatusC* device_status, MRContactStatusC* device_cstatus);
+    // Persistent horizon: every world of the work-queue schedule runs all
+    // `steps` control steps inside one launch, each world's next step starting
+    // when its previous one ends (device inputs [steps][E][max_nv], [steps][E]).
+    void enqueue_horizon(const float* device_commands, const uint32_t* device_masks, uint32_t steps);
+    // Wall-clock microseconds of each control step of each world in the last
+    // advance, [steps][E] (step timing on).
+    void read_world_step_microseconds(float* out, uint32_t steps) const;
     // Bytes of static scratch enqueue_static needs for this world.
     size_t static_scratch_bytes() const;
     // Waits for the stream and summarises the statuses of the last enqueue.
@@ -150,12 +157,15 @@ private:
     const void* graph_cstatus_ = nullptr;
     const void* graph_summary_ = nullptr;
     uint32_t graph_schedule_ = 0;  // schedule_version_ the graph was captured with
-    void launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, bool events, const StepBuffers& buf);
+    void launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, bool events, const StepBuffers& buf,
+                      bool horizon = false);
+    bool last_horizon_ = false;  // the last advance ran as one persistent horizon launch
     // where the last advance wrote its statuses and summary
     const MRWorldStatusC* last_status_ = nullptr;
     const MRContactStatusC* last_cstatus_ = nullptr;
     const AdvanceSummaryDev* last_summary_ = nullptr;
     int device_ = -1;  // the device current at creation
+    float us_per_tick_ = 0.0f;  // of the device's constant-rate clock, measured when step timing is enabled
     void drop_graph();
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_begin_ = nullptr, ev_end_ = nullptr;
@@ -167,6 +177,7 @@ private:
     // per-advance inputs / outputs
     DeviceBuffer d_commands_, d_masks_, d_status_, d_cstatus_, d_accel_, d_ev_pairs_, d_ev_counts_;
     // scratch
+    DeviceBuffer d_step_us_;
     DeviceBuffer d_stage_h_, d_stage_p_, d_jrow_, d_resp_, d_cold_, d_hot_, d_stage_cycles_, d_summary_, d_samples_, d_queue_, d_queue_head_;
     bool stage_profiling_ = false;
     uint32_t last_steps_ = 0;

@@ -181,7 +181,8 @@ class MlpWeightsC(ct.Structure):
     """Mirror of MRMlpWeightsC (include/mr_mlp.h): device pointers."""
     _fields_ = [("w1", ct.c_void_p), ("b1", ct.c_void_p), ("w2", ct.c_void_p), ("b2", ct.c_void_p),
                 ("w3", ct.c_void_p), ("b3", ct.c_void_p), ("in_shift", ct.c_void_p), ("in_scale", ct.c_void_p),
-                ("out_scale", ct.c_void_p), ("min_correction", ct.c_float)]
+                ("out_scale", ct.c_void_p), ("min_correction", ct.c_float),
+                ("max_normal_residual", ct.c_float)]
 
 
 class MlpTraceC(ct.Structure):
@@ -264,6 +265,7 @@ assert ct.sizeof(WorldStepConfigC) == 88
 assert ct.sizeof(WorldModelDescC) == 216
 assert ct.sizeof(WorldAdvanceC) == 40
 assert ct.sizeof(WorldLayoutC) == 40
+assert ct.sizeof(MlpWeightsC) == 80
 # the reference's task-rollout records (c_api.h), LP64
 assert ct.sizeof(TaskRolloutDynamicSphereC) == 36
 assert ct.sizeof(TaskRolloutConfigC) == 104
@@ -331,6 +333,8 @@ _SIGNATURES: Final = {
     "mr_world_advance": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_void_p, ct.c_size_t, ct.c_uint32,
                           ct.POINTER(WorldAdvanceC)], ct.c_int),
     "mr_world_enqueue_device": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32], ct.c_int),
+    "mr_world_enqueue_device_horizon": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32], ct.c_int),
+    "mr_world_read_world_step_microseconds": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
     "mr_world_enqueue_device_static": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32, ct.c_void_p,
                                         ct.c_void_p], ct.c_int),
     "mr_world_wait": ([ct.c_void_p, ct.POINTER(WorldAdvanceC)], ct.c_int),
@@ -635,6 +639,20 @@ class World:
         self._b.check(self._b.lib.mr_world_enqueue_device(self._h, commands.data_ptr(), mask_ptr, steps))
         self.last_steps = steps
 
+    def enqueue_device_horizon(self, commands, reset_masks=None) -> None:
+        """mr_world_enqueue_device_horizon: with a work-queue schedule, every scheduled world runs all
+        control steps of `commands` ([steps][E][max_nv] on the GPU) inside one persistent launch."""
+        steps = self._device_commands(commands)
+        mask_ptr = self._device_masks(reset_masks, steps)
+        self._b.check(self._b.lib.mr_world_enqueue_device_horizon(self._h, commands.data_ptr(), mask_ptr, steps))
+        self.last_steps = steps
+
+    def world_step_microseconds(self) -> np.ndarray:
+        """[steps][E] wall-clock microseconds of each control step of each world (step timing on)."""
+        out = np.zeros((self.last_steps, self.environment_count), np.float32)
+        self._b.check(self._b.lib.mr_world_read_world_step_microseconds(self._h, _ptr(out), out.size))
+        return out
+
     def enqueue_device_static(self, commands, status, contact_status, reset_masks=None) -> None:
         """mr_world_enqueue_device_static: enqueue_device that allocates no device memory. The
         per-step scratch comes from the library's static device array; `status` and
@@ -732,9 +750,11 @@ class World:
         self._b.check(self._b.lib.mr_world_set_mlp_capture(self._h, 1 if enabled else 0))
 
     def mlp_samples(self) -> np.ndarray:
-        """[E][constraint_capacity][12]: 8 features, converged impulse, valid flag (last captured step)."""
+        """[E][constraint_capacity][20]: 8 features, converged impulse (3), valid flag, the warm start the
+        solve began from (3), 1.0 where the network replaced the plain warm start, the network's scaled
+        correction (3) and 1.0 where its features were in bounds (last captured step)."""
         cap = self.config.constraint_capacity
-        out = np.zeros((self.environment_count, cap, 12), np.float32)
+        out = np.zeros((self.environment_count, cap, 20), np.float32)
         self._b.check(self._b.lib.mr_world_read_mlp_samples(self._h, _ptr(out), out.size))
         return out
 

@@ -2,8 +2,9 @@
 over time and are served from one resident pool of worlds kept in HBM.
 
 Admission: a request takes N free slots of the pool when they exist and the
-pool's device memory, at the measured bytes per slot, fits the HBM budget;
-otherwise it waits (first come, first served). Its slots start from the
+device memory of all admitted requests' worlds, at the measured bytes per
+slot, stays within the HBM budget; otherwise it waits (first come, first
+served). A request whose own worlds exceed the budget or the pool is refused. Its slots start from the
 initial state through the control-step kernel's reset mask. Every tick
 advances all active slots by one control step in a single launch of the
 work-queue kernel over exactly those slots (inactive slots are not touched),
@@ -35,6 +36,7 @@ class Request:
     done: int = 0
     admitted: float = -1.0
     finished: float = -1.0
+    first_tick: int = -1  # the tick of its first control step (its command-bank row is first_tick % bank)
 
 
 def poisson_requests(rate: float, count: int, seed: int = 0, worlds=(1, 1024), horizon=(10, 500)) -> list:
@@ -67,6 +69,8 @@ class ServeReport:
     capacity_slots: int
     bytes_per_slot: float
     hbm_budget_bytes: float
+    max_admitted_bytes: float = 0.0
+    budget_blocked_ticks: int = 0  # ticks on which an arrived request waited on the budget with enough free slots
     tick_ms: list = field(default_factory=list, repr=False)
     results: dict = field(default_factory=dict, repr=False)
 
@@ -84,8 +88,7 @@ class StepServer:
         E = model.environment_count
         layout = self.world.layout()
         self.bytes_per_slot = float(layout.retained_device_bytes) / E
-        self.hbm_budget = float(hbm_budget_bytes)
-        self.capacity = int(min(E, self.hbm_budget // max(self.bytes_per_slot, 1.0)))
+        self.set_hbm_budget(hbm_budget_bytes)
         q0, v0, s0 = model.initial_state()
         self.world.set_reset_state(q0, v0, s0)
         self.grid = self.world.resident_workgroups()
@@ -93,6 +96,11 @@ class StepServer:
         self.commands = torch.from_numpy(model.commands(0, command_bank)).to("cuda")
         self.mask_host = np.zeros((1, E), np.uint32)
         self.mask = torch.zeros((1, E), dtype=torch.int32, device="cuda")
+
+    def set_hbm_budget(self, hbm_budget_bytes: float) -> None:
+        """Device-memory budget of the admitted worlds; capacity is the number of slots it covers."""
+        self.hbm_budget = float(hbm_budget_bytes)
+        self.capacity = int(min(self.model.environment_count, self.hbm_budget // max(self.bytes_per_slot, 1.0)))
 
     def serve(self, requests, idle_sleep: bool = True, return_states: bool = False) -> ServeReport:
         """Serves `requests` to completion. With `return_states`, the final (q, v, scene) rows of
@@ -102,9 +110,10 @@ class StepServer:
 
         too_big = [r.rid for r in requests if r.worlds > self.capacity]
         if too_big:
-            raise ValueError(f"requests {too_big} need more worlds than the pool's {self.capacity} slots")
+            raise ValueError(f"requests {too_big} need more worlds than the pool and the HBM budget hold "
+                             f"({self.capacity} slots)")
         pending = deque(sorted(requests, key=lambda r: r.arrival))
-        free = deque(range(self.capacity))
+        free = deque(range(self.model.environment_count))
         active = []
         tick_ms, active_counts = [], []
         env_steps = 0
@@ -112,13 +121,22 @@ class StepServer:
         t0 = time.perf_counter()
         tick = 0
         changed = True  # the active set differs from the one the schedule holds
+        admitted_bytes, max_admitted_bytes, budget_blocked = 0.0, 0.0, 0
         while pending or active:
             now = time.perf_counter() - t0
             admitted = []
             while pending and pending[0].arrival <= now and len(free) >= pending[0].worlds:
-                r = pending.popleft()
+                r = pending[0]
+                need = r.worlds * self.bytes_per_slot
+                if admitted_bytes + need > self.hbm_budget:
+                    budget_blocked += 1
+                    break
+                pending.popleft()
                 r.slots = np.array([free.popleft() for _ in range(r.worlds)], np.uint32)
                 r.admitted = now
+                r.first_tick = tick
+                admitted_bytes += need
+                max_admitted_bytes = max(max_admitted_bytes, admitted_bytes)
                 admitted.append(r)
             if not active and not admitted:
                 if pending and idle_sleep:
@@ -157,6 +175,7 @@ class StepServer:
                         states = states if states is not None else self.world.state()
                         results[r.rid] = tuple(a[r.slots.astype(np.int64)].copy() for a in states)
                     free.extend(int(s) for s in r.slots)
+                    admitted_bytes -= r.worlds * self.bytes_per_slot
                     changed = True
                 else:
                     still.append(r)
@@ -172,6 +191,7 @@ class StepServer:
             max_active_worlds=int(max(active_counts)) if active_counts else 0,
             queue_wait_s_p50=float(np.percentile(waits, 50)), queue_wait_s_p99=float(np.percentile(waits, 99)),
             capacity_slots=self.capacity, bytes_per_slot=self.bytes_per_slot, hbm_budget_bytes=self.hbm_budget,
+            max_admitted_bytes=max_admitted_bytes, budget_blocked_ticks=budget_blocked,
             tick_ms=tick_ms, results=results)
 
 

@@ -79,6 +79,9 @@ class WarmStartMLP:
         self.out_scale = torch.ones(OUT, dtype=torch.float32, device=device)
         self.step = 0
         self.min_correction = 2.0  # impulse units; see MRMlpWeightsC.min_correction
+        # a converged learned solve with a larger normal residual is re-solved from the plain warm
+        # start (MRMlpWeightsC.max_normal_residual); the bound evaluate() calls healthy
+        self.max_normal_residual = 1.0
         self._bufs = {}
 
     # -------------------------------------------------------------- views
@@ -86,7 +89,7 @@ class WarmStartMLP:
         return nat.MlpWeightsC(_p(self.bf16["w1"]), _p(self.master["b1"]), _p(self.bf16["w2"]),
                                _p(self.master["b2"]), _p(self.bf16["w3"]), _p(self.master["b3"]),
                                _p(self.in_shift), _p(self.in_scale), _p(self.out_scale),
-                               float(self.min_correction))
+                               float(self.min_correction), float(self.max_normal_residual))
 
     def set_normalization(self, features, targets) -> None:
         """Feature mean / inverse std and target std from training data (rows x 8, rows x 3)."""
@@ -251,7 +254,7 @@ def collect_samples(model, cfg, control_steps: int, settle_steps: int = 40, max_
     feats, targs, total = [], [], 0
     for t in range(control_steps):
         w.advance(cmds[t:t + 1])
-        rows = w.mlp_samples().reshape(-1, 12)
+        rows = w.mlp_samples().reshape(-1, 20)
         rows = rows[rows[:, 11] == 1.0]
         feats.append(rows[:, :8].copy())
         targs.append(rows[:, 8:11].copy())

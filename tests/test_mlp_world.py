@@ -163,6 +163,108 @@ def test_forced_fallback_reproduces_the_plain_path(gpu):
     assert np.all(fc["diagnostics"][..., 2] == pc["residuals"][..., 3] / 3.0)
 
 
+def _project_cone(lam, mu_u, mu_v):
+    """The cone projection the kernel applies to a learned warm start (normal clamp, radial
+    elliptic scaling of the tangents), in float32."""
+    lam = lam.astype(np.float32).copy()
+    f32 = np.float32
+    lam[:, 0] = np.maximum(lam[:, 0], f32(0.0))
+    lu, lv = np.float32(mu_u) * lam[:, 0], np.float32(mu_v) * lam[:, 0]
+    ok = (lu > 0) & (lv > 0)
+    lam[~ok, 1:] = 0.0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ns = (lam[:, 1] * lam[:, 1]) / (lu * lu) + (lam[:, 2] * lam[:, 2]) / (lv * lv)
+        s = (f32(1.0) / np.sqrt(ns)).astype(np.float32)
+    big = ok & (ns > 1.0)
+    lam[big, 1] *= s[big]
+    lam[big, 2] *= s[big]
+    return lam
+
+
+@pytest.mark.gpu
+def test_in_solver_inference_matches_the_mlp_oracle(gpu):
+    """Trained weights inside the solver, against the CPU restatement oracle/mlp_oracle.py on the
+    features the kernel captured at the start of each contact's cone solve:
+      * the network's scaled correction (normalize, forward with BF16 operands, scale) agrees
+        with the oracle within 2e-3 of each output's scale, the forward-pass tolerance (BF16
+        operands, FP32 against float64 accumulation), and the in-bounds flag agrees exactly;
+      * given the kernel's own correction, the decision and the warm start are exact: a contact
+        is replaced iff its features are in bounds and some component exceeds min_correction,
+        and then its warm start is the previous impulse plus the correction projected onto the
+        mu = 0.8 cone (normal clamp, radial tangent scaling), bit for bit; otherwise it keeps
+        the previous impulse bit for bit."""
+    import os
+    import sys
+
+    import metalrobo as mr
+    from metalrobo import warmstart as ws
+
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import mlp_oracle as mo
+
+    model, cfg = chain_world(envs=32, residual_tolerance=1.0e-5, max_iterations=128, capture_substep=1)
+    mats = model.materials()["friction"]
+    assert np.all(mats[:, :2] == np.float32(0.8))
+    f, t = ws.collect_samples(model, cfg, control_steps=8, settle_steps=40)
+    mlp = ws.WarmStartMLP(seed=5)
+    mlp.fit(f, t, steps=150, batch=4096)
+    gate = np.float32(0.05)
+    mlp.min_correction = float(gate)
+    mlp.max_normal_residual = 0.0
+    q0, v0, s0, h0, p0, c0 = settled(model)
+    w = mr.World(model, cfg)
+    w.set_state(q0, v0, s0)
+    w.write_manifolds(h0, p0, c0)
+    w.set_warm_start_mlp(mlp)
+    w.set_mlp_capture(True)
+    w.advance(model.commands(40, 1))
+    rows = w.mlp_samples().reshape(-1, 20)
+    rows = rows[rows[:, 11] == 1.0]
+    assert len(rows) > 200
+    st = mlp.numpy_state()
+    x = rows[:, :8]
+    xn = ((x - st["in_shift"]) * st["in_scale"]).astype(np.float32)
+    in_range = np.all(np.abs(xn) <= 8.0, axis=1)
+    np.testing.assert_array_equal(rows[:, 19] == 1.0, in_range)
+    d_ref = mo.forward(st, x)["y_hat"] * st["out_scale"].astype(np.float64)
+    d_gpu = rows[:, 16:19]
+    for k in range(3):
+        scale = max(float(np.abs(d_ref[in_range, k]).max()), 1e-12)
+        err = float(np.abs(d_gpu[in_range, k] - d_ref[in_range, k]).max())
+        assert err <= 2e-3 * scale, (k, err, scale)
+    changed = in_range & (np.abs(d_gpu).max(axis=1) > gate)
+    replaced = rows[:, 15] == 1.0
+    print(f"in-solver inference: {len(rows)} contacts, {int(in_range.sum())} in bounds, "
+          f"{int(replaced.sum())} warm starts replaced", flush=True)
+    assert replaced.sum() > 0 and (~replaced).sum() > 0
+    np.testing.assert_array_equal(replaced, changed)
+    keep = ~replaced
+    assert np.array_equal(rows[keep, 12:15].view(np.uint32), rows[keep, 5:8].view(np.uint32))
+    mu = np.sqrt(np.float32(0.8) * np.float32(0.8))  # the kernel's material mixing, sqrt(mu_a mu_b)
+    expect = _project_cone((rows[:, 5:8] + d_gpu).astype(np.float32), mu, mu)
+    assert np.array_equal(rows[replaced, 12:15].view(np.uint32), expect[replaced].view(np.uint32))
+    w.close()
+
+
+@pytest.mark.gpu
+def test_residual_guard_falls_back_to_the_plain_path(gpu):
+    """A learned solve that reaches the tolerance but leaves a normal residual above
+    max_normal_residual is solved again from the plain warm start. With the bound at 1e-30
+    every microstep the network touched falls back, so the MLP instance reproduces the plain
+    path bit for bit and counts the residual fallbacks (contact status diagnostics.w) among
+    all fallbacks (diagnostics.z)."""
+    model, cfg = chain_world(residual_tolerance=1.0e-5, max_iterations=128)
+    mlp = zero_mlp()
+    mlp.master["b3"].fill_(0.25)  # a correction that replaces every warm start
+    mlp.max_normal_residual = 1.0e-30
+    _, plain = run(model, cfg, 3)
+    _, guarded = run(model, cfg, 3, setup=lambda w: w.set_warm_start_mlp(mlp))
+    assert_same(plain[:7], guarded[:7])
+    gc = guarded[7]
+    assert gc["diagnostics"][..., 3].sum() > 0
+    assert np.all(gc["diagnostics"][..., 3] <= gc["diagnostics"][..., 2])
+
+
 @pytest.mark.gpu
 def test_trained_warm_start_against_the_plain_one(gpu):
     """Trains the MLP on samples the plain solve produced (chain-and-boxes
@@ -187,7 +289,9 @@ def test_trained_warm_start_against_the_plain_one(gpu):
     ev = ws.evaluate(ev_model, cfg, mlp, control_steps=5, settle_steps=40)
     plain, learned = ev["plain"]["sweeps_per_step"], ev["mlp"]["sweeps_per_step"]
     print(f"sweeps per control step: plain {plain:.1f}, learned {learned:.1f} "
-          f"({100.0 * (1.0 - learned / plain):+.1f} % saved), fallbacks {ev['mlp']['fallbacks_per_step']:.3f}")
+          f"({100.0 * (1.0 - learned / plain):+.1f} % saved), fallbacks {ev['mlp']['fallbacks_per_step']:.3f}, "
+          f"healthy worlds leaving the bound {ev['mlp']['healthy_worlds_leaving_bound']}, "
+          f"ms per step plain {ev['plain']['ms_per_step']:.2f} learned {ev['mlp']['ms_per_step']:.2f}", flush=True)
     assert ev["healthy_worlds"] > 0
     assert learned <= 2.0 * plain + 1e-9
 

@@ -165,3 +165,55 @@ def test_learned_warm_start_capture_refuses_a_work_queue(gpu):
     w.set_schedule(np.arange(8), grid=4)
     with pytest.raises(mr.MetalRoboError, match="work-queue"):
         w.set_mlp_capture(True)
+
+
+@pytest.mark.parametrize("kind,envs,grid", [(mr.MR_SYNTHETIC_RANDOM_TREES, 64, 24),
+                                            (mr.MR_SYNTHETIC_HUMANOID_TERRAIN, 48, 16)])
+def test_persistent_horizon_matches_one_launch_per_step(gpu, kind, envs, grid):
+    """mr_world_enqueue_device_horizon: every world runs all control steps in one persistent
+    launch (fewer workgroups than worlds, reversed order, adaptive residual solve with the
+    reordered queue, a reset mask mid-horizon). States, manifolds, statuses and contact
+    statuses equal one plain launch per control step bit for bit, and the per-world step
+    times are positive and within the launch."""
+    import torch
+
+    print(f"horizon: kind {kind}, {envs} worlds, grid {grid}", flush=True)
+    model = mr.WorldModel.synthetic(kind, envs, 5)
+    cfg = model.default_config()
+    cfg.residual_tolerance = 1.0e-5
+    cfg.max_iterations = 64
+    steps = 6
+    host = model.commands(0, steps)
+    masks = np.zeros((steps, envs), np.uint32)
+    masks[3, 1] = 1
+    masks[4, envs - 2] = 1
+    ref = mr.World(model, cfg)
+    ref.set_step_timing(True)
+    ref.advance(host, masks)
+    # one launch per step: every world's step ends within its launch
+    assert np.all(ref.world_step_microseconds() <= ref.step_milliseconds()[:, None] * 1e3 * 1.01)
+    hz = mr.World(model, cfg)
+    hz.set_schedule(np.arange(envs)[::-1], grid=grid)
+    hz.set_schedule_reorder(True)
+    hz.set_step_timing(True)
+    cmds = torch.from_numpy(host).cuda()
+    m_dev = torch.from_numpy(masks.view(np.int32)).cuda()
+    hz.enqueue_device_horizon(cmds, m_dev)
+    adv = hz.wait()
+    assert same(snapshot(ref), snapshot(hz))
+    for a, b in ((ref.statuses(), hz.statuses()), (ref.contact_statuses(), hz.contact_statuses())):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    assert ref.contact_statuses()["constraints"].sum() > 0
+    assert adv.control_step_count == steps and adv.failed_environment_steps == int(
+        np.sum(ref.statuses()["code"] != 0))
+    us = hz.world_step_microseconds()
+    assert us.shape == (steps, envs) and np.all(us > 0.0)
+    launch_us = hz.last_kernel_milliseconds() * steps * 1e3  # the per-step average times the steps
+    print(f"world step times: max {us.max():.1f} us, largest per-world sum {us.sum(axis=0).max():.1f} us, "
+          f"launch {launch_us:.1f} us", flush=True)
+    assert us.sum(axis=0).max() <= launch_us * 1.01
+    with pytest.raises(mr.MetalRoboError):
+        hz.step_milliseconds()
+    plain = mr.World(model, cfg)
+    with pytest.raises(mr.MetalRoboError, match="work-queue schedule"):
+        plain.enqueue_device_horizon(cmds)
