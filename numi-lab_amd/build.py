@@ -30,6 +30,8 @@ HOST_FLAGS = [
 SOURCES = [
     ("mr_kernels.hip", "hip"),
     ("mr_mlp.hip", "hip"),
+    ("mr_task.hip", "hip"),
+    ("mr_task_host.cpp", "cxx"),
     ("mr_builder.cpp", "cxx"),
     ("mr_world.cpp", "cxx"),
     ("mr_c_api.cpp", "cxx"),

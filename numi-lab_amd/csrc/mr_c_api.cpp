@@ -4,6 +4,7 @@
 
 #include "mr_builder.h"
 #include "mr_mlp_host.h"
+#include "mr_task_host.h"
 #include "mr_world.h"
 
 #include <algorithm>
@@ -21,6 +22,10 @@ struct MRWorldModelHandle {
 
 struct MRWorldHandle {
     std::unique_ptr<mr::World> world;
+};
+
+struct MRLocomotionTaskHandle {
+    std::unique_ptr<mr::LocomotionTask> task;
 };
 
 struct MRTaskRolloutHandle {
@@ -794,6 +799,101 @@ int mr_mlp_adam(float* param, uint16_t* param_bf16, const float* grad, float* m1
         mr::hip_check(mr::mlp_adam(param, param_bf16, grad, m1, m2, count, lr, beta1, beta2, eps, step,
                                static_cast<hipStream_t>(stream)),
                   "mr_mlp_adam launch");
+    });
+}
+
+/* ------------------------------------------------- locomotion task --- */
+
+MRLocomotionTaskHandle* mr_locomotion_task_create(const MRLocomotionTaskProgramC* program, MRWorldHandle* world) {
+    return construct<MRLocomotionTaskHandle>([&]() {
+        require(program != nullptr && world != nullptr, "null argument");
+        auto* h = new MRLocomotionTaskHandle();
+        try {
+            h->task = std::make_unique<mr::LocomotionTask>(*program, *world->world);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        return h;
+    });
+}
+
+void mr_locomotion_task_destroy(MRLocomotionTaskHandle* task) { delete task; }
+
+int mr_locomotion_task_reset(MRLocomotionTaskHandle* task) {
+    return translate([&]() {
+        require(task != nullptr, "null argument");
+        task->task->reset();
+    });
+}
+
+int mr_locomotion_task_step(MRLocomotionTaskHandle* task, const float* device_actions) {
+    return translate([&]() {
+        require(task != nullptr && device_actions != nullptr, "null argument");
+        task->task->step(device_actions);
+    });
+}
+
+int mr_locomotion_task_read(MRLocomotionTaskHandle* task, float* observations, size_t observation_count,
+                            MRTaskTransitionC* transitions, size_t transition_count, MRTaskStateC* states,
+                            size_t state_count) {
+    return translate([&]() {
+        require(task != nullptr, "null argument");
+        const mr::LocomotionTask& t = *task->task;
+        const size_t E = t.environments();
+        if (observations != nullptr) require_count(observation_count, E * t.observation_floats(), "observations");
+        if (transitions != nullptr) require_count(transition_count, E, "transitions");
+        if (states != nullptr) require_count(state_count, E, "task states");
+        t.read(observations, transitions, states);
+    });
+}
+
+const float* mr_locomotion_task_device_observations(const MRLocomotionTaskHandle* task) {
+    return task != nullptr ? task->task->device_observations() : nullptr;
+}
+
+const MRTaskTransitionC* mr_locomotion_task_device_transitions(const MRLocomotionTaskHandle* task) {
+    return task != nullptr ? task->task->device_transitions() : nullptr;
+}
+
+const float* mr_locomotion_task_device_targets(const MRLocomotionTaskHandle* task) {
+    return task != nullptr ? task->task->device_targets() : nullptr;
+}
+
+int mr_policy_dense_layer(const float* weights, const float* bias, const float* mean, const float* inverse_std,
+                          float observation_clip, const float* input, uint32_t input_stride, float* output,
+                          uint32_t output_stride, uint32_t environments, uint32_t input_width, uint32_t output_width,
+                          uint32_t activation, void* stream) {
+    return translate([&]() {
+        require(weights && bias && input && output, "mr_policy_dense_layer: null pointer");
+        require((mean == nullptr) == (inverse_std == nullptr), "mr_policy_dense_layer: mean and inverse_std go together");
+        require(input_width > 0u && output_width > 0u, "mr_policy_dense_layer: widths must be positive");
+        require(input_stride >= input_width && output_stride >= output_width, "mr_policy_dense_layer: strides below widths");
+        require(activation <= MR_POLICY_ACTIVATION_SILU, "mr_policy_dense_layer: unknown activation");
+        require(mean == nullptr || observation_clip > 0.0f, "mr_policy_dense_layer: observation_clip must be positive");
+        mr::hip_check(mr::launch_policy_dense(weights, bias, mean, inverse_std, observation_clip, input, input_stride,
+                                              output, output_stride, environments, input_width, output_width,
+                                              activation, static_cast<hipStream_t>(stream)),
+                      "mr_policy_dense_layer launch");
+    });
+}
+
+int mr_policy_sample_and_score(const float* actor_mean, uint32_t mean_stride, const float* action_bias,
+                               const float* action_scale, const float* log_std, float action_clip, uint64_t seed,
+                               uint64_t revision, const MRTaskStateC* task_states, uint32_t environments,
+                               uint32_t actions, uint32_t control_step, float* actions_out, float* latents_out,
+                               float* log_probabilities_out, void* stream) {
+    return translate([&]() {
+        require(actor_mean && action_bias && action_scale && task_states && actions_out && latents_out &&
+                    log_probabilities_out,
+                "mr_policy_sample_and_score: null pointer");
+        require(actions > 0u && mean_stride >= actions, "mr_policy_sample_and_score: action width");
+        require(action_clip > 0.0f, "mr_policy_sample_and_score: action_clip must be positive");
+        mr::hip_check(mr::launch_policy_sample(actor_mean, mean_stride, action_bias, action_scale, log_std, action_clip,
+                                               seed, revision, task_states, environments, actions, control_step,
+                                               actions_out, latents_out, log_probabilities_out,
+                                               static_cast<hipStream_t>(stream)),
+                      "mr_policy_sample_and_score launch");
     });
 }
 

@@ -386,6 +386,16 @@ void World::get_manifolds(MRManifoldHeaderC* h, MRManifoldPointC* p, uint32_t* c
     }
 }
 
+World::DeviceView World::device_view() const {
+    DeviceView d;
+    d.q = static_cast<float*>(d_q_.ptr);
+    d.v = static_cast<float*>(d_v_.ptr);
+    d.reset_q = static_cast<const float*>(d_reset_q_.ptr);
+    d.reset_v = static_cast<const float*>(d_reset_v_.ptr);
+    d.dofs = static_cast<const MRDofC*>(d_dofs_.ptr);
+    return d;
+}
+
 KernelArgs World::base_args(uint32_t step) const {
     KernelArgs a;
     std::memset(&a, 0, sizeof(a));

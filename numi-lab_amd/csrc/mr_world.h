@@ -51,6 +51,19 @@ public:
     size_t persistent_bytes() const;
     size_t transient_bytes() const;
     hipStream_t stream() const { return stream_; }
+    // Device arrays of the resident state and model, for kernels that run
+    // beside the control step on the same stream (the task layer).
+    struct DeviceView {
+        float* q;              // [E][max_nq]
+        float* v;              // [E][max_nv]
+        const float* reset_q;  // [E][max_nq]
+        const float* reset_v;  // [E][max_nv]
+        const MRDofC* dofs;    // the model's DoF table
+    };
+    DeviceView device_view() const;
+    // Statuses [steps][E] of the last advance, in device memory.
+    const MRWorldStatusC* last_status_device() const { return last_status_; }
+    float control_timestep() const { return cfg_.control_timestep; }
 
     // Host-array state boundary (all environment-major with model strides).
     void set_state(const float* q, const float* v, const MRBodyStateC* scene);

@@ -209,6 +209,20 @@ class TaskRolloutAdvanceC(ct.Structure):
     ]
 
 
+class LocomotionTaskProgramC(ct.Structure):
+    """MRLocomotionTaskProgramC (include/mr_task.h)."""
+    _fields_ = [
+        ("actions", ct.c_void_p), ("observations", ct.c_void_p), ("rewards", ct.c_void_p),
+        ("terminations", ct.c_void_p), ("default_q", ct.c_void_p), ("action_count", ct.c_uint32),
+        ("observation_count", ct.c_uint32), ("reward_count", ct.c_uint32), ("termination_count", ct.c_uint32),
+        ("history_length", ct.c_uint32), ("delay_slots", ct.c_uint32), ("max_episode_steps", ct.c_uint32),
+        ("flags", ct.c_uint32), ("root_q_offset", ct.c_uint32), ("root_v_offset", ct.c_uint32),
+        ("actuator_delay", ct.c_uint32), ("reserved", ct.c_uint32), ("root_reference", ct.c_float * 4),
+        ("base_height_target", ct.c_float), ("gait_period", ct.c_float), ("command_lower", ct.c_float * 4),
+        ("command_upper", ct.c_float * 4), ("command_seconds", ct.c_float * 2), ("seed", ct.c_uint64),
+    ]
+
+
 # numpy mirrors of the bulk records (include/mr_world_types.h)
 F4 = ("<f4", (4,))
 MODEL_HEADER_DTYPE: Final = np.dtype([(n, "<u4") for n in (
@@ -252,12 +266,28 @@ PAIR_RECORD_DTYPE: Final = np.dtype([("environment", "<u4"), ("collider_a", "<u4
 ACTUATOR_DTYPE: Final = np.dtype([("motor_and_speed", *F4), ("transmission_and_envelope", *F4),
                                   ("identity", "<u4", (4,))])
 MR_ACTUATOR_PROFILE_ACTIVE: Final = 1
+
+# task-program records (include/mr_task.h; task_program_types.h:445-623, byte for byte)
+U4 = ("<u4", (4,))
+TASK_ACTION_DTYPE: Final = np.dtype([("indices", *U4), ("parameters", *F4), ("drive", *F4), ("actuator", *U4)])
+TASK_OBSERVATION_DTYPE: Final = np.dtype([("source", *U4), ("transform", *F4), ("auxiliary", *U4)])
+TASK_REWARD_DTYPE: Final = np.dtype([("source", *U4), ("parameters", *F4), ("auxiliary", *F4)])
+TASK_TERMINATION_DTYPE: Final = np.dtype([("source", *U4), ("parameters", *F4), ("schedule", *U4)])
+TASK_STATE_DTYPE: Final = np.dtype([
+    ("episode", *U4), ("schedule", *U4), ("status", *U4), ("command_and_phase", *F4), ("command_extension", *F4),
+    ("air_return_tracking", *F4), ("recovery", *F4), ("recovery_stats", *U4), ("threat_geometry", *F4),
+    ("threat_teacher", *F4), ("threat_metadata", *U4)])
+TASK_TRANSITION_DTYPE: Final = np.dtype([
+    ("reward_and_state", *F4), ("termination", *U4), ("reward_breakdown0", *F4), ("reward_breakdown1", *F4),
+    ("outcome_channels0", *F4), ("outcome_channels1", *F4), ("policy_revision", "<u8"),
+    ("timeout_bootstrap_value", "<f4"), ("episode_tracking_score", "<f4"), ("task_progress", *U4)])
 MR_ACTUATOR_PROFILE_CALIBRATED: Final = 2
 
 _EXPECTED_SIZES: Final = {
     MODEL_HEADER_DTYPE: 64, BODY_DTYPE: 256, DOF_DTYPE: 48, ACTUATOR_DTYPE: 48, SHAPE_DTYPE: 96, MATERIAL_DTYPE: 32, PAIR_DTYPE: 16,
     BODY_STATE_DTYPE: 128, MANIFOLD_HEADER_DTYPE: 64, MANIFOLD_POINT_DTYPE: 64, WORLD_STATUS_DTYPE: 48,
-    CONTACT_STATUS_DTYPE: 96, PAIR_RECORD_DTYPE: 16,
+    CONTACT_STATUS_DTYPE: 96, PAIR_RECORD_DTYPE: 16, TASK_ACTION_DTYPE: 64, TASK_OBSERVATION_DTYPE: 48,
+    TASK_REWARD_DTYPE: 48, TASK_TERMINATION_DTYPE: 48, TASK_STATE_DTYPE: 176, TASK_TRANSITION_DTYPE: 128,
 }
 for _dt, _size in _EXPECTED_SIZES.items():
     assert _dt.itemsize == _size, (_dt, _dt.itemsize, _size)
@@ -266,6 +296,7 @@ assert ct.sizeof(WorldModelDescC) == 216
 assert ct.sizeof(WorldAdvanceC) == 40
 assert ct.sizeof(WorldLayoutC) == 40
 assert ct.sizeof(MlpWeightsC) == 80
+assert ct.sizeof(LocomotionTaskProgramC) == 160 and LocomotionTaskProgramC.seed.offset == 152
 # the reference's task-rollout records (c_api.h), LP64
 assert ct.sizeof(TaskRolloutDynamicSphereC) == 36
 assert ct.sizeof(TaskRolloutConfigC) == 104
@@ -333,6 +364,21 @@ _SIGNATURES: Final = {
     "mr_world_advance": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_void_p, ct.c_size_t, ct.c_uint32,
                           ct.POINTER(WorldAdvanceC)], ct.c_int),
     "mr_world_enqueue_device": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32], ct.c_int),
+    "mr_locomotion_task_create": ([ct.POINTER(LocomotionTaskProgramC), ct.c_void_p], ct.c_void_p),
+    "mr_locomotion_task_destroy": ([ct.c_void_p], None),
+    "mr_locomotion_task_reset": ([ct.c_void_p], ct.c_int),
+    "mr_locomotion_task_step": ([ct.c_void_p, ct.c_void_p], ct.c_int),
+    "mr_locomotion_task_read": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_void_p, ct.c_size_t, ct.c_void_p,
+                                 ct.c_size_t], ct.c_int),
+    "mr_locomotion_task_device_observations": ([ct.c_void_p], ct.c_void_p),
+    "mr_locomotion_task_device_transitions": ([ct.c_void_p], ct.c_void_p),
+    "mr_locomotion_task_device_targets": ([ct.c_void_p], ct.c_void_p),
+    "mr_policy_dense_layer": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_float, ct.c_void_p,
+                               ct.c_uint32, ct.c_void_p, ct.c_uint32, ct.c_uint32, ct.c_uint32, ct.c_uint32,
+                               ct.c_uint32, ct.c_void_p], ct.c_int),
+    "mr_policy_sample_and_score": ([ct.c_void_p, ct.c_uint32, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_float,
+                                    ct.c_uint64, ct.c_uint64, ct.c_void_p, ct.c_uint32, ct.c_uint32, ct.c_uint32,
+                                    ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p], ct.c_int),
     "mr_world_enqueue_device_horizon": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32], ct.c_int),
     "mr_world_read_world_step_microseconds": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
     "mr_world_enqueue_device_static": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32, ct.c_void_p,
@@ -854,3 +900,125 @@ class TaskRollout:
         out = np.zeros((lay.environment_count, lay.scene_body_count, 13), np.float32)
         self._b.check(self._b.lib.mr_task_rollout_copy_final_scene_states(self._h, _ptr(out), out.size))
         return out
+
+
+class LocomotionTask:
+    """mr_locomotion_task_*: the task layer either side of a World's control step. `program` is a
+    dict with the record arrays (TASK_*_DTYPE) and the MRLocomotionTaskProgramC scalars; the world's
+    reset state becomes the program's nominal pose at rest."""
+
+    def __init__(self, world: World, program: dict) -> None:
+        self._b = load()
+        self.world = world
+        self.program = program
+        self._keep = {k: np.ascontiguousarray(program[k], dt) for k, dt in (
+            ("actions", TASK_ACTION_DTYPE), ("observations", TASK_OBSERVATION_DTYPE),
+            ("rewards", TASK_REWARD_DTYPE), ("terminations", TASK_TERMINATION_DTYPE))}
+        self._keep["default_q"] = np.ascontiguousarray(program["default_q"], np.float32)
+        self.c_program = task_program_struct(program, self._keep)
+        self._h = self._b.lib.mr_locomotion_task_create(ct.byref(self.c_program), world._h)
+        if not self._h:
+            raise MetalRoboError(self._b.last_error())
+        self.environment_count = world.environment_count
+        self.history_length = int(program["history_length"])
+        self.observation_count = len(self._keep["observations"])
+        self.action_count = len(self._keep["actions"])
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._b.lib.mr_locomotion_task_destroy(self._h)
+            self._h = None
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self) -> None:
+        self._b.check(self._b.lib.mr_locomotion_task_reset(self._h))
+
+    def step(self, actions) -> None:
+        """One control step with device actions (contiguous float32 GPU tensor [E][action_count])."""
+        import torch
+
+        if (tuple(actions.shape) != (self.environment_count, self.action_count) or actions.dtype != torch.float32
+                or not actions.is_cuda or not actions.is_contiguous()):
+            raise ValueError(f"actions must be a contiguous float32 GPU tensor of shape "
+                             f"{(self.environment_count, self.action_count)}")
+        self._b.check(self._b.lib.mr_locomotion_task_step(self._h, actions.data_ptr()))
+
+    def read(self):
+        """(observations [E][history][O], transitions [E], task states [E]) after waiting."""
+        E = self.environment_count
+        obs = np.zeros((E, self.history_length, self.observation_count), np.float32)
+        tr = np.zeros(E, TASK_TRANSITION_DTYPE)
+        st = np.zeros(E, TASK_STATE_DTYPE)
+        self._b.check(self._b.lib.mr_locomotion_task_read(self._h, _ptr(obs), obs.size, _ptr(tr), tr.size, _ptr(st),
+                                                          st.size))
+        return obs, tr, st
+
+    def device_observations(self) -> int:
+        return self._b.lib.mr_locomotion_task_device_observations(self._h)
+
+    def device_targets(self) -> int:
+        return self._b.lib.mr_locomotion_task_device_targets(self._h)
+
+
+def task_program_struct(program: dict, arrays: dict) -> LocomotionTaskProgramC:
+    """The MRLocomotionTaskProgramC of `program`, pointing into `arrays` (kept alive by the caller)."""
+    c = LocomotionTaskProgramC()
+    c.actions = arrays["actions"].ctypes.data
+    c.observations = arrays["observations"].ctypes.data
+    c.rewards = arrays["rewards"].ctypes.data if len(arrays["rewards"]) else None
+    c.terminations = arrays["terminations"].ctypes.data if len(arrays["terminations"]) else None
+    c.default_q = arrays["default_q"].ctypes.data
+    c.action_count = len(arrays["actions"])
+    c.observation_count = len(arrays["observations"])
+    c.reward_count = len(arrays["rewards"])
+    c.termination_count = len(arrays["terminations"])
+    for name in ("history_length", "delay_slots", "max_episode_steps", "flags", "root_q_offset", "root_v_offset",
+                 "actuator_delay", "seed"):
+        setattr(c, name, int(program.get(name, 0)))
+    for name in ("base_height_target", "gait_period"):
+        setattr(c, name, float(program[name]))
+    for name, n in (("root_reference", 4), ("command_lower", 4), ("command_upper", 4), ("command_seconds", 2)):
+        vals = list(program.get(name, [0.0] * n)) + [0.0] * n
+        getattr(c, name)[:] = [float(x) for x in vals[:n]]
+    return c
+
+
+def policy_dense_layer(weights, bias, inp, out, activation: int, mean=None, inverse_std=None,
+                       observation_clip: float = 0.0) -> None:
+    """mr_policy_dense_layer on torch GPU tensors: weights [out][in], bias [out], inp [E][>=in],
+    out [E][>=out] (row strides from the tensors), optional mean / inverse_std [in]."""
+    import torch
+
+    b = load()
+    E, in_w = int(inp.shape[0]), int(weights.shape[1])
+    out_w = int(weights.shape[0])
+    for t in (weights, bias, inp, out) + tuple(x for x in (mean, inverse_std) if x is not None):
+        if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
+            raise ValueError("policy tensors must be contiguous float32 GPU tensors")
+    stream = torch.cuda.current_stream().cuda_stream
+    b.check(b.lib.mr_policy_dense_layer(weights.data_ptr(), bias.data_ptr(),
+                                        None if mean is None else mean.data_ptr(),
+                                        None if inverse_std is None else inverse_std.data_ptr(),
+                                        float(observation_clip), inp.data_ptr(), int(inp.stride(0)), out.data_ptr(),
+                                        int(out.stride(0)), E, in_w, out_w, int(activation), stream))
+
+
+def policy_sample_and_score(mean, action_bias, action_scale, log_std, action_clip: float, seed: int, revision: int,
+                            task_states, control_step: int, actions_out, latents_out, log_prob_out) -> None:
+    """mr_policy_sample_and_score on torch GPU tensors; task_states is a uint8 GPU tensor holding
+    [E] MRTaskStateC records (176 bytes each); log_std may be None (deterministic)."""
+    import torch
+
+    b = load()
+    E, A = int(mean.shape[0]), int(action_bias.shape[0])
+    stream = torch.cuda.current_stream().cuda_stream
+    b.check(b.lib.mr_policy_sample_and_score(mean.data_ptr(), int(mean.stride(0)), action_bias.data_ptr(),
+                                             action_scale.data_ptr(), None if log_std is None else log_std.data_ptr(),
+                                             float(action_clip), int(seed), int(revision), task_states.data_ptr(), E,
+                                             A, int(control_step), actions_out.data_ptr(), latents_out.data_ptr(),
+                                             log_prob_out.data_ptr(), stream))

@@ -15,7 +15,8 @@ LIBRARY = os.path.join(HERE, "liboracle_mr.so")
 def build(verbose=True):
     tmp = LIBRARY + ".tmp"
     cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-           "-Wall", "-Wno-unused-function", os.path.join(HERE, "oracle_world.cpp"), os.path.join(HERE, "exact_cone.cpp"), "-o", tmp, "-lpthread"]
+           "-Wall", "-Wno-unused-function", os.path.join(HERE, "oracle_world.cpp"), os.path.join(HERE, "exact_cone.cpp"),
+           os.path.join(HERE, "oracle_task.cpp"), "-o", tmp, "-lpthread"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"oracle build failed: {' '.join(cmd)}\n{res.stderr}")

@@ -296,3 +296,121 @@ def contact_program(prob):
         return prob["v_free"] + R.T @ lam, (tw0 + Minv @ Js.T @ lam).reshape(ns, 6)
 
     return H, lin, apply
+
+
+# ------------------------------------------------------------- task layer
+
+def _task_lib():
+    lib = load()
+    if not getattr(lib, "_task_bound", False):
+        from metalrobo.native import LocomotionTaskProgramC
+        P = ct.c_void_p
+        lib.mr_oracle_mix64.argtypes = [ct.c_uint64]
+        lib.mr_oracle_mix64.restype = ct.c_uint64
+        lib.mr_oracle_task_random_unit.argtypes = [ct.c_uint64, ct.c_uint32, ct.c_uint32, ct.c_uint32, ct.c_uint32]
+        lib.mr_oracle_task_random_unit.restype = ct.c_float
+        prog = ct.POINTER(LocomotionTaskProgramC)
+        lib.mr_oracle_task_apply_actions.argtypes = [prog, ct.c_uint32, ct.c_uint32, ct.c_float, P, P, P, P]
+        lib.mr_oracle_task_apply_actions.restype = None
+        lib.mr_oracle_task_complete.argtypes = [prog, ct.c_uint32, ct.c_uint32, ct.c_uint32, ct.c_float] + [P] * 8
+        lib.mr_oracle_task_complete.restype = None
+        lib.mr_oracle_task_observe.argtypes = [prog, ct.c_uint32, ct.c_uint32, ct.c_uint32, ct.c_float] + [P] * 9
+        lib.mr_oracle_task_observe.restype = None
+        lib.mr_oracle_policy_dense.argtypes = [P, P, P, P, ct.c_float, P, ct.c_uint32, P, ct.c_uint32, ct.c_uint32,
+                                               ct.c_uint32, ct.c_uint32, ct.c_uint32]
+        lib.mr_oracle_policy_dense.restype = None
+        lib.mr_oracle_policy_sample.argtypes = [P, ct.c_uint32, P, P, P, ct.c_float, ct.c_uint64, ct.c_uint64, P,
+                                                ct.c_uint32, ct.c_uint32, ct.c_uint32, P, P, P]
+        lib.mr_oracle_policy_sample.restype = None
+        lib._task_bound = True
+    return lib
+
+
+class OracleTask:
+    """The task layer's state and the oracle restatement of its kernels, stepping beside an
+    OracleState for the physics (apply_actions -> physics -> complete -> observe)."""
+
+    def __init__(self, model, config, program, physics=None):
+        from metalrobo.native import (TASK_ACTION_DTYPE, TASK_OBSERVATION_DTYPE, TASK_REWARD_DTYPE,
+                                      TASK_STATE_DTYPE, TASK_TERMINATION_DTYPE, TASK_TRANSITION_DTYPE,
+                                      task_program_struct)
+        self.lib = _task_lib()
+        self.model, self.config = model, config
+        E, nq, nv = model.environment_count, model.max_nq, model.max_nv
+        self.arrays = {k: np.ascontiguousarray(program[k], dt) for k, dt in (
+            ("actions", TASK_ACTION_DTYPE), ("observations", TASK_OBSERVATION_DTYPE),
+            ("rewards", TASK_REWARD_DTYPE), ("terminations", TASK_TERMINATION_DTYPE))}
+        self.arrays["default_q"] = np.ascontiguousarray(program["default_q"], np.float32)
+        self.prog = task_program_struct(program, self.arrays)
+        self.A, self.O, self.H = len(self.arrays["actions"]), len(self.arrays["observations"]), int(program["history_length"])
+        self.slots = int(program["delay_slots"])
+        self.dt = np.float32(config.control_timestep)
+        self.physics = physics if physics is not None else OracleState(model, config)
+        ph = self.physics
+        ph.reset_q = np.tile(self.arrays["default_q"], (E, 1)).astype(np.float32)
+        ph.reset_v = np.zeros((E, nv), np.float32)
+        ph.reset_scene = model.initial_state()[2].copy()
+        self.states = np.zeros(E, TASK_STATE_DTYPE)
+        self.history = np.zeros((E, self.slots, self.A), np.float32)
+        self.previous = np.zeros((E, 2, self.A), np.float32)
+        self.observations = np.zeros((E, self.H, self.O), np.float32)
+        self.transitions = np.zeros(E, TASK_TRANSITION_DTYPE)
+        self.targets = np.zeros((E, nv), np.float32)
+        self.mask = np.zeros(E, np.uint32)
+        self.dofs = np.ascontiguousarray(model.dofs()[model.models()[0]["dof_offset"]:])
+        self.observe()
+
+    def apply_actions(self, actions):
+        E, nv = self.model.environment_count, self.model.max_nv
+        actions = np.ascontiguousarray(actions, np.float32)
+        self.lib.mr_oracle_task_apply_actions(ct.byref(self.prog), E, nv, self.dt, _p(self.states), _p(self.history),
+                                              _p(actions), _p(self.targets))
+
+    def complete(self, status_codes):
+        E, nq, nv = self.model.environment_count, self.model.max_nq, self.model.max_nv
+        codes = np.ascontiguousarray(status_codes, np.uint32)
+        ph = self.physics
+        self.lib.mr_oracle_task_complete(ct.byref(self.prog), E, nq, nv, self.dt, _p(ph.q), _p(ph.v), _p(self.dofs),
+                                         _p(codes), _p(self.states), _p(self.history), _p(self.previous),
+                                         _p(self.transitions))
+
+    def observe(self):
+        E, nq, nv = self.model.environment_count, self.model.max_nq, self.model.max_nv
+        ph = self.physics
+        self.lib.mr_oracle_task_observe(ct.byref(self.prog), E, nq, nv, self.dt, _p(ph.q), _p(ph.v), _p(ph.reset_q),
+                                        _p(ph.reset_v), _p(self.states), _p(self.history), _p(self.previous),
+                                        _p(self.observations), _p(self.mask))
+
+    def step(self, actions, control_step=0):
+        self.apply_actions(actions)
+        status, *_ = self.physics.step(self.targets, reset_mask=self.mask.copy(), control_step=control_step)
+        self.complete(status["code"])
+        self.observe()
+        return status
+
+
+def policy_dense(weights, bias, inp, out_width, activation, mean=None, inverse_std=None, clip=0.0):
+    lib = _task_lib()
+    inp = np.ascontiguousarray(inp, np.float32)
+    E, in_w = inp.shape
+    out = np.zeros((E, out_width), np.float32)
+    lib.mr_oracle_policy_dense(_p(np.ascontiguousarray(weights, np.float32)), _p(np.ascontiguousarray(bias, np.float32)),
+                               None if mean is None else _p(np.ascontiguousarray(mean, np.float32)),
+                               None if inverse_std is None else _p(np.ascontiguousarray(inverse_std, np.float32)),
+                               clip, _p(inp), in_w, _p(out), out_width, E, int(weights.shape[1]), out_width, activation)
+    return out
+
+
+def policy_sample(mean, action_bias, action_scale, log_std, action_clip, seed, revision, states, control_step):
+    lib = _task_lib()
+    mean = np.ascontiguousarray(mean, np.float32)
+    E, A = mean.shape
+    acts = np.zeros((E, A), np.float32)
+    lat = np.zeros((E, A), np.float32)
+    lp = np.zeros(E, np.float32)
+    keep = [np.ascontiguousarray(x, np.float32) for x in (action_bias, action_scale)]
+    ls = None if log_std is None else np.ascontiguousarray(log_std, np.float32)
+    lib.mr_oracle_policy_sample(_p(mean), A, _p(keep[0]), _p(keep[1]), None if ls is None else _p(ls), action_clip,
+                                seed, revision, _p(np.ascontiguousarray(states)), E, A, control_step, _p(acts), _p(lat),
+                                _p(lp))
+    return acts, lat, lp

@@ -268,11 +268,13 @@ def test_residual_guard_falls_back_to_the_plain_path(gpu):
 @pytest.mark.gpu
 def test_trained_warm_start_against_the_plain_one(gpu):
     """Trains the MLP on samples the plain solve produced (chain-and-boxes
-    worlds), then runs held-out worlds at residual 1e-5 with and without it.
-    It shows the gain (sweeps saved per control step) and checks what the
-    fallback guarantees whatever the network predicts: every healthy world
-    ends each step at the tolerance or after a fallback, and the learned path
-    costs at most the fallback's bound of twice the plain sweeps."""
+    worlds), then runs held-out worlds at residual 1e-5 with and without it,
+    and asserts the sign of what the trained network gains. Measured: it
+    saves no sweeps on this workload (about 2.5 % more, with fallbacks on a
+    few percent of microsteps), so the saving is negative and the learned
+    warm start stays off by default. With the normal-residual guard, no
+    world that stays within the healthy bound under the plain warm start
+    leaves it under the learned one."""
     import metalrobo as mr
     from metalrobo import warmstart as ws
 
@@ -293,7 +295,9 @@ def test_trained_warm_start_against_the_plain_one(gpu):
           f"healthy worlds leaving the bound {ev['mlp']['healthy_worlds_leaving_bound']}, "
           f"ms per step plain {ev['plain']['ms_per_step']:.2f} learned {ev['mlp']['ms_per_step']:.2f}", flush=True)
     assert ev["healthy_worlds"] > 0
-    assert learned <= 2.0 * plain + 1e-9
+    saving = 1.0 - learned / plain
+    assert saving < 0.0, f"the trained network now saves {100.0 * saving:.1f} % of the sweeps: update the record"
+    assert ev["mlp"]["healthy_worlds_leaving_bound"] == 0
 
 
 @pytest.mark.gpu
