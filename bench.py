@@ -38,7 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "numi-lab_amd"))
 
 HBM_PEAK_GBPS = 8000.0
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "control_step_queue_humanoid_4096_traffic.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "control_horizon_queue_humanoid_4096_traffic.md")
 
 
 def bytes_per_env_step(nq, nv, s_dyn, s_static, m):
@@ -50,12 +50,13 @@ def bytes_per_env_step(nq, nv, s_dyn, s_static, m):
 
 
 def read_traffic_profile(path):
-    """HBM bytes per launch of the headline kernel from the committed counter profile, and the
-    mean manifolds per world of the run it was measured on (the same command)."""
+    """HBM bytes per env-step of the headline kernel from the committed counter profile (FETCH_SIZE
+    + WRITE_SIZE of the driver's command, scaled by the copy calibration), and the mean manifolds
+    per world of the run it was measured on."""
     if not os.path.isfile(path):
         return None, None
     text = open(path).read()
-    m = re.search(r"hbm_bytes_per_launch\s*[=:]\s*([0-9.eE+]+)", text)
+    m = re.search(r"hbm_bytes_per_env_step\s*[=:]\s*([0-9.eE+]+)", text)
     k = re.search(r"mean_manifolds\s*[=:]\s*([0-9.eE+]+)", text)
     return (float(m.group(1)) if m else None), (float(k.group(1)) if k else None)
 
@@ -473,7 +474,9 @@ def main():
     s_static = int(len(scene) - s_dyn)
     b_env = bytes_per_env_step(nq, nv, s_dyn, s_static, mean_manifolds)
     achieved = E * steps_per_launch * b_env / avg_launch_s / 1e9
-    traffic, traffic_m = read_traffic_profile(TRAFFIC_PROFILE)
+    traffic_per_env_step, traffic_m = read_traffic_profile(TRAFFIC_PROFILE)
+    traffic = (round(traffic_per_env_step * E * steps_per_launch, 1)
+               if traffic_per_env_step is not None and args.horizon > 0 else None)
     p99 = head_sum["world_step_latency_ms"]["p99"]
     vs_plain = round(value / plain["env_steps_per_s"], 4) if plain else None
 
@@ -523,7 +526,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 6), "traffic": traffic,
                      "kernel": kernel_name,
-                     "traffic_mean_manifolds": traffic_m,
+                     "traffic_mean_manifolds": traffic_m, "traffic_bytes_per_env_step": traffic_per_env_step,
                      "algorithmic_bytes_per_env_step": round(b_env, 1),
                      "algorithmic_bytes_per_launch": round(E * steps_per_launch * b_env, 1),
                      "kernel_ms_per_launch": round(avg_launch_s * 1e3, 4)},

@@ -415,7 +415,8 @@ typedef struct MRWorldStepConfigC {
     float residual_tolerance;         /* >0: stop sweeps once final |dlambda| <= tol */
     uint32_t max_iterations;          /* sweep cap when residual_tolerance > 0 */
     uint32_t solver;                  /* MRSolverMode */
-    uint32_t reserved;
+    float maximum_normal_impulse;     /* > 0: cap of every contact's normal impulse (the cone's
+                                         maximumNormalImpulse, MetalWorldContact.metal:488-493); 0: none */
 } MRWorldStepConfigC;
 
 #ifdef __cplusplus

@@ -1624,8 +1624,11 @@ MRD_INL bool build_manifold(const Env& e, const MRCompiledPairC& pair, const Raw
 
 // ------------------------------------------------------------ cone solve
 
-MRD_INL void project_cone(float lam[3], float mu_u, float mu_v) {
+// Normal clamp (and the optional maximum normal impulse), then radial scaling
+// of the tangents onto the elliptic cone (MetalWorldContact.metal:482-509).
+MRD_INL void project_cone(float lam[3], float mu_u, float mu_v, float max_normal) {
     lam[0] = fmaxd(lam[0], 0.0f);
+    if (max_normal > 0.0f) lam[0] = fmind(lam[0], max_normal);
     const float lu = mu_u * lam[0];
     const float lv = mu_v * lam[0];
     if (!(lu > 0.0f) || !(lv > 0.0f)) {
@@ -1965,7 +1968,7 @@ MRD_INL bool mlp_warm_start(const Env& e, uint32_t lane, uint32_t ncons, uint32_
                 lam[k] = changed ? prev + d : prev;
             }
             const HotConstraint& hk = e.hot[c];
-            if (changed) project_cone(lam, hk.mu_u, hk.mu_v);
+            if (changed) project_cone(lam, hk.mu_u, hk.mu_v, e.a->cfg.maximum_normal_impulse);
             if (changed && isfin(lam[0]) && isfin(lam[1]) && isfin(lam[2])) {
                 e.lam[3u * c] = lam[0];
                 e.lam[3u * c + 1u] = lam[1];
@@ -2769,7 +2772,7 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
                 if (!invert3x3(W, hk.winv)) report_failure(&s_fail, c, MR_STEP_FACTORIZATION_FAILED, c);
             }
             float lam[3] = {e.lam[3u * c], e.lam[3u * c + 1u], e.lam[3u * c + 2u]};
-            project_cone(lam, hk.mu_u, hk.mu_v);
+            project_cone(lam, hk.mu_u, hk.mu_v, e.a->cfg.maximum_normal_impulse);
             e.lam[3u * c] = lam[0];
             e.lam[3u * c + 1u] = lam[1];
             e.lam[3u * c + 2u] = lam[2];
@@ -2855,7 +2858,7 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
                         float prop[3] = {lam[0], lam[1], lam[2]};
                         for (int r = 0; r < 3; ++r)
                             for (int q = 0; q < 3; ++q) prop[r] += sb.winv[3 * r + q] * rhs[q];
-                        project_cone(prop, sb.mu_u, sb.mu_v);
+                        project_cone(prop, sb.mu_u, sb.mu_v, e.a->cfg.maximum_normal_impulse);
                         const float relax = e.jrelax[c];
                         float d[3], cand[3];
                         for (int r = 0; r < 3; ++r) {
@@ -2991,7 +2994,7 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
                         float cand[3] = {lam0, lam1, lam2};
                         for (int r = 0; r < 3; ++r)
                             for (int q = 0; q < 3; ++q) cand[r] += sb.winv[3 * r + q] * rhs[q];
-                        project_cone(cand, sb.mu_u, sb.mu_v);
+                        project_cone(cand, sb.mu_u, sb.mu_v, e.a->cfg.maximum_normal_impulse);
                         const float d0 = cand[0] - lam0, d1 = cand[1] - lam1, d2 = cand[2] - lam2;
                         if (wave_any(!isfin(cand[0]) | !isfin(cand[1]) | !isfin(cand[2]) | !isfin(d0) | !isfin(d1) | !isfin(d2))) {
                             if (lane == 0u) s_fail = ((unsigned long long)MR_STEP_NONFINITE_RESULT << 24) | (c & 0xffffffu);

@@ -196,7 +196,7 @@ void LocomotionTask::step(const float* device_actions) {
         throw std::runtime_error("locomotion task: the world must step one workgroup per environment");
     hipStream_t s = world_.stream();
     hip_check(launch_task_apply_actions(args_, device_actions, s), "task apply_actions");
-    world_.enqueue(args_.targets, args_.reset_mask, 1u, true);
+    world_.enqueue_chained(args_.targets, args_.reset_mask, 1u);
     hip_check(launch_task_complete(args_, world_.last_status_device(), s), "task complete");
     hip_check(launch_task_observe(args_, s), "task observe");
 }

@@ -3,6 +3,7 @@
 #include "mr_world.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -61,6 +62,8 @@ World::World(const ModelData& model, const MRWorldStepConfigC& config) : model_(
         throw std::runtime_error("max_iterations must be positive with a residual tolerance");
     if (cfg_.solver != MR_SOLVER_ORDERED_SWEEPS && cfg_.solver != MR_SOLVER_RELAXED_JACOBI)
         throw std::runtime_error("unknown solver mode");
+    if (!(cfg_.maximum_normal_impulse >= 0.0f) || !std::isfinite(cfg_.maximum_normal_impulse))
+        throw std::runtime_error("maximum_normal_impulse must be finite and non-negative (0: no cap)");
     if (cfg_.pair_capacity == 0u || cfg_.pair_capacity > 4096u) throw std::runtime_error("pair_capacity in [1, 4096]");
     if (cfg_.manifold_capacity == 0u || cfg_.manifold_capacity > 4096u)
         throw std::runtime_error("manifold_capacity in [1, 4096]");
@@ -633,6 +636,7 @@ void World::drop_graph() {
 void World::enqueue_graph(const float* device_commands, uint32_t steps) {
     if (steps == 0u || device_commands == nullptr) throw std::runtime_error("enqueue_graph: commands and steps required");
     if (stage_profiling_) throw std::runtime_error("enqueue_graph: stage profiling is not captured in graphs");
+    begin_submission();
     // the buffers the launches point at exist before the comparison and the capture
     hip_check(hipStreamSynchronize(stream_), "sync");
     d_status_.reserve((size_t)steps * env_ * sizeof(MRWorldStatusC));
@@ -672,11 +676,27 @@ void World::enqueue_graph(const float* device_commands, uint32_t steps) {
     last_status_ = static_cast<const MRWorldStatusC*>(d_status_.ptr);
     last_cstatus_ = static_cast<const MRContactStatusC*>(d_cstatus_.ptr);
     last_summary_ = static_cast<const AdvanceSummaryDev*>(d_summary_.ptr);
+    in_flight_ = true;
+}
+
+void World::begin_submission() const {
+    if (in_flight_)
+        throw std::runtime_error("context busy: the world's resident state has a submission in flight; "
+                                 "call mr_world_wait before the next one");
+}
+
+void World::enqueue_chained(const float* device_commands, const uint32_t* device_masks, uint32_t steps) {
+    if (in_flight_)
+        throw std::runtime_error("context busy: the world's resident state has a submission in flight; "
+                                 "call mr_world_wait before the next one");
+    enqueue(device_commands, device_masks, steps, true);
+    in_flight_ = false;  // the chain orders itself on the stream; the caller waits through its own readback
 }
 
 void World::enqueue(const float* commands, const uint32_t* masks, uint32_t steps, bool device_inputs) {
     if (steps == 0u) throw std::runtime_error("control_step_count must be positive");
     if (commands == nullptr) throw std::runtime_error("commands are required");
+    begin_submission();
     const size_t E = env_;
     const size_t per_step = E * model_.max_nv;
     const float* cmd = commands;
@@ -711,6 +731,7 @@ void World::enqueue(const float* commands, const uint32_t* masks, uint32_t steps
     d_summary_.reserve(sizeof(AdvanceSummaryDev));
     launch_steps(cmd, msk, steps, true, own_buffers());
     last_steps_ = steps;
+    in_flight_ = true;
 }
 
 void World::enqueue_horizon(const float* device_commands, const uint32_t* device_masks, uint32_t steps) {
@@ -720,6 +741,7 @@ void World::enqueue_horizon(const float* device_commands, const uint32_t* device
         throw std::runtime_error("enqueue_horizon: needs a work-queue schedule (mr_world_set_schedule with grid > 0)");
     if (mlp_instance())
         throw std::runtime_error("enqueue_horizon: the learned warm start runs one workgroup per environment");
+    begin_submission();
     const size_t E = env_;
     if (stage_profiling_)
         hip_check(hipMemsetAsync(d_stage_cycles_.ptr, 0, E * kStageCount * sizeof(unsigned long long), stream_),
@@ -730,6 +752,7 @@ void World::enqueue_horizon(const float* device_commands, const uint32_t* device
     d_summary_.reserve(sizeof(AdvanceSummaryDev));
     launch_steps(device_commands, device_masks, steps, true, own_buffers(), true);
     last_steps_ = steps;
+    in_flight_ = true;
 }
 
 void World::read_world_step_microseconds(float* out, uint32_t steps) const {
@@ -771,6 +794,7 @@ void World::enqueue_static(const float* device_commands, const uint32_t* device_
     if (need > capacity)
         throw std::runtime_error("enqueue_device_static: the world needs " + std::to_string(need) +
                                  " bytes of scratch; the static array holds " + std::to_string(capacity));
+    begin_submission();
     const size_t E = env_, pcap = cfg_.pair_capacity, ccap = cfg_.constraint_capacity;
     const size_t row_stride = std::max(1u, model_.max_nv);
     unsigned char* at = static_cast<unsigned char*>(base);
@@ -795,10 +819,12 @@ void World::enqueue_static(const float* device_commands, const uint32_t* device_
                   "clear stage cycles");
     launch_steps(device_commands, device_masks, steps, true, b);
     last_steps_ = steps;
+    in_flight_ = true;
 }
 
 AdvanceSummary World::wait() {
     hip_check(hipStreamSynchronize(stream_), "stream synchronize");
+    in_flight_ = false;
     hip_check(hipEventElapsedTime(&last_kernel_ms_, ev_begin_, ev_end_), "event elapsed");
     AdvanceSummary s;
     s.control_step_count = last_steps_;

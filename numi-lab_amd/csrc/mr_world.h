@@ -77,6 +77,10 @@ public:
     // the pointers are device memory already resident in HBM; otherwise they
     // are host arrays copied before the first launch.
     void enqueue(const float* commands, const uint32_t* masks, uint32_t steps, bool device_inputs);
+    // enqueue() for work that orders itself on the world's stream (the task
+    // layer's steps): no in-flight bookkeeping, but refused while a caller's
+    // submission waits for mr_world_wait.
+    void enqueue_chained(const float* device_commands, const uint32_t* device_masks, uint32_t steps);
     // enqueue() on device inputs that allocates nothing: the per-step scratch
     // and the advance summary come from the library's static device array
     // (static_scratch), the statuses go to the caller's device buffers
@@ -92,8 +96,10 @@ public:
     void read_world_step_microseconds(float* out, uint32_t steps) const;
     // Bytes of static scratch enqueue_static needs for this world.
     size_t static_scratch_bytes() const;
-    // Waits for the stream and summarises the statuses of the last enqueue.
+    // Waits for the stream and summarises the statuses of the last enqueue;
+    // the world then accepts the next submission.
     AdvanceSummary wait();
+    bool in_flight() const { return in_flight_; }
 
     void read_statuses(MRWorldStatusC* out, uint32_t steps) const;  // [steps][E] of the last enqueue
     void read_contact_statuses(MRContactStatusC* out, uint32_t steps) const;
@@ -173,6 +179,11 @@ private:
     void launch_steps(const float* cmd, const uint32_t* msk, uint32_t steps, bool events, const StepBuffers& buf,
                       bool horizon = false);
     bool last_horizon_ = false;  // the last advance ran as one persistent horizon launch
+    // A resident world takes one submission at a time: set by every enqueue,
+    // cleared by wait() (the reference's contextBusy rule for a resident
+    // continuation, MetalWorld.mm:16590-16603).
+    bool in_flight_ = false;
+    void begin_submission() const;  // throws while a submission is in flight
     // where the last advance wrote its statuses and summary
     const MRWorldStatusC* last_status_ = nullptr;
     const MRContactStatusC* last_cstatus_ = nullptr;

@@ -85,7 +85,7 @@ class WorldStepConfigC(ct.Structure):
         ("manifold_merge_distance", ct.c_float), ("manifold_normal_cosine", ct.c_float),
         ("contact_slop", ct.c_float), ("max_depenetration_velocity", ct.c_float),
         ("residual_tolerance", ct.c_float), ("max_iterations", ct.c_uint32),
-        ("solver", ct.c_uint32), ("reserved", ct.c_uint32),
+        ("solver", ct.c_uint32), ("maximum_normal_impulse", ct.c_float),
     ]
 
     def copy(self) -> "WorldStepConfigC":

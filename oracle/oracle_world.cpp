@@ -1573,8 +1573,9 @@ float fold_dot(const float* a, const float* b, uint32_t n) {
 }
 
 // MetalWorldContact.metal:482-509 (projectFrictionCone).
-void project_cone(float lam[3], float mu_u, float mu_v) {
+void project_cone(float lam[3], float mu_u, float mu_v, float max_normal) {
     lam[0] = fmaxd(lam[0], 0.0f);
+    if (max_normal > 0.0f) lam[0] = fmind(lam[0], max_normal);
     const float lu = mu_u * lam[0];
     const float lv = mu_v * lam[0];
     if (!(lu > 0.0f) || !(lv > 0.0f)) {
@@ -2197,7 +2198,7 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
         // warm start: project, then apply every warm impulse once (:6402-6470)
         for (uint32_t c = 0; c < ncons; ++c) {
             Constraint& k = cons[c];
-            project_cone(k.lambda, k.mu_u, k.mu_v);
+            project_cone(k.lambda, k.mu_u, k.mu_v, ctx.cfg->maximum_normal_impulse);
             for (int r = 0; r < 3; ++r) delta[3u * c + r] = k.lambda[r];
         }
         apply_all();
@@ -2221,7 +2222,7 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
                 float prop[3] = {k.lambda[0], k.lambda[1], k.lambda[2]};
                 for (int r = 0; r < 3; ++r)
                     for (int q = 0; q < 3; ++q) prop[r] += k.winv[r][q] * rhs[q];
-                project_cone(prop, k.mu_u, k.mu_v);
+                project_cone(prop, k.mu_u, k.mu_v, ctx.cfg->maximum_normal_impulse);
                 float d[3], cand[3];
                 for (int r = 0; r < 3; ++r) {
                     d[r] = relax[c] * (prop[r] - k.lambda[r]);
@@ -2248,7 +2249,7 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
     } else {
     for (uint32_t c = 0; c < ncons; ++c) {
         Constraint& k = cons[c];
-        project_cone(k.lambda, k.mu_u, k.mu_v);
+        project_cone(k.lambda, k.mu_u, k.mu_v, ctx.cfg->maximum_normal_impulse);
         apply_delta(k, c, k.lambda);
     }
     for (uint32_t it = 0; it < max_iters; ++it) {
@@ -2262,7 +2263,7 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
             float cand[3] = {k.lambda[0], k.lambda[1], k.lambda[2]};
             for (int r = 0; r < 3; ++r)
                 for (int s = 0; s < 3; ++s) cand[r] += k.winv[r][s] * rhs[s];
-            project_cone(cand, k.mu_u, k.mu_v);
+            project_cone(cand, k.mu_u, k.mu_v, ctx.cfg->maximum_normal_impulse);
             const float d[3] = {cand[0] - k.lambda[0], cand[1] - k.lambda[1], cand[2] - k.lambda[2]};
             if (!std::isfinite(cand[0]) || !std::isfinite(cand[1]) || !std::isfinite(cand[2]) ||
                 !std::isfinite(d[0]) || !std::isfinite(d[1]) || !std::isfinite(d[2])) {

@@ -197,3 +197,35 @@ def test_full_size_humanoid_batch_matches_oracle_on_sampled_worlds(gpu, solver):
             np.testing.assert_array_equal(mp[e, :n][f], ost.mp[e, :n][f], err_msg=f"env {e} {f}")
         assert int(g_status["code"][e]) == 0
     assert sum(int(mc[e]) for e in sample) > 0
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_maximum_normal_impulse_matches_oracle(gpu, solver):
+    """A normal-impulse cap that binds (humanoid feet and chain-box contacts) in both solver
+    modes: the GPU trajectory and stored impulses equal the oracle's bit for bit, and the
+    cap holds on every stored impulse."""
+    for kind, envs, steps, cap in ((mr.MR_SYNTHETIC_HUMANOID_TERRAIN, 6, 12, 0.05),
+                                   (mr.MR_SYNTHETIC_CHAIN_BOXES, 6, 60, 0.02)):
+        model = mr.WorldModel.synthetic(kind, envs, 4)
+        cfg = model.default_config()
+        cfg.solver = solver
+        cfg.maximum_normal_impulse = cap
+        world = mr.World(model, cfg)
+        ost = ol.OracleState(model, cfg)
+        cmds = model.commands(0, steps)
+        world.advance(cmds)
+        for k in range(steps):
+            ost.step(cmds[k], control_step=k)
+        q, v, s = world.state()
+        mh, mp, mc = world.manifolds()
+        assert np.array_equal(q.view(np.uint32), ost.q.view(np.uint32)), kind
+        assert np.array_equal(v.view(np.uint32), ost.v.view(np.uint32)), kind
+        assert np.array_equal(mc, ost.mcount)
+        bound = 0
+        for e in range(envs):
+            n = int(mc[e])
+            lam = mp[e, :n]["impulses"][..., 0]
+            np.testing.assert_array_equal(lam.view(np.uint32), ost.mp[e, :n]["impulses"][..., 0].view(np.uint32))
+            assert np.all(lam <= np.float32(cap))
+            bound += int(np.sum(lam == np.float32(cap)))
+        assert bound > 0, f"the cap never bound ({kind})"

@@ -129,3 +129,38 @@ def test_adaptive_mode_reaches_the_tolerance_or_the_cap():
         res = float(cs["residuals"][0, 0])
         assert 1 <= it <= 64
         assert res <= 1.0e-7 or it == 64, (it, res)
+
+
+def test_maximum_normal_impulse_caps_every_contact():
+    """The cone's maximum normal impulse (MetalWorldContact.metal:488-493): with the cap below
+    the share of the weight each of the four corner contacts carries, every stored normal
+    impulse equals the cap and the box sinks at g - 4 cap / (m h) per microstep; with the cap
+    above it, the resting box is unchanged bit for bit."""
+    w, cfg, st = box_on_plane()
+    run(st, 40)
+    h = cfg.control_timestep / cfg.physics_substeps
+    share = 9.81 * h / 4.0
+    snap = [a.copy() for a in (st.q, st.v, st.scene, st.mh, st.mp, st.mcount)]
+    free = ol.OracleState(w.model, cfg)
+    free.q, free.v, free.scene, free.mh, free.mp, free.mcount = [a.copy() for a in snap]
+    run(free, 1)
+    cfg_hi = w.config()
+    cfg_hi.maximum_normal_impulse = 10.0 * share
+    hi = ol.OracleState(w.model, cfg_hi)
+    hi.q, hi.v, hi.scene, hi.mh, hi.mp, hi.mcount = [a.copy() for a in snap]
+    run(hi, 1)
+    assert np.array_equal(hi.scene.view(np.uint8), free.scene.view(np.uint8))
+    cap = 0.25 * share
+    cfg_lo = w.config()
+    cfg_lo.maximum_normal_impulse = cap
+    lo = ol.OracleState(w.model, cfg_lo)
+    lo.q, lo.v, lo.scene, lo.mh, lo.mp, lo.mcount = [a.copy() for a in snap]
+    vz0 = float(lo.scene[0, 0]["linear_velocity_inv_mass"][2])
+    cs = run(lo, 1)[-1]
+    lam = lo.mp[0, 0]["impulses"][:4, 0]
+    np.testing.assert_array_equal(lam, np.float32(cap))
+    assert cs["residuals"][0, 1] > 0.0  # the capped normal rows are no longer satisfied
+    vz = float(lo.scene[0, 0]["linear_velocity_inv_mass"][2])
+    steps = cfg.physics_substeps
+    expect = vz0 + steps * (-9.81 * h + 4.0 * cap)  # unit mass
+    assert abs(vz - expect) < 1e-4, (vz, expect)

@@ -217,3 +217,31 @@ def test_persistent_horizon_matches_one_launch_per_step(gpu, kind, envs, grid):
     plain = mr.World(model, cfg)
     with pytest.raises(mr.MetalRoboError, match="work-queue schedule"):
         plain.enqueue_device_horizon(cmds)
+
+
+def test_one_submission_in_flight_per_world(gpu):
+    """A resident world takes one submission at a time (the reference's contextBusy rule for a
+    resident continuation): a second enqueue before mr_world_wait is refused and changes
+    nothing; after the wait the next one is accepted, for every enqueue form."""
+    import torch
+
+    model, cfg = trees(envs=16)
+    w = mr.World(model, cfg)
+    cmds = torch.from_numpy(model.commands(0, 2)).cuda()
+    w.enqueue_device(cmds)
+    with pytest.raises(mr.MetalRoboError, match="context busy"):
+        w.enqueue_device(cmds)
+    with pytest.raises(mr.MetalRoboError, match="context busy"):
+        w.advance(model.commands(0, 1))
+    w.wait()
+    ref = mr.World(model, cfg)
+    ref.advance(model.commands(0, 2))
+    assert same(snapshot(ref), snapshot(w))  # the refused submissions never ran
+    w.set_schedule(np.arange(16), grid=8)
+    w.enqueue_device_horizon(cmds)
+    with pytest.raises(mr.MetalRoboError, match="context busy"):
+        w.enqueue_graph(cmds)
+    w.wait()
+    w.enqueue_graph(cmds)
+    w.wait()
+    w.advance(model.commands(0, 1))
