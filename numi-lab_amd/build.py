@@ -32,6 +32,8 @@ SOURCES = [
     ("mr_mlp.hip", "hip"),
     ("mr_task.hip", "hip"),
     ("mr_task_host.cpp", "cxx"),
+    ("mr_packs.cpp", "cxx"),
+    ("mr_policy.cpp", "cxx"),
     ("mr_builder.cpp", "cxx"),
     ("mr_world.cpp", "cxx"),
     ("mr_c_api.cpp", "cxx"),

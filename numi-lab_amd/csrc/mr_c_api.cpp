@@ -4,6 +4,8 @@
 
 #include "mr_builder.h"
 #include "mr_mlp_host.h"
+#include "mr_packs.h"
+#include "mr_policy.h"
 #include "mr_task_host.h"
 #include "mr_world.h"
 
@@ -26,6 +28,16 @@ struct MRWorldHandle {
 
 struct MRLocomotionTaskHandle {
     std::unique_ptr<mr::LocomotionTask> task;
+};
+
+struct MRPolicyPackHandle {
+    mr::PolicyPack pack;
+    std::vector<MRPolicyDenseLayerC> layers, critic_layers;
+    MRPolicyPackC view{};
+};
+
+struct MRPolicyHandle {
+    std::unique_ptr<mr::Policy> policy;
 };
 
 struct MRTaskRolloutHandle {
@@ -829,7 +841,7 @@ int mr_locomotion_task_reset(MRLocomotionTaskHandle* task) {
 
 int mr_locomotion_task_step(MRLocomotionTaskHandle* task, const float* device_actions) {
     return translate([&]() {
-        require(task != nullptr && device_actions != nullptr, "null argument");
+        require(task != nullptr, "null argument");
         task->task->step(device_actions);
     });
 }
@@ -894,6 +906,188 @@ int mr_policy_sample_and_score(const float* actor_mean, uint32_t mean_stride, co
                                                actions_out, latents_out, log_probabilities_out,
                                                static_cast<hipStream_t>(stream)),
                       "mr_policy_sample_and_score launch");
+    });
+}
+
+// ---- learning packs and compiled policies ----
+
+uint64_t mr_learning_pack_content_hash(const void* payload, size_t byte_count) {
+    if (payload == nullptr && byte_count != 0u) return 0u;
+    return mr::learning_pack_hash(payload, byte_count);
+}
+
+int mr_write_policy_pack(const MRPolicyPackC* policy, const char* policy_pack_path) {
+    return translate([&]() {
+        require(policy != nullptr && policy_pack_path != nullptr && policy_pack_path[0] != '\0',
+                "PolicyPack and output path are required");
+        (void)mr::write_policy_pack(mr::policy_pack_from_c(*policy), policy_pack_path);
+    });
+}
+
+MRPolicyPackHandle* mr_read_policy_pack(const char* policy_pack_path) {
+    return construct<MRPolicyPackHandle>([&]() {
+        require(policy_pack_path != nullptr && policy_pack_path[0] != '\0', "PolicyPack path is empty");
+        auto h = std::make_unique<MRPolicyPackHandle>();
+        h->pack = mr::read_policy_pack(policy_pack_path);
+        const mr::PolicyPack& p = h->pack;
+        const auto layers = [](const std::vector<mr::DenseLayer>& in, std::vector<MRPolicyDenseLayerC>& out) {
+            out.resize(in.size());
+            for (size_t i = 0; i < in.size(); ++i)
+                out[i] = MRPolicyDenseLayerC{in[i].input_count, in[i].output_count, in[i].activation,
+                                             in[i].weights.data(), in[i].weights.size(), in[i].bias.data(),
+                                             in[i].bias.size()};
+        };
+        layers(p.layers, h->layers);
+        layers(p.critic_layers, h->critic_layers);
+        MRPolicyPackC& v = h->view;
+        v.id = p.id.c_str();
+        v.revision = p.revision;
+        v.contract_version = p.contract[0];
+        v.world_fingerprint = p.contract[1];
+        v.task_fingerprint = p.contract[2];
+        v.observation_fingerprint = p.contract[3];
+        v.action_fingerprint = p.contract[4];
+        v.observation_mean = p.observation_mean.data();
+        v.observation_mean_count = p.observation_mean.size();
+        v.observation_inverse_standard_deviation = p.observation_inverse_std.data();
+        v.observation_inverse_standard_deviation_count = p.observation_inverse_std.size();
+        v.layers = h->layers.data();
+        v.layer_count = h->layers.size();
+        v.critic_observation_mean = p.critic_observation_mean.data();
+        v.critic_observation_mean_count = p.critic_observation_mean.size();
+        v.critic_observation_inverse_standard_deviation = p.critic_observation_inverse_std.data();
+        v.critic_observation_inverse_standard_deviation_count = p.critic_observation_inverse_std.size();
+        v.critic_layers = h->critic_layers.data();
+        v.critic_layer_count = h->critic_layers.size();
+        v.action_log_standard_deviation = p.action_log_std.data();
+        v.action_log_standard_deviation_count = p.action_log_std.size();
+        v.action_bias = p.action_bias.data();
+        v.action_bias_count = p.action_bias.size();
+        v.action_scale = p.action_scale.data();
+        v.action_scale_count = p.action_scale.size();
+        v.observation_clip = p.observation_clip;
+        v.action_clip = p.action_clip;
+        return h.release();
+    });
+}
+
+const MRPolicyPackC* mr_policy_pack_view(const MRPolicyPackHandle* pack) {
+    return pack != nullptr ? &pack->view : nullptr;
+}
+
+uint64_t mr_policy_pack_content_hash(const MRPolicyPackHandle* pack) {
+    return pack != nullptr ? pack->pack.content_hash : 0u;
+}
+
+uint32_t mr_policy_pack_format_version(const MRPolicyPackHandle* pack) {
+    return pack != nullptr ? pack->pack.format_version : 0u;
+}
+
+void mr_policy_pack_destroy(MRPolicyPackHandle* pack) { delete pack; }
+
+int mr_write_policy_rollout_pack(const MRPolicyRolloutPackC* pack, const char* output_path) {
+    return translate([&]() {
+        require(pack != nullptr && output_path != nullptr && output_path[0] != '\0',
+                "PolicyRolloutPack and output path are required");
+        (void)mr::write_policy_rollout_pack(*pack, output_path);
+    });
+}
+
+MRPolicyHandle* mr_policy_create(const MRPolicyPackC* policy, uint32_t environments) {
+    return construct<MRPolicyHandle>([&]() {
+        require(policy != nullptr, "null policy");
+        auto h = std::make_unique<MRPolicyHandle>();
+        h->policy = std::make_unique<mr::Policy>(mr::policy_pack_from_c(*policy), environments);
+        return h.release();
+    });
+}
+
+void mr_policy_destroy(MRPolicyHandle* policy) { delete policy; }
+
+uint64_t mr_policy_revision(const MRPolicyHandle* policy) {
+    return policy != nullptr ? policy->policy->revision() : 0u;
+}
+
+uint32_t mr_policy_actor_input_count(const MRPolicyHandle* policy) {
+    return policy != nullptr ? policy->policy->actor_input() : 0u;
+}
+
+uint32_t mr_policy_critic_input_count(const MRPolicyHandle* policy) {
+    return policy != nullptr ? policy->policy->critic_input() : 0u;
+}
+
+uint32_t mr_policy_action_count(const MRPolicyHandle* policy) {
+    return policy != nullptr ? policy->policy->actions() : 0u;
+}
+
+int mr_policy_act(MRPolicyHandle* policy, const float* actor_observations, const float* critic_observations,
+                  const MRTaskStateC* task_states, uint32_t control_step, uint64_t seed, float* actions,
+                  float* latents, float* log_probabilities, float* values, void* stream) {
+    return translate([&]() {
+        require(policy != nullptr, "null policy");
+        policy->policy->act(actor_observations, critic_observations, task_states, control_step, seed, actions,
+                            latents, log_probabilities, values, static_cast<hipStream_t>(stream));
+    });
+}
+
+int mr_locomotion_task_set_policy(MRLocomotionTaskHandle* task, const MRPolicyPackC* policy) {
+    return translate([&]() {
+        require(task != nullptr, "null task");
+        require(policy != nullptr, "task policy is null");
+        task->task->set_policy(mr::policy_pack_from_c(*policy));
+    });
+}
+
+int mr_locomotion_task_load_policy_pack(MRLocomotionTaskHandle* task, const char* policy_pack_path) {
+    return translate([&]() {
+        require(task != nullptr, "null task");
+        require(policy_pack_path != nullptr && policy_pack_path[0] != '\0', "PolicyPack path is empty");
+        task->task->set_policy(mr::read_policy_pack(policy_pack_path));
+    });
+}
+
+uint64_t mr_locomotion_task_policy_revision(const MRLocomotionTaskHandle* task) {
+    return task != nullptr ? task->task->policy_revision() : 0u;
+}
+
+int mr_locomotion_task_clear_policy(MRLocomotionTaskHandle* task) {
+    return translate([&]() {
+        require(task != nullptr, "null task");
+        task->task->clear_policy();
+    });
+}
+
+const float* mr_locomotion_task_device_policy_actions(const MRLocomotionTaskHandle* task) {
+    return task != nullptr ? task->task->device_policy_actions() : nullptr;
+}
+
+const float* mr_locomotion_task_device_policy_latents(const MRLocomotionTaskHandle* task) {
+    return task != nullptr ? task->task->device_policy_latents() : nullptr;
+}
+
+const float* mr_locomotion_task_device_policy_log_probabilities(const MRLocomotionTaskHandle* task) {
+    return task != nullptr ? task->task->device_policy_log_probabilities() : nullptr;
+}
+
+int mr_locomotion_task_read_policy_outputs(MRLocomotionTaskHandle* task, float* actions, size_t action_count,
+                                           float* latents, size_t latent_count, float* log_probabilities,
+                                           size_t log_probability_count) {
+    return translate([&]() {
+        require(task != nullptr, "null task");
+        task->task->read_policy_outputs(actions, action_count, latents, latent_count, log_probabilities,
+                                        log_probability_count);
+    });
+}
+
+int mr_locomotion_task_write_policy_rollout_pack(const MRLocomotionTaskHandle* task,
+                                                 const MRPolicyRolloutBatchC* batch, const char* batch_id,
+                                                 const char* output_path) {
+    return translate([&]() {
+        require(task != nullptr, "null task");
+        require(batch != nullptr && batch_id != nullptr && batch_id[0] != '\0' && output_path != nullptr &&
+                    output_path[0] != '\0',
+                "policy rollout batch, identity, and output path are required");
+        task->task->write_policy_rollout_pack(*batch, batch_id, output_path);
     });
 }
 

@@ -367,6 +367,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_task_complete_kernel(mr::Tas
     tr.termination[1] = timeout ? 1u : 0u;
     tr.termination[2] = physics_error ? 1u : 0u;
     tr.termination[3] = reason;
+    tr.policy_revision = a.policy_revision;
     for (int k = 0; k < 4; ++k) {
         tr.reward_breakdown0[k] = b0[k];
         tr.reward_breakdown1[k] = b1[k];

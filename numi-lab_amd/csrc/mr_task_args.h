@@ -27,6 +27,7 @@ struct TaskArgs {
     float root_reference[3];
     float command_lower[4], command_upper[4], command_seconds[2];
     unsigned long long seed;
+    unsigned long long policy_revision;  // stamped on every transition
     // world state (device)
     const float* q;        // [E][max_nq]
     const float* v;        // [E][max_nv]

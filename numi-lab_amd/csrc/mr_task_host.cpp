@@ -2,6 +2,7 @@
 // tables and the per-control-step sequence apply_actions -> physics ->
 // complete -> observe on the world's stream.
 #include "mr_task_host.h"
+#include "mr_policy.h"
 
 #include <algorithm>
 #include <cmath>
@@ -109,6 +110,28 @@ LocomotionTask::LocomotionTask(const MRLocomotionTaskProgramC& p, World& world) 
     upload(d_default_q_, dq.data(), dq.size());
     upload(d_fraction_, fraction.data(), fraction.size());
 
+    // the program's content hash: scalar fields (pointers zeroed), then each table and the nominal pose
+    {
+        MRLocomotionTaskProgramC scalars = p;
+        scalars.actions = nullptr;
+        scalars.observations = nullptr;
+        scalars.rewards = nullptr;
+        scalars.terminations = nullptr;
+        scalars.default_q = nullptr;
+        std::vector<uint8_t> bytes;
+        const auto add = [&](const void* data, size_t n) {
+            const auto* b = static_cast<const uint8_t*>(data);
+            bytes.insert(bytes.end(), b, b + n);
+        };
+        add(&scalars, sizeof(scalars));
+        add(p.actions, p.action_count * sizeof(MRTaskActionBindingC));
+        add(p.observations, p.observation_count * sizeof(MRTaskObservationOperatorC));
+        add(p.rewards, p.reward_count * sizeof(MRTaskRewardOperatorC));
+        add(p.terminations, p.termination_count * sizeof(MRTaskTerminationOperatorC));
+        add(p.default_q, nq * sizeof(float));
+        fingerprint_ = learning_pack_hash(bytes.data(), bytes.size());
+    }
+
     TaskArgs& a = args_;
     a.actions = static_cast<const MRTaskActionBindingC*>(d_actions_.ptr);
     a.observations = static_cast<const MRTaskObservationOperatorC*>(d_observations_.ptr);
@@ -190,11 +213,69 @@ void LocomotionTask::reset() {
     hip_check(launch_task_observe(args_, s), "task observe");
 }
 
+LocomotionTask::~LocomotionTask() = default;
+
+void LocomotionTask::set_policy(const PolicyPack& pack) {
+    auto compiled = std::make_unique<Policy>(pack, args_.environments, observation_floats(), args_.action_count);
+    const size_t n = (size_t)args_.environments * args_.action_count * sizeof(float);
+    d_policy_actions_.reserve(n);
+    d_policy_latents_.reserve(n);
+    d_policy_log_probabilities_.reserve((size_t)args_.environments * sizeof(float));
+    policy_ = std::move(compiled);
+    args_.policy_revision = policy_->revision();
+}
+
+void LocomotionTask::clear_policy() {
+    policy_.reset();
+    args_.policy_revision = 0;
+}
+
+uint64_t LocomotionTask::policy_revision() const { return policy_ ? policy_->revision() : 0u; }
+
+const float* LocomotionTask::device_policy_actions() const {
+    return policy_ ? static_cast<const float*>(d_policy_actions_.ptr) : nullptr;
+}
+
+const float* LocomotionTask::device_policy_latents() const {
+    return policy_ ? static_cast<const float*>(d_policy_latents_.ptr) : nullptr;
+}
+
+const float* LocomotionTask::device_policy_log_probabilities() const {
+    return policy_ ? static_cast<const float*>(d_policy_log_probabilities_.ptr) : nullptr;
+}
+
+void LocomotionTask::read_policy_outputs(float* actions, size_t action_count, float* latents, size_t latent_count,
+                                         float* log_probabilities, size_t log_probability_count) const {
+    if (!policy_) throw std::runtime_error("locomotion task: no policy is installed");
+    const size_t E = args_.environments, EA = E * args_.action_count;
+    const auto copy = [&](float* out, size_t count, const DeviceBuffer& b, size_t need, const char* what) {
+        if (out == nullptr) return;
+        if (count != need)
+            throw std::invalid_argument(std::string("locomotion task: ") + what + " count must be " +
+                                        std::to_string(need));
+        hip_check(hipMemcpy(out, b.ptr, need * sizeof(float), hipMemcpyDeviceToHost), what);
+    };
+    hip_check(hipStreamSynchronize(world_.stream()), "sync");
+    copy(actions, action_count, d_policy_actions_, EA, "policy actions");
+    copy(latents, latent_count, d_policy_latents_, EA, "policy latents");
+    copy(log_probabilities, log_probability_count, d_policy_log_probabilities_, E, "policy log probabilities");
+}
+
 void LocomotionTask::step(const float* device_actions) {
-    if (device_actions == nullptr) throw std::runtime_error("locomotion task: actions are required");
     if (world_.queue_grid() != 0u)
         throw std::runtime_error("locomotion task: the world must step one workgroup per environment");
     hipStream_t s = world_.stream();
+    if (policy_) {
+        if (device_actions != nullptr)
+            throw std::runtime_error("locomotion task: a policy is installed, so the step takes no actions");
+        auto* actions = static_cast<float*>(d_policy_actions_.ptr);
+        policy_->act(args_.observations_out, nullptr, args_.states, 0u, args_.seed, actions,
+                     static_cast<float*>(d_policy_latents_.ptr),
+                     static_cast<float*>(d_policy_log_probabilities_.ptr), nullptr, s);
+        device_actions = actions;
+    } else if (device_actions == nullptr) {
+        throw std::runtime_error("locomotion task: actions are required");
+    }
     hip_check(launch_task_apply_actions(args_, device_actions, s), "task apply_actions");
     world_.enqueue_chained(args_.targets, args_.reset_mask, 1u);
     hip_check(launch_task_complete(args_, world_.last_status_device(), s), "task complete");
@@ -228,3 +309,59 @@ static_assert(sizeof(MRTaskTransitionC) == 128, "MRTaskTransitionGPU layout");
 
 static_assert(sizeof(MRLocomotionTaskProgramC) == 160 && offsetof(MRLocomotionTaskProgramC, seed) == 152,
               "MRLocomotionTaskProgramC layout (mirrored by the ctypes binding)");
+
+namespace mr {
+
+void LocomotionTask::write_policy_rollout_pack(const MRPolicyRolloutBatchC& batch, const std::string& id,
+                                               const std::string& path) const {
+    if (!policy_) throw std::invalid_argument("a compiled policy must be installed before publishing a rollout pack");
+    if (batch.transition_count != 0 && batch.transitions == nullptr)
+        throw std::invalid_argument("rollout transition pointer is null");
+    // MRTaskTransitionC -> MRLearningTransitionC as c_api.cpp:4420-4447 maps them
+    std::vector<MRLearningTransitionC> transitions(batch.transition_count);
+    for (size_t i = 0; i < batch.transition_count; ++i) {
+        const MRTaskTransitionC& s = batch.transitions[i];
+        MRLearningTransitionC& d = transitions[i];
+        d = MRLearningTransitionC{};
+        d.reward_and_bootstrap[0] = s.reward_and_state[0];
+        d.reward_and_bootstrap[1] = s.timeout_bootstrap_value;
+        for (int k = 0; k < 4; ++k) {
+            d.termination[k] = s.termination[k];
+            d.context[k] = s.task_progress[k];
+        }
+        d.policy_revision = s.policy_revision;
+    }
+    MRPolicyRolloutPackC k{};
+    k.id = id.c_str();
+    k.task_fingerprint = fingerprint_;
+    k.policy_fingerprint = policy_->fingerprint();
+    k.policy_revision = policy_->revision();
+    k.environment_count = args_.environments;
+    k.control_step_count = batch.control_step_count;
+    k.actor_observation_count = observation_floats();
+    // without a critic network the critic reads the actor observation
+    k.critic_observation_count = policy_->critic_input() != 0 ? policy_->critic_input() : observation_floats();
+    k.action_count = args_.action_count;
+    k.motion_feature_count = 0;
+    k.actor_observations = batch.actor_observations;
+    k.actor_observation_count_total = batch.actor_observation_count;
+    k.critic_observations = batch.critic_observations;
+    k.critic_observation_count_total = batch.critic_observation_count;
+    k.motion_features = batch.motion_features;
+    k.motion_feature_count_total = batch.motion_feature_count;
+    k.teacher_actions = batch.teacher_actions;
+    k.teacher_action_count = batch.teacher_action_count;
+    k.latents = batch.latents;
+    k.latent_count = batch.latent_count;
+    k.log_probabilities = batch.log_probabilities;
+    k.log_probability_count = batch.log_probability_count;
+    k.values = batch.values;
+    k.value_count = batch.value_count;
+    k.bootstrap_values = batch.bootstrap_values;
+    k.bootstrap_value_count = batch.bootstrap_value_count;
+    k.transitions = transitions.data();
+    k.transition_count = transitions.size();
+    mr::write_policy_rollout_pack(k, path);
+}
+
+} // namespace mr

@@ -5,6 +5,7 @@ libmetalrobo_amd.so; this package is the ctypes boundary."""
 from .native import (  # noqa: F401
     BODY_STATE_DTYPE,
     CONTACT_STATUS_DTYPE,
+    LEARNING_TRANSITION_DTYPE,
     MANIFOLD_HEADER_DTYPE,
     MANIFOLD_POINT_DTYPE,
     MR_RUN_SOURCE_SYNTHETIC_CHAIN_BOXES,
@@ -32,4 +33,12 @@ from .native import (  # noqa: F401
     WorldStepConfigC,
     load,
     version,
+)
+from .packs import (  # noqa: F401
+    DenseLayer,
+    Policy,
+    PolicyPack,
+    learning_pack_content_hash,
+    read_policy_rollout_pack,
+    write_policy_rollout_pack,
 )

@@ -223,6 +223,73 @@ class LocomotionTaskProgramC(ct.Structure):
     ]
 
 
+class PolicyDenseLayerC(ct.Structure):
+    """MRPolicyDenseLayerC (include/mr_packs.h, c_api.h:334-342)."""
+    _fields_ = [("input_count", ct.c_uint32), ("output_count", ct.c_uint32), ("activation", ct.c_uint32),
+                ("weights", ct.c_void_p), ("weight_count", ct.c_size_t), ("bias", ct.c_void_p),
+                ("bias_count", ct.c_size_t)]
+
+
+_POLICY_TABLES = (("observation_mean", "observation_mean_count"),
+                  ("observation_inverse_standard_deviation", "observation_inverse_standard_deviation_count"),
+                  ("layers", "layer_count"), ("critic_observation_mean", "critic_observation_mean_count"),
+                  ("critic_observation_inverse_standard_deviation",
+                   "critic_observation_inverse_standard_deviation_count"),
+                  ("critic_layers", "critic_layer_count"),
+                  ("action_log_standard_deviation", "action_log_standard_deviation_count"),
+                  ("action_bias", "action_bias_count"), ("action_scale", "action_scale_count"))
+
+
+class PolicyPackC(ct.Structure):
+    """MRPolicyPackC (include/mr_packs.h, c_api.h:344-372)."""
+    _fields_ = ([("id", ct.c_char_p), ("revision", ct.c_uint64), ("contract_version", ct.c_uint64),
+                 ("world_fingerprint", ct.c_uint64), ("task_fingerprint", ct.c_uint64),
+                 ("observation_fingerprint", ct.c_uint64), ("action_fingerprint", ct.c_uint64)]
+                + [f for table, count in _POLICY_TABLES for f in ((table, ct.c_void_p), (count, ct.c_size_t))]
+                + [("observation_clip", ct.c_float), ("action_clip", ct.c_float)])
+
+
+_ROLLOUT_BATCH_TABLES = (("actor_observations", "actor_observation_count"),
+                         ("critic_observations", "critic_observation_count"),
+                         ("motion_features", "motion_feature_count"), ("teacher_actions", "teacher_action_count"),
+                         ("latents", "latent_count"), ("log_probabilities", "log_probability_count"),
+                         ("values", "value_count"), ("bootstrap_values", "bootstrap_value_count"),
+                         ("transitions", "transition_count"))
+
+
+class PolicyRolloutBatchC(ct.Structure):
+    """MRPolicyRolloutBatchC (include/mr_packs.h, c_api.h:374-394)."""
+    _fields_ = [("control_step_count", ct.c_uint32)] + [
+        fld for table, count in _ROLLOUT_BATCH_TABLES for fld in ((table, ct.c_void_p), (count, ct.c_size_t))]
+
+
+class PolicyOutcomeDescriptorC(ct.Structure):
+    """MRPolicyOutcomeDescriptorC (include/mr_packs.h)."""
+    _fields_ = [("id", ct.c_char_p), ("unit", ct.c_char_p), ("direction", ct.c_uint32)]
+
+
+class PolicyRolloutPackC(ct.Structure):
+    """MRPolicyRolloutPackC (include/mr_packs.h)."""
+    _fields_ = [("id", ct.c_char_p), ("task_fingerprint", ct.c_uint64), ("policy_fingerprint", ct.c_uint64),
+                ("policy_revision", ct.c_uint64)] + [(n, ct.c_uint32) for n in (
+                    "environment_count", "control_step_count", "actor_observation_count", "critic_observation_count",
+                    "action_count", "motion_feature_count")] + [
+                ("actor_observations", ct.c_void_p), ("actor_observation_count_total", ct.c_size_t),
+                ("critic_observations", ct.c_void_p), ("critic_observation_count_total", ct.c_size_t),
+                ("motion_features", ct.c_void_p), ("motion_feature_count_total", ct.c_size_t),
+                ("teacher_actions", ct.c_void_p), ("teacher_action_count", ct.c_size_t),
+                ("latents", ct.c_void_p), ("latent_count", ct.c_size_t),
+                ("log_probabilities", ct.c_void_p), ("log_probability_count", ct.c_size_t),
+                ("values", ct.c_void_p), ("value_count", ct.c_size_t),
+                ("bootstrap_values", ct.c_void_p), ("bootstrap_value_count", ct.c_size_t),
+                ("outcomes", ct.c_void_p), ("outcome_count", ct.c_size_t),
+                ("outcome_values", ct.c_void_p), ("outcome_value_count", ct.c_size_t),
+                ("transitions", ct.c_void_p), ("transition_count", ct.c_size_t)]
+
+
+assert ct.sizeof(PolicyDenseLayerC) == 48 and ct.sizeof(PolicyPackC) == 208 and ct.sizeof(PolicyRolloutBatchC) == 152
+
+
 # numpy mirrors of the bulk records (include/mr_world_types.h)
 F4 = ("<f4", (4,))
 MODEL_HEADER_DTYPE: Final = np.dtype([(n, "<u4") for n in (
@@ -281,6 +348,10 @@ TASK_TRANSITION_DTYPE: Final = np.dtype([
     ("reward_and_state", *F4), ("termination", *U4), ("reward_breakdown0", *F4), ("reward_breakdown1", *F4),
     ("outcome_channels0", *F4), ("outcome_channels1", *F4), ("policy_revision", "<u8"),
     ("timeout_bootstrap_value", "<f4"), ("episode_tracking_score", "<f4"), ("task_progress", *U4)])
+# MRLearningTransitionC (include/mr_packs.h; task_program_types.h:628-637)
+LEARNING_TRANSITION_DTYPE: Final = np.dtype([("reward_and_bootstrap", *F4), ("termination", *U4), ("context", *U4),
+                                             ("policy_revision", "<u8"), ("reserved", "<u8")])
+assert LEARNING_TRANSITION_DTYPE.itemsize == 64
 MR_ACTUATOR_PROFILE_CALIBRATED: Final = 2
 
 _EXPECTED_SIZES: Final = {
@@ -379,6 +450,33 @@ _SIGNATURES: Final = {
     "mr_policy_sample_and_score": ([ct.c_void_p, ct.c_uint32, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_float,
                                     ct.c_uint64, ct.c_uint64, ct.c_void_p, ct.c_uint32, ct.c_uint32, ct.c_uint32,
                                     ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p], ct.c_int),
+    "mr_learning_pack_content_hash": ([ct.c_void_p, ct.c_size_t], ct.c_uint64),
+    "mr_write_policy_pack": ([ct.POINTER(PolicyPackC), ct.c_char_p], ct.c_int),
+    "mr_read_policy_pack": ([ct.c_char_p], ct.c_void_p),
+    "mr_policy_pack_view": ([ct.c_void_p], ct.POINTER(PolicyPackC)),
+    "mr_policy_pack_content_hash": ([ct.c_void_p], ct.c_uint64),
+    "mr_policy_pack_format_version": ([ct.c_void_p], ct.c_uint32),
+    "mr_policy_pack_destroy": ([ct.c_void_p], None),
+    "mr_write_policy_rollout_pack": ([ct.POINTER(PolicyRolloutPackC), ct.c_char_p], ct.c_int),
+    "mr_policy_create": ([ct.POINTER(PolicyPackC), ct.c_uint32], ct.c_void_p),
+    "mr_policy_destroy": ([ct.c_void_p], None),
+    "mr_policy_revision": ([ct.c_void_p], ct.c_uint64),
+    "mr_policy_actor_input_count": ([ct.c_void_p], ct.c_uint32),
+    "mr_policy_critic_input_count": ([ct.c_void_p], ct.c_uint32),
+    "mr_policy_action_count": ([ct.c_void_p], ct.c_uint32),
+    "mr_policy_act": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32, ct.c_uint64, ct.c_void_p,
+                       ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p], ct.c_int),
+    "mr_locomotion_task_set_policy": ([ct.c_void_p, ct.POINTER(PolicyPackC)], ct.c_int),
+    "mr_locomotion_task_load_policy_pack": ([ct.c_void_p, ct.c_char_p], ct.c_int),
+    "mr_locomotion_task_policy_revision": ([ct.c_void_p], ct.c_uint64),
+    "mr_locomotion_task_clear_policy": ([ct.c_void_p], ct.c_int),
+    "mr_locomotion_task_device_policy_actions": ([ct.c_void_p], ct.c_void_p),
+    "mr_locomotion_task_device_policy_latents": ([ct.c_void_p], ct.c_void_p),
+    "mr_locomotion_task_device_policy_log_probabilities": ([ct.c_void_p], ct.c_void_p),
+    "mr_locomotion_task_read_policy_outputs": ([ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_void_p, ct.c_size_t,
+                                                ct.c_void_p, ct.c_size_t], ct.c_int),
+    "mr_locomotion_task_write_policy_rollout_pack": ([ct.c_void_p, ct.POINTER(PolicyRolloutBatchC), ct.c_char_p,
+                                                      ct.c_char_p], ct.c_int),
     "mr_world_enqueue_device_horizon": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32], ct.c_int),
     "mr_world_read_world_step_microseconds": ([ct.c_void_p, ct.c_void_p, ct.c_size_t], ct.c_int),
     "mr_world_enqueue_device_static": ([ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32, ct.c_void_p,
@@ -938,15 +1036,66 @@ class LocomotionTask:
     def reset(self) -> None:
         self._b.check(self._b.lib.mr_locomotion_task_reset(self._h))
 
-    def step(self, actions) -> None:
-        """One control step with device actions (contiguous float32 GPU tensor [E][action_count])."""
+    def step(self, actions=None) -> None:
+        """One control step with device actions (contiguous float32 GPU tensor [E][action_count]),
+        or None when a policy is installed."""
         import torch
 
+        if actions is None:
+            self._b.check(self._b.lib.mr_locomotion_task_step(self._h, None))
+            return
         if (tuple(actions.shape) != (self.environment_count, self.action_count) or actions.dtype != torch.float32
                 or not actions.is_cuda or not actions.is_contiguous()):
             raise ValueError(f"actions must be a contiguous float32 GPU tensor of shape "
                              f"{(self.environment_count, self.action_count)}")
         self._b.check(self._b.lib.mr_locomotion_task_step(self._h, actions.data_ptr()))
+
+    def set_policy(self, pack) -> None:
+        """mr_locomotion_task_set_policy with a metalrobo.packs.PolicyPack."""
+        c, keep = pack.to_c()
+        self._b.check(self._b.lib.mr_locomotion_task_set_policy(self._h, ct.byref(c)))
+        del keep
+
+    def load_policy_pack(self, path: str) -> None:
+        self._b.check(self._b.lib.mr_locomotion_task_load_policy_pack(self._h, str(path).encode()))
+
+    def clear_policy(self) -> None:
+        self._b.check(self._b.lib.mr_locomotion_task_clear_policy(self._h))
+
+    def policy_revision(self) -> int:
+        return int(self._b.lib.mr_locomotion_task_policy_revision(self._h))
+
+    def read_policy_outputs(self):
+        """(actions [E][A], latents [E][A], log probabilities [E]) of the last policy step, after waiting."""
+        E, A = self.environment_count, self.action_count
+        actions = np.zeros((E, A), np.float32)
+        latents = np.zeros((E, A), np.float32)
+        logp = np.zeros(E, np.float32)
+        self._b.check(self._b.lib.mr_locomotion_task_read_policy_outputs(
+            self._h, _ptr(actions), actions.size, _ptr(latents), latents.size, _ptr(logp), logp.size))
+        return actions, latents, logp
+
+    def write_policy_rollout_pack(self, path: str, batch_id: str, control_step_count: int, actor_observations,
+                                  critic_observations, latents, log_probabilities, values, bootstrap_values,
+                                  transitions, teacher_actions=None) -> None:
+        """mr_locomotion_task_write_policy_rollout_pack from host numpy tables (step-major)."""
+        keep = []
+        c = PolicyRolloutBatchC()
+        c.control_step_count = int(control_step_count)
+        for (name, count), arr in zip(_ROLLOUT_BATCH_TABLES, (
+                actor_observations, critic_observations, None, teacher_actions, latents, log_probabilities, values,
+                bootstrap_values, transitions)):
+            if arr is None:
+                setattr(c, name, None)
+                setattr(c, count, 0)
+                continue
+            a = (np.ascontiguousarray(arr, TASK_TRANSITION_DTYPE) if name == "transitions"
+                 else np.ascontiguousarray(np.asarray(arr, np.float32).reshape(-1)))
+            keep.append(a)
+            setattr(c, name, a.ctypes.data if a.size else None)
+            setattr(c, count, a.size)
+        self._b.check(self._b.lib.mr_locomotion_task_write_policy_rollout_pack(
+            self._h, ct.byref(c), batch_id.encode(), str(path).encode()))
 
     def read(self):
         """(observations [E][history][O], transitions [E], task states [E]) after waiting."""

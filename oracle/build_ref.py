@@ -8,11 +8,19 @@ Compiled reference files (under /root/reference, read in place, never copied):
   src/core/ReferenceConicSolver.cpp  exact-cone contact solve
   src/core/EngineModel.cpp           EngineModel::valid, used by the dynamics
   src/core/ConstraintIR.cpp          validateConstraintIR, used by EngineModel::valid
+  src/core/ConstraintSolver.cpp      projected Gauss-Seidel over free bodies
+                                     (solveContactConstraints)
 They need only the C++ standard library and the reference's own headers
 (include/metalrobo), so g++ builds them directly; nothing else of the
 reference's build is run. Only tests/ load the result.
+
+It also copies one data file the reference ships, a trained G1 locomotion
+policy pack (artifacts/g1-legs-locomotion-60min-20260805/retained-policy/
+candidate.policypack), to oracle/_ref/g1_legs_locomotion.policypack, the
+golden input of the policy-pack tests.
 """
 import os
+import shutil
 import subprocess
 import sys
 
@@ -21,7 +29,23 @@ REFERENCE = "/root/reference"
 OUT_DIR = os.path.join(HERE, "_ref")
 LIBRARY = os.path.join(OUT_DIR, "libmr_ref.so")
 SOURCES = ["ArticulatedDynamics.cpp", "Collision.cpp", "ReferenceConicSolver.cpp", "EngineModel.cpp",
-           "ConstraintIR.cpp"]
+           "ConstraintIR.cpp", "ConstraintSolver.cpp"]
+
+
+POLICY_PACK = os.path.join(REFERENCE, "artifacts", "g1-legs-locomotion-60min-20260805", "retained-policy",
+                           "candidate.policypack")
+POLICY_PACK_COPY = os.path.join(OUT_DIR, "g1_legs_locomotion.policypack")
+
+
+def copy_policy_pack(verbose=True):
+    """Copies the reference's trained policy pack into oracle/_ref when it is present."""
+    if not os.path.isfile(POLICY_PACK):
+        return None
+    os.makedirs(OUT_DIR, exist_ok=True)
+    shutil.copyfile(POLICY_PACK, POLICY_PACK_COPY)
+    if verbose:
+        print(f"copied {POLICY_PACK_COPY}")
+    return POLICY_PACK_COPY
 
 
 def available() -> bool:
@@ -57,6 +81,7 @@ def build(verbose=True):
         os.remove(o)
     if verbose:
         print(f"built {LIBRARY}")
+    copy_policy_pack(verbose)
     return LIBRARY
 
 

@@ -241,11 +241,12 @@ void mr_oracle_task_apply_actions(const MRLocomotionTaskProgramC* pp, uint32_t E
 }
 
 // complete for environments [0, E): q [E][max_nq], v [E][max_nv] after the
-// physics step, dofs the model's DoF table, status_codes [E] of the step.
+// physics step, dofs the model's DoF table, status_codes [E] of the step;
+// policy_revision is stamped on every transition.
 void mr_oracle_task_complete(const MRLocomotionTaskProgramC* pp, uint32_t E, uint32_t max_nq, uint32_t max_nv,
                              float dt, const float* qs, const float* vs, const MRDofC* dofs,
                              const uint32_t* status_codes, MRTaskStateC* states, const float* history,
-                             float* previous_joint, MRTaskTransitionC* transitions) {
+                             float* previous_joint, MRTaskTransitionC* transitions, uint64_t policy_revision) {
     const MRLocomotionTaskProgramC& p = *pp;
     const uint32_t A = p.action_count, slots = p.delay_slots;
     for (uint32_t env = 0; env < E; ++env) {
@@ -397,6 +398,7 @@ void mr_oracle_task_complete(const MRLocomotionTaskProgramC* pp, uint32_t E, uin
         tr.termination[1] = timeout ? 1u : 0u;
         tr.termination[2] = physics_error ? 1u : 0u;
         tr.termination[3] = reason;
+        tr.policy_revision = policy_revision;
         for (int k = 0; k < 4; ++k) {
             tr.reward_breakdown0[k] = b0[k];
             tr.reward_breakdown1[k] = b1[k];

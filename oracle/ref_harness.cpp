@@ -10,7 +10,9 @@
 //   * collideCpuReference                 (Collision.hpp, Collision.cpp:2959-3228)
 //     — FP64 SAP broadphase, analytic narrowphase, persistent manifolds;
 //   * solveReferenceConicProblem          (ReferenceConicSolver.hpp,
-//     ReferenceConicSolver.cpp:182) — exact circular-cone contact solve.
+//     ReferenceConicSolver.cpp:182) — exact circular-cone contact solve;
+//   * solveContactConstraints             (ConstraintSolver.hpp:101-105,
+//     ConstraintSolver.cpp:1047) — projected Gauss-Seidel over free bodies.
 //
 // Arguments are plain pointers and sizes. Records that cross this boundary
 // are declared here with the byte layout of the matching reference records,
@@ -19,6 +21,7 @@
 
 #include "metalrobo/ArticulatedDynamics.hpp"
 #include "metalrobo/Collision.hpp"
+#include "metalrobo/ConstraintSolver.hpp"
 #include "metalrobo/EngineModel.hpp"
 #include "metalrobo/ReferenceConicSolver.hpp"
 
@@ -26,6 +29,7 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <span>
 #include <string>
 #include <vector>
 
@@ -352,6 +356,46 @@ int mr_ref_conic_solve(uint32_t nv, const double* inverse_mass, const double* fr
         }
         for (uint32_t i = 0; i < nv; ++i) velocity[i] = s.velocity[i];
         for (size_t i = 0; i < s.impulses.size(); ++i) impulses[i] = s.impulses[i];
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(err, errlen, e.what());
+        return -1;
+    }
+}
+
+static_assert(sizeof(MRBodyStateGPU) == 128, "MRBodyStateGPU layout");
+static_assert(sizeof(MRContactConstraintGPU) == 144, "MRContactConstraintGPU layout");
+
+// Projected Gauss-Seidel contact solve of free bodies (solveContactConstraints,
+// ConstraintSolver.cpp:1047-1233), in place: bodies [body_count] and contacts
+// [contact_count] use the reference's MRBodyStateGPU / MRContactConstraintGPU
+// byte layouts; velocities and impulses are updated. Runs exactly `iterations`
+// sweeps (deterministic mode, no early exit). info [4] = (iterations, last
+// sweep's maximum impulse change, maximum normal residual, maximum cone
+// violation). Returns the reference's status code (0 or a failure code).
+int mr_ref_pgs_solve(uint32_t body_count, void* bodies, uint32_t contact_count, void* contacts, double timestep,
+                     double error_reduction, double penetration_slop, double max_depenetration_velocity,
+                     uint32_t iterations, double* info, char* err, size_t errlen) {
+    try {
+        ContactSolverConfig cfg;
+        cfg.timestep = timestep;
+        cfg.errorReduction = error_reduction;
+        cfg.penetrationSlop = penetration_slop;
+        cfg.maxDepenetrationVelocity = max_depenetration_velocity;
+        cfg.velocityIterations = iterations;
+        cfg.enableWarmStart = false;
+        cfg.deterministic = true;
+        std::span<MRBodyStateGPU> b(static_cast<MRBodyStateGPU*>(bodies), body_count);
+        std::span<MRContactConstraintGPU> c(static_cast<MRContactConstraintGPU*>(contacts), contact_count);
+        const ContactSolverDiagnostics d = solveContactConstraints(b, c, cfg);
+        info[0] = (double)d.iterations;
+        info[1] = d.maximumImpulseDelta;
+        info[2] = d.maximumNormalResidual;
+        info[3] = d.maximumConeViolation;
+        if (!d.succeeded()) {
+            set_error(err, errlen, "contact solve status " + std::to_string((uint32_t)d.code));
+            return (int)d.code;
+        }
         return 0;
     } catch (const std::exception& e) {
         set_error(err, errlen, e.what());

@@ -1,10 +1,12 @@
-"""Writes tests/golden/task_rollout_layout.json: sizeof and offsetof of every
-field of the task-rollout records, as the C compiler lays them out from a
-header (the reference's include/metalrobo/c_api.h when it is present).
+"""Writes tests/golden/task_rollout_layout.json and
+tests/golden/policy_pack_layout.json: sizeof and offsetof of every field of
+the task-rollout records and of the policy-pack records, as the C compiler
+lays them out from a header (the reference's include/metalrobo/c_api.h when
+it is present).
 
 The records and their field names are those of the reference header,
-c_api.h:55-235; the field lists below are read from this repository's
-include/mr_c_api.h, which declares the same records.
+c_api.h:55-235 and 334-394; the field lists are read from this repository's
+include/mr_c_api.h and include/mr_packs.h, which declare the same records.
 
 Usage: python tests/golden/task_rollout_layout.py [path/to/reference/include]
 """
@@ -21,14 +23,16 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 RECORDS = ["MRTaskRolloutDynamicSphereC", "MRTaskRolloutConfigC", "MRTaskRolloutLayoutC", "MRRunManifestC",
            "MRTaskRolloutStageHighWaterC", "MRTaskRolloutAdvanceC"]
 FIXTURE = os.path.join(HERE, "task_rollout_layout.json")
+PACK_RECORDS = ["MRPolicyDenseLayerC", "MRPolicyPackC", "MRPolicyRolloutBatchC"]
+PACK_FIXTURE = os.path.join(HERE, "policy_pack_layout.json")
 
 
-def record_fields(header):
+def record_fields(header, records=RECORDS):
     """Field names of each record, in declaration order."""
     text = re.sub(r"/\*.*?\*/", "", open(header).read(), flags=re.S)
     text = re.sub(r"//[^\n]*", "", text)
     out = {}
-    for name in RECORDS:
+    for name in records:
         m = re.search(r"typedef struct(?: MR_ALIGN16)? " + name + r" \{(.*?)\} " + name + ";", text, re.S)
         if m is None:
             raise RuntimeError(f"{name} not found in {header}")
@@ -69,6 +73,11 @@ def main():
     json.dump({"source": "include/metalrobo/c_api.h:55-235 (LP64, gcc)", "records": data}, open(FIXTURE, "w"),
               indent=1, sort_keys=True)
     print(f"wrote {FIXTURE}")
+    fields = record_fields(os.path.join(ROOT, "include", "mr_packs.h"), PACK_RECORDS)
+    data = layout(ref_include, "metalrobo/c_api.h", fields)
+    json.dump({"source": "include/metalrobo/c_api.h:334-394 (LP64, gcc)", "records": data}, open(PACK_FIXTURE, "w"),
+              indent=1, sort_keys=True)
+    print(f"wrote {PACK_FIXTURE}")
 
 
 if __name__ == "__main__":

@@ -312,7 +312,8 @@ def _task_lib():
         prog = ct.POINTER(LocomotionTaskProgramC)
         lib.mr_oracle_task_apply_actions.argtypes = [prog, ct.c_uint32, ct.c_uint32, ct.c_float, P, P, P, P]
         lib.mr_oracle_task_apply_actions.restype = None
-        lib.mr_oracle_task_complete.argtypes = [prog, ct.c_uint32, ct.c_uint32, ct.c_uint32, ct.c_float] + [P] * 8
+        lib.mr_oracle_task_complete.argtypes = ([prog, ct.c_uint32, ct.c_uint32, ct.c_uint32, ct.c_float] + [P] * 8
+                                                + [ct.c_uint64])
         lib.mr_oracle_task_complete.restype = None
         lib.mr_oracle_task_observe.argtypes = [prog, ct.c_uint32, ct.c_uint32, ct.c_uint32, ct.c_float] + [P] * 9
         lib.mr_oracle_task_observe.restype = None
@@ -351,6 +352,7 @@ class OracleTask:
         ph.reset_v = np.zeros((E, nv), np.float32)
         ph.reset_scene = model.initial_state()[2].copy()
         self.states = np.zeros(E, TASK_STATE_DTYPE)
+        self.policy_revision = 0  # stamped on every transition
         self.history = np.zeros((E, self.slots, self.A), np.float32)
         self.previous = np.zeros((E, 2, self.A), np.float32)
         self.observations = np.zeros((E, self.H, self.O), np.float32)
@@ -372,7 +374,7 @@ class OracleTask:
         ph = self.physics
         self.lib.mr_oracle_task_complete(ct.byref(self.prog), E, nq, nv, self.dt, _p(ph.q), _p(ph.v), _p(self.dofs),
                                          _p(codes), _p(self.states), _p(self.history), _p(self.previous),
-                                         _p(self.transitions))
+                                         _p(self.transitions), self.policy_revision)
 
     def observe(self):
         E, nq, nv = self.model.environment_count, self.model.max_nq, self.model.max_nv

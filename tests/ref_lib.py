@@ -42,6 +42,9 @@ def load():
         lib.mr_ref_conic_solve.argtypes = [ct.c_uint32, P, P, ct.c_uint32, P, P, P, P, P, ct.c_uint32,
                                            ct.c_double, P, P, P, ct.c_char_p, ct.c_size_t]
         lib.mr_ref_conic_solve.restype = ct.c_int
+        lib.mr_ref_pgs_solve.argtypes = [ct.c_uint32, P, ct.c_uint32, P, ct.c_double, ct.c_double, ct.c_double,
+                                         ct.c_double, ct.c_uint32, P, ct.c_char_p, ct.c_size_t]
+        lib.mr_ref_pgs_solve.restype = ct.c_int
         _lib = lib
     return _lib
 
@@ -136,3 +139,30 @@ def conic_solve(inverse_mass, free_velocity, jacobian, target, regularization, f
     rc = lib.mr_ref_conic_solve(nv, _p(M), _p(vf), k, _p(J), _p(tg), _p(rg), _p(w), _p(mu), max_iterations,
                                 tolerance, _p(vel), _p(lam), _p(info), err, 512)
     return rc, vel, lam, info, err.value.decode()
+
+
+# The reference's free-body state and contact constraint records (engine_types.h:418-434, 1538-1563).
+PGS_BODY_DTYPE = np.dtype([("position", "<f4", 4), ("orientation", "<f4", 4), ("linear_velocity_inverse_mass", "<f4", 4),
+                           ("angular_velocity", "<f4", 4), ("inverse_inertia_world", "<f4", (3, 4)),
+                           ("flags_and_indices", "<u4", 4)])
+PGS_CONTACT_DTYPE = np.dtype([("body_a", "<u4"), ("body_b", "<u4"), ("flags", "<u4"), ("island", "<u4"),
+                              ("pair_key", "<u8"), ("feature_key", "<u8"), ("point_and_separation", "<f4", 4),
+                              ("normal", "<f4", 4), ("tangent", "<f4", 4), ("friction", "<f4", 4),
+                              ("response", "<f4", 4), ("target_velocity_and_pre_solve_normal", "<f4", 4),
+                              ("impulses", "<f4", 4)])
+assert PGS_BODY_DTYPE.itemsize == 128 and PGS_CONTACT_DTYPE.itemsize == 144
+
+
+def pgs_solve(bodies, contacts, timestep, error_reduction, penetration_slop, max_depenetration_velocity,
+              iterations):
+    """Reference projected Gauss-Seidel over free bodies, in place on copies.
+    Returns (rc, bodies, contacts, info = (iterations, last maximum impulse change,
+    normal residual, cone violation), err)."""
+    lib = load()
+    b = np.array(bodies, dtype=PGS_BODY_DTYPE, copy=True)
+    c = np.array(contacts, dtype=PGS_CONTACT_DTYPE, copy=True)
+    info = np.zeros(4, np.float64)
+    err = ct.create_string_buffer(512)
+    rc = lib.mr_ref_pgs_solve(len(b), _p(b), len(c), _p(c), timestep, error_reduction, penetration_slop,
+                              max_depenetration_velocity, iterations, _p(info), err, 512)
+    return rc, b, c, info, err.value.decode()

@@ -154,6 +154,30 @@ def test_task_rollout_records_have_the_reference_layout():
             assert getattr(cls, f).offset == golden[name][f], (name, f)
 
 
+def test_policy_pack_records_have_the_reference_layout():
+    """MRPolicyDenseLayerC, MRPolicyPackC and MRPolicyRolloutBatchC sit at the
+    reference header's offsets (fixture tests/golden/policy_pack_layout.json,
+    written from c_api.h:334-394), in C and in the ctypes mirrors; where the
+    reference is present its header declares the same fields in order."""
+    import importlib.util
+    import json
+
+    spec = importlib.util.spec_from_file_location("trl", os.path.join(ROOT, "tests", "golden", "task_rollout_layout.py"))
+    trl = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(trl)
+    golden = json.load(open(os.path.join(ROOT, "tests", "golden", "policy_pack_layout.json")))["records"]
+    fields = trl.record_fields(os.path.join(ROOT, "include", "mr_packs.h"), trl.PACK_RECORDS)
+    assert trl.layout(os.path.join(ROOT, "include"), "mr_packs.h", fields) == golden
+    for name, cls in {"MRPolicyDenseLayerC": nat.PolicyDenseLayerC, "MRPolicyPackC": nat.PolicyPackC,
+                      "MRPolicyRolloutBatchC": nat.PolicyRolloutBatchC}.items():
+        assert ct.sizeof(cls) == golden[name]["size"], name
+        assert [f for f, _ in cls._fields_] == fields[name], name
+        for f in fields[name]:
+            assert getattr(cls, f).offset == golden[name][f], (name, f)
+    if os.path.isfile("/root/reference/include/metalrobo/c_api.h"):
+        assert trl.record_fields("/root/reference/include/metalrobo/c_api.h", trl.PACK_RECORDS) == fields
+
+
 @pytest.mark.skipif(not os.path.isfile("/root/reference/include/metalrobo/c_api.h"),
                     reason="the reference header is not present here")
 def test_task_rollout_layout_fixture_matches_the_reference_header():

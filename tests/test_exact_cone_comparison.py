@@ -92,3 +92,25 @@ def test_converged_sweeps_keep_the_same_gap():
     report("humanoid (adaptive, |dlambda| <= 1e-5, cap 128)", adaptive)
     assert np.all(np.isfinite(adaptive[:, 0]))
     assert abs(adaptive[:, 0].mean() - fixed[:, 0].mean()) <= 0.5 * fixed[:, 0].mean()
+
+
+def test_gap_holds_across_worlds():
+    """The comparison is not a property of world 0: every world of a humanoid
+    batch and of a random-tree batch stays within the horizon-level gap, and
+    the solve moves the velocity towards the exact one in most contact steps."""
+    for kind, envs, steps, seed, bound in ((mr.MR_SYNTHETIC_HUMANOID_TERRAIN, 6, 40, 2024, 0.25),
+                                           (mr.MR_SYNTHETIC_RANDOM_TREES, 6, 40, 11, 0.4)):
+        per_world = []
+        for env in range(envs):
+            rows = horizon_errors(kind, envs, steps, seed, env, exact_tolerance=1.0e-6)
+            if len(rows) == 0:
+                continue
+            assert np.all(np.isfinite(rows[:, 0]))
+            assert np.max(rows[:, 4]) <= 1.0e-4, "captured velocities disagree with the program's response map"
+            per_world.append((rows[:, 0].mean(), np.mean(rows[:, 0] < rows[:, 1])))
+        assert len(per_world) >= envs // 2
+        means = np.array(per_world)
+        print(f"kind {kind}: per-world mean exact-cone gap {np.round(means[:, 0], 3).tolist()}, "
+              f"fraction of steps moved towards the exact velocity {np.round(means[:, 1], 2).tolist()}", flush=True)
+        assert means[:, 0].max() <= bound, means[:, 0]
+        assert means[:, 1].mean() >= 0.85
