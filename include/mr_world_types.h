@@ -50,7 +50,15 @@ enum MRShapeType {
     MR_SHAPE_HEIGHTFIELD = 7u
 };
 
-enum MRShapeFlags { MR_SHAPE_FLAG_SIMULATION_DISABLED = 1u };
+/* MR_SHAPE_FLAG_ENABLE_CCD opts a collider into the hybrid mode's continuous
+ * collision query (MRShapeGPU flags, engine_types.h; Collision.metal:7205). */
+enum MRShapeFlags { MR_SHAPE_FLAG_SIMULATION_DISABLED = 1u, MR_SHAPE_FLAG_ENABLE_CCD = 2u };
+
+/* MetalWorldCCDMode (MetalWorld.hpp:420): speculative contacts only, or
+ * speculative contacts plus microsteps split at the earliest time of impact
+ * of CCD-enabled pairs. */
+enum MRCcdMode { MR_CCD_SPECULATIVE = 0u, MR_CCD_HYBRID = 1u };
+#define MR_CCD_MAX_EVENTS 16u
 
 enum MRMotionType {
     MR_MOTION_STATIC = 0u,
@@ -363,7 +371,9 @@ typedef struct MRContactStatusC {
      * cone sweeps run over the control step (all microsteps, fallbacks included) */
     mr_float4 residuals;
     /* max penetration, retained fraction, microsteps whose learned warm start
-     * fell back to the plain one, unused */
+     * fell back to the plain one, and w: those fallbacks the residual guard made
+     * or, in the hybrid CCD mode (which excludes the learned warm start), the
+     * segments that ended at a time of impact over the control step */
     mr_float4 diagnostics;
 } MRContactStatusC;
 
@@ -417,6 +427,15 @@ typedef struct MRWorldStepConfigC {
     uint32_t solver;                  /* MRSolverMode */
     float maximum_normal_impulse;     /* > 0: cap of every contact's normal impulse (the cone's
                                          maximumNormalImpulse, MetalWorldContact.metal:488-493); 0: none */
+    /* continuous collision (MetalWorld.hpp:420-426, MetalWorld.mm:3157-3215) */
+    uint32_t ccd_mode;                /* MRCcdMode */
+    uint32_t max_ccd_events;          /* impact segments per microstep, <= MR_CCD_MAX_EVENTS (default 4) */
+    uint32_t max_ccd_iterations;      /* conservative-advancement iterations per pair (default 16) */
+    uint32_t ccd_reserved;
+    float ccd_distance_tolerance;     /* m: a pair this close counts as touching (default 1e-3) */
+    float ccd_time_tolerance;         /* s: shortest impact segment (default 1e-5) */
+    float ccd_minimum_advance;        /* s: smallest conservative-advancement step (default 1e-6) */
+    float ccd_reserved1;
 } MRWorldStepConfigC;
 
 #ifdef __cplusplus
@@ -437,6 +456,6 @@ MR_STATIC_ASSERT(sizeof(MRManifoldPointC) == 64, "MRManifoldPointC size");
 MR_STATIC_ASSERT(sizeof(MRWorldStatusC) == 48, "MRWorldStatusC size");
 MR_STATIC_ASSERT(sizeof(MRContactStatusC) == 96, "MRContactStatusC size");
 MR_STATIC_ASSERT(sizeof(MRPairRecordC) == 16, "MRPairRecordC size");
-MR_STATIC_ASSERT(sizeof(MRWorldStepConfigC) == 88, "MRWorldStepConfigC size");
+MR_STATIC_ASSERT(sizeof(MRWorldStepConfigC) == 120, "MRWorldStepConfigC size");
 
 #endif /* MR_WORLD_TYPES_H */

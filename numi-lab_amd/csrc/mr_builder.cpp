@@ -1053,6 +1053,12 @@ ModelData copy_model(const MRWorldModelDescC& d, float microstep) {
     m.default_config.contact_slop = 0.001f;
     m.default_config.max_depenetration_velocity = 1.0f;
     m.default_config.max_iterations = 64u;
+    m.default_config.ccd_mode = MR_CCD_SPECULATIVE;
+    m.default_config.max_ccd_events = 4u;
+    m.default_config.max_ccd_iterations = 16u;
+    m.default_config.ccd_distance_tolerance = 1.0e-3f;
+    m.default_config.ccd_time_tolerance = 1.0e-5f;
+    m.default_config.ccd_minimum_advance = 1.0e-6f;
     return m;
 }
 

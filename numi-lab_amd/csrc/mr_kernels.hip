@@ -778,11 +778,9 @@ MRD_INL void inflate_bounds(Collider& c) {
     c.upper = add(c.upper, pad);
 }
 
-MRD_INL bool project_collider(const Env& e, uint32_t s, Collider& c) {
+// Collider of shape s with its body at (bpos, borient).
+MRD_INL bool project_collider_pose(const Env& e, uint32_t s, V3 bpos, Q4 borient, Collider& c) {
     const MRShapeC& sh = e.shapes[s];
-    V3 bpos;
-    Q4 borient;
-    body_pose(e, sh.body, bpos, borient);
     Q4 br, lr;
     if (!checked_quat(borient, br) || !checked_quat(q4(sh.local_rotation), lr)) return false;
     c.type = sh.shape_type;
@@ -841,6 +839,13 @@ MRD_INL bool project_collider(const Env& e, uint32_t s, Collider& c) {
         return false;
     }
     return finite3(c.lower) && finite3(c.upper);
+}
+
+MRD_INL bool project_collider(const Env& e, uint32_t s, Collider& c) {
+    V3 bpos;
+    Q4 borient;
+    body_pose(e, e.shapes[s].body, bpos, borient);
+    return project_collider_pose(e, s, bpos, borient, c);
 }
 
 MRD_INL float collider_scale(const Collider& c) {
@@ -1368,6 +1373,243 @@ MRD_INL void generate_contacts(const Env& e, uint32_t ia, uint32_t ib, uint32_t 
                                        feature_key(fs.type, q), h.feature);
         }
     }
+}
+
+
+// ------------------------------------------------ continuous collision
+//
+// The hybrid mode splits a microstep at the earliest time of impact (TOI) of
+// the CCD-enabled compiled pairs (mr_world_resolve_ccd and the event select /
+// finalize kernels, Collision.metal:6707-6950, 7157-7330, 7657-7900). Over the
+// remaining time T of the microstep every body moves rigidly: links with the
+// COM and angular velocity of the current kinematics, dynamic scene bodies
+// with their velocity plus T * gravity, kinematic bodies with theirs, static
+// bodies not at all; rotations follow exp(t w) at a constant rate. A pair whose
+// swept bounds overlap and whose colliders start farther apart than their
+// contact distance (the sum of the contact offsets; within it the segment's
+// speculative contacts hold the pair at any normal speed) gets a TOI: closed
+// form for two spheres (centres moving linearly),
+// conservative advancement otherwise, stepping t by distance / (closing-speed
+// bound), at least the minimum advance, and reporting the first t at which the
+// pair is within the tolerance. A step never closes the gap past contact, so
+// the pair is touching there without penetrating (up to speed x minimum
+// advance). The closing-speed bound is |v_a - v_b| + |w_a| r_a + |w_b| r_b, r
+// the largest distance from a body's COM to a point of its shape (exact for
+// the constant-rate rotation); against a static plane only the normal
+// component of the velocity counts. A pair that does not resolve within the iteration
+// cap is safe when both bodies move less than their contact offset plus the
+// tolerance (the speculative contacts cover it), and fails the step otherwise.
+
+struct CcdMotion {
+    V3 p0;
+    Q4 r0;
+    V3 v, w;
+};
+
+MRD_INL CcdMotion ccd_motion(const Env& e, uint32_t b, float T) {
+    const V3 zero{0.0f, 0.0f, 0.0f};
+    CcdMotion m;
+    if (b < e.nlinks) {
+        m.p0 = e.pos[b];
+        m.r0 = e.rot[b];
+        m.v = e.vl[b];
+        m.w = e.w[b];
+        return m;
+    }
+    const MRBodyStateC& s = e.scene[b - e.nlinks];
+    m.p0 = f3(s.position);
+    m.r0 = q4(s.orientation);
+    const uint32_t mt = e.topo->motion[b];
+    if (mt == MR_MOTION_DYNAMIC) {
+        m.v = add(f3(s.linear_velocity_inv_mass), mul(T, e.gravity));
+        m.w = f3(s.angular_velocity);
+    } else if (mt == MR_MOTION_KINEMATIC) {
+        m.v = f3(s.linear_velocity_inv_mass);
+        m.w = f3(s.angular_velocity);
+    } else {
+        m.v = zero;
+        m.w = zero;
+    }
+    return m;
+}
+
+MRD_INL bool ccd_collider(const Env& e, uint32_t s, const CcdMotion& m, float t, Collider& c) {
+    const V3 p = add(m.p0, mul(t, m.v));
+    Q4 r;
+    if (!normalized_quat(qmul(rotation_increment(mul(t, m.w)), m.r0), r)) return false;
+    return project_collider_pose(e, s, p, r, c);
+}
+
+// Largest distance from the body's COM to a point of shape s; planes and
+// heightfields are unbounded.
+MRD_INL float ccd_reach(const MRShapeC& sh) {
+    const float offset = length(f3(sh.local_position));
+    if (sh.shape_type == MR_SHAPE_SPHERE) return offset + sh.dimensions.x;
+    if (sh.shape_type == MR_SHAPE_CAPSULE) return offset + sh.dimensions.x + sh.dimensions.y;
+    if (sh.shape_type == MR_SHAPE_BOX) return offset + length(f3(sh.dimensions));
+    return __builtin_inff();
+}
+
+// Distance bound of a body's shape point over the remaining time T.
+MRD_INL float ccd_motion_bound(const CcdMotion& m, const MRShapeC& sh, float T) {
+    const float wl = length(m.w), vl = length(m.v);
+    if (!(wl > 0.0f) && !(vl > 0.0f)) return 0.0f;
+    const float reach = wl > 0.0f ? ccd_reach(sh) : 0.0f;
+    return T * (vl + wl * reach);
+}
+
+// Separation of two colliders at any distance (negative when they overlap):
+// the analytic pair functions without the contact-distance cut-off. For two
+// boxes the largest separation over the 15 separating axes, a lower bound of
+// their distance. Heightfields have no such query.
+MRD_INL float box_box_separation(const Collider& A, const Collider& B) {
+    const V3 d = sub(B.center, A.center);
+    float best = -__builtin_inff();
+    for (int i = 0; i < 3; ++i) {
+        const V3 ax = box_axis(A, i);
+        best = fmaxd(best, fabs_(dot(d, ax)) - comp(A.half_extents, i) - box_projection_radius(B, ax));
+    }
+    for (int j = 0; j < 3; ++j) {
+        const V3 ax = box_axis(B, j);
+        best = fmaxd(best, fabs_(dot(d, ax)) - box_projection_radius(A, ax) - comp(B.half_extents, j));
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const V3 c = cross(box_axis(A, i), box_axis(B, j));
+            const float l = length(c);
+            if (!(l > 1.0e-6f)) continue;
+            const V3 ax = mul(1.0f / l, c);
+            best = fmaxd(best, fabs_(dot(d, ax)) - box_projection_radius(A, ax) - box_projection_radius(B, ax));
+        }
+    return best;
+}
+
+MRD_INL bool ccd_separation(uint32_t ia, uint32_t ib, uint32_t cls, const Collider& A, const Collider& B, float& d) {
+    if (cls == MR_PAIR_SPHERE_SPHERE) {
+        d = length(sub(B.center, A.center)) - A.radius - B.radius;
+    } else if (cls == MR_PAIR_SPHERE_CAPSULE) {
+        const Collider& sp = A.type == MR_SHAPE_SPHERE ? A : B;
+        const Collider& cp = A.type == MR_SHAPE_SPHERE ? B : A;
+        const SegPoint cl = closest_on_segment(sp.center, cp.p0, cp.p1);
+        d = length(sub(cl.point, sp.center)) - sp.radius - cp.radius;
+    } else if (cls == MR_PAIR_CAPSULE_CAPSULE) {
+        const SegPair cl = closest_segments(A.p0, A.p1, B.p0, B.p1);
+        d = length(sub(cl.pb, cl.pa)) - A.radius - B.radius;
+    } else if (cls == MR_PAIR_SPHERE_BOX) {
+        const bool sphere_a = A.type == MR_SHAPE_SPHERE;
+        V3 n, bp;
+        uint32_t feat;
+        sphere_box(sphere_a ? A : B, sphere_a ? B : A, n, bp, d, feat);
+    } else if (cls == MR_PAIR_CAPSULE_BOX) {
+        const bool cap_a = A.type == MR_SHAPE_CAPSULE;
+        V3 n, cpp, bp;
+        float t;
+        uint32_t feat;
+        capsule_box(cap_a ? A : B, cap_a ? B : A, cap_a ? ia : ib, cap_a ? ib : ia, n, cpp, bp, d, t, feat);
+    } else if (cls == MR_PAIR_BOX_BOX) {
+        d = box_box_separation(A, B);
+    } else if (cls == MR_PAIR_SPHERE_PLANE || cls == MR_PAIR_CAPSULE_PLANE || cls == MR_PAIR_BOX_PLANE) {
+        const Collider& pl = A.type == MR_SHAPE_PLANE ? A : B;
+        const Collider& fs = A.type == MR_SHAPE_PLANE ? B : A;
+        if (cls == MR_PAIR_SPHERE_PLANE) {
+            d = dot(pl.plane_normal, sub(fs.center, pl.center)) - fs.radius;
+        } else if (cls == MR_PAIR_CAPSULE_PLANE) {
+            d = fmind(dot(pl.plane_normal, sub(fs.p0, pl.center)), dot(pl.plane_normal, sub(fs.p1, pl.center))) -
+                fs.radius;
+        } else {
+            d = __builtin_inff();
+            for (uint32_t vi = 0; vi < 8u; ++vi) d = fmind(d, dot(pl.plane_normal, sub(box_vertex(fs, vi), pl.center)));
+        }
+    } else {
+        return false;
+    }
+    return isfin(d);
+}
+
+enum : uint32_t { kCcdNone = 0u, kCcdImpact = 1u, kCcdUnsafe = 2u };
+
+// TOI query of compiled pair k over the remaining time T. Returns kCcdImpact
+// with toi (seconds), kCcdUnsafe for an unresolved pair the speculative
+// contacts do not cover, kCcdNone otherwise (no CCD flag, apart over the whole
+// interval, or touching already: the discrete contacts handle that).
+MRD_INL uint32_t ccd_pair(const Env& e, uint32_t k, float T, float& toi) {
+    const MRWorldStepConfigC& cfg = e.a->cfg;
+    const MRCompiledPairC& pr = e.pairs[k];
+    const MRShapeC& sa = e.shapes[pr.collider_a];
+    const MRShapeC& sb = e.shapes[pr.collider_b];
+    if (((sa.flags | sb.flags) & MR_SHAPE_FLAG_ENABLE_CCD) == 0u ||
+        ((sa.flags | sb.flags) & MR_SHAPE_FLAG_SIMULATION_DISABLED) != 0u)
+        return kCcdNone;
+    const CcdMotion ma = ccd_motion(e, sa.body, T), mb = ccd_motion(e, sb.body, T);
+    const float tol = cfg.ccd_distance_tolerance;
+    const float motion_a = ccd_motion_bound(ma, sa, T), motion_b = ccd_motion_bound(mb, sb, T);
+    const bool speculative_safe = motion_a <= sa.contact_rest_radius.x + tol && motion_b <= sb.contact_rest_radius.x + tol;
+    Collider a0, b0, a1, b1;
+    if (!ccd_collider(e, pr.collider_a, ma, 0.0f, a0) || !ccd_collider(e, pr.collider_b, mb, 0.0f, b0) ||
+        !ccd_collider(e, pr.collider_a, ma, T, a1) || !ccd_collider(e, pr.collider_b, mb, T, b1))
+        return speculative_safe ? kCcdNone : kCcdUnsafe;
+    // swept bounds: the union of the start and end bounds
+    Collider sa_sw = a0, sb_sw = b0;
+    sa_sw.lower = vmin(a0.lower, a1.lower);
+    sa_sw.upper = vmax(a0.upper, a1.upper);
+    sb_sw.lower = vmin(b0.lower, b1.lower);
+    sb_sw.upper = vmax(b0.upper, b1.upper);
+    if (!pair_may_overlap(sa_sw, sb_sw)) return kCcdNone;
+    float d;
+    if (!ccd_separation(pr.collider_a, pr.collider_b, pr.pair_class, a0, b0, d))
+        return speculative_safe ? kCcdNone : kCcdUnsafe;
+    // within the contact distance the segment's speculative contacts hold the pair
+    if (d <= fmaxd(tol, sa.contact_rest_radius.x + sb.contact_rest_radius.x)) return kCcdNone;
+    if (pr.pair_class == MR_PAIR_SPHERE_SPHERE) {
+        const V3 rel0 = sub(b0.center, a0.center);
+        const V3 drel = sub(sub(b1.center, b0.center), sub(a1.center, a0.center));
+        const float radius = a0.radius + b0.radius;
+        const float aa = dot(drel, drel), bb = 2.0f * dot(rel0, drel), cc = dot(rel0, rel0) - radius * radius;
+        const float disc = bb * bb - 4.0f * aa * cc;
+        if (aa > kTiny && disc >= 0.0f) {
+            const float alpha = (-bb - sqrt_rn(disc)) / (2.0f * aa);
+            if (alpha >= 0.0f && alpha <= 1.0f) {
+                toi = alpha * T;
+                return kCcdImpact;
+            }
+        }
+        return kCcdNone;
+    }
+    // Closing-speed bound: |v_a - v_b| + |w_a| r_a + |w_b| r_b; against a
+    // static plane only the motion along its normal closes the gap.
+    const bool plane_a = a0.type == MR_SHAPE_PLANE, plane_b = b0.type == MR_SHAPE_PLANE;
+    float total;
+    if ((plane_a && !(motion_a > 0.0f)) || (plane_b && !(motion_b > 0.0f))) {
+        const V3 n = plane_a ? a0.plane_normal : b0.plane_normal;
+        const CcdMotion& fm = plane_a ? mb : ma;
+        const float wl = length(fm.w);
+        const float reach = wl > 0.0f ? ccd_reach(plane_a ? sb : sa) : 0.0f;
+        total = T * (fabs_(dot(fm.v, n)) + wl * reach);
+    } else {
+        const float wa = length(ma.w), wb = length(mb.w);
+        const float spin = (wa > 0.0f ? wa * ccd_reach(sa) : 0.0f) + (wb > 0.0f ? wb * ccd_reach(sb) : 0.0f);
+        total = T * (length(sub(ma.v, mb.v)) + spin);
+    }
+    total = fmaxd(total, tol);
+    const float min_alpha = fmind(1.0f, cfg.ccd_minimum_advance / T);
+    float alpha = 0.0f;
+    for (uint32_t it = 0; it < cfg.max_ccd_iterations; ++it) {
+        if (d <= tol) {
+            toi = alpha * T;
+            return kCcdImpact;
+        }
+        alpha += fmaxd(d / total, min_alpha);
+        if (alpha > 1.0f) return kCcdNone;
+        Collider a, b;
+        if (!ccd_collider(e, pr.collider_a, ma, alpha * T, a) || !ccd_collider(e, pr.collider_b, mb, alpha * T, b) ||
+            !ccd_separation(pr.collider_a, pr.collider_b, pr.pair_class, a, b, d))
+            return speculative_safe ? kCcdNone : kCcdUnsafe;
+    }
+    if (d <= tol) {
+        toi = alpha * T;
+        return kCcdImpact;
+    }
+    return speculative_safe ? kCcdNone : kCcdUnsafe;
 }
 
 // --------------------------------------------------------------- manifold
@@ -1993,7 +2235,8 @@ MRD_INL bool mlp_warm_start(const Env& e, uint32_t lane, uint32_t ncons, uint32_
 // the plain instance carries none of that code.
 // `step` selects the control step of a multi-step launch: the command, reset
 // mask and status records of step `step` ([steps][E] each); 0 elsewhere.
-template <bool kMlp> __device__ __forceinline__ void control_step(const KernelArgs& args, uint32_t env,
+template <bool kMlp, bool kCcd = false>
+__device__ __forceinline__ void control_step(const KernelArgs& args, uint32_t env,
                                                                  uint32_t step = 0u) {
     extern __shared__ __align__(16) unsigned char lds_base[];
     __shared__ unsigned long long s_fail;
@@ -2190,20 +2433,33 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
     uint32_t successful = 0u;
     float min_piv = __builtin_inff(), max_piv = 0.0f, max_acc = 0.0f;
     uint32_t failing_substep = MR_INVALID_INDEX;
-    const float h = e.h;
+    float h = e.h;
     const MRWorldStepConfigC& cfg = A.cfg;
     const bool floating = e.root_type == MR_ROOT_FLOATING;
+    uint32_t impact_segments = 0u;  // segments that ended at a time of impact (hybrid CCD)
 
-    for (uint32_t substep = 0; substep < cfg.physics_substeps; ++substep) {
-        const bool final_sub = substep + 1u == cfg.physics_substeps;
-        const bool capture = cfg.capture_evidence != 0u && substep == cfg.capture_substep && A.evidence_pairs != nullptr;
+    // A microstep is one segment, or (hybrid CCD) a sequence of segments ending
+    // at successive times of impact; each segment runs every stage below with
+    // h = its duration, and the microstep counts once its last segment commits.
+    float remaining = A.h;
+    uint32_t segment = 0u;
+    for (uint32_t substep = 0; substep < cfg.physics_substeps;) {
+        if constexpr (kCcd) {
+            h = remaining;
+            e.h = h;
+        }
+        bool impact = false;
         // ---------------------------------------------------------- drives
         // Implicit drives refresh from the current state every microstep; an
         // effort command is clamped once, at the control step's start velocity,
-        // and held (MetalWorld.metal:184-270 prepare, 280-366 refresh).
-        if (substep == 0u || cfg.actuation_mode == MR_ACTUATION_IMPLICIT_POSITION_DRIVE)
-            for (uint32_t j = lane; j < nv; j += kLanes) e.effort[j] = drive_effort(e, j);
-        __syncthreads();
+        // and held (MetalWorld.metal:184-270 prepare, 280-366 refresh). They
+        // depend on h, so the hybrid mode refreshes them once the segment's
+        // duration is known, below.
+        if constexpr (!kCcd) {
+            if (substep == 0u || cfg.actuation_mode == MR_ACTUATION_IMPLICIT_POSITION_DRIVE)
+                for (uint32_t j = lane; j < nv; j += kLanes) e.effort[j] = drive_effort(e, j);
+            __syncthreads();
+        }
         // ------------------------------------------------------ kinematics
         if (lane == 0u) {
             if (floating) {
@@ -2238,6 +2494,48 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
         }
         clk.mark(mr::kStageKinematics);
         if (s_fail != ~0ull) break;
+        // ----------------------------------------------- time of impact
+        if constexpr (kCcd) {
+            uint32_t my_bits = ~0u, my_pair = ~0u, my_unsafe = ~0u;
+            for (uint32_t k = lane; k < e.npairs; k += kLanes) {
+                float toi = 0.0f;
+                const uint32_t r = ccd_pair(e, k, remaining, toi);
+                if (r == kCcdImpact) {
+                    const uint32_t bits = __float_as_uint(fmaxd(toi, 0.0f));
+                    if (bits < my_bits) {
+                        my_bits = bits;
+                        my_pair = k;
+                    }
+                } else if (r == kCcdUnsafe && my_unsafe == ~0u) {
+                    my_unsafe = k;
+                }
+            }
+            // earliest (TOI bits, pair) over the wave: non-negative floats order as their bits
+            const uint32_t best_bits = wave_min_u32(my_bits);
+            const uint32_t best_pair = wave_min_u32(my_bits == best_bits ? my_pair : ~0u);
+            const uint32_t unsafe = wave_min_u32(my_unsafe);
+            if (unsafe != ~0u) report_failure(&s_fail, unsafe, MR_STEP_DID_NOT_CONVERGE, unsafe);
+            if (best_bits != ~0u) {
+                const float tau = fmaxd(__uint_as_float(best_bits), cfg.ccd_time_tolerance);
+                if (tau < remaining - cfg.ccd_time_tolerance) {
+                    if (segment >= cfg.max_ccd_events)
+                        report_failure(&s_fail, best_pair, MR_STEP_CCD_EVENT_BUDGET_EXHAUSTED, best_pair);
+                    impact = true;
+                    h = tau;
+                    e.h = h;
+                }
+            }
+            __syncthreads();
+            if (s_fail != ~0ull) break;
+        }
+        const bool final_sub = substep + 1u == cfg.physics_substeps && !impact;
+        const bool capture = cfg.capture_evidence != 0u && substep == cfg.capture_substep && segment == 0u &&
+            A.evidence_pairs != nullptr;
+        if constexpr (kCcd) {
+            if ((substep == 0u && segment == 0u) || cfg.actuation_mode == MR_ACTUATION_IMPLICIT_POSITION_DRIVE)
+                for (uint32_t j = lane; j < nv; j += kLanes) e.effort[j] = drive_effort(e, j);
+            __syncthreads();
+        }
         // -------------------------------------------------------------- ABA
         for (uint32_t i = lane; i < nl; i += kLanes) aba_init_link(e, i);
         __syncthreads();
@@ -2348,8 +2646,11 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
             st.flags_indices[2] = b;
             world_inverse_inertia(bp, rq, st);
             if (bp.motion_type == MR_MOTION_DYNAMIC) {
-                const V3 lv = add(mul(bp.damping_scale.x, f3(st.linear_velocity_inv_mass)), mul(h, e.gravity));
-                const V3 av = mul(bp.damping_scale.y, f3(st.angular_velocity));
+                // velocity damping is a per-microstep factor: its first segment applies it
+                const float dl = !kCcd || segment == 0u ? bp.damping_scale.x : 1.0f;
+                const float da = !kCcd || segment == 0u ? bp.damping_scale.y : 1.0f;
+                const V3 lv = add(mul(dl, f3(st.linear_velocity_inv_mass)), mul(h, e.gravity));
+                const V3 av = mul(da, f3(st.angular_velocity));
                 st.linear_velocity_inv_mass.x = lv.x;
                 st.linear_velocity_inv_mass.y = lv.y;
                 st.linear_velocity_inv_mass.z = lv.z;
@@ -3272,7 +3573,7 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
         mcount_src = nman;
         src_buf = dst_buf;
         dst_buf = dst_buf == (cur + 1u) % 3u ? alt_buf : (cur + 1u) % 3u;
-        successful = substep + 1u;
+        if (!impact) successful = substep + 1u;
         min_piv = fmind(min_piv, s_fmax[0]);
         max_piv = fmaxd(max_piv, s_fmax[1]);
         float my_acc = 0.0f;
@@ -3280,7 +3581,19 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
         max_acc = fmaxd(max_acc, wave_max(my_acc));
         __syncthreads();
         clk.mark(mr::kStageIntegration);
+        if constexpr (kCcd) {
+            if (impact) {
+                remaining -= h;
+                ++segment;
+                ++impact_segments;  // counted once the segment commits
+                continue;
+            }
+            remaining = A.h;
+            segment = 0u;
+        }
+        ++substep;
     }
+    if constexpr (kCcd) e.h = A.h;
 
     // ------------------------------------------- transactional commit
     __syncthreads();
@@ -3321,6 +3634,7 @@ template <bool kMlp> __device__ __forceinline__ void control_step(const KernelAr
             ws.diagnostics = mr_float4{0.0f, 0.0f, 0.0f, 0.0f};
         }
         A.status[step_rec] = ws;
+        if constexpr (kCcd) cs.diagnostics.w = (float)impact_segments;
         cs.code = code;
         if (failed) {
             if (code == MR_STEP_PAIR_CAPACITY_OVERFLOW) cs.first_failing_pair = findex;
@@ -3446,6 +3760,14 @@ mr_control_step_mlp_kernel(KernelArgs args) {
     control_step<true>(args, blockIdx.x);
 }
 
+// The hybrid CCD mode: one workgroup per environment, microsteps split at
+// times of impact. A separate instance, so the speculative kernels carry none
+// of its registers.
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+mr_control_step_ccd_kernel(KernelArgs args) {
+    control_step<false, true>(args, blockIdx.x);
+}
+
 // Advance summary: one thread clears it, then a grid-stride pass over the
 // [steps][environments] status records folds it with per-wave reductions and
 // one atomic per wave (kernels rather than a memset, so the sequence is also
@@ -3536,6 +3858,8 @@ hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipSt
     }
     if (mlp)
         hipLaunchKernelGGL(mr_control_step_mlp_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
+    else if (args.cfg.ccd_mode == MR_CCD_HYBRID)
+        hipLaunchKernelGGL(mr_control_step_ccd_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
     else
         hipLaunchKernelGGL(mr_control_step_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
     return hipGetLastError();
@@ -3649,6 +3973,9 @@ hipError_t configure_control_step(uint32_t lds_bytes, uint32_t mlp_lds_bytes) {
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
         if (e != hipSuccess) return e;
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mr_control_horizon_queue_kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mr_control_step_ccd_kernel),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
         if (e != hipSuccess) return e;
         plain_max = lds_bytes;

@@ -64,6 +64,16 @@ World::World(const ModelData& model, const MRWorldStepConfigC& config) : model_(
         throw std::runtime_error("unknown solver mode");
     if (!(cfg_.maximum_normal_impulse >= 0.0f) || !std::isfinite(cfg_.maximum_normal_impulse))
         throw std::runtime_error("maximum_normal_impulse must be finite and non-negative (0: no cap)");
+    if (cfg_.ccd_mode != MR_CCD_SPECULATIVE && cfg_.ccd_mode != MR_CCD_HYBRID) throw std::runtime_error("unknown ccd_mode");
+    if (cfg_.ccd_mode == MR_CCD_HYBRID) {
+        if (cfg_.max_ccd_events == 0u || cfg_.max_ccd_events > MR_CCD_MAX_EVENTS)
+            throw std::runtime_error("max_ccd_events must be in [1, 16] in the hybrid CCD mode");
+        if (cfg_.max_ccd_iterations == 0u || cfg_.max_ccd_iterations > 256u)
+            throw std::runtime_error("max_ccd_iterations must be in [1, 256] in the hybrid CCD mode");
+        for (float x : {cfg_.ccd_distance_tolerance, cfg_.ccd_time_tolerance, cfg_.ccd_minimum_advance})
+            if (!(x > 0.0f) || !std::isfinite(x))
+                throw std::runtime_error("CCD tolerances and the minimum advance must be finite and positive");
+    }
     if (cfg_.pair_capacity == 0u || cfg_.pair_capacity > 4096u) throw std::runtime_error("pair_capacity in [1, 4096]");
     if (cfg_.manifold_capacity == 0u || cfg_.manifold_capacity > 4096u)
         throw std::runtime_error("manifold_capacity in [1, 4096]");
@@ -469,6 +479,8 @@ void World::set_warm_start_mlp(const MRMlpWeightsC* weights) {
         throw std::runtime_error("warm-start MLP: every weight pointer must be set");
     if (cfg_.solver != MR_SOLVER_ORDERED_SWEEPS)
         throw std::runtime_error("warm-start MLP: the learned warm start runs with the ordered sweeps solver");
+    if (cfg_.ccd_mode != MR_CCD_SPECULATIVE)
+        throw std::runtime_error("warm-start MLP: not available in the hybrid CCD mode");
     if (lds_mlp_.total == 0u)
         throw std::runtime_error("warm-start MLP: the world's LDS plus the MLP tile exceeds 160 KiB");
     if (queue_grid_ > 0u)
@@ -484,6 +496,8 @@ void World::set_mlp_capture(bool enabled) {
     if (enabled) {
         if (lds_mlp_.total == 0u)
             throw std::runtime_error("MLP sample capture: the world's LDS plus the MLP tile exceeds 160 KiB");
+        if (cfg_.ccd_mode != MR_CCD_SPECULATIVE)
+            throw std::runtime_error("MLP sample capture: not available in the hybrid CCD mode");
         if (queue_grid_ > 0u)
             throw std::runtime_error("MLP sample capture: runs one workgroup per environment; clear the work-queue "
                                      "schedule first (mr_world_set_schedule with grid 0)");
@@ -593,6 +607,8 @@ void World::set_schedule(const uint32_t* order, uint32_t count, uint32_t grid) {
     }
     if (order == nullptr || count == 0u) throw std::runtime_error("set_schedule: the work queue needs an order");
     if (mlp_instance()) throw std::runtime_error("set_schedule: the learned warm start runs one workgroup per environment");
+    if (cfg_.ccd_mode != MR_CCD_SPECULATIVE)
+        throw std::runtime_error("set_schedule: the hybrid CCD mode runs one workgroup per environment");
     std::vector<uint8_t> seen(env_, 0u);
     for (uint32_t i = 0; i < count; ++i) {
         if (order[i] >= env_) throw std::runtime_error("set_schedule: environment index out of range");

@@ -86,6 +86,9 @@ class WorldStepConfigC(ct.Structure):
         ("contact_slop", ct.c_float), ("max_depenetration_velocity", ct.c_float),
         ("residual_tolerance", ct.c_float), ("max_iterations", ct.c_uint32),
         ("solver", ct.c_uint32), ("maximum_normal_impulse", ct.c_float),
+        ("ccd_mode", ct.c_uint32), ("max_ccd_events", ct.c_uint32), ("max_ccd_iterations", ct.c_uint32),
+        ("ccd_reserved", ct.c_uint32), ("ccd_distance_tolerance", ct.c_float), ("ccd_time_tolerance", ct.c_float),
+        ("ccd_minimum_advance", ct.c_float), ("ccd_reserved1", ct.c_float),
     ]
 
     def copy(self) -> "WorldStepConfigC":
@@ -362,7 +365,7 @@ _EXPECTED_SIZES: Final = {
 }
 for _dt, _size in _EXPECTED_SIZES.items():
     assert _dt.itemsize == _size, (_dt, _dt.itemsize, _size)
-assert ct.sizeof(WorldStepConfigC) == 88
+assert ct.sizeof(WorldStepConfigC) == 120
 assert ct.sizeof(WorldModelDescC) == 216
 assert ct.sizeof(WorldAdvanceC) == 40
 assert ct.sizeof(WorldLayoutC) == 40

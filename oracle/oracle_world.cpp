@@ -700,17 +700,17 @@ void inflate_bounds(Collider& c) {
 
 // Collision.metal:321-600 (makeWorldShape) for the primitive types, plus the
 // static axis-aligned heightfield.
-bool project_collider(const StepContext& ctx, const Work& wk, uint32_t s, Collider& c) {
+// Collider of shape s with its body at (bpos, borient).
+bool project_collider_pose(const StepContext& ctx, uint32_t s, V3 bpos, Q4 borient, Collider& c) {
     const MRShapeC& sh = ctx.m.shapes[s];
-    const MRBodyStateC& body = wk.bodies[sh.body];
     Q4 br, lr;
-    if (!checked_quat(q4(body.orientation), br) || !checked_quat(q4(sh.local_rotation), lr))
+    if (!checked_quat(borient, br) || !checked_quat(q4(sh.local_rotation), lr))
         return false;
     c.type = sh.shape_type;
     c.body = sh.body;
     c.disabled = (sh.flags & MR_SHAPE_FLAG_SIMULATION_DISABLED) != 0u ? 1u : 0u;
     c.rotation = qmul(br, lr);
-    c.center = add(f3(body.position), qrot(br, f3(sh.local_position)));
+    c.center = add(bpos, qrot(br, f3(sh.local_position)));
     c.contact_offset = sh.contact_rest_radius.x;
     c.radius = 0.0f;
     c.half_length = 0.0f;
@@ -769,6 +769,11 @@ bool project_collider(const StepContext& ctx, const Work& wk, uint32_t s, Collid
         return false;
     }
     return finite3(c.lower) && finite3(c.upper);
+}
+
+bool project_collider(const StepContext& ctx, const Work& wk, uint32_t s, Collider& c) {
+    const MRBodyStateC& body = wk.bodies[ctx.m.shapes[s].body];
+    return project_collider_pose(ctx, s, f3(body.position), q4(body.orientation), c);
 }
 
 float collider_scale(const Collider& c) {
@@ -1706,15 +1711,286 @@ struct EnvState {
     uint32_t* mcount;
 };
 
+// ---- continuous collision: the hybrid mode's time-of-impact query
+// (mr_world_resolve_ccd and the event select kernel, Collision.metal:6707-6950,
+// 7157-7330, 7657-7800). Over the remaining time T of a microstep every body
+// moves rigidly from the segment's start: links with the COM and angular
+// velocity of the current kinematics, dynamic scene bodies with their velocity
+// plus T * gravity, kinematic bodies with theirs, static bodies not at all;
+// rotations follow exp(t w). Pairs starting within their contact distance are
+// left to the segment's speculative contacts. Two spheres get the closed-form
+// TOI of their linearly moving centres; other pairs conservative advancement,
+// stepping t by distance / (closing-speed bound |v_a - v_b| + |w_a| r_a +
+// |w_b| r_b, or the normal speed against a static plane) until the pair is
+// within the distance tolerance. The earliest (TOI, pair) ends the segment.
+
+struct CcdMotion {
+    V3 p0;
+    Q4 r0;
+    V3 v, w;
+};
+
+CcdMotion ccd_motion(const StepContext& ctx, const Work& wk, const EnvState& st, uint32_t b, float T) {
+    const uint32_t nlinks = ctx.m.header->link_count;
+    CcdMotion m;
+    if (b < nlinks) {
+        m.p0 = wk.pos[b];
+        m.r0 = wk.rot[b];
+        m.v = wk.vl[b];
+        m.w = wk.w[b];
+        return m;
+    }
+    const MRBodyStateC& s = st.scene[b - nlinks];
+    m.p0 = f3(s.position);
+    m.r0 = q4(s.orientation);
+    const uint32_t mt = ctx.m.bodies[b].motion_type;
+    if (mt == MR_MOTION_DYNAMIC) {
+        m.v = add(f3(s.linear_velocity_inv_mass), mul(T, ctx.m.gravity));
+        m.w = f3(s.angular_velocity);
+    } else if (mt == MR_MOTION_KINEMATIC) {
+        m.v = f3(s.linear_velocity_inv_mass);
+        m.w = f3(s.angular_velocity);
+    } else {
+        m.v = V3{0.0f, 0.0f, 0.0f};
+        m.w = V3{0.0f, 0.0f, 0.0f};
+    }
+    return m;
+}
+
+bool ccd_collider(const StepContext& ctx, uint32_t s, const CcdMotion& m, float t, Collider& c) {
+    const V3 p = add(m.p0, mul(t, m.v));
+    Q4 r;
+    if (!normalized_quat(qmul(rotation_increment(mul(t, m.w)), m.r0), r)) return false;
+    return project_collider_pose(ctx, s, p, r, c);
+}
+
+float ccd_reach(const MRShapeC& sh) {
+    const float offset = length(f3(sh.local_position));
+    if (sh.shape_type == MR_SHAPE_SPHERE) return offset + sh.dimensions.x;
+    if (sh.shape_type == MR_SHAPE_CAPSULE) return offset + sh.dimensions.x + sh.dimensions.y;
+    if (sh.shape_type == MR_SHAPE_BOX) return offset + length(f3(sh.dimensions));
+    return INFINITY;
+}
+
+float ccd_motion_bound(const CcdMotion& m, const MRShapeC& sh, float T) {
+    const float wl = length(m.w), vl = length(m.v);
+    if (!(wl > 0.0f) && !(vl > 0.0f)) return 0.0f;
+    const float reach = wl > 0.0f ? ccd_reach(sh) : 0.0f;
+    return T * (vl + wl * reach);
+}
+
+float box_box_separation(const Collider& A, const Collider& B) {
+    const V3 d = sub(B.center, A.center);
+    float best = -INFINITY;
+    for (int i = 0; i < 3; ++i) {
+        const V3 ax = box_axis(A, i);
+        best = fmaxd(best, std::fabs(dot(d, ax)) - comp(A.half_extents, i) - box_projection_radius(B, ax));
+    }
+    for (int j = 0; j < 3; ++j) {
+        const V3 ax = box_axis(B, j);
+        best = fmaxd(best, std::fabs(dot(d, ax)) - box_projection_radius(A, ax) - comp(B.half_extents, j));
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const V3 c = cross(box_axis(A, i), box_axis(B, j));
+            const float l = length(c);
+            if (!(l > 1.0e-6f)) continue;
+            const V3 ax = mul(1.0f / l, c);
+            best = fmaxd(best, std::fabs(dot(d, ax)) - box_projection_radius(A, ax) - box_projection_radius(B, ax));
+        }
+    return best;
+}
+
+// Separation at any distance: the analytic pair functions without the
+// contact-distance cut-off; false for pairs without such a query (heightfield).
+bool ccd_separation(uint32_t ia, uint32_t ib, uint32_t cls, const Collider& A, const Collider& B, float& d) {
+    if (cls == MR_PAIR_SPHERE_SPHERE) {
+        d = length(sub(B.center, A.center)) - A.radius - B.radius;
+    } else if (cls == MR_PAIR_SPHERE_CAPSULE) {
+        const Collider& sp = A.type == MR_SHAPE_SPHERE ? A : B;
+        const Collider& cp = A.type == MR_SHAPE_SPHERE ? B : A;
+        const SegPoint cl = closest_on_segment(sp.center, cp.p0, cp.p1);
+        d = length(sub(cl.point, sp.center)) - sp.radius - cp.radius;
+    } else if (cls == MR_PAIR_CAPSULE_CAPSULE) {
+        const SegPair cl = closest_segments(A.p0, A.p1, B.p0, B.p1);
+        d = length(sub(cl.pb, cl.pa)) - A.radius - B.radius;
+    } else if (cls == MR_PAIR_SPHERE_BOX) {
+        const bool sphere_a = A.type == MR_SHAPE_SPHERE;
+        V3 n, bp;
+        uint32_t feat;
+        sphere_box(sphere_a ? A : B, sphere_a ? B : A, n, bp, d, feat);
+    } else if (cls == MR_PAIR_CAPSULE_BOX) {
+        const bool cap_a = A.type == MR_SHAPE_CAPSULE;
+        V3 n, cpp, bp;
+        float t;
+        uint32_t feat;
+        capsule_box(cap_a ? A : B, cap_a ? B : A, cap_a ? ia : ib, cap_a ? ib : ia, n, cpp, bp, d, t, feat);
+    } else if (cls == MR_PAIR_BOX_BOX) {
+        d = box_box_separation(A, B);
+    } else if (cls == MR_PAIR_SPHERE_PLANE || cls == MR_PAIR_CAPSULE_PLANE || cls == MR_PAIR_BOX_PLANE) {
+        const Collider& pl = A.type == MR_SHAPE_PLANE ? A : B;
+        const Collider& fs = A.type == MR_SHAPE_PLANE ? B : A;
+        if (cls == MR_PAIR_SPHERE_PLANE) {
+            d = dot(pl.plane_normal, sub(fs.center, pl.center)) - fs.radius;
+        } else if (cls == MR_PAIR_CAPSULE_PLANE) {
+            d = fmind(dot(pl.plane_normal, sub(fs.p0, pl.center)), dot(pl.plane_normal, sub(fs.p1, pl.center))) -
+                fs.radius;
+        } else {
+            d = INFINITY;
+            for (uint32_t vi = 0; vi < 8u; ++vi) d = fmind(d, dot(pl.plane_normal, sub(box_vertex(fs, vi), pl.center)));
+        }
+    } else {
+        return false;
+    }
+    return std::isfinite(d);
+}
+
+enum : uint32_t { kCcdNone = 0u, kCcdImpact = 1u, kCcdUnsafe = 2u };
+
+uint32_t ccd_pair(const StepContext& ctx, const Work& wk, const EnvState& st, uint32_t k, float T, float& toi) {
+    const MRWorldStepConfigC& cfg = *ctx.cfg;
+    const MRCompiledPairC& pr = ctx.m.pairs[k];
+    const MRShapeC& sa = ctx.m.shapes[pr.collider_a];
+    const MRShapeC& sb = ctx.m.shapes[pr.collider_b];
+    if (((sa.flags | sb.flags) & MR_SHAPE_FLAG_ENABLE_CCD) == 0u ||
+        ((sa.flags | sb.flags) & MR_SHAPE_FLAG_SIMULATION_DISABLED) != 0u)
+        return kCcdNone;
+    const CcdMotion ma = ccd_motion(ctx, wk, st, sa.body, T), mb = ccd_motion(ctx, wk, st, sb.body, T);
+    const float tol = cfg.ccd_distance_tolerance;
+    const float motion_a = ccd_motion_bound(ma, sa, T), motion_b = ccd_motion_bound(mb, sb, T);
+    const bool speculative_safe = motion_a <= sa.contact_rest_radius.x + tol && motion_b <= sb.contact_rest_radius.x + tol;
+    Collider a0, b0, a1, b1;
+    if (!ccd_collider(ctx, pr.collider_a, ma, 0.0f, a0) || !ccd_collider(ctx, pr.collider_b, mb, 0.0f, b0) ||
+        !ccd_collider(ctx, pr.collider_a, ma, T, a1) || !ccd_collider(ctx, pr.collider_b, mb, T, b1))
+        return speculative_safe ? kCcdNone : kCcdUnsafe;
+    Collider a_sw = a0, b_sw = b0;
+    a_sw.lower = vmin(a0.lower, a1.lower);
+    a_sw.upper = vmax(a0.upper, a1.upper);
+    b_sw.lower = vmin(b0.lower, b1.lower);
+    b_sw.upper = vmax(b0.upper, b1.upper);
+    if (!pair_may_overlap(a_sw, b_sw)) return kCcdNone;
+    float d;
+    if (!ccd_separation(pr.collider_a, pr.collider_b, pr.pair_class, a0, b0, d))
+        return speculative_safe ? kCcdNone : kCcdUnsafe;
+    // within the contact distance the segment's speculative contacts hold the pair
+    if (d <= fmaxd(tol, sa.contact_rest_radius.x + sb.contact_rest_radius.x)) return kCcdNone;
+    if (pr.pair_class == MR_PAIR_SPHERE_SPHERE) {
+        const V3 rel0 = sub(b0.center, a0.center);
+        const V3 drel = sub(sub(b1.center, b0.center), sub(a1.center, a0.center));
+        const float radius = a0.radius + b0.radius;
+        const float aa = dot(drel, drel), bb = 2.0f * dot(rel0, drel), cc = dot(rel0, rel0) - radius * radius;
+        const float disc = bb * bb - 4.0f * aa * cc;
+        if (aa > kTiny && disc >= 0.0f) {
+            const float alpha = (-bb - std::sqrt(disc)) / (2.0f * aa);
+            if (alpha >= 0.0f && alpha <= 1.0f) {
+                toi = alpha * T;
+                return kCcdImpact;
+            }
+        }
+        return kCcdNone;
+    }
+    // Closing-speed bound: |v_a - v_b| + |w_a| r_a + |w_b| r_b; against a
+    // static plane only the motion along its normal closes the gap.
+    const bool plane_a = a0.type == MR_SHAPE_PLANE, plane_b = b0.type == MR_SHAPE_PLANE;
+    float total;
+    if ((plane_a && !(motion_a > 0.0f)) || (plane_b && !(motion_b > 0.0f))) {
+        const V3 n = plane_a ? a0.plane_normal : b0.plane_normal;
+        const CcdMotion& fm = plane_a ? mb : ma;
+        const float wl = length(fm.w);
+        const float reach = wl > 0.0f ? ccd_reach(plane_a ? sb : sa) : 0.0f;
+        total = T * (std::fabs(dot(fm.v, n)) + wl * reach);
+    } else {
+        const float wa = length(ma.w), wb = length(mb.w);
+        const float spin = (wa > 0.0f ? wa * ccd_reach(sa) : 0.0f) + (wb > 0.0f ? wb * ccd_reach(sb) : 0.0f);
+        total = T * (length(sub(ma.v, mb.v)) + spin);
+    }
+    total = fmaxd(total, tol);
+    const float min_alpha = fmind(1.0f, cfg.ccd_minimum_advance / T);
+    float alpha = 0.0f;
+    for (uint32_t it = 0; it < cfg.max_ccd_iterations; ++it) {
+        if (d <= tol) {
+            toi = alpha * T;
+            return kCcdImpact;
+        }
+        alpha += fmaxd(d / total, min_alpha);
+        if (alpha > 1.0f) return kCcdNone;
+        Collider a, b;
+        if (!ccd_collider(ctx, pr.collider_a, ma, alpha * T, a) || !ccd_collider(ctx, pr.collider_b, mb, alpha * T, b) ||
+            !ccd_separation(pr.collider_a, pr.collider_b, pr.pair_class, a, b, d))
+            return speculative_safe ? kCcdNone : kCcdUnsafe;
+    }
+    if (d <= tol) {
+        toi = alpha * T;
+        return kCcdImpact;
+    }
+    return speculative_safe ? kCcdNone : kCcdUnsafe;
+}
+
+// The segment the hybrid mode runs next from the current state: its duration
+// (the remaining time, or the earliest TOI clamped below by the time
+// tolerance) and whether it ends at an impact. False with f set when a pair is
+// unresolved and not covered by the speculative contacts, or when an impact
+// would need more than max_ccd_events segments in this microstep; the failure
+// of the smaller pair index wins.
+bool ccd_segment(const StepContext& ctx, EnvState& st, Work& wk, float remaining, uint32_t segment, Failure& f,
+                 bool& impact, float& duration) {
+    impact = false;
+    duration = remaining;
+    if (!forward_kinematics(ctx, st.q, st.v, wk, f)) return false;
+    uint32_t best_bits = ~0u, best_pair = ~0u, unsafe = ~0u;
+    for (uint32_t k = 0; k < ctx.m.header->pair_count; ++k) {
+        float toi = 0.0f;
+        const uint32_t r = ccd_pair(ctx, wk, st, k, remaining, toi);
+        if (r == kCcdImpact) {
+            const float t = fmaxd(toi, 0.0f);
+            uint32_t bits;
+            std::memcpy(&bits, &t, sizeof(bits));
+            if (bits < best_bits || (bits == best_bits && k < best_pair)) {
+                best_bits = bits;
+                best_pair = k;
+            }
+        } else if (r == kCcdUnsafe && unsafe == ~0u) {
+            unsafe = k;
+        }
+    }
+    uint32_t fail_pair = ~0u, fail_code = MR_STEP_SUCCESS;
+    if (unsafe != ~0u) {
+        fail_pair = unsafe;
+        fail_code = MR_STEP_DID_NOT_CONVERGE;
+    }
+    if (best_bits != ~0u) {
+        float t;
+        std::memcpy(&t, &best_bits, sizeof(t));
+        const float tau = fmaxd(t, ctx.cfg->ccd_time_tolerance);
+        if (tau < remaining - ctx.cfg->ccd_time_tolerance) {
+            if (segment >= ctx.cfg->max_ccd_events && best_pair < fail_pair) {
+                fail_pair = best_pair;
+                fail_code = MR_STEP_CCD_EVENT_BUDGET_EXHAUSTED;
+            }
+            impact = true;
+            duration = tau;
+        }
+    }
+    if (fail_code != MR_STEP_SUCCESS) {
+        f.code = fail_code;
+        f.index = fail_pair;
+        return false;
+    }
+    return true;
+}
+
 struct MicroOutputs {
     MRContactStatusC* cstatus;
     MRPairRecordC* pairs;  // evidence (capacity = pair_capacity) or null
     uint32_t* pair_count;
 };
 
+// One microstep, or one segment of a hybrid-CCD microstep, of duration ctx.h;
+// `damp` applies the scene bodies' per-microstep velocity damping (the first
+// segment of a microstep).
 bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool first_sub, bool final_sub,
                bool capture, MicroOutputs& out, Work& wk, Failure& f, std::vector<Constraint>& cons,
-               std::vector<float>& jrow, std::vector<float>& resp) {
+               std::vector<float>& jrow, std::vector<float>& resp, bool damp) {
     const MRModelHeaderC& hd = *ctx.m.header;
     const uint32_t nv = hd.nv;
     const uint32_t nlinks = hd.link_count;
@@ -1758,8 +2034,10 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
         s.flags_indices[2] = b;
         world_inverse_inertia(bp, rq, s);
         if (bp.motion_type == MR_MOTION_DYNAMIC) {
-            const V3 lv = add(mul(bp.damping_scale.x, f3(s.linear_velocity_inv_mass)), mul(h, ctx.m.gravity));
-            const V3 av = mul(bp.damping_scale.y, f3(s.angular_velocity));
+            const float dl = damp ? bp.damping_scale.x : 1.0f;
+            const float da = damp ? bp.damping_scale.y : 1.0f;
+            const V3 lv = add(mul(dl, f3(s.linear_velocity_inv_mass)), mul(h, ctx.m.gravity));
+            const V3 av = mul(da, f3(s.angular_velocity));
             s.linear_velocity_inv_mass.x = lv.x;
             s.linear_velocity_inv_mass.y = lv.y;
             s.linear_velocity_inv_mass.z = lv.z;
@@ -2512,21 +2790,48 @@ int mr_oracle_control_step(const MRWorldModelDescC* desc, const MRWorldStepConfi
         mo.pair_count = &ev_count;
         const float* cmd = command + (size_t)env * max_nv;
         Failure fail;
+        const bool hybrid = cfg->ccd_mode == MR_CCD_HYBRID;
+        const float h_full = ctx.h;
+        uint32_t impact_segments = 0u;
         for (uint32_t sub = 0; sub < cfg->physics_substeps; ++sub) {
             const bool capture = cfg->capture_evidence != 0u && sub == cfg->capture_substep;
-            if (!microstep(ctx, st, cmd, sub == 0u, sub + 1u == cfg->physics_substeps, capture, mo, *wk, fail, cons,
-                           jrow, resp)) {
+            // hybrid CCD: segments ending at successive times of impact, each a
+            // full microstep of its own duration
+            float remaining = h_full;
+            bool failed = false;
+            for (uint32_t seg = 0;; ++seg) {
+                bool impact = false;
+                float duration = remaining;
+                if (hybrid && !ccd_segment(ctx, st, *wk, remaining, seg, fail, impact, duration)) {
+                    failed = true;
+                    break;
+                }
+                ctx.h = duration;
+                const bool ok = microstep(ctx, st, cmd, sub == 0u && seg == 0u,
+                                          sub + 1u == cfg->physics_substeps && !impact, capture && seg == 0u, mo,
+                                          *wk, fail, cons, jrow, resp, seg == 0u);
+                ctx.h = h_full;
+                if (!ok) {
+                    failed = true;
+                    break;
+                }
+                min_piv = fmind(min_piv, wk->minPivot);
+                max_piv = fmaxd(max_piv, wk->maxPivot);
+                for (uint32_t j = 0; j < hd.nv; ++j) max_acc = fmaxd(max_acc, std::fabs(wk->accel[j]));
+                if (!impact) break;
+                ++impact_segments;
+                remaining -= duration;
+            }
+            if (failed) {
                 ws_out.failing_substep = sub;
                 ws_out.failing_index = fail.index;
                 break;
             }
             ws_out.successful_substeps = sub + 1u;
-            min_piv = fmind(min_piv, wk->minPivot);
-            max_piv = fmaxd(max_piv, wk->maxPivot);
-            for (uint32_t j = 0; j < hd.nv; ++j) max_acc = fmaxd(max_acc, std::fabs(wk->accel[j]));
         }
         ws_out.code = fail.code;
         cs.code = fail.code;
+        if (hybrid) cs.diagnostics.w = (float)impact_segments;
         if (fail.code == MR_STEP_SUCCESS) {
             for (uint32_t i = 0; i < hd.nq; ++i) eq[i] = wq[i];
             for (uint32_t i = 0; i < hd.nv; ++i) ev[i] = wv[i];
