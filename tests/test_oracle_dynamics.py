@@ -153,3 +153,34 @@ def test_pendulum_energy_is_conserved_without_damping():
         assert status["code"][0] == 0
     e1 = tm.kinetic_energy(st.q[0], st.v[0], arm) + tm.potential_energy(st.q[0])
     assert abs(e1 - e0) < 2e-2 * max(1.0, abs(e0)), (e0, e1)
+
+
+def test_chain_blow_up_is_the_step_size_limit_of_explicit_velocity_terms():
+    """The chain + boxes workload's rare non-finite worlds come from the
+    microstep, not from the contacts or the kernel: world 158 (seed 7) spins
+    its effort-driven links until h |w| per microstep passes ~2 rad and
+    blows up at 4 substeps (h = 5 ms); at 8 substeps (h = 2.5 ms) the same
+    world, the same torques, stays finite. Semi-implicit Euler integrates the
+    velocity-product (Coriolis, centrifugal) terms explicitly, which is
+    stable only while h |w| stays of order 1."""
+    model = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_CHAIN_BOXES, 160, 7)
+    cmds = model.commands(0, 60)
+    outcome = {}
+    for subs in (4, 8):
+        cfg = model.default_config()
+        cfg.physics_substeps = subs
+        st = ol.OracleState(model, cfg)
+        h = cfg.control_timestep / subs
+        peak_hw, code = 0.0, 0
+        for k in range(60):
+            status, *_ = st.step(cmds[k], control_step=k, env_begin=158, env_end=159)
+            code = int(status["code"][158])
+            if code != 0:
+                break
+            peak_hw = max(peak_hw, h * float(np.abs(st.v[158]).max()))
+        outcome[subs] = (code, k, peak_hw)
+    code4, step4, hw4 = outcome[4]
+    assert code4 == 3 and step4 == 37, outcome  # MR_STEP_NONFINITE_RESULT at control step 37
+    assert hw4 > 2.0, outcome                    # the last accepted step was already past the limit
+    code8, _, hw8 = outcome[8]
+    assert code8 == 0 and hw8 < 1.0, outcome
