@@ -47,12 +47,18 @@ enum MRShapeType {
     MR_SHAPE_CAPSULE = 1u,
     MR_SHAPE_BOX = 2u,
     MR_SHAPE_PLANE = 3u,
+    MR_SHAPE_CONVEX = 5u,         /* convex hull of a geometry's vertices (gpu_types.h:41-50) */
+    MR_SHAPE_TRIANGLE_MESH = 6u,  /* static triangle mesh with a BVH4 */
     MR_SHAPE_HEIGHTFIELD = 7u
 };
 
 /* MR_SHAPE_FLAG_ENABLE_CCD opts a collider into the hybrid mode's continuous
  * collision query (MRShapeGPU flags, engine_types.h; Collision.metal:7205). */
-enum MRShapeFlags { MR_SHAPE_FLAG_SIMULATION_DISABLED = 1u, MR_SHAPE_FLAG_ENABLE_CCD = 2u };
+enum MRShapeFlags {
+    MR_SHAPE_FLAG_SIMULATION_DISABLED = 1u,
+    MR_SHAPE_FLAG_ENABLE_CCD = 2u,
+    MR_SHAPE_FLAG_MESH_TWO_SIDED = 4u /* triangle meshes are one-sided unless set */
+};
 
 /* MetalWorldCCDMode (MetalWorld.hpp:420): speculative contacts only, or
  * speculative contacts plus microsteps split at the earliest time of impact
@@ -88,7 +94,9 @@ enum MRPairClass {
     MR_PAIR_SPHERE_BOX = 8u,
     MR_PAIR_CAPSULE_BOX = 9u,
     MR_PAIR_BOX_BOX = 10u,
-    MR_PAIR_HEIGHTFIELD = 12u
+    MR_PAIR_CONVEX = 11u,      /* a convex hull with a sphere, capsule, box, plane or hull */
+    MR_PAIR_HEIGHTFIELD = 12u,
+    MR_PAIR_MESH = 13u         /* a static triangle mesh with a sphere, capsule, box or hull */
 };
 
 /* engine_types.h:171-186 */
@@ -212,15 +220,53 @@ typedef struct MRShapeC {
     uint32_t collision_group;
     uint32_t collision_mask;
     uint32_t slot_generation;
-    uint32_t reserved;
+    uint32_t geometry;         /* convex and mesh shapes: index into the model's geometry table */
     mr_float4 local_position;
     mr_float4 local_rotation;
     /* Sphere: x=radius. Capsule: x=radius, y=half length (local +Y).
-     * Box: xyz=half extents. Heightfield: unused (the world header holds it). */
+     * Box: xyz=half extents. Convex and mesh: xyz=scale of the geometry's
+     * vertices. Heightfield: unused (the world header holds it). */
     mr_float4 dimensions;
     /* contact offset, rest offset, bounding radius, reserved */
     mr_float4 contact_rest_radius;
 } MRShapeC;
+
+/* Cooked geometry of convex and mesh shapes (MRGeometryHeaderGPU,
+ * engine_types.h:1360-1383, in this build's own compact form). Vertices are
+ * local, before the shape's scale. The model compiler fills the bounds and a
+ * mesh's BVH4 (node_offset, node_count); callers leave them zero. */
+#define MR_MAX_CONVEX_VERTICES 64u
+#define MR_MESH_LEAF_TRIANGLES 4u
+typedef struct MRGeometryC {
+    uint32_t kind;            /* MR_SHAPE_CONVEX or MR_SHAPE_TRIANGLE_MESH */
+    uint32_t flags;
+    uint32_t vertex_offset;   /* into the vertex table */
+    uint32_t vertex_count;    /* convex: 4 .. MR_MAX_CONVEX_VERTICES */
+    uint32_t triangle_offset; /* meshes: into the triangle table */
+    uint32_t triangle_count;
+    uint32_t node_offset;     /* meshes: into the BVH node table (model compiler) */
+    uint32_t node_count;
+    mr_float4 local_lower;    /* vertex bounds (model compiler) */
+    mr_float4 local_upper;
+} MRGeometryC;
+
+/* One mesh triangle: vertex indices local to its geometry, and the stable
+ * feature id its contacts carry (MRMeshTriangleGPU, engine_types.h:1411-1418). */
+typedef struct MRTriangleC {
+    uint32_t vertices[3];
+    uint32_t feature;
+} MRTriangleC;
+
+/* BVH4 node: the four children's bounds in structure-of-arrays form. A child
+ * is an inner node (count 0: child = node index within the geometry), a leaf
+ * (count 1 .. MR_MESH_LEAF_TRIANGLES triangles from child, local to the
+ * geometry) or empty (child = MR_INVALID_INDEX). */
+typedef struct MRBvhNodeC {
+    mr_float4 lower_x, lower_y, lower_z;
+    mr_float4 upper_x, upper_y, upper_z;
+    uint32_t child[4];
+    uint32_t count[4];
+} MRBvhNodeC;
 
 /* engine_types.h:1349-1356, with per-axis coefficients for elliptic cones. */
 typedef struct MRMaterialC {
@@ -300,6 +346,17 @@ typedef struct MRWorldModelDescC {
     const MRActuatorProfileC* actuators;
     uint32_t actuator_total;           /* dof_total, or 0 with actuators null */
     uint32_t reserved_actuators;
+    /* geometry of convex and mesh shapes, shared by every model block */
+    const MRGeometryC* geometries;     /* [geometry_total] */
+    const mr_float4* geometry_vertices; /* [geometry_vertex_total], xyz */
+    const MRTriangleC* triangles;      /* [triangle_total] */
+    /* the meshes' BVH4 nodes: filled by mr_world_model_describe (with the
+     * triangles in leaf order), ignored by mr_world_model_create, which builds them */
+    const MRBvhNodeC* bvh_nodes;
+    uint32_t geometry_total;
+    uint32_t geometry_vertex_total;
+    uint32_t triangle_total;
+    uint32_t bvh_node_total;
 } MRWorldModelDescC;
 
 /* ---------------------------------------------------------------- state -- */
@@ -448,6 +505,9 @@ MR_STATIC_ASSERT(sizeof(MRDofC) == 48, "MRDofC size");
 MR_STATIC_ASSERT(sizeof(MRActuatorProfileC) == 48, "MRActuatorProfileC size");
 MR_STATIC_ASSERT(sizeof(MRShapeC) == 96, "MRShapeC size");
 MR_STATIC_ASSERT(sizeof(MRMaterialC) == 32, "MRMaterialC size");
+MR_STATIC_ASSERT(sizeof(MRGeometryC) == 64, "MRGeometryC size");
+MR_STATIC_ASSERT(sizeof(MRTriangleC) == 16, "MRTriangleC size");
+MR_STATIC_ASSERT(sizeof(MRBvhNodeC) == 128, "MRBvhNodeC size");
 MR_STATIC_ASSERT(sizeof(MRCompiledPairC) == 16, "MRCompiledPairC size");
 MR_STATIC_ASSERT(sizeof(MRModelHeaderC) == 64, "MRModelHeaderC size");
 MR_STATIC_ASSERT(sizeof(MRBodyStateC) == 128, "MRBodyStateC size");

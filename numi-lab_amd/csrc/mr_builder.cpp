@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <string>
 
 namespace mr {
 
@@ -767,7 +768,164 @@ uint32_t pair_class(uint32_t a, uint32_t b) {
     if (lo == MR_SHAPE_CAPSULE && hi == MR_SHAPE_BOX) return MR_PAIR_CAPSULE_BOX;
     if (lo == MR_SHAPE_BOX && hi == MR_SHAPE_BOX) return MR_PAIR_BOX_BOX;
     if (hi == MR_SHAPE_HEIGHTFIELD && lo <= MR_SHAPE_BOX) return MR_PAIR_HEIGHTFIELD;
+    if (hi == MR_SHAPE_CONVEX && lo <= MR_SHAPE_PLANE) return MR_PAIR_CONVEX;
+    if (lo == MR_SHAPE_CONVEX && hi == MR_SHAPE_CONVEX) return MR_PAIR_CONVEX;
+    if (hi == MR_SHAPE_TRIANGLE_MESH && (lo <= MR_SHAPE_BOX || lo == MR_SHAPE_CONVEX)) return MR_PAIR_MESH;
     return MR_PAIR_UNSUPPORTED;
+}
+
+namespace {
+
+struct BvhBuilder {
+    const mr_float4* verts;       // the geometry's vertices
+    std::vector<MRTriangleC>& tris;  // the geometry's triangles, reordered in place into leaf order
+    std::vector<MRBvhNodeC>& nodes;  // the geometry's nodes (indices local to it)
+
+    void bounds(uint32_t first, uint32_t count, float lo[3], float hi[3]) const {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = INFINITY;
+            hi[k] = -INFINITY;
+        }
+        for (uint32_t t = first; t < first + count; ++t)
+            for (uint32_t v : tris[t].vertices) {
+                const float p[3] = {verts[v].x, verts[v].y, verts[v].z};
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = std::min(lo[k], p[k]);
+                    hi[k] = std::max(hi[k], p[k]);
+                }
+            }
+    }
+    // Splits [first, first + count) at the median centroid along the longest
+    // centroid axis; ties keep the previous order (stable).
+    uint32_t split(uint32_t first, uint32_t count) {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        auto centroid = [&](const MRTriangleC& t, int k) {
+            const float* a = &verts[t.vertices[0]].x;
+            const float* b = &verts[t.vertices[1]].x;
+            const float* c = &verts[t.vertices[2]].x;
+            return (a[k] + b[k] + c[k]) / 3.0f;
+        };
+        for (uint32_t t = first; t < first + count; ++t)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], centroid(tris[t], k));
+                hi[k] = std::max(hi[k], centroid(tris[t], k));
+            }
+        int axis = 0;
+        for (int k = 1; k < 3; ++k)
+            if (hi[k] - lo[k] > hi[axis] - lo[axis]) axis = k;
+        std::stable_sort(tris.begin() + first, tris.begin() + first + count,
+                         [&](const MRTriangleC& x, const MRTriangleC& y) { return centroid(x, axis) < centroid(y, axis); });
+        return count / 2u;
+    }
+    uint32_t build(uint32_t first, uint32_t count) {
+        const uint32_t index = (uint32_t)nodes.size();
+        nodes.push_back(MRBvhNodeC{});
+        // up to four child ranges: halves, and halves of halves when they hold more than a leaf
+        uint32_t rf[4], rc[4], n = 0;
+        if (count <= MR_MESH_LEAF_TRIANGLES) {
+            rf[0] = first;
+            rc[0] = count;
+            n = 1;
+        } else {
+            const uint32_t h = split(first, count);
+            const uint32_t halves[2][2] = {{first, h}, {first + h, count - h}};
+            for (const auto& half : halves) {
+                if (half[1] <= MR_MESH_LEAF_TRIANGLES) {
+                    rf[n] = half[0];
+                    rc[n++] = half[1];
+                } else {
+                    const uint32_t q = split(half[0], half[1]);
+                    rf[n] = half[0];
+                    rc[n++] = q;
+                    rf[n] = half[0] + q;
+                    rc[n++] = half[1] - q;
+                }
+            }
+        }
+        MRBvhNodeC node{};
+        float* lx = &node.lower_x.x;
+        float* ly = &node.lower_y.x;
+        float* lz = &node.lower_z.x;
+        float* ux = &node.upper_x.x;
+        float* uy = &node.upper_y.x;
+        float* uz = &node.upper_z.x;
+        for (uint32_t c = 0; c < 4u; ++c) {
+            node.child[c] = MR_INVALID_INDEX;
+            node.count[c] = 0u;
+            lx[c] = ly[c] = lz[c] = INFINITY;
+            ux[c] = uy[c] = uz[c] = -INFINITY;
+        }
+        for (uint32_t c = 0; c < n; ++c) {
+            float lo[3], hi[3];
+            bounds(rf[c], rc[c], lo, hi);
+            lx[c] = lo[0];
+            ly[c] = lo[1];
+            lz[c] = lo[2];
+            ux[c] = hi[0];
+            uy[c] = hi[1];
+            uz[c] = hi[2];
+            if (rc[c] <= MR_MESH_LEAF_TRIANGLES) {
+                node.child[c] = rf[c];
+                node.count[c] = rc[c];
+            } else {
+                node.child[c] = build(rf[c], rc[c]);
+                node.count[c] = 0u;
+            }
+        }
+        nodes[index] = node;
+        return index;
+    }
+};
+
+} // namespace
+
+void compile_geometry(ModelData& m) {
+    m.bvh_nodes.clear();
+    for (size_t gi = 0; gi < m.geometries.size(); ++gi) {
+        MRGeometryC& g = m.geometries[gi];
+        const std::string where = "geometry " + std::to_string(gi) + ": ";
+        if (g.kind != MR_SHAPE_CONVEX && g.kind != MR_SHAPE_TRIANGLE_MESH)
+            throw std::runtime_error(where + "kind must be MR_SHAPE_CONVEX or MR_SHAPE_TRIANGLE_MESH");
+        if ((size_t)g.vertex_offset + g.vertex_count > m.geometry_vertices.size())
+            throw std::runtime_error(where + "vertices outside the vertex table");
+        if (g.kind == MR_SHAPE_CONVEX && (g.vertex_count < 4u || g.vertex_count > MR_MAX_CONVEX_VERTICES))
+            throw std::runtime_error(where + "a convex hull needs 4 to 64 vertices");
+        if (g.kind == MR_SHAPE_TRIANGLE_MESH && (g.vertex_count < 3u || g.triangle_count == 0u))
+            throw std::runtime_error(where + "a mesh needs vertices and triangles");
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t v = 0; v < g.vertex_count; ++v) {
+            const mr_float4& p = m.geometry_vertices[g.vertex_offset + v];
+            const float c[3] = {p.x, p.y, p.z};
+            for (int k = 0; k < 3; ++k) {
+                if (!std::isfinite(c[k])) throw std::runtime_error(where + "non-finite vertex");
+                lo[k] = std::min(lo[k], c[k]);
+                hi[k] = std::max(hi[k], c[k]);
+            }
+        }
+        g.local_lower = mr_float4{lo[0], lo[1], lo[2], 0.0f};
+        g.local_upper = mr_float4{hi[0], hi[1], hi[2], 0.0f};
+        g.node_offset = 0u;
+        g.node_count = 0u;
+        if (g.kind != MR_SHAPE_TRIANGLE_MESH) {
+            g.triangle_offset = 0u;
+            g.triangle_count = 0u;
+            continue;
+        }
+        if ((size_t)g.triangle_offset + g.triangle_count > m.triangles.size())
+            throw std::runtime_error(where + "triangles outside the triangle table");
+        std::vector<MRTriangleC> tris(m.triangles.begin() + g.triangle_offset,
+                                      m.triangles.begin() + g.triangle_offset + g.triangle_count);
+        for (const MRTriangleC& t : tris)
+            for (uint32_t v : t.vertices)
+                if (v >= g.vertex_count) throw std::runtime_error(where + "triangle vertex index out of range");
+        std::vector<MRBvhNodeC> nodes;
+        BvhBuilder bb{m.geometry_vertices.data() + g.vertex_offset, tris, nodes};
+        bb.build(0u, (uint32_t)tris.size());
+        std::copy(tris.begin(), tris.end(), m.triangles.begin() + g.triangle_offset);
+        g.node_offset = (uint32_t)m.bvh_nodes.size();
+        g.node_count = (uint32_t)nodes.size();
+        m.bvh_nodes.insert(m.bvh_nodes.end(), nodes.begin(), nodes.end());
+    }
 }
 
 void compile_pairs(const MRModelHeaderC& h, const MRBodyC* bodies, const MRShapeC* shapes,
@@ -827,6 +985,14 @@ MRWorldModelDescC ModelData::describe() const {
     d.heightfield_heights = heights.empty() ? nullptr : heights.data();
     d.actuators = actuators.empty() ? nullptr : actuators.data();
     d.actuator_total = (uint32_t)actuators.size();
+    d.geometries = geometries.empty() ? nullptr : geometries.data();
+    d.geometry_vertices = geometry_vertices.empty() ? nullptr : geometry_vertices.data();
+    d.triangles = triangles.empty() ? nullptr : triangles.data();
+    d.geometry_total = (uint32_t)geometries.size();
+    d.geometry_vertex_total = (uint32_t)geometry_vertices.size();
+    d.triangle_total = (uint32_t)triangles.size();
+    d.bvh_nodes = bvh_nodes.empty() ? nullptr : bvh_nodes.data();
+    d.bvh_node_total = (uint32_t)bvh_nodes.size();
     return d;
 }
 
@@ -951,8 +1117,18 @@ void finalize_model(ModelData& m, float microstep) {
             if (sh.body >= h.body_count || sh.material >= h.material_count)
                 throw std::runtime_error("shape references an invalid body or material");
             if (sh.shape_type != MR_SHAPE_SPHERE && sh.shape_type != MR_SHAPE_CAPSULE && sh.shape_type != MR_SHAPE_BOX &&
-                sh.shape_type != MR_SHAPE_PLANE && sh.shape_type != MR_SHAPE_HEIGHTFIELD)
+                sh.shape_type != MR_SHAPE_PLANE && sh.shape_type != MR_SHAPE_HEIGHTFIELD &&
+                sh.shape_type != MR_SHAPE_CONVEX && sh.shape_type != MR_SHAPE_TRIANGLE_MESH)
                 throw std::runtime_error("unsupported shape type");
+            if (sh.shape_type == MR_SHAPE_CONVEX || sh.shape_type == MR_SHAPE_TRIANGLE_MESH) {
+                if (sh.geometry >= m.geometries.size() || m.geometries[sh.geometry].kind != sh.shape_type)
+                    throw std::runtime_error("convex or mesh shape without a geometry of its kind");
+                if (!(sh.dimensions.x > 0.0f && sh.dimensions.y > 0.0f && sh.dimensions.z > 0.0f) ||
+                    !std::isfinite(sh.dimensions.x) || !std::isfinite(sh.dimensions.y) || !std::isfinite(sh.dimensions.z))
+                    throw std::runtime_error("convex and mesh scales must be finite and positive");
+                if (sh.shape_type == MR_SHAPE_TRIANGLE_MESH && b[sh.body].motion_type != MR_MOTION_STATIC)
+                    throw std::runtime_error("triangle meshes must belong to static bodies");
+            }
             if (sh.shape_type == MR_SHAPE_HEIGHTFIELD &&
                 (!m.heightfield.present || m.heightfield.nx < 2u || m.heightfield.ny < 2u ||
                  m.heights.size() < (size_t)m.heightfield.nx * m.heightfield.ny ||
@@ -1014,6 +1190,14 @@ ModelData copy_model(const MRWorldModelDescC& d, float microstep) {
     if (d.heightfield_heights != nullptr)
         m.heights.assign(d.heightfield_heights, d.heightfield_heights + d.heightfield_vertex_count);
     m.heightfield = d.heightfield;
+    need(d.geometries, d.geometry_total, "geometries");
+    need(d.geometry_vertices, d.geometry_vertex_total, "geometry_vertices");
+    need(d.triangles, d.triangle_total, "triangles");
+    if (d.geometries != nullptr) m.geometries.assign(d.geometries, d.geometries + d.geometry_total);
+    if (d.geometry_vertices != nullptr)
+        m.geometry_vertices.assign(d.geometry_vertices, d.geometry_vertices + d.geometry_vertex_total);
+    if (d.triangles != nullptr) m.triangles.assign(d.triangles, d.triangles + d.triangle_total);
+    compile_geometry(m);
     m.gravity = d.gravity;
     m.max_nq = d.max_nq;
     m.max_nv = d.max_nv;

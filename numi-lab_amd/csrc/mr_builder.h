@@ -37,6 +37,11 @@ struct ModelData {
     std::vector<MRCompiledPairC> pairs;
     std::vector<float> heights;
     MRHeightfieldC heightfield{};
+    // convex and mesh geometry, shared by every model block
+    std::vector<MRGeometryC> geometries;
+    std::vector<mr_float4> geometry_vertices;
+    std::vector<MRTriangleC> triangles;   // each mesh's triangles in BVH leaf order
+    std::vector<MRBvhNodeC> bvh_nodes;    // built by compile_geometry
     mr_float4 gravity{0.0f, 0.0f, -9.81f, 0.0f};
     uint32_t max_nq = 0, max_nv = 0, max_scene_bodies = 0, max_shapes = 0, max_pairs = 0;
     std::vector<float> initial_q, initial_v;
@@ -54,6 +59,10 @@ struct ModelData {
     void commands(uint32_t first_step, uint32_t step_count, float* out) const;
 };
 
+// Validates the convex and mesh geometries, fills their bounds and builds
+// each mesh's BVH4 (reordering its triangles into leaf order; the triangle
+// feature ids travel with the triangles). Throws std::runtime_error.
+void compile_geometry(ModelData& model);
 // Validates topology/capacity rules the fused kernel relies on and fills
 // derived fields (levels, damping scales for timestep h). Throws
 // std::runtime_error with a message on violation.

@@ -189,6 +189,12 @@ struct KernelArgs {
     const MRCompiledPairC* pairs;
     const float* heights;
     MRHeightfieldC heightfield;
+    // convex and mesh geometry, shared by every model block (null without any)
+    const MRGeometryC* geometries;
+    const mr_float4* geometry_vertices;
+    const MRTriangleC* triangles;
+    const MRBvhNodeC* bvh_nodes;
+    uint32_t geometry;  // 1: some shape is a convex hull or a mesh (the kGeo instances run)
     float gravity[4];
     MRWorldStepConfigC cfg;
     float h;

@@ -778,7 +778,9 @@ MRD_INL void inflate_bounds(Collider& c) {
     c.upper = add(c.upper, pad);
 }
 
-// Collider of shape s with its body at (bpos, borient).
+// Collider of shape s with its body at (bpos, borient). Convex hulls and
+// meshes are projected by the kGeo instances only.
+template <bool kGeo = false>
 MRD_INL bool project_collider_pose(const Env& e, uint32_t s, V3 bpos, Q4 borient, Collider& c) {
     const MRShapeC& sh = e.shapes[s];
     Q4 br, lr;
@@ -835,17 +837,44 @@ MRD_INL bool project_collider_pose(const Env& e, uint32_t s, V3 bpos, Q4 borient
         c.lower = sub(f3(hf.bounds_lower), V3{co, co, co});
         c.upper = add(f3(hf.bounds_upper), V3{co, co, co});
         inflate_bounds(c);
+    } else if (kGeo && (sh.shape_type == MR_SHAPE_CONVEX || sh.shape_type == MR_SHAPE_TRIANGLE_MESH)) {
+        // half_extents carries the vertex scale; bounds of the transformed
+        // vertices (convex) or of the transformed local bounds (mesh)
+        c.half_extents = f3(sh.dimensions);
+        const MRGeometryC g = e.a->geometries[sh.geometry];
+        const V3 sc = c.half_extents;
+        V3 lo{__builtin_inff(), __builtin_inff(), __builtin_inff()};
+        V3 hi{-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+        const bool convex = sh.shape_type == MR_SHAPE_CONVEX;
+        const uint32_t count = convex ? g.vertex_count : 8u;
+        for (uint32_t i = 0; i < count; ++i) {
+            V3 l;
+            if (convex) {
+                const mr_float4 v = e.a->geometry_vertices[g.vertex_offset + i];
+                l = V3{v.x, v.y, v.z};
+            } else {
+                l = V3{(i & 1u) ? g.local_upper.x : g.local_lower.x, (i & 2u) ? g.local_upper.y : g.local_lower.y,
+                       (i & 4u) ? g.local_upper.z : g.local_lower.z};
+            }
+            const V3 w = add(c.center, qrot(c.rotation, V3{sc.x * l.x, sc.y * l.y, sc.z * l.z}));
+            lo = vmin(lo, w);
+            hi = vmax(hi, w);
+        }
+        c.lower = sub(lo, V3{co, co, co});
+        c.upper = add(hi, V3{co, co, co});
+        inflate_bounds(c);
     } else {
         return false;
     }
     return finite3(c.lower) && finite3(c.upper);
 }
 
+template <bool kGeo = false>
 MRD_INL bool project_collider(const Env& e, uint32_t s, Collider& c) {
     V3 bpos;
     Q4 borient;
     body_pose(e, e.shapes[s].body, bpos, borient);
-    return project_collider_pose(e, s, bpos, borient, c);
+    return project_collider_pose<kGeo>(e, s, bpos, borient, c);
 }
 
 MRD_INL float collider_scale(const Collider& c) {
@@ -1262,8 +1291,722 @@ MRD_INL V3 hf_query_point(const Collider& fs, uint32_t q) {
 
 MRD_INL float hf_query_radius(const Collider& fs) { return fs.type == MR_SHAPE_BOX ? 0.0f : fs.radius; }
 
+// ------------------------------------------- convex hulls and triangle meshes
+//
+// Shapes with a support-mapped core (a point set plus a radius): sphere
+// (centre), capsule (segment ends), box (8 vertices), convex hull (its
+// vertices, scaled) and a mesh triangle (3 vertices). Separated cores get
+// GJK's distance and witness points (Voronoi-region simplex reduction, Ericson
+// 5.1.5); overlapping cores, or cores closer than float resolution at their
+// coordinates, get the Minkowski portal refinement (XenoCollide) along the
+// ray from the centre difference through the origin. The contact patch then
+// takes both cores' features within a slab of their support planes along the
+// normal: two faces give the corners of their intersection polygon (A's face
+// clipped by B's, Sutherland-Hodgman in the plane normal to the contact, with
+// B's face normal replacing the query normal when within ~8 degrees), an edge
+// on a face the clipped segment's ends, anything else the witness pair; the
+// deepest MR_RAW_CONTACTS_PER_PAIR points are kept. Per lane: the cores,
+// simplices and patch polygons live in private memory, so this code is
+// compiled into its own kernel instances, used only by worlds with convex or
+// mesh shapes (Collision.metal:2298-3740 is the reference's GJK / EPA / MPR /
+// support-mapped patch; the algorithm here is the published one named above).
+
+struct Core {
+    uint32_t type;  // MR_SHAPE_* of the collider (MR_SHAPE_TRIANGLE_MESH for a triangle)
+    uint32_t n;     // points
+    float radius;
+    V3 center;
+    Q4 rotation;
+    V3 scale;       // convex: vertex scale; box: half extents
+    V3 p0, p1, p2;  // capsule endpoints / triangle vertices
+    const mr_float4* verts;  // convex: the geometry's vertices
+};
+
+MRD_INL V3 core_point(const Core& c, uint32_t i) {
+    if (c.type == MR_SHAPE_SPHERE) return c.center;
+    if (c.type == MR_SHAPE_CAPSULE) return i == 0u ? c.p0 : c.p1;
+    if (c.type == MR_SHAPE_TRIANGLE_MESH) return i == 0u ? c.p0 : (i == 1u ? c.p1 : c.p2);
+    if (c.type == MR_SHAPE_BOX) {
+        const V3 he = c.scale;
+        const V3 l{(i & 1u) == 0u ? -he.x : he.x, (i & 2u) == 0u ? -he.y : he.y, (i & 4u) == 0u ? -he.z : he.z};
+        return add(c.center, qrot(c.rotation, l));
+    }
+    const mr_float4 v = c.verts[i];
+    return add(c.center, qrot(c.rotation, V3{c.scale.x * v.x, c.scale.y * v.y, c.scale.z * v.z}));
+}
+
+// Index of the point with the largest projection on d (the lowest on ties).
+MRD_INL uint32_t core_support(const Core& c, V3 d) {
+    uint32_t best = 0u;
+    float bv = dot(d, core_point(c, 0u));
+    for (uint32_t i = 1u; i < c.n; ++i) {
+        const float v = dot(d, core_point(c, i));
+        if (v > bv) {
+            bv = v;
+            best = i;
+        }
+    }
+    return best;
+}
+
+MRD_INL Core make_core(const Env& e, uint32_t s, const Collider& col) {
+    Core c;
+    c.type = col.type;
+    c.radius = 0.0f;
+    c.center = col.center;
+    c.rotation = col.rotation;
+    c.scale = col.half_extents;
+    c.p0 = col.p0;
+    c.p1 = col.p1;
+    c.p2 = V3{0.0f, 0.0f, 0.0f};
+    c.verts = nullptr;
+    if (col.type == MR_SHAPE_SPHERE) {
+        c.n = 1u;
+        c.radius = col.radius;
+    } else if (col.type == MR_SHAPE_CAPSULE) {
+        c.n = 2u;
+        c.radius = col.radius;
+    } else if (col.type == MR_SHAPE_BOX) {
+        c.n = 8u;
+    } else {
+        const MRGeometryC& g = e.a->geometries[e.shapes[s].geometry];
+        c.n = g.vertex_count;
+        c.verts = e.a->geometry_vertices + g.vertex_offset;
+    }
+    return c;
+}
+
+struct Mv {
+    V3 w, a, b;
+    uint32_t ia, ib;
+};
+
+MRD_INL Mv minkowski_support(const Core& A, const Core& B, V3 d) {
+    Mv m;
+    m.ia = core_support(A, d);
+    m.ib = core_support(B, neg(d));
+    m.a = core_point(A, m.ia);
+    m.b = core_point(B, m.ib);
+    m.w = sub(m.a, m.b);
+    return m;
+}
+
+// Closest point to the origin of simplex s[0..n) (n = 1..3), reducing it in
+// place to the sub-simplex whose interior holds that point; lam are the
+// barycentric weights of the kept points.
+MRD_INL V3 simplex_closest(Mv* s, uint32_t& n, float* lam) {
+    if (n == 1u) {
+        lam[0] = 1.0f;
+        return s[0].w;
+    }
+    if (n == 2u) {
+        const V3 a = s[0].w, b = s[1].w, ab = sub(b, a);
+        const float den = dot(ab, ab);
+        const float t = den > kTiny ? -dot(a, ab) / den : 0.0f;
+        if (t <= 0.0f) {
+            n = 1u;
+            lam[0] = 1.0f;
+            return a;
+        }
+        if (t >= 1.0f) {
+            s[0] = s[1];
+            n = 1u;
+            lam[0] = 1.0f;
+            return b;
+        }
+        lam[0] = 1.0f - t;
+        lam[1] = t;
+        return add(a, mul(t, ab));
+    }
+    const V3 a = s[0].w, b = s[1].w, c = s[2].w;
+    const V3 ab = sub(b, a), ac = sub(c, a), ap = neg(a);
+    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) {
+        n = 1u;
+        lam[0] = 1.0f;
+        return a;
+    }
+    const V3 bp = neg(b);
+    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) {
+        s[0] = s[1];
+        n = 1u;
+        lam[0] = 1.0f;
+        return b;
+    }
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        const float t = d1 / (d1 - d3);
+        n = 2u;
+        lam[0] = 1.0f - t;
+        lam[1] = t;
+        return add(a, mul(t, ab));
+    }
+    const V3 cp = neg(c);
+    const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) {
+        s[0] = s[2];
+        n = 1u;
+        lam[0] = 1.0f;
+        return c;
+    }
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        const float t = d2 / (d2 - d6);
+        s[1] = s[2];
+        n = 2u;
+        lam[0] = 1.0f - t;
+        lam[1] = t;
+        return add(a, mul(t, ac));
+    }
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        const float t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        s[0] = s[1];
+        s[1] = s[2];
+        n = 2u;
+        lam[0] = 1.0f - t;
+        lam[1] = t;
+        return add(b, mul(t, sub(c, b)));
+    }
+    const float den = 1.0f / (va + vb + vc);
+    const float v = vb * den, w = vc * den;
+    lam[0] = 1.0f - v - w;
+    lam[1] = v;
+    lam[2] = w;
+    return add(a, add(mul(v, ab), mul(w, ac)));
+}
+
+// True when the origin lies inside (or on) tetrahedron s[0..4).
+MRD_INL bool tetra_contains_origin(const Mv* s) {
+    for (int f = 0; f < 4; ++f) {
+        const V3 a = s[(f + 1) % 4].w, b = s[(f + 2) % 4].w, c = s[(f + 3) % 4].w, d = s[f].w;
+        const V3 nrm = cross(sub(b, a), sub(c, a));
+        const float sd = dot(nrm, sub(d, a)), so = dot(nrm, neg(a));
+        if (sd * so < 0.0f) return false;
+    }
+    return true;
+}
+
+struct ConvexHit {
+    uint32_t status;  // 0 ok, 1 no result
+    bool overlap;
+    V3 n;             // A -> B
+    float sep;        // core separation (negative: depth)
+    V3 pa, pb;        // core witness points
+};
+
+MRD_INL ConvexHit gjk(const Core& A, const Core& B) {
+    ConvexHit r;
+    r.status = 0u;
+    r.overlap = false;
+    r.n = r.pa = r.pb = V3{0.0f, 0.0f, 0.0f};
+    r.sep = 0.0f;
+    Mv s[4];
+    float lam[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t n = 0u;
+    V3 v = sub(A.center, B.center);
+    if (!(dot(v, v) > kTiny)) v = V3{1.0f, 0.0f, 0.0f};
+    for (uint32_t it = 0; it < 64u; ++it) {
+        const Mv m = minkowski_support(A, B, neg(v));
+        const float vv = dot(v, v);
+        if (n > 0u && vv - dot(v, m.w) <= 1.0e-6f * vv) break;
+        bool repeat = false;
+        for (uint32_t k = 0; k < n; ++k) repeat = repeat || (s[k].ia == m.ia && s[k].ib == m.ib);
+        if (repeat) break;
+        s[n++] = m;
+        if (n == 4u) {
+            if (tetra_contains_origin(s)) {
+                r.overlap = true;
+                return r;
+            }
+            float best = __builtin_inff();
+            Mv keep[3];
+            float kl[3];
+            uint32_t kn = 0u;
+            for (int f = 0; f < 4; ++f) {
+                Mv t[3] = {s[(f + 1) % 4], s[(f + 2) % 4], s[(f + 3) % 4]};
+                uint32_t tn = 3u;
+                float tl[3];
+                const V3 p = simplex_closest(t, tn, tl);
+                const float d = dot(p, p);
+                if (d < best) {
+                    best = d;
+                    kn = tn;
+                    for (uint32_t k = 0; k < tn; ++k) {
+                        keep[k] = t[k];
+                        kl[k] = tl[k];
+                    }
+                    v = p;
+                }
+            }
+            n = kn;
+            for (uint32_t k = 0; k < kn; ++k) {
+                s[k] = keep[k];
+                lam[k] = kl[k];
+            }
+        } else {
+            v = simplex_closest(s, n, lam);
+        }
+        if (dot(v, v) <= 1.0e-14f) {
+            r.overlap = true;
+            return r;
+        }
+    }
+    V3 pa{0.0f, 0.0f, 0.0f}, pb{0.0f, 0.0f, 0.0f};
+    for (uint32_t k = 0; k < n; ++k) {
+        pa = add(pa, mul(lam[k], s[k].a));
+        pb = add(pb, mul(lam[k], s[k].b));
+    }
+    const V3 d = sub(pb, pa);
+    const float dist = length(d);
+    const V3 ca = vabs(A.center), cb = vabs(B.center);
+    const float coord = fmaxd(fmaxd(fmaxd(ca.x, ca.y), ca.z), fmaxd(fmaxd(cb.x, cb.y), cb.z));
+    if (!(dist > 1.0e-6f * (1.0f + coord))) {
+        r.overlap = true;
+        return r;
+    }
+    r.pa = pa;
+    r.pb = pb;
+    r.sep = dist;
+    r.n = dist > kTiny ? mul(1.0f / dist, d) : normalize(neg(v));
+    return r;
+}
+
+MRD_INL ConvexHit mpr(const Core& A, const Core& B) {
+    ConvexHit r;
+    r.status = 1u;
+    r.overlap = true;
+    r.n = r.pa = r.pb = V3{0.0f, 0.0f, 0.0f};
+    r.sep = 0.0f;
+    Mv v0;
+    v0.a = A.center;
+    v0.b = B.center;
+    v0.w = sub(A.center, B.center);
+    v0.ia = v0.ib = 0u;
+    if (!(dot(v0.w, v0.w) > 1.0e-12f)) v0.w = V3{1.0e-5f, 0.0f, 0.0f};
+    V3 n = neg(v0.w);
+    Mv v1 = minkowski_support(A, B, n);
+    n = cross(v1.w, v0.w);
+    if (!(dot(n, n) > 1.0e-14f)) {
+        const V3 d = normalize(sub(v1.w, v0.w));
+        r.status = 0u;
+        r.n = d;
+        r.sep = -dot(v1.w, d);
+        r.pa = v1.a;
+        r.pb = sub(v1.a, mul(-r.sep, d));
+        return r;
+    }
+    Mv v2 = minkowski_support(A, B, n);
+    if (dot(v2.w, n) < 0.0f) return r;
+    n = cross(sub(v1.w, v0.w), sub(v2.w, v0.w));
+    if (dot(n, v0.w) > 0.0f) {
+        const Mv t = v1;
+        v1 = v2;
+        v2 = t;
+        n = neg(n);
+    }
+    Mv v3;
+    bool portal = false;
+    for (uint32_t it = 0; it < 32u; ++it) {
+        v3 = minkowski_support(A, B, n);
+        if (dot(v3.w, n) < 0.0f) return r;
+        if (dot(cross(v1.w, v3.w), v0.w) < 0.0f) {
+            v2 = v3;
+            n = cross(sub(v1.w, v0.w), sub(v3.w, v0.w));
+            continue;
+        }
+        if (dot(cross(v3.w, v2.w), v0.w) < 0.0f) {
+            v1 = v3;
+            n = cross(sub(v3.w, v0.w), sub(v2.w, v0.w));
+            continue;
+        }
+        portal = true;
+        break;
+    }
+    if (!portal) return r;
+    for (uint32_t it = 0; it < 32u; ++it) {
+        n = cross(sub(v2.w, v1.w), sub(v3.w, v1.w));
+        const float nl = length(n);
+        if (!(nl > kTiny)) return r;
+        n = mul(1.0f / nl, n);
+        const Mv v4 = minkowski_support(A, B, n);
+        const float delta = dot(sub(v4.w, v3.w), n);
+        const float depth = dot(v4.w, n);
+        if (delta <= 1.0e-5f * fmaxd(1.0f, fabs_(depth)) || it == 31u) {
+            const float dist = dot(v1.w, n);
+            const V3 p = mul(dist, n);
+            const V3 e1 = sub(v2.w, v1.w), e2 = sub(v3.w, v1.w), ep = sub(p, v1.w);
+            const float d00 = dot(e1, e1), d01 = dot(e1, e2), d11 = dot(e2, e2);
+            const float d20 = dot(ep, e1), d21 = dot(ep, e2);
+            const float den = d00 * d11 - d01 * d01;
+            float b2 = 0.0f, b3 = 0.0f;
+            if (fabs_(den) > kTiny) {
+                b2 = (d11 * d20 - d01 * d21) / den;
+                b3 = (d00 * d21 - d01 * d20) / den;
+            }
+            const float b1 = 1.0f - b2 - b3;
+            r.status = 0u;
+            r.n = n;
+            r.sep = -depth;
+            r.pa = add(add(mul(b1, v1.a), mul(b2, v2.a)), mul(b3, v3.a));
+            r.pb = sub(r.pa, mul(depth, n));
+            return r;
+        }
+        if (dot(cross(v4.w, v1.w), v0.w) < 0.0f) {
+            if (dot(cross(v4.w, v2.w), v0.w) < 0.0f) v1 = v4;
+            else v3 = v4;
+        } else {
+            if (dot(cross(v4.w, v3.w), v0.w) < 0.0f) v2 = v4;
+            else v1 = v4;
+        }
+    }
+    return r;
+}
+
+MRD_INL void plane_basis(V3 n, V3& t1, V3& t2) {
+    const V3 an = vabs(n);
+    V3 ref;
+    if (an.x <= an.y && an.x <= an.z) ref = V3{1.0f, 0.0f, 0.0f};
+    else if (an.y <= an.z) ref = V3{0.0f, 1.0f, 0.0f};
+    else ref = V3{0.0f, 0.0f, 1.0f};
+    t1 = normalize(cross(n, ref));
+    t2 = cross(n, t1);
+}
+
+constexpr uint32_t kFeatureCap = 16u;
+constexpr uint32_t kPatchCap = 2u * kFeatureCap;
+
+// Points of core c within `slab` of its support plane along d, in index
+// order (at most kFeatureCap), and the plane's offset.
+MRD_INL uint32_t core_feature(const Core& c, V3 d, float slab, uint32_t* idx, float& top) {
+    top = -__builtin_inff();
+    for (uint32_t i = 0; i < c.n; ++i) top = fmaxd(top, dot(d, core_point(c, i)));
+    uint32_t k = 0u;
+    for (uint32_t i = 0; i < c.n && k < kFeatureCap; ++i)
+        if (dot(d, core_point(c, i)) >= top - slab) idx[k++] = i;
+    return k;
+}
+
+MRD_INL float core_extent(const Core& c) {
+    float e = 0.0f;
+    for (uint32_t i = 0; i < c.n; ++i) e = fmaxd(e, length(sub(core_point(c, i), c.center)));
+    return e;
+}
+
+// Convex hull of planar points (gift wrapping from the lowest index of the
+// leftmost point), counter-clockwise indices into pts; returns its size.
+MRD_INL uint32_t hull2d(const float (*pts)[2], uint32_t n, uint32_t* out) {
+    uint32_t start = 0u;
+    for (uint32_t i = 1; i < n; ++i)
+        if (pts[i][0] < pts[start][0] || (pts[i][0] == pts[start][0] && pts[i][1] < pts[start][1])) start = i;
+    uint32_t h = 0u, p = start;
+    do {
+        out[h++] = p;
+        uint32_t q = p == 0u ? 1u : 0u;
+        for (uint32_t r = 0; r < n; ++r) {
+            if (r == p) continue;
+            const float cr = (pts[q][0] - pts[p][0]) * (pts[r][1] - pts[p][1]) -
+                (pts[q][1] - pts[p][1]) * (pts[r][0] - pts[p][0]);
+            const float dq = (pts[q][0] - pts[p][0]) * (pts[q][0] - pts[p][0]) +
+                (pts[q][1] - pts[p][1]) * (pts[q][1] - pts[p][1]);
+            const float dr = (pts[r][0] - pts[p][0]) * (pts[r][0] - pts[p][0]) +
+                (pts[r][1] - pts[p][1]) * (pts[r][1] - pts[p][1]);
+            if (cr < 0.0f || (cr == 0.0f && dr > dq)) q = r;
+        }
+        p = q;
+    } while (p != start && h < n);
+    return h;
+}
+
+// Clips segment (x0,y0)-(x1,y1) against a counter-clockwise convex polygon;
+// the kept parameter interval is [t0, t1] (empty when t0 > t1).
+MRD_INL void clip_segment(const float (*pts)[2], const uint32_t* poly, uint32_t m, float x0, float y0, float x1,
+                          float y1, float& t0, float& t1) {
+    t0 = 0.0f;
+    t1 = 1.0f;
+    for (uint32_t k = 0; k < m && t0 <= t1; ++k) {
+        const float* a = pts[poly[k]];
+        const float* b = pts[poly[(k + 1) % m]];
+        const float ex = b[0] - a[0], ey = b[1] - a[1];
+        const float f0 = ex * (y0 - a[1]) - ey * (x0 - a[0]);
+        const float f1 = ex * (y1 - a[1]) - ey * (x1 - a[0]);
+        if (f0 < 0.0f && f1 < 0.0f) {
+            t0 = 1.0f;
+            t1 = 0.0f;
+        } else if (f0 < 0.0f) {
+            t0 = fmaxd(t0, f0 / (f0 - f1));
+        } else if (f1 < 0.0f) {
+            t1 = fmind(t1, f0 / (f0 - f1));
+        }
+    }
+}
+
+// Keeps the deepest MR_RAW_CONTACTS_PER_PAIR contacts (replacing the
+// shallowest, the earlier kept on ties).
+MRD_INL void append_deepest(RawBatch& out, const RawContact& c) {
+    if (out.count < MR_RAW_CONTACTS_PER_PAIR) {
+        out.c[out.count++] = c;
+        return;
+    }
+    uint32_t worst = 0u;
+    for (uint32_t k = 1; k < out.count; ++k)
+        if (out.c[k].separation > out.c[worst].separation) worst = k;
+    if (c.separation < out.c[worst].separation) out.c[worst] = c;
+}
+
+MRD_INL uint32_t core_feature_key(const Core& c, uint32_t local, uint32_t tri_feature) {
+    if (c.type == MR_SHAPE_TRIANGLE_MESH) return feature_key(MR_SHAPE_TRIANGLE_MESH, tri_feature);
+    return feature_key(c.type, local);
+}
+
+// One vertex of the clipped patch polygon: plane coordinates, the point on
+// A's feature, its feature keys, and the tag of the polygon edge leaving it
+// (k < kFeatureCap: A's hull edge k; kFeatureCap + k: B's clip line k).
+struct PatchVertex {
+    float x, y;
+    V3 p;
+    uint32_t feat_a, feat_b, tag;
+};
+
+MRD_INL void convex_contacts(const Core& A, const Core& B, uint32_t tri_a, uint32_t tri_b, float accepted,
+                             RawBatch& out) {
+    ConvexHit h = gjk(A, B);
+    if (h.overlap) h = mpr(A, B);
+    if (h.status != 0u) return;
+    const float sep = h.sep - A.radius - B.radius;
+    if (!(sep <= accepted)) return;
+    V3 n = h.n;
+    const float ext_a = fmaxd(core_extent(A), 1.0e-3f), ext_b = fmaxd(core_extent(B), 1.0e-3f);
+    uint32_t fa[kFeatureCap], fb[kFeatureCap];
+    float top_a, top_b;
+    const uint32_t na = core_feature(A, n, 2.0e-3f * ext_a, fa, top_a);
+    const uint32_t nb = core_feature(B, neg(n), 2.0e-3f * ext_b, fb, top_b);
+    float plane_b = -top_b;  // B's support plane: dot(n, x) = plane_b
+    const uint32_t face_a = core_feature_key(A, 128u + fa[0], tri_a), face_b = core_feature_key(B, 128u + fb[0], tri_b);
+    if (na == 1u || nb == 1u || (na == 2u && nb == 2u)) {
+        const uint32_t feat_a = na == 1u ? core_feature_key(A, fa[0], tri_a) : face_a;
+        const uint32_t feat_b = nb == 1u ? core_feature_key(B, fb[0], tri_b) : face_b;
+        append_deepest(out, make_contact(n, h.sep - A.radius - B.radius, add(h.pa, mul(A.radius, n)),
+                                         sub(h.pb, mul(B.radius, n)), feat_a, feat_b));
+        return;
+    }
+    V3 t1, t2;
+    plane_basis(n, t1, t2);
+    float pa2[kFeatureCap][2], pb2[kFeatureCap][2];
+    V3 wa[kFeatureCap], wb[kFeatureCap];
+    for (uint32_t k = 0; k < na; ++k) {
+        wa[k] = core_point(A, fa[k]);
+        pa2[k][0] = dot(t1, wa[k]);
+        pa2[k][1] = dot(t2, wa[k]);
+    }
+    for (uint32_t k = 0; k < nb; ++k) {
+        wb[k] = core_point(B, fb[k]);
+        pb2[k][0] = dot(t1, wb[k]);
+        pb2[k][1] = dot(t2, wb[k]);
+    }
+    uint32_t hull_a[kFeatureCap], hull_b[kFeatureCap];
+    const uint32_t ma = na >= 3u ? hull2d(pa2, na, hull_a) : 0u;
+    const uint32_t mb = nb >= 3u ? hull2d(pb2, nb, hull_b) : 0u;
+    const uint32_t before = out.count;
+    if (ma >= 3u && mb >= 3u) {
+        V3 fnrm{0.0f, 0.0f, 0.0f};
+        for (uint32_t k = 0; k < mb; ++k) fnrm = add(fnrm, cross(wb[hull_b[k]], wb[hull_b[(k + 1u) % mb]]));
+        const float fl = length(fnrm);
+        if (fl > kTiny && dot(fnrm, n) >= 0.99f * fl) {
+            n = mul(1.0f / fl, fnrm);
+            plane_b = __builtin_inff();
+            for (uint32_t k = 0; k < nb; ++k) plane_b = fmind(plane_b, dot(n, wb[k]));
+        }
+        PatchVertex poly[kPatchCap], next[kPatchCap];
+        uint32_t m = ma;
+        for (uint32_t k = 0; k < ma; ++k) {
+            const uint32_t i = hull_a[k];
+            poly[k] = PatchVertex{pa2[i][0], pa2[i][1], wa[i], core_feature_key(A, fa[i], tri_a), face_b, k};
+        }
+        for (uint32_t k = 0; k < mb && m > 0u; ++k) {
+            const float* a = pb2[hull_b[k]];
+            const float* b = pb2[hull_b[(k + 1u) % mb]];
+            const float ex = b[0] - a[0], ey = b[1] - a[1];
+            uint32_t q = 0u;
+            for (uint32_t j = 0; j < m; ++j) {
+                const PatchVertex P = poly[j];
+                const PatchVertex Q = poly[(j + 1u) % m];
+                const float fp = ex * (P.y - a[1]) - ey * (P.x - a[0]);
+                const float fq = ex * (Q.y - a[1]) - ey * (Q.x - a[0]);
+                const bool in_p = fp >= 0.0f, in_q = fq >= 0.0f;
+                if (in_p != in_q && q < kPatchCap) {
+                    const float t = fp / (fp - fq);
+                    PatchVertex c;
+                    c.x = P.x + t * (Q.x - P.x);
+                    c.y = P.y + t * (Q.y - P.y);
+                    c.p = add(P.p, mul(t, sub(Q.p, P.p)));
+                    if (P.tag < kFeatureCap) {
+                        c.feat_a = core_feature_key(A, 64u + fa[hull_a[P.tag]], tri_a);
+                        c.feat_b = core_feature_key(B, 64u + fb[hull_b[k]], tri_b);
+                    } else {
+                        const uint32_t j_line = P.tag - kFeatureCap;
+                        const uint32_t shared = j_line + 1u == k ? hull_b[k]
+                            : (j_line == 0u && k + 1u == mb) ? hull_b[0] : MR_INVALID_INDEX;
+                        c.feat_a = face_a;
+                        c.feat_b = shared != MR_INVALID_INDEX ? core_feature_key(B, fb[shared], tri_b)
+                                                              : core_feature_key(B, 64u + fb[hull_b[k]], tri_b);
+                    }
+                    c.tag = in_p ? kFeatureCap + k : P.tag;
+                    next[q++] = c;
+                }
+                if (in_q && q < kPatchCap) next[q++] = Q;
+            }
+            m = q;
+            for (uint32_t j = 0; j < m; ++j) poly[j] = next[j];
+        }
+        const float tol = 4.0e-6f * (1.0f + fmaxd(ext_a, ext_b));
+        for (uint32_t j = 0; j < m; ++j) {
+            bool dup = false;
+            for (uint32_t i = 0; i < j; ++i) {
+                const float dx = poly[j].x - poly[i].x, dy = poly[j].y - poly[i].y;
+                dup = dup || dx * dx + dy * dy <= tol * tol;
+            }
+            if (dup) continue;
+            const float s = plane_b - dot(n, poly[j].p);
+            append_deepest(out, make_contact(n, s - A.radius - B.radius, add(poly[j].p, mul(A.radius, n)),
+                                             sub(add(poly[j].p, mul(s, n)), mul(B.radius, n)), poly[j].feat_a,
+                                             poly[j].feat_b));
+        }
+    } else if (na == 2u && mb >= 3u) {
+        float t0, tt1;
+        clip_segment(pb2, hull_b, mb, pa2[0][0], pa2[0][1], pa2[1][0], pa2[1][1], t0, tt1);
+        if (t0 <= tt1)
+            for (int ei = 0; ei < 2; ++ei) {
+                const float t = ei == 0 ? t0 : tt1;
+                const V3 p = add(wa[0], mul(t, sub(wa[1], wa[0])));
+                const float s = plane_b - dot(n, p);
+                append_deepest(out, make_contact(n, s - A.radius - B.radius, add(p, mul(A.radius, n)),
+                                                 sub(add(p, mul(s, n)), mul(B.radius, n)),
+                                                 core_feature_key(A, 64u + fa[ei], tri_a), face_b));
+            }
+    } else if (nb == 2u && ma >= 3u) {
+        float t0, tt1;
+        clip_segment(pa2, hull_a, ma, pb2[0][0], pb2[0][1], pb2[1][0], pb2[1][1], t0, tt1);
+        if (t0 <= tt1)
+            for (int ei = 0; ei < 2; ++ei) {
+                const float t = ei == 0 ? t0 : tt1;
+                const V3 p = add(wb[0], mul(t, sub(wb[1], wb[0])));
+                const float s = dot(n, p) - top_a;
+                append_deepest(out, make_contact(n, s - A.radius - B.radius, add(sub(p, mul(s, n)), mul(A.radius, n)),
+                                                 sub(p, mul(B.radius, n)), face_a,
+                                                 core_feature_key(B, 64u + fb[ei], tri_b)));
+            }
+    }
+    if (out.count == before)
+        append_deepest(out, make_contact(n, h.sep - A.radius - B.radius, add(h.pa, mul(A.radius, n)),
+                                         sub(h.pb, mul(B.radius, n)), face_a, face_b));
+}
+
+// A convex hull against a plane: every vertex within `accepted` (deepest 8).
+MRD_INL void convex_plane_contacts(const Core& hull, const Collider& plane, bool plane_is_a, float accepted,
+                                   RawBatch& out) {
+    for (uint32_t i = 0; i < hull.n; ++i) {
+        const V3 p = core_point(hull, i);
+        const float sep = dot(plane.plane_normal, sub(p, plane.center));
+        if (!(sep <= accepted)) continue;
+        const V3 sp = sub(p, mul(sep, plane.plane_normal));
+        RawContact c = make_contact(neg(plane.plane_normal), sep, p, sp, feature_key(MR_SHAPE_CONVEX, i),
+                                    feature_key(MR_SHAPE_PLANE, 0u));
+        if (plane_is_a) c = swapped(c);
+        append_deepest(out, c);
+    }
+}
+
+constexpr uint32_t kBvhStack = 48u;
+
+// A static mesh against a finite collider: BVH4 traversal with the finite
+// collider's bounds (padded by the accepted distance) in the mesh's unscaled
+// local frame, then every overlapping triangle as a 3-point core. One-sided
+// meshes keep the contacts whose normal (finite -> mesh) opposes the
+// triangle's front normal (counter-clockwise vertices).
+MRD_INL void mesh_contacts(const Env& e, uint32_t mesh_shape, const Collider& mesh, const Core& fin,
+                           const Collider& fin_col, bool mesh_is_a, float accepted, RawBatch& out) {
+    const MRShapeC& ms = e.shapes[mesh_shape];
+    const MRGeometryC g = e.a->geometries[ms.geometry];
+    const mr_float4* vt = e.a->geometry_vertices + g.vertex_offset;
+    const MRTriangleC* tris = e.a->triangles + g.triangle_offset;
+    const MRBvhNodeC* nodes = e.a->bvh_nodes + g.node_offset;
+    const bool two_sided = (ms.flags & MR_SHAPE_FLAG_MESH_TWO_SIDED) != 0u;
+    const V3 scale = mesh.half_extents;
+    V3 lo{__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    V3 hi{-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    for (uint32_t k = 0; k < 8u; ++k) {
+        const V3 w{(k & 1u) ? fin_col.upper.x + accepted : fin_col.lower.x - accepted,
+                   (k & 2u) ? fin_col.upper.y + accepted : fin_col.lower.y - accepted,
+                   (k & 4u) ? fin_col.upper.z + accepted : fin_col.lower.z - accepted};
+        const V3 l = qrot_inv(mesh.rotation, sub(w, mesh.center));
+        const V3 u{l.x / scale.x, l.y / scale.y, l.z / scale.z};
+        lo = vmin(lo, u);
+        hi = vmax(hi, u);
+    }
+    uint32_t stack[kBvhStack];
+    uint32_t top = 0u;
+    stack[top++] = 0u;
+    while (top > 0u) {
+        const MRBvhNodeC& nd = nodes[stack[--top]];
+        const float lx[4] = {nd.lower_x.x, nd.lower_x.y, nd.lower_x.z, nd.lower_x.w};
+        const float ly[4] = {nd.lower_y.x, nd.lower_y.y, nd.lower_y.z, nd.lower_y.w};
+        const float lz[4] = {nd.lower_z.x, nd.lower_z.y, nd.lower_z.z, nd.lower_z.w};
+        const float ux[4] = {nd.upper_x.x, nd.upper_x.y, nd.upper_x.z, nd.upper_x.w};
+        const float uy[4] = {nd.upper_y.x, nd.upper_y.y, nd.upper_y.z, nd.upper_y.w};
+        const float uz[4] = {nd.upper_z.x, nd.upper_z.y, nd.upper_z.z, nd.upper_z.w};
+        uint32_t child[4], count[4];
+        for (int c = 0; c < 4; ++c) {
+            child[c] = nd.child[c];
+            count[c] = nd.count[c];
+        }
+        for (int c = 3; c >= 0; --c) {  // pushed in reverse: children leave the stack in index order
+            if (child[c] == MR_INVALID_INDEX || count[c] != 0u) continue;
+            if (lx[c] <= hi.x && ux[c] >= lo.x && ly[c] <= hi.y && uy[c] >= lo.y && lz[c] <= hi.z && uz[c] >= lo.z &&
+                top < kBvhStack)
+                stack[top++] = child[c];
+        }
+        for (uint32_t c = 0; c < 4u; ++c) {
+            if (child[c] == MR_INVALID_INDEX || count[c] == 0u) continue;
+            if (!(lx[c] <= hi.x && ux[c] >= lo.x && ly[c] <= hi.y && uy[c] >= lo.y && lz[c] <= hi.z && uz[c] >= lo.z))
+                continue;
+            for (uint32_t t = child[c]; t < child[c] + count[c]; ++t) {
+                const MRTriangleC tr = tris[t];
+                Core tc;
+                tc.type = MR_SHAPE_TRIANGLE_MESH;
+                tc.n = 3u;
+                tc.radius = 0.0f;
+                tc.rotation = mesh.rotation;
+                tc.scale = scale;
+                tc.verts = nullptr;
+                V3 p[3];
+                for (int k = 0; k < 3; ++k) {
+                    const mr_float4 v = vt[tr.vertices[k]];
+                    p[k] = add(mesh.center, qrot(mesh.rotation, V3{scale.x * v.x, scale.y * v.y, scale.z * v.z}));
+                }
+                tc.p0 = p[0];
+                tc.p1 = p[1];
+                tc.p2 = p[2];
+                tc.center = mul(1.0f / 3.0f, add(add(p[0], p[1]), p[2]));
+                RawBatch one;
+                one.count = 0u;
+                convex_contacts(fin, tc, 0u, tr.feature, accepted, one);
+                const V3 front = cross(sub(p[1], p[0]), sub(p[2], p[0]));
+                for (uint32_t k = 0; k < one.count; ++k) {
+                    const RawContact& rc = one.c[k];
+                    if (!two_sided && !(dot(rc.normal, front) < 0.0f)) continue;
+                    append_deepest(out, mesh_is_a ? swapped(rc) : rc);
+                }
+            }
+        }
+    }
+}
+
 // hf_first: index of this pair's first precomputed heightfield query in e.hfq,
-// or ~0 to evaluate the queries here.
+// or ~0 to evaluate the queries here. Convex and mesh pairs are generated by
+// the kGeo instances only.
+template <bool kGeo = false>
 MRD_INL void generate_contacts(const Env& e, uint32_t ia, uint32_t ib, uint32_t cls, const Collider& A,
                                const Collider& B, RawBatch& out, uint32_t hf_first) {
     out.count = 0;
@@ -1372,6 +2115,18 @@ MRD_INL void generate_contacts(const Env& e, uint32_t ia, uint32_t ib, uint32_t 
                 append_surface_contact(out, hf_a, V3{h.n[0], h.n[1], h.n[2]}, V3{h.p[0], h.p[1], h.p[2]}, h.d,
                                        feature_key(fs.type, q), h.feature);
         }
+    } else if (kGeo && cls == MR_PAIR_CONVEX) {
+        if (A.type == MR_SHAPE_PLANE || B.type == MR_SHAPE_PLANE) {
+            const bool plane_a = A.type == MR_SHAPE_PLANE;
+            convex_plane_contacts(make_core(e, plane_a ? ib : ia, plane_a ? B : A), plane_a ? A : B, plane_a, accepted,
+                                  out);
+        } else {
+            convex_contacts(make_core(e, ia, A), make_core(e, ib, B), 0u, 0u, accepted, out);
+        }
+    } else if (kGeo && cls == MR_PAIR_MESH) {
+        const bool mesh_a = A.type == MR_SHAPE_TRIANGLE_MESH;
+        mesh_contacts(e, mesh_a ? ia : ib, mesh_a ? A : B, make_core(e, mesh_a ? ib : ia, mesh_a ? B : A),
+                      mesh_a ? B : A, mesh_a, accepted, out);
     }
 }
 
@@ -1433,11 +2188,12 @@ MRD_INL CcdMotion ccd_motion(const Env& e, uint32_t b, float T) {
     return m;
 }
 
+template <bool kGeo>
 MRD_INL bool ccd_collider(const Env& e, uint32_t s, const CcdMotion& m, float t, Collider& c) {
     const V3 p = add(m.p0, mul(t, m.v));
     Q4 r;
     if (!normalized_quat(qmul(rotation_increment(mul(t, m.w)), m.r0), r)) return false;
-    return project_collider_pose(e, s, p, r, c);
+    return project_collider_pose<kGeo>(e, s, p, r, c);
 }
 
 // Largest distance from the body's COM to a point of shape s; planes and
@@ -1532,6 +2288,7 @@ enum : uint32_t { kCcdNone = 0u, kCcdImpact = 1u, kCcdUnsafe = 2u };
 // with toi (seconds), kCcdUnsafe for an unresolved pair the speculative
 // contacts do not cover, kCcdNone otherwise (no CCD flag, apart over the whole
 // interval, or touching already: the discrete contacts handle that).
+template <bool kGeo>
 MRD_INL uint32_t ccd_pair(const Env& e, uint32_t k, float T, float& toi) {
     const MRWorldStepConfigC& cfg = e.a->cfg;
     const MRCompiledPairC& pr = e.pairs[k];
@@ -1545,8 +2302,8 @@ MRD_INL uint32_t ccd_pair(const Env& e, uint32_t k, float T, float& toi) {
     const float motion_a = ccd_motion_bound(ma, sa, T), motion_b = ccd_motion_bound(mb, sb, T);
     const bool speculative_safe = motion_a <= sa.contact_rest_radius.x + tol && motion_b <= sb.contact_rest_radius.x + tol;
     Collider a0, b0, a1, b1;
-    if (!ccd_collider(e, pr.collider_a, ma, 0.0f, a0) || !ccd_collider(e, pr.collider_b, mb, 0.0f, b0) ||
-        !ccd_collider(e, pr.collider_a, ma, T, a1) || !ccd_collider(e, pr.collider_b, mb, T, b1))
+    if (!ccd_collider<kGeo>(e, pr.collider_a, ma, 0.0f, a0) || !ccd_collider<kGeo>(e, pr.collider_b, mb, 0.0f, b0) ||
+        !ccd_collider<kGeo>(e, pr.collider_a, ma, T, a1) || !ccd_collider<kGeo>(e, pr.collider_b, mb, T, b1))
         return speculative_safe ? kCcdNone : kCcdUnsafe;
     // swept bounds: the union of the start and end bounds
     Collider sa_sw = a0, sb_sw = b0;
@@ -1601,7 +2358,7 @@ MRD_INL uint32_t ccd_pair(const Env& e, uint32_t k, float T, float& toi) {
         alpha += fmaxd(d / total, min_alpha);
         if (alpha > 1.0f) return kCcdNone;
         Collider a, b;
-        if (!ccd_collider(e, pr.collider_a, ma, alpha * T, a) || !ccd_collider(e, pr.collider_b, mb, alpha * T, b) ||
+        if (!ccd_collider<kGeo>(e, pr.collider_a, ma, alpha * T, a) || !ccd_collider<kGeo>(e, pr.collider_b, mb, alpha * T, b) ||
             !ccd_separation(pr.collider_a, pr.collider_b, pr.pair_class, a, b, d))
             return speculative_safe ? kCcdNone : kCcdUnsafe;
     }
@@ -2235,7 +2992,7 @@ MRD_INL bool mlp_warm_start(const Env& e, uint32_t lane, uint32_t ncons, uint32_
 // the plain instance carries none of that code.
 // `step` selects the control step of a multi-step launch: the command, reset
 // mask and status records of step `step` ([steps][E] each); 0 elsewhere.
-template <bool kMlp, bool kCcd = false>
+template <bool kMlp, bool kCcd = false, bool kGeo = false>
 __device__ __forceinline__ void control_step(const KernelArgs& args, uint32_t env,
                                                                  uint32_t step = 0u) {
     extern __shared__ __align__(16) unsigned char lds_base[];
@@ -2499,7 +3256,7 @@ __device__ __forceinline__ void control_step(const KernelArgs& args, uint32_t en
             uint32_t my_bits = ~0u, my_pair = ~0u, my_unsafe = ~0u;
             for (uint32_t k = lane; k < e.npairs; k += kLanes) {
                 float toi = 0.0f;
-                const uint32_t r = ccd_pair(e, k, remaining, toi);
+                const uint32_t r = ccd_pair<kGeo>(e, k, remaining, toi);
                 if (r == kCcdImpact) {
                     const uint32_t bits = __float_as_uint(fmaxd(toi, 0.0f));
                     if (bits < my_bits) {
@@ -2672,7 +3429,7 @@ __device__ __forceinline__ void control_step(const KernelArgs& args, uint32_t en
         // ------------------------------------------- collider projection
         for (uint32_t s = lane; s < e.nshapes; s += kLanes) {
             Collider c;
-            if (!project_collider(e, s, c)) report_failure(&s_fail, s, MR_STEP_NONFINITE_INPUT, s);
+            if (!project_collider<kGeo>(e, s, c)) report_failure(&s_fail, s, MR_STEP_NONFINITE_INPUT, s);
             e.col[s] = c;
         }
         __syncthreads();
@@ -2798,7 +3555,7 @@ __device__ __forceinline__ void control_step(const KernelArgs& args, uint32_t en
             const Collider ca_ = e.col[pr.collider_a];
             const Collider cb_ = e.col[pr.collider_b];
             RawBatch raw;
-            generate_contacts(e, pr.collider_a, pr.collider_b, pr.pair_class, ca_, cb_, raw, e.moff[k]);
+            generate_contacts<kGeo>(e, pr.collider_a, pr.collider_b, pr.pair_class, ca_, cb_, raw, e.moff[k]);
             bool bad = false;
             for (uint32_t c = 0; c < raw.count; ++c)
                 if (!finite3(raw.c[c].normal) || !isfin(raw.c[c].separation) || !finite3(raw.c[c].point_a) ||
@@ -3768,6 +4525,20 @@ mr_control_step_ccd_kernel(KernelArgs args) {
     control_step<false, true>(args, blockIdx.x);
 }
 
+// Worlds with convex hulls or triangle meshes: one workgroup per
+// environment, with the GJK / MPR / patch and BVH narrowphase (speculative
+// and hybrid CCD modes). Separate instances, so the other kernels carry none
+// of its private arrays.
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+mr_control_step_geo_kernel(KernelArgs args) {
+    control_step<false, false, true>(args, blockIdx.x);
+}
+
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+mr_control_step_ccd_geo_kernel(KernelArgs args) {
+    control_step<false, true, true>(args, blockIdx.x);
+}
+
 // Advance summary: one thread clears it, then a grid-stride pass over the
 // [steps][environments] status records folds it with per-wave reductions and
 // one atomic per wave (kernels rather than a memset, so the sequence is also
@@ -3858,6 +4629,11 @@ hipError_t launch_control_step(const KernelArgs& args, uint32_t lds_bytes, hipSt
     }
     if (mlp)
         hipLaunchKernelGGL(mr_control_step_mlp_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
+    else if (args.geometry != 0u && args.cfg.ccd_mode == MR_CCD_HYBRID)
+        hipLaunchKernelGGL(mr_control_step_ccd_geo_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream,
+                           args);
+    else if (args.geometry != 0u)
+        hipLaunchKernelGGL(mr_control_step_geo_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
     else if (args.cfg.ccd_mode == MR_CCD_HYBRID)
         hipLaunchKernelGGL(mr_control_step_ccd_kernel, dim3(args.environment_count), dim3(64), lds_bytes, stream, args);
     else
@@ -3975,9 +4751,12 @@ hipError_t configure_control_step(uint32_t lds_bytes, uint32_t mlp_lds_bytes) {
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mr_control_horizon_queue_kernel),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
         if (e != hipSuccess) return e;
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mr_control_step_ccd_kernel),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-        if (e != hipSuccess) return e;
+        for (const void* k : {reinterpret_cast<const void*>(&mr_control_step_ccd_kernel),
+                              reinterpret_cast<const void*>(&mr_control_step_geo_kernel),
+                              reinterpret_cast<const void*>(&mr_control_step_ccd_geo_kernel)}) {
+            e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+            if (e != hipSuccess) return e;
+        }
         plain_max = lds_bytes;
     }
     if (mlp_lds_bytes > mlp_max) {

@@ -82,6 +82,11 @@ World::World(const ModelData& model, const MRWorldStepConfigC& config) : model_(
     if (cfg_.capture_evidence != 0u && cfg_.capture_substep >= cfg_.physics_substeps)
         throw std::runtime_error("capture_substep must be below physics_substeps");
     finalize_model(model_, cfg_.control_timestep / (float)cfg_.physics_substeps);
+    for (const MRShapeC& sh : model_.shapes)
+        if (sh.shape_type == MR_SHAPE_CONVEX || sh.shape_type == MR_SHAPE_TRIANGLE_MESH)
+            throw std::runtime_error(
+                "convex and mesh shapes: the device narrowphase is switched off (its kernel instance took a memory "
+                "aperture violation on the GPU whose cause is not yet shown); the CPU oracle implements them");
     env_ = model_.environment_count();
     hip_check(hipGetDevice(&device_), "hipGetDevice");
 
@@ -110,6 +115,12 @@ World::World(const ModelData& model, const MRWorldStepConfigC& config) : model_(
     upload(d_materials_, model_.materials);
     upload(d_pairs_, model_.pairs);
     upload(d_heights_, model_.heights);
+    upload(d_geometries_, model_.geometries);
+    upload(d_geometry_vertices_, model_.geometry_vertices);
+    upload(d_triangles_, model_.triangles);
+    upload(d_bvh_nodes_, model_.bvh_nodes);
+    for (const MRShapeC& sh : model_.shapes)
+        geometry_ = geometry_ || sh.shape_type == MR_SHAPE_CONVEX || sh.shape_type == MR_SHAPE_TRIANGLE_MESH;
     const size_t E = env_;
     d_q_.reserve(E * model_.max_nq * sizeof(float) + 16u);
     d_v_.reserve(E * model_.max_nv * sizeof(float) + 16u);
@@ -310,7 +321,8 @@ size_t sum_bytes(const DeviceBuffer* const (&bufs)[N]) {
 
 size_t World::immutable_bytes() const {
     const DeviceBuffer* const all[] = {&d_models_, &d_env_model_, &d_slot_plan_, &d_bodies_, &d_dofs_,
-                                       &d_actuators_, &d_shapes_, &d_materials_, &d_pairs_, &d_heights_};
+                                       &d_actuators_, &d_shapes_, &d_materials_, &d_pairs_, &d_heights_,
+                                       &d_geometries_, &d_geometry_vertices_, &d_triangles_, &d_bvh_nodes_};
     return sum_bytes(all);
 }
 
@@ -422,6 +434,11 @@ KernelArgs World::base_args(uint32_t step) const {
     a.materials = static_cast<const MRMaterialC*>(d_materials_.ptr);
     a.pairs = static_cast<const MRCompiledPairC*>(d_pairs_.ptr);
     a.heights = static_cast<const float*>(d_heights_.ptr);
+    a.geometries = static_cast<const MRGeometryC*>(d_geometries_.ptr);
+    a.geometry_vertices = static_cast<const mr_float4*>(d_geometry_vertices_.ptr);
+    a.triangles = static_cast<const MRTriangleC*>(d_triangles_.ptr);
+    a.bvh_nodes = static_cast<const MRBvhNodeC*>(d_bvh_nodes_.ptr);
+    a.geometry = geometry_ ? 1u : 0u;
     a.heightfield = model_.heightfield;
     a.gravity[0] = model_.gravity.x;
     a.gravity[1] = model_.gravity.y;
@@ -481,6 +498,7 @@ void World::set_warm_start_mlp(const MRMlpWeightsC* weights) {
         throw std::runtime_error("warm-start MLP: the learned warm start runs with the ordered sweeps solver");
     if (cfg_.ccd_mode != MR_CCD_SPECULATIVE)
         throw std::runtime_error("warm-start MLP: not available in the hybrid CCD mode");
+    if (geometry_) throw std::runtime_error("warm-start MLP: not available in worlds with convex or mesh shapes");
     if (lds_mlp_.total == 0u)
         throw std::runtime_error("warm-start MLP: the world's LDS plus the MLP tile exceeds 160 KiB");
     if (queue_grid_ > 0u)
@@ -498,6 +516,8 @@ void World::set_mlp_capture(bool enabled) {
             throw std::runtime_error("MLP sample capture: the world's LDS plus the MLP tile exceeds 160 KiB");
         if (cfg_.ccd_mode != MR_CCD_SPECULATIVE)
             throw std::runtime_error("MLP sample capture: not available in the hybrid CCD mode");
+        if (geometry_)
+            throw std::runtime_error("MLP sample capture: not available in worlds with convex or mesh shapes");
         if (queue_grid_ > 0u)
             throw std::runtime_error("MLP sample capture: runs one workgroup per environment; clear the work-queue "
                                      "schedule first (mr_world_set_schedule with grid 0)");
@@ -609,6 +629,8 @@ void World::set_schedule(const uint32_t* order, uint32_t count, uint32_t grid) {
     if (mlp_instance()) throw std::runtime_error("set_schedule: the learned warm start runs one workgroup per environment");
     if (cfg_.ccd_mode != MR_CCD_SPECULATIVE)
         throw std::runtime_error("set_schedule: the hybrid CCD mode runs one workgroup per environment");
+    if (geometry_)
+        throw std::runtime_error("set_schedule: worlds with convex or mesh shapes run one workgroup per environment");
     std::vector<uint8_t> seen(env_, 0u);
     for (uint32_t i = 0; i < count; ++i) {
         if (order[i] >= env_) throw std::runtime_error("set_schedule: environment index out of range");

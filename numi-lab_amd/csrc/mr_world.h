@@ -195,6 +195,8 @@ private:
     hipEvent_t ev_begin_ = nullptr, ev_end_ = nullptr;
     // model
     DeviceBuffer d_models_, d_env_model_, d_slot_plan_, d_bodies_, d_dofs_, d_actuators_, d_shapes_, d_materials_, d_pairs_, d_heights_;
+    DeviceBuffer d_geometries_, d_geometry_vertices_, d_triangles_, d_bvh_nodes_;
+    bool geometry_ = false;  // some shape is a convex hull or a triangle mesh
     // state
     DeviceBuffer d_q_, d_v_, d_scene_, d_mh_, d_mp_, d_mcount_, d_mcurrent_;
     DeviceBuffer d_reset_q_, d_reset_v_, d_reset_scene_;

@@ -71,6 +71,9 @@ class WorldModelDescC(ct.Structure):
         ("material_total", ct.c_uint32), ("pair_total", ct.c_uint32), ("heightfield_vertex_count", ct.c_uint32),
         ("heightfield", HeightfieldC), ("heightfield_heights", ct.c_void_p),
         ("actuators", ct.c_void_p), ("actuator_total", ct.c_uint32), ("reserved_actuators", ct.c_uint32),
+        ("geometries", ct.c_void_p), ("geometry_vertices", ct.c_void_p), ("triangles", ct.c_void_p),
+        ("bvh_nodes", ct.c_void_p), ("geometry_total", ct.c_uint32), ("geometry_vertex_total", ct.c_uint32),
+        ("triangle_total", ct.c_uint32), ("bvh_node_total", ct.c_uint32),
     ]
 
 
@@ -308,9 +311,16 @@ DOF_DTYPE: Final = np.dtype([("body", "<u4"), ("q_index", "<u4"), ("flags", "<u4
                              ("limits", *F4), ("drive", *F4)])
 SHAPE_DTYPE: Final = np.dtype([
     ("body", "<u4"), ("shape_type", "<u4"), ("material", "<u4"), ("flags", "<u4"), ("collision_group", "<u4"),
-    ("collision_mask", "<u4"), ("slot_generation", "<u4"), ("reserved", "<u4"), ("local_position", *F4),
+    ("collision_mask", "<u4"), ("slot_generation", "<u4"), ("geometry", "<u4"), ("local_position", *F4),
     ("local_rotation", *F4), ("dimensions", *F4), ("contact_rest_radius", *F4)])
 MATERIAL_DTYPE: Final = np.dtype([("friction", *F4), ("response", *F4)])
+GEOMETRY_DTYPE: Final = np.dtype([(n, "<u4") for n in (
+    "kind", "flags", "vertex_offset", "vertex_count", "triangle_offset", "triangle_count", "node_offset",
+    "node_count")] + [("local_lower", *F4), ("local_upper", *F4)])
+TRIANGLE_DTYPE: Final = np.dtype([("vertices", "<u4", (3,)), ("feature", "<u4")])
+BVH_NODE_DTYPE: Final = np.dtype([(n, *F4) for n in ("lower_x", "lower_y", "lower_z", "upper_x", "upper_y",
+                                                     "upper_z")] + [("child", "<u4", (4,)), ("count", "<u4", (4,))])
+assert GEOMETRY_DTYPE.itemsize == 64 and TRIANGLE_DTYPE.itemsize == 16 and BVH_NODE_DTYPE.itemsize == 128
 PAIR_DTYPE: Final = np.dtype([("collider_a", "<u4"), ("collider_b", "<u4"), ("pair_class", "<u4"),
                               ("reserved", "<u4")])
 BODY_STATE_DTYPE: Final = np.dtype([
@@ -366,7 +376,7 @@ _EXPECTED_SIZES: Final = {
 for _dt, _size in _EXPECTED_SIZES.items():
     assert _dt.itemsize == _size, (_dt, _dt.itemsize, _size)
 assert ct.sizeof(WorldStepConfigC) == 120
-assert ct.sizeof(WorldModelDescC) == 216
+assert ct.sizeof(WorldModelDescC) == 264
 assert ct.sizeof(WorldAdvanceC) == 40
 assert ct.sizeof(WorldLayoutC) == 40
 assert ct.sizeof(MlpWeightsC) == 80

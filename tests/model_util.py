@@ -37,7 +37,8 @@ def body(mass, inertia, motion=2, parent=INVALID, joint_type=15, q_index=INVALID
     return b
 
 
-def shape(body_index, shape_type, dims, pos=(0, 0, 0), rot=(0, 0, 0, 1), contact_offset=0.01, material=0):
+def shape(body_index, shape_type, dims, pos=(0, 0, 0), rot=(0, 0, 0, 1), contact_offset=0.01, material=0,
+          geometry=0, flags=0):
     s = np.zeros(1, nat.SHAPE_DTYPE)[0]
     s["body"] = body_index
     s["shape_type"] = shape_type
@@ -48,7 +49,31 @@ def shape(body_index, shape_type, dims, pos=(0, 0, 0), rot=(0, 0, 0, 1), contact
     s["local_rotation"] = rot
     s["dimensions"][:len(dims)] = dims
     s["contact_rest_radius"] = [contact_offset, 0, 0, 0]
+    s["geometry"] = geometry
+    s["flags"] = flags
     return s
+
+
+def geometry_tables(geoms):
+    """(geometries, vertices, triangles) arrays from [(kind, vertices [n][3], triangles [m][3] or None)];
+    triangle feature ids are their indices."""
+    g = np.zeros(len(geoms), nat.GEOMETRY_DTYPE)
+    verts, tris = [], []
+    for k, (kind, vs, ts) in enumerate(geoms):
+        g[k]["kind"] = kind
+        g[k]["vertex_offset"] = len(verts)
+        g[k]["vertex_count"] = len(vs)
+        verts += [[*v, 0.0] for v in vs]
+        if ts is not None:
+            g[k]["triangle_offset"] = len(tris)
+            g[k]["triangle_count"] = len(ts)
+            tris += [(tuple(t), i) for i, t in enumerate(ts)]
+    vt = np.array(verts, np.float32).reshape(-1, 4)
+    tr = np.zeros(len(tris), nat.TRIANGLE_DTYPE)
+    for i, (t, f) in enumerate(tris):
+        tr[i]["vertices"] = t
+        tr[i]["feature"] = f
+    return g, vt, tr
 
 
 def dof(body_index, q_index, flags=0, limits=(0, 0, 0, 0), drive=(0, 0, 0, 0)):
@@ -76,6 +101,10 @@ def pair_class(a, b):
     table = {(0, 0): 1, (0, 3): 2, (1, 3): 3, (2, 3): 4, (0, 1): 6, (1, 1): 7, (0, 2): 8, (1, 2): 9, (2, 2): 10}
     if hi == 7 and lo <= 2:
         return 12
+    if (hi == 5 and lo <= 3) or (lo, hi) == (5, 5):
+        return 11  # convex hull
+    if hi == 6 and (lo <= 2 or lo == 5):
+        return 13  # triangle mesh
     return table.get((lo, hi), 0)
 
 
@@ -123,7 +152,7 @@ class CustomWorld:
 
     def __init__(self, bodies, dofs, shapes, link_count, root_type, nq, nv, materials=None, envs=1,
                  gravity=(0, 0, -9.81), microstep=0.005, exclusion_distance=1, heights=None, hf_origin=None,
-                 hf_spacing=None, actuators=None):
+                 hf_spacing=None, actuators=None, geometries=None):
         self.bodies = np.array(bodies, nat.BODY_DTYPE)
         self.dofs = np.array(dofs, nat.DOF_DTYPE) if len(dofs) else np.zeros(1, nat.DOF_DTYPE)
         self.shapes = np.array(shapes, nat.SHAPE_DTYPE) if len(shapes) else np.zeros(1, nat.SHAPE_DTYPE)
@@ -186,6 +215,15 @@ class CustomWorld:
             self.actuators = np.array(actuators, nat.ACTUATOR_DTYPE)
             desc.actuators = self.actuators.ctypes.data
             desc.actuator_total = len(self.actuators)
+        if geometries is not None:
+            self.geometries, self.geometry_vertices, self.triangles = geometry_tables(geometries)
+            desc.geometries = self.geometries.ctypes.data
+            desc.geometry_total = len(self.geometries)
+            desc.geometry_vertices = self.geometry_vertices.ctypes.data
+            desc.geometry_vertex_total = len(self.geometry_vertices)
+            if len(self.triangles):
+                desc.triangles = self.triangles.ctypes.data
+                desc.triangle_total = len(self.triangles)
         self._desc = desc  # kept (with the arrays it points into) for tests that alter a copy
         b = mr.load()
         h = b.lib.mr_world_model_create(ct.byref(desc), microstep)
