@@ -378,9 +378,9 @@ void build_random_trees(ModelData& m, uint32_t E, uint64_t seed) {
     m.max_nq = max_dof;
     m.max_nv = max_dof;
     m.max_scene_bodies = max_free + 1u;
-    // slightly compliant contacts (1e-7 m/N): bounds the impulse a contact can take where its
-    // Delassus block is nearly singular (a point near a fixed joint)
-    m.materials.push_back(MRMaterialC{f4(0.8, 0.8, 0.8, 0.8), f4(0.0, 0.5, 1.0e-7, 0.0)});
+    // rigid contacts: with the links of a tree kept out of contact with each other (the pair
+    // filter below), no contact joins two links of one tree, and no compliance is needed
+    m.materials.push_back(MRMaterialC{f4(0.8, 0.8, 0.8, 0.8), f4(0.0, 0.5, 0.0, 0.0)});
     m.initial_q.assign((size_t)E * m.max_nq, 0.0f);
     m.initial_v.assign((size_t)E * m.max_nv, 0.0f);
     m.initial_scene.resize((size_t)E * m.max_scene_bodies);
@@ -430,7 +430,10 @@ void build_random_trees(ModelData& m, uint32_t E, uint64_t seed) {
             const double w = 2.0 * radius;
             MRBodyC b = make_body(mass, mass * (len[i] * len[i] + w * w) / 12.0, mass * (w * w + w * w) / 12.0,
                                   mass * (len[i] * len[i] + w * w) / 12.0, MR_MOTION_DYNAMIC);
-            b.damping = f4(0.1, 0.05, 0.0, 0.0);  // viscous drag on every link (N s/m, N m s)
+            // viscous drag on every link (N s/m, N m s): bounds the energy the random torques pump
+            // in over long horizons, which would otherwise carry some links past the microstep's
+            // stability limit for the explicit velocity-product terms (h |qdot| of order 1)
+            b.damping = f4(0.1, 0.05, 0.0, 0.0);
             b.parent = parent;
             b.joint_type = MR_JOINT_REVOLUTE;
             b.q_index = i - 1u;
@@ -457,7 +460,9 @@ void build_random_trees(ModelData& m, uint32_t E, uint64_t seed) {
             m.bodies.push_back(b);
             MRDofC d = make_dof(i, i - 1u, MR_DOF_FLAG_ACTUATED | MR_DOF_FLAG_EFFORT_LIMIT);
             d.limits = f4(0.0, 0.0, 0.0, 20.0);
-            d.drive = f4(0.0, 0.0, 0.1, 0.0);  // armature (reflected rotor inertia), kg m^2
+            // armature (reflected rotor inertia), kg m^2: a 0.1 kg distal link alone has about
+            // 1e-4 kg m^2 about its joint, and 1 N m torques would spin it past h |qdot| = 1 rad
+            d.drive = f4(0.0, 0.0, 0.1, 0.0);
             m.dofs.push_back(d);
             m.shapes.push_back(make_shape(i, MR_SHAPE_CAPSULE, 0u, D3{0, 0, 0}, kIdentity,
                                           D3{radius, 0.5 * len[i], 0.0}, 0.01));
@@ -534,6 +539,8 @@ void build_random_trees(ModelData& m, uint32_t E, uint64_t seed) {
             scene_state(D3{0, 0, 0}, kIdentity, plane, links + free_count);
         h.pair_offset = (uint32_t)m.pairs.size();
         std::vector<MRCompiledPairC> pairs;
+        // no contacts between links of one tree: a contact next to a joint inside the tree has a
+        // nearly singular Delassus block, and rigid contacts there fail to factorise or blow up
         compile_pairs(h, &m.bodies[h.body_offset], &m.shapes[h.shape_offset], MR_MAX_LINKS, pairs);
         h.pair_count = (uint32_t)pairs.size();
         m.pairs.insert(m.pairs.end(), pairs.begin(), pairs.end());
