@@ -920,11 +920,18 @@ void compile_geometry(ModelData& m) {
         }
         if ((size_t)g.triangle_offset + g.triangle_count > m.triangles.size())
             throw std::runtime_error(where + "triangles outside the triangle table");
+        // The median splits keep the BVH4 balanced: 2^24 triangles give at most 12 levels,
+        // and a depth-first traversal then holds at most 3 x 12 + 1 = 37 < 48 pending nodes.
+        if (g.triangle_count > (1u << 24))
+            throw std::runtime_error(where + "a mesh holds at most 2^24 triangles");
         std::vector<MRTriangleC> tris(m.triangles.begin() + g.triangle_offset,
                                       m.triangles.begin() + g.triangle_offset + g.triangle_count);
-        for (const MRTriangleC& t : tris)
+        for (const MRTriangleC& t : tris) {
             for (uint32_t v : t.vertices)
                 if (v >= g.vertex_count) throw std::runtime_error(where + "triangle vertex index out of range");
+            // contact features carry 28 bits of local id
+            if (t.feature >= (1u << 28)) throw std::runtime_error(where + "triangle feature ids must be below 2^28");
+        }
         std::vector<MRBvhNodeC> nodes;
         BvhBuilder bb{m.geometry_vertices.data() + g.vertex_offset, tris, nodes};
         bb.build(0u, (uint32_t)tris.size());

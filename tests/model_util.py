@@ -216,7 +216,9 @@ class CustomWorld:
             desc.actuators = self.actuators.ctypes.data
             desc.actuator_total = len(self.actuators)
         if geometries is not None:
-            self.geometries, self.geometry_vertices, self.triangles = geometry_tables(geometries)
+            # a list of (kind, vertices, triangles), or the three tables already built
+            tables = geometries if isinstance(geometries, tuple) else geometry_tables(geometries)
+            self.geometries, self.geometry_vertices, self.triangles = tables
             desc.geometries = self.geometries.ctypes.data
             desc.geometry_total = len(self.geometries)
             desc.geometry_vertices = self.geometry_vertices.ctypes.data

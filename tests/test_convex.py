@@ -418,6 +418,24 @@ DEVICE_PATH_OFF = ("the device convex / mesh narrowphase is switched off: its ke
                    "aperture violation on the first scene (hull_tumble) whose cause is not yet shown")
 
 
+@pytest.mark.parametrize("case", ["few_vertices", "vertex_index", "feature_id", "kind"])
+def test_geometry_compiler_refuses_malformed_geometry(case):
+    geoms = [(CONVEX, box_vertices(HE), None), (MESH, [[-1, -1, 0], [1, -1, 0], [1, 1, 0]], [[0, 1, 2]])]
+    specs = [(CONVEX, [1, 1, 1], (0, 0, 0.2), (0, 0, 0, 1), 0, 0), (MESH, [1, 1, 1], (0, 0, 0), (0, 0, 0, 1), 1, 0)]
+    if case == "few_vertices":
+        geoms[0] = (CONVEX, box_vertices(HE)[:3], None)
+    elif case == "vertex_index":
+        geoms[1] = (MESH, geoms[1][1], [[0, 1, 3]])
+    elif case == "feature_id":
+        g, v, t = mu.geometry_tables(geoms)
+        t[0]["feature"] = 1 << 28
+        geoms = (g, v, t)
+    else:
+        specs[0] = (CONVEX, [1, 1, 1], (0, 0, 0.2), (0, 0, 0, 1), 1, 0)  # a hull naming the mesh geometry
+    with pytest.raises(mr.MetalRoboError):
+        scene(specs, geoms, plane=False)
+
+
 def test_world_creation_refuses_convex_and_mesh_shapes_while_the_device_path_is_off():
     w, _ = scene([(CONVEX, [1, 1, 1], (0.0, 0.0, 0.2), (0, 0, 0, 1), 0, 0)], HULL_BOX)
     with pytest.raises(mr.MetalRoboError, match="switched off"):
