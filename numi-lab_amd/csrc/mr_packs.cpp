@@ -92,6 +92,7 @@ public:
     }
     bool count(uint64_t& n) { return scalar(n) && n <= kMaximumPayload; }
     bool done() const { return at_ == n_; }
+    size_t remaining() const { return n_ - at_; }
 
 private:
     const uint8_t* p_;
@@ -126,7 +127,10 @@ void write_layers(Sink& s, const std::vector<DenseLayer>& layers) {
 
 bool read_layers(Source& s, std::vector<DenseLayer>& layers) {
     uint64_t n = 0;
-    if (!s.count(n)) return false;
+    // a layer takes at least its three counts and two table lengths: bound the
+    // allocation by what the payload can still hold
+    constexpr size_t kLayerMinimumBytes = 3u * sizeof(uint32_t) + 2u * sizeof(uint64_t);
+    if (!s.count(n) || n > s.remaining() / kLayerMinimumBytes) return false;
     layers.resize((size_t)n);
     for (DenseLayer& l : layers)
         if (!(s.scalar(l.input_count) && s.scalar(l.output_count) && s.scalar(l.activation) && s.floats(l.weights) &&

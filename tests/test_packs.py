@@ -176,6 +176,12 @@ def test_policy_pack_reader_rejections(tmp_path):
     }
     extra = payload + b"\0\0\0\0"
     cases["trailing"] = (struct.pack("<8sIIQQ", b"MRLEARN\0", 4, 2, len(extra), fnv1a64(extra)) + extra, "malformed")
+    # a valid hash over a layer count no payload could hold (2^31 layers): refused before allocating
+    at = 8 + struct.unpack_from("<Q", payload, 0)[0] + 8  # id, revision
+    for _ in range(2):  # observation mean and inverse std
+        at += 8 + 4 * struct.unpack_from("<Q", payload, at)[0]
+    huge = payload[:at] + struct.pack("<Q", 1 << 31) + payload[at + 8:]
+    cases["layer_count"] = (struct.pack("<8sIIQQ", b"MRLEARN\0", 4, 2, len(huge), fnv1a64(huge)) + huge, "malformed")
     for name, (data, match) in cases.items():
         path = tmp_path / f"{name}.policypack"
         path.write_bytes(data)
