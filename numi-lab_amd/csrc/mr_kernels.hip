@@ -2743,14 +2743,32 @@ MRD_INL V3 scene_cross_response(const MRBodyStateC& b, V3 xc, V3 xj, V3 d) {
     return add(mul(b.linear_velocity_inv_mass.w, d), cross(dw, sub(xc, f3(b.position))));
 }
 
+// sum_k x[k] * y[k] added in index order (the order the oracle uses), with
+// the loads of eight terms issued before their products are added, so that
+// their memory latencies overlap instead of following one another.
+MRD_INL float serial_dot(const float* x, const float* y, uint32_t n) {
+    float a = 0.0f;
+    uint32_t k = 0;
+    for (; k + 8u <= n; k += 8u) {
+        float xs[8], ys[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            xs[i] = x[k + i];
+            ys[i] = y[k + i];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a = a + xs[i] * ys[i];
+    }
+    for (; k < n; ++k) a = a + x[k] * y[k];
+    return a;
+}
+
 // W_nn(c, j): normal velocity at contact c per unit normal impulse of contact j.
 MRD_INL float jacobi_normal_cross(const Env& e, uint32_t c, uint32_t j, uint32_t mnv) {
     const uint32_t pc = e.cpair[c], pj = e.cpair[j];
     float w = 0.0f;
     if (((pc >> 16) & mr::kHotArticulated) != 0u && ((pj >> 16) & mr::kHotArticulated) != 0u) {
-        const float* x = e.jrow + (size_t)c * 3u * mnv;
-        const float* y = e.resp + (size_t)j * 3u * mnv;
-        for (uint32_t k = 0; k < e.nv; ++k) w = w + x[k] * y[k];
+        w = serial_dot(e.jrow + (size_t)c * 3u * mnv, e.resp + (size_t)j * 3u * mnv, e.nv);
     }
     const uint32_t cb[2] = {pc & 0xffu, (pc >> 8) & 0xffu}, jb[2] = {pj & 0xffu, (pj >> 8) & 0xffu};
     for (int x = 0; x < 2; ++x) {
@@ -3897,11 +3915,7 @@ __device__ __forceinline__ void control_step(const KernelArgs& args, uint32_t en
                         float rel[3] = {0.0f, 0.0f, 0.0f};
                         if ((kind & mr::kHotArticulated) != 0u) {
                             const float* jb = e.jrow + (size_t)c * 3u * mnv;
-                            for (int r = 0; r < 3; ++r) {
-                                float a = 0.0f;
-                                for (uint32_t k = 0; k < nv; ++k) a = a + jb[(size_t)r * mnv + k] * e.vcand[k];
-                                rel[r] = a;
-                            }
+                            for (int r = 0; r < 3; ++r) rel[r] = serial_dot(jb + (size_t)r * mnv, e.vcand, nv);
                         }
                         if ((kind & (mr::kHotDynamicA | mr::kHotDynamicB)) != 0u) {
                             V3 va, vb;
