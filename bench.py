@@ -297,6 +297,140 @@ STATUS_NAMES = {1: "fixed_budget_complete", 2: "nonfinite_input", 3: "nonfinite_
                 11: "did_not_converge", 12: "unsupported", 13: "ccd_capacity_overflow"}
 
 
+def task_leg(mr, args):
+    """The task layer either side of the step, driven by a device policy: 4096 humanoid worlds run
+    observe -> dense-layer policy (actor 256-128 ELU, critic of the same shape, stochastic head) ->
+    action sampling and scoring -> apply_actions -> control step -> complete, all on the world's
+    stream. The synthetic policy has random weights of that shape. The same worlds stepped with
+    plain PD targets and no task give the layer's cost."""
+    import torch
+    from metalrobo import packs as pk
+    from metalrobo import task as tk
+
+    E, K, W = args.task_envs, args.task_steps, 5
+    model = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_HUMANOID_TERRAIN, E, args.seed)
+    cfg = model.default_config()
+    world = mr.World(model, cfg)
+    prog = tk.humanoid_locomotion_program(model, seed=args.seed)
+    task = mr.LocomotionTask(world, prog)
+    O, A = task.history_length * task.observation_count, task.action_count
+    rng = np.random.default_rng(args.seed)
+
+    def layer(i, o, act):
+        return pk.DenseLayer((rng.standard_normal((o, i)) / np.sqrt(i)).astype(np.float32),
+                             (0.05 * rng.standard_normal(o)).astype(np.float32), act)
+
+    elu, ident = pk.ACTIVATION_ELU, pk.ACTIVATION_IDENTITY
+    pack = pk.PolicyPack(
+        id="bench_policy", revision=1, contract=(1, 1, 2, 3, 4),
+        layers=[layer(O, 256, elu), layer(256, 128, elu), layer(128, A, ident)],
+        observation_mean=np.zeros(O, np.float32), observation_inverse_std=np.ones(O, np.float32),
+        critic_layers=[layer(O, 256, elu), layer(256, 128, elu), layer(128, 1, ident)],
+        action_log_std=np.full(A, -1.0, np.float32), action_bias=np.zeros(A, np.float32),
+        action_scale=np.ones(A, np.float32), observation_clip=5.0, action_clip=1.0)
+    task.set_policy(pack)
+    for _ in range(W):
+        task.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        task.step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    _, tr, _ = task.read()
+    # the task enqueues its control steps itself: read the last step's statuses through the C ABI
+    from metalrobo import native as nat
+    stat = np.zeros(E, nat.WORLD_STATUS_DTYPE)
+    mr.load().check(mr.load().lib.mr_world_read_statuses(world._h, stat.ctypes.data, stat.size))
+    task.close()
+    world.close()
+    plain = mr.World(model, cfg)
+    plain.set_step_timing(True)
+    commands = torch.from_numpy(model.commands(0, W + K)).cuda()
+    plain.enqueue_device(commands[:W])
+    plain.wait()
+    run = timed_run(plain, commands, W, K, E)
+    plain.close()
+    return {"workload": "humanoid29_heightfield locomotion task with a device policy", "worlds": E, "steps": K,
+            "policy": {"actor": [O, 256, 128, A], "critic": [O, 256, 128, 1], "activation": "elu",
+                       "stochastic": True, "weights": "random (synthetic)"},
+            "env_steps_per_s": round(E * K / dt, 1), "ms_per_step": round(1000.0 * dt / K, 4),
+            "world_only_env_steps_per_s": round(run["env_steps_per_s"], 1),
+            "task_and_policy_share_of_step": round(1.0 - (E * K / dt) / run["env_steps_per_s"], 4),
+            "failed_env_steps_last_step": int((stat["code"] != 0).sum()),
+            "terminated_last_step": int((tr["termination"] != 0).sum())}
+
+
+def ccd_leg(mr, args):
+    """Hybrid continuous collision on the chain-and-boxes worlds with every free box CCD-enabled and
+    thrown at the ground (horizontal U(-8, 8) m/s, downward U(20, 30) m/s): the same control steps
+    from the same state in the speculative and the hybrid mode. Reports throughput, the impact
+    segments the hybrid mode splits microsteps at, status codes, and the deepest penetration seen.
+    The hybrid mode transfers the thrown boxes' momentum into the light, undamped chain links that
+    the speculative mode lets them pass through; a few chains are then spun past the microstep's
+    stability limit for explicit velocity terms (the workload's own failure mechanism), and those
+    worlds fail non-finite and roll back."""
+    import ctypes as ct
+
+    import torch
+    from metalrobo import native as nat
+
+    E, K = args.ccd_envs, args.ccd_steps
+    base = mr.WorldModel.synthetic(mr.MR_SYNTHETIC_CHAIN_BOXES, E, args.seed)
+    bcfg = base.default_config()
+    shapes = base.shapes().copy()
+    boxes = shapes["shape_type"] == 2
+    shapes["flags"][boxes] |= 2  # MR_SHAPE_FLAG_ENABLE_CCD
+    desc = nat.WorldModelDescC.from_buffer_copy(base.desc)
+    desc.shapes = shapes.ctypes.data
+    b = mr.load()
+    h = b.lib.mr_world_model_create(ct.byref(desc), bcfg.control_timestep / bcfg.physics_substeps)
+    if not h:
+        raise mr.MetalRoboError(b.last_error())
+    model = mr.WorldModel(h, b)
+    q, v, scene = base.initial_state()
+    rng = np.random.default_rng(args.seed)
+    free = scene["flags_indices"][..., 0] == 2  # dynamic scene bodies
+    n = int(free.sum())
+    lin = scene["linear_velocity_inv_mass"]
+    lin[free, 0] = rng.uniform(-8.0, 8.0, n)
+    lin[free, 1] = rng.uniform(-8.0, 8.0, n)
+    lin[free, 2] = -rng.uniform(20.0, 30.0, n)
+    scene["linear_velocity_inv_mass"] = lin
+    commands = torch.from_numpy(base.commands(0, K)).cuda()
+    out = {"workload": "chain7_boxes20_plane, boxes CCD-enabled and thrown at 20-30 m/s", "worlds": E, "steps": K}
+    for mode, name in ((0, "speculative"), (1, "hybrid")):
+        cfg = bcfg.copy()
+        cfg.ccd_mode = mode
+        cfg.max_ccd_events = 8
+        cfg.max_ccd_iterations = 16
+        cfg.ccd_distance_tolerance = 1.0e-3
+        cfg.ccd_time_tolerance = 1.0e-5
+        cfg.ccd_minimum_advance = 1.0e-6
+        world = mr.World(model, cfg)
+        world.set_state(q, v, scene)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        impacts, codes, pen = 0.0, np.zeros(16, np.int64), 0.0
+        for k in range(K):
+            world.enqueue_device(commands[k:k + 1])
+            world.wait()
+            cs, st = world.contact_statuses()[0], world.statuses()[0]
+            impacts += float(cs["diagnostics"][:, 3].sum())
+            ok = st["code"] == 0  # a failed step's contact status describes a rolled-back state
+            if ok.any():
+                pen = max(pen, float(cs["diagnostics"][ok, 0].max()))
+            codes += np.bincount(st["code"], minlength=16)[:16]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        world.close()
+        out[name] = {"env_steps_per_s": round(E * K / dt, 1), "ms_per_step": round(1000.0 * dt / K, 3),
+                     "impact_segments_per_env_step": round(impacts / (E * K), 4),
+                     "max_penetration_m_successful_steps": round(pen, 5),
+                     "status_codes": {str(c): int(x) for c, x in enumerate(codes) if x}, "thrown_boxes": n}
+    return out
+
+
 def server_leg(mr, args):
     """Step server on the 29-DoF humanoid: Poisson arrivals at 200 requests/s, N log-uniform in
     [1, 1024] worlds, horizons 10-500 control steps, a resident pool admitted by a 200 GB HBM
@@ -368,6 +502,12 @@ def main():
     ap.add_argument("--skip-comparisons", action="store_true",
                     help="time only the headline path (no plain-at-residual or fixed-budget legs)")
     ap.add_argument("--skip-server", action="store_true", help="skip the step-server leg")
+    ap.add_argument("--skip-task", action="store_true", help="skip the policy-driven task-layer leg")
+    ap.add_argument("--task-envs", type=int, default=4096)
+    ap.add_argument("--task-steps", type=int, default=50)
+    ap.add_argument("--skip-ccd", action="store_true", help="skip the hybrid continuous-collision leg")
+    ap.add_argument("--ccd-envs", type=int, default=4096)
+    ap.add_argument("--ccd-steps", type=int, default=20)
     ap.add_argument("--serve-pool", type=int, default=16384)
     ap.add_argument("--serve-requests", type=int, default=96)
     ap.add_argument("--serve-serial-requests", type=int, default=16)
@@ -562,6 +702,10 @@ def main():
         out["server"] = server_leg(mr, args)
     if not args.skip_warm_start:
         out["warm_start"] = warm_start_leg(mr, args)
+    if not args.skip_task:
+        out["task"] = task_leg(mr, args)
+    if not args.skip_ccd:
+        out["ccd"] = ccd_leg(mr, args)
     if not args.skip_cpu_baseline:
         threads, how = host_cores()
         if args.cpu_threads:

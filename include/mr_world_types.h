@@ -385,7 +385,10 @@ typedef struct MRManifoldHeaderC {
 typedef struct MRManifoldPointC {
     mr_float4 local_anchor_a;
     mr_float4 local_anchor_b;
-    mr_float4 impulses;          /* normal, tangent u, tangent v, unused */
+    /* normal, tangent u, tangent v; w: 0, or in the hybrid CCD mode the duration (s)
+     * of the segment whose solve produced them (a warm start scales them by the
+     * next segment's duration over it) */
+    mr_float4 impulses;
     uint32_t feature_and_life[4]; /* feature A, feature B, lifetime, flags */
 } MRManifoldPointC;
 

@@ -2362,11 +2362,14 @@ MRD_INL uint32_t ccd_pair(const Env& e, uint32_t k, float T, float& toi) {
             !ccd_separation(pr.collider_a, pr.collider_b, pr.pair_class, a, b, d))
             return speculative_safe ? kCcdNone : kCcdUnsafe;
     }
-    if (d <= tol) {
+    if (d <= tol || !speculative_safe) {
+        // Within the tolerance: the time of impact. Otherwise the iterations ran
+        // out with the pair separated up to alpha T: the segment ends there and
+        // the pair is queried again from that state.
         toi = alpha * T;
         return kCcdImpact;
     }
-    return speculative_safe ? kCcdNone : kCcdUnsafe;
+    return kCcdNone;
 }
 
 // --------------------------------------------------------------- manifold
@@ -3709,9 +3712,18 @@ __device__ __forceinline__ void control_step(const KernelArgs& args, uint32_t en
                 ck.compliance = ma.response.z + mb.response.z;
                 ck.dissipation = ma.response.w + mb.response.w;
                 const bool warm = cfg.warm_start != 0u;
-                e.lam[3u * c + 0u] = warm ? mp.impulses.x : 0.0f;
-                e.lam[3u * c + 1u] = warm ? mp.impulses.y : 0.0f;
-                e.lam[3u * c + 2u] = warm ? mp.impulses.z : 0.0f;
+                float wi[3] = {mp.impulses.x, mp.impulses.y, mp.impulses.z};
+                if constexpr (kCcd) {
+                    // impulses scale with the step: a segment warm-starts from the last
+                    // segment's impulses times the ratio of the two durations
+                    if (mp.impulses.w > 0.0f) {
+                        const float s = h / mp.impulses.w;
+                        for (int r = 0; r < 3; ++r) wi[r] = wi[r] * s;
+                    }
+                }
+                e.lam[3u * c + 0u] = warm ? wi[0] : 0.0f;
+                e.lam[3u * c + 1u] = warm ? wi[1] : 0.0f;
+                e.lam[3u * c + 2u] = warm ? wi[2] : 0.0f;
                 e.cpair[c] = sa.body | (sb.body << 8) | (hk.kind << 16);
                 e.hot[c] = hk;
                 e.cold[c] = ck;
@@ -4206,7 +4218,7 @@ __device__ __forceinline__ void control_step(const KernelArgs& args, uint32_t en
             res_c[c] = viol;
             const ColdConstraint& ck = e.cold[c];
             new_p[(size_t)ck.manifold * MR_MANIFOLD_POINT_CAPACITY + ck.point].impulses =
-                mr_float4{lamr[0], lamr[1], lamr[2], 0.0f};
+                mr_float4{lamr[0], lamr[1], lamr[2], kCcd ? h : 0.0f};
         }
         __syncthreads();
         // Pass 3, uniform: maxima in constraint order (the oracle's order,

@@ -2678,11 +2678,14 @@ uint32_t ccd_pair(const StepContext& ctx, const Work& wk, const EnvState& st, ui
             !ccd_separation(pr.collider_a, pr.collider_b, pr.pair_class, a, b, d))
             return speculative_safe ? kCcdNone : kCcdUnsafe;
     }
-    if (d <= tol) {
+    if (d <= tol || !speculative_safe) {
+        // Within the tolerance: the time of impact. Otherwise the iterations ran
+        // out with the pair separated up to alpha T: the segment ends there and
+        // the pair is queried again from that state.
         toi = alpha * T;
         return kCcdImpact;
     }
-    return speculative_safe ? kCcdNone : kCcdUnsafe;
+    return kCcdNone;
 }
 
 // The segment the hybrid mode runs next from the current state: its duration
@@ -2948,7 +2951,14 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
             c.restitution_threshold = fmaxd(ma.response.y, mb.response.y);
             c.compliance = ma.response.z + mb.response.z;
             c.dissipation = ma.response.w + mb.response.w;
-            if (ctx.cfg->warm_start != 0u) {
+            if (ctx.cfg->warm_start != 0u && ctx.cfg->ccd_mode == MR_CCD_HYBRID && mp.impulses.w > 0.0f) {
+                // impulses scale with the step: a segment warm-starts from the last
+                // segment's impulses times the ratio of the two durations
+                const float s = h / mp.impulses.w;
+                c.lambda[0] = mp.impulses.x * s;
+                c.lambda[1] = mp.impulses.y * s;
+                c.lambda[2] = mp.impulses.z * s;
+            } else if (ctx.cfg->warm_start != 0u) {
                 c.lambda[0] = mp.impulses.x;
                 c.lambda[1] = mp.impulses.y;
                 c.lambda[2] = mp.impulses.z;
@@ -3352,7 +3362,7 @@ bool microstep(const StepContext& ctx, EnvState& st, const float* command, bool 
             viol = std::sqrt(k.lambda[1] * k.lambda[1] + k.lambda[2] * k.lambda[2]);
         max_cone = fmaxd(max_cone, viol);
         MRManifoldPointC& mp = new_p[k.manifold * MR_MANIFOLD_POINT_CAPACITY + k.point];
-        mp.impulses = mr_float4{k.lambda[0], k.lambda[1], k.lambda[2], 0.0f};
+        mp.impulses = mr_float4{k.lambda[0], k.lambda[1], k.lambda[2], ctx.cfg->ccd_mode == MR_CCD_HYBRID ? h : 0.0f};
     }
     if (!std::isfinite(sweep_delta) || !std::isfinite(max_normal) || !std::isfinite(max_cone)) {
         f.code = MR_STEP_NONFINITE_RESULT;

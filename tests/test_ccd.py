@@ -154,6 +154,54 @@ def test_event_budget():
     assert float(st.scene[0, 0]["position"][0]) < 0.5 and float(st.scene[0, 1]["position"][0]) > -0.6
 
 
+def spinning_bullet():
+    """A 4 cm box spinning at 300 rad/s while flying at 60 m/s at the wall: its
+    rotation makes the closing-speed bound conservative, so a time-of-impact
+    query advances in several steps."""
+    w = scene_world([(BOX, [0.02, 0.02, 0.02], 0.1)], [(BOX, [0.005, 0.5, 0.5], (0.5, 0, 0), (0, 0, 0, 1))])
+    return w, [mover((0.0, 0, 0), (60.0, 0, 0), 0.1, ang=(0.0, 0.0, 300.0)), static((0.5, 0, 0))]
+
+
+def test_an_unresolved_pair_splits_the_microstep_instead_of_failing():
+    """With one conservative-advancement iteration per query the spinning
+    bullet's pair is not resolved within its query (which used to fail the step
+    with MR_STEP_DID_NOT_CONVERGE). The query still proves the pair separated up
+    to the time it advanced, so the microstep ends there; from that state the
+    pair is within its contact distance and the segment's speculative contacts
+    stop the bullet at the wall."""
+    w, states = spinning_bullet()
+    for iterations in (16, 1):
+        st, out = run_oracle(w, ccd_config(w, max_ccd_iterations=iterations, max_ccd_events=16), states)
+        assert out[0][0]["code"][0] == 0, iterations
+        assert out[0][1]["diagnostics"][0, 3] >= 1.0  # the microstep was split
+        x = float(st.scene[0, 0]["position"][0])
+        assert 0.5 - 0.005 - 0.04 < x < 0.5, (iterations, x)  # at the wall's near face, not through it
+
+
+def test_hybrid_impulses_record_their_segment_duration():
+    """A manifold's impulses come from one segment's solve; in the hybrid mode
+    the point records that segment's duration (impulses.w), and the next
+    segment's warm start scales them by its own duration over it. The
+    speculative mode keeps w = 0 and unscaled warm starts."""
+    w, states = bullet_and_wall()
+    cfg = ccd_config(w)
+    st, out = run_oracle(w, cfg, states)
+    mh = st.mh[0, :int(st.mcount[0])]
+    pts = st.mp[0, :int(st.mcount[0])]
+    live = np.arange(4)[None, :] < mh["pair_and_count"][:, 3:4]
+    assert live.any()
+    h = cfg.control_timestep / cfg.physics_substeps
+    wv = pts["impulses"][..., 3][live]
+    assert (wv > 0.0).all() and (wv <= h * (1.0 + 1e-6)).all(), wv
+    # speculative mode: a slow sphere that comes to rest against the wall
+    slow = [mover((0.5 - 0.005 - 0.02 - 0.002, 0, 0), (2.0, 0, 0), 0.1), static((0.5, 0, 0))]
+    st, out = run_oracle(w, w.config(), slow)
+    mh = st.mh[0, :int(st.mcount[0])]
+    live = np.arange(4)[None, :] < mh["pair_and_count"][:, 3:4]
+    assert live.any()
+    assert (st.mp[0, :int(st.mcount[0])]["impulses"][..., 3][live] == 0.0).all()
+
+
 def heightfield_world(ccd=True):
     heights = np.zeros((8, 8), np.float32)
     w = scene_world([(SPHERE, [0.05], 1.0)], [(HEIGHTFIELD, [0, 0, 0], (0, 0, 0), (0, 0, 0, 1))], ccd=ccd,
