@@ -411,22 +411,24 @@ def ccd_leg(mr, args):
         world.set_state(q, v, scene)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        impacts, codes, pen = 0.0, np.zeros(16, np.int64), 0.0
+        impacts, codes, pens = 0.0, np.zeros(16, np.int64), []
         for k in range(K):
             world.enqueue_device(commands[k:k + 1])
             world.wait()
             cs, st = world.contact_statuses()[0], world.statuses()[0]
             impacts += float(cs["diagnostics"][:, 3].sum())
             ok = st["code"] == 0  # a failed step's contact status describes a rolled-back state
-            if ok.any():
-                pen = max(pen, float(cs["diagnostics"][ok, 0].max()))
+            pens.append(cs["diagnostics"][ok, 0])
             codes += np.bincount(st["code"], minlength=16)[:16]
+        pens = np.concatenate(pens)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         world.close()
         out[name] = {"env_steps_per_s": round(E * K / dt, 1), "ms_per_step": round(1000.0 * dt / K, 3),
                      "impact_segments_per_env_step": round(impacts / (E * K), 4),
-                     "max_penetration_m_successful_steps": round(pen, 5),
+                     "penetration_m_successful_steps": {"p50": round(float(np.percentile(pens, 50)), 5),
+                                                        "p99": round(float(np.percentile(pens, 99)), 5),
+                                                        "max": round(float(pens.max()), 5)},
                      "status_codes": {str(c): int(x) for c, x in enumerate(codes) if x}, "thrown_boxes": n}
     return out
 
