@@ -294,10 +294,14 @@ def gpu_matches(w, cfg, states, steps):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["bullet", "head_on", "budget", "heightfield"])
+@pytest.mark.parametrize("name", ["bullet", "head_on", "budget", "heightfield", "split"])
 def test_gpu_ccd_scenes_match_oracle(gpu, name):
     print(f"ccd scene {name}", flush=True)
-    if name == "bullet":
+    if name == "split":
+        # one advancement iteration: the query ends the segment at the time it proved safe
+        w, states = spinning_bullet()
+        cfg = ccd_config(w, max_ccd_iterations=1, max_ccd_events=16)
+    elif name == "bullet":
         w, states = bullet_and_wall()
         cfg = ccd_config(w)
     elif name == "head_on":
