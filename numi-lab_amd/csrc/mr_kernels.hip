@@ -4545,8 +4545,10 @@ mr_control_step_mlp_kernel(KernelArgs args) {
 
 // The hybrid CCD mode: one workgroup per environment, microsteps split at
 // times of impact. A separate instance, so the speculative kernels carry none
-// of its registers.
-extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+// of its registers. Two wavefronts per SIMD: at one, the compiler moves state
+// into AGPRs instead of spilling, and the chain7+boxes hybrid workload ran
+// 19 % slower (502k against 597k env-steps/s at 4096 worlds).
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 mr_control_step_ccd_kernel(KernelArgs args) {
     control_step<false, true>(args, blockIdx.x);
 }
