@@ -1,16 +1,16 @@
 // Locomotion task layer and policy inference on gfx950: the kernels either
 // side of the physics control step.
 //
-// Reference (semantics restated, code this build's own):
+// Reference (semantics restated; the code is written independently):
 //   mr_task_apply_actions_kernel    src/metal/LocomotionTask.metal:3442-3674
 //   mr_task_complete_kernel         src/metal/LocomotionTask.metal:3904-6746 (locomotion subset)
 //   mr_task_observe_kernel          src/metal/LocomotionTask.metal:2184-3440, 594-1330 (locomotion subset)
-//   mr_policy_dense_kernel          src/metal/PolicyInference.metal:78-170
+//   mr_policy_dense_tile_kernel     src/metal/PolicyInference.metal:78-170
 //   mr_policy_sample_kernel         src/metal/PolicyInference.metal:172-268
 //
 // The task kernels run one thread per environment (a few hundred FLOPs per
 // environment each, beside the physics step's millions); the dense layer
-// runs one thread per (environment, neuron) with the weights read through L2.
+// runs register-tiled over environments x neurons with LDS-staged operands.
 
 #include "mr_device_math.h"
 #include "mr_task_args.h"
@@ -517,25 +517,77 @@ __device__ __forceinline__ float policy_normal(unsigned long long key, uint32_t 
 
 } // namespace
 
-// One thread per (environment, output neuron); the features are accumulated
-// as a chain of fused multiply-adds in feature order, starting from the bias.
-extern "C" __global__ void __launch_bounds__(256)
-mr_policy_dense_kernel(const float* weights, const float* bias, const float* mean, const float* inverse_std,
-                       float clip, const float* input, uint32_t input_stride, float* output, uint32_t output_stride,
-                       uint32_t environments, uint32_t in_width, uint32_t out_width, uint32_t act) {
-    const uint64_t flat = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (flat >= (uint64_t)environments * out_width) return;
-    const uint32_t env = (uint32_t)(flat / out_width);
-    const uint32_t neuron = (uint32_t)(flat - (uint64_t)env * out_width);
-    const float* x = input + (size_t)env * input_stride;
-    const float* w = weights + (size_t)neuron * in_width;
-    float value = bias[neuron];
-    for (uint32_t f = 0; f < in_width; ++f) {
-        float sample = x[f];
-        if (mean != nullptr) sample = fminf(fmaxf((sample - mean[f]) * inverse_std[f], -clip), clip);
-        value = fmaf(w[f], sample, value);
+// Dense layer over a tile of environments x output neurons per workgroup of
+// 256 threads, each thread holding a 4 x 4 block of outputs in registers.
+// The features pass in chunks of 16: the chunk's normalised inputs and its
+// weights are staged in LDS once per workgroup (transposed, so a thread reads
+// its four inputs and four weights of one feature as two 16-byte loads).
+// Every output is still one chain of fused multiply-adds in feature order,
+// starting from the bias, so the result does not depend on the tile shape.
+constexpr uint32_t kDenseChunk = 16u;
+
+template <uint32_t kNeurons>
+__global__ void __launch_bounds__(256)
+mr_policy_dense_tile_kernel(const float* weights, const float* bias, const float* mean, const float* inverse_std,
+                            float clip, const float* input, uint32_t input_stride, float* output,
+                            uint32_t output_stride, uint32_t environments, uint32_t in_width, uint32_t out_width,
+                            uint32_t act) {
+    constexpr uint32_t kEnvs = 4096u / kNeurons;        // (kEnvs / 4) * (kNeurons / 4) = 256 threads
+    constexpr uint32_t kPad = 4u;                       // keeps rows 16-byte aligned, spreads the stores
+    __shared__ __align__(16) float xs[kDenseChunk][kEnvs + kPad];
+    __shared__ __align__(16) float ws[kDenseChunk][kNeurons + kPad];
+    const uint32_t t = threadIdx.x;
+    const uint32_t env0 = blockIdx.x * kEnvs;
+    const uint32_t n0 = blockIdx.y * kNeurons;
+    const uint32_t tn = t % (kNeurons / 4u);
+    const uint32_t te = t / (kNeurons / 4u);
+    float acc[4][4];
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) {
+        const uint32_t n = n0 + tn * 4u + j;
+        const float b = n < out_width ? bias[n] : 0.0f;
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i) acc[i][j] = b;
     }
-    output[(size_t)env * output_stride + neuron] = activation(act, value);
+    for (uint32_t f0 = 0; f0 < in_width; f0 += kDenseChunk) {
+        const uint32_t span = min(kDenseChunk, in_width - f0);
+        __syncthreads();
+        for (uint32_t q = t; q < kEnvs * kDenseChunk; q += 256u) {
+            const uint32_t e = q / kDenseChunk, k = q % kDenseChunk;
+            float sample = 0.0f;
+            if (env0 + e < environments && k < span) {
+                sample = input[(size_t)(env0 + e) * input_stride + f0 + k];
+                if (mean != nullptr)
+                    sample = fminf(fmaxf((sample - mean[f0 + k]) * inverse_std[f0 + k], -clip), clip);
+            }
+            xs[k][e] = sample;
+        }
+        for (uint32_t q = t; q < kNeurons * kDenseChunk; q += 256u) {
+            const uint32_t n = q / kDenseChunk, k = q % kDenseChunk;
+            ws[k][n] = (n0 + n < out_width && k < span) ? weights[(size_t)(n0 + n) * in_width + f0 + k] : 0.0f;
+        }
+        __syncthreads();
+        for (uint32_t k = 0; k < span; ++k) {
+            const float4 xv = *reinterpret_cast<const float4*>(&xs[k][te * 4u]);
+            const float4 wv = *reinterpret_cast<const float4*>(&ws[k][tn * 4u]);
+            const float xa[4] = {xv.x, xv.y, xv.z, xv.w};
+            const float wa[4] = {wv.x, wv.y, wv.z, wv.w};
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; ++i)
+#pragma unroll
+                for (uint32_t j = 0; j < 4u; ++j) acc[i][j] = fmaf(wa[j], xa[i], acc[i][j]);
+        }
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < 4u; ++i) {
+        const uint32_t env = env0 + te * 4u + i;
+        if (env >= environments) continue;
+#pragma unroll
+        for (uint32_t j = 0; j < 4u; ++j) {
+            const uint32_t n = n0 + tn * 4u + j;
+            if (n < out_width) output[(size_t)env * output_stride + n] = activation(act, acc[i][j]);
+        }
+    }
 }
 
 // One thread per environment: Gaussian sample of the actor's mean, its log
@@ -592,11 +644,19 @@ hipError_t launch_policy_dense(const float* weights, const float* bias, const fl
                                float clip, const float* input, uint32_t input_stride, float* output,
                                uint32_t output_stride, uint32_t environments, uint32_t in_width, uint32_t out_width,
                                uint32_t act, hipStream_t stream) {
-    const uint64_t n = (uint64_t)environments * out_width;
-    if (n == 0u) return hipSuccess;
-    hipLaunchKernelGGL(mr_policy_dense_kernel, dim3((uint32_t)((n + 255u) / 256u)), dim3(256), 0, stream, weights,
-                       bias, mean, inverse_std, clip, input, input_stride, output, output_stride, environments,
-                       in_width, out_width, act);
+    if (environments == 0u || out_width == 0u) return hipSuccess;
+    // the neuron tile is the smallest of 64 / 32 / 16 / 8 that covers the
+    // layer (or 64), so narrow heads do not run mostly empty tiles
+    const auto launch = [&](auto kernel, uint32_t neurons) {
+        const uint32_t envs = 4096u / neurons;
+        const dim3 grid((environments + envs - 1u) / envs, (out_width + neurons - 1u) / neurons);
+        hipLaunchKernelGGL(kernel, grid, dim3(256), 0, stream, weights, bias, mean, inverse_std, clip, input,
+                           input_stride, output, output_stride, environments, in_width, out_width, act);
+    };
+    if (out_width > 32u) launch(mr_policy_dense_tile_kernel<64u>, 64u);
+    else if (out_width > 16u) launch(mr_policy_dense_tile_kernel<32u>, 32u);
+    else if (out_width > 8u) launch(mr_policy_dense_tile_kernel<16u>, 16u);
+    else launch(mr_policy_dense_tile_kernel<8u>, 8u);
     return hipGetLastError();
 }
 

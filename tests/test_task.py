@@ -334,6 +334,24 @@ def test_gpu_policy_layers_match_oracle(gpu):
     mr.native.policy_dense_layer(dev(w), dev(b), dev(x), out, 0)
     ref = ol.policy_dense(w, b, x, O, 0)
     assert np.array_equal(out.cpu().numpy()[:, :O].view(np.uint32), ref.view(np.uint32))
+    # ragged shapes: every neuron-tile width (64 / 32 / 16 / 8), partial
+    # environment tiles, feature counts off the 16-feature chunk, one feature
+    for e, i, o in ((1, 7, 1), (65, 294, 29), (1100, 128, 128), (513, 17, 9), (300, 33, 64), (70, 1, 3),
+                    (4099, 40, 16), (129, 16, 257)):
+        wr = (rng.standard_normal((o, i)) / math.sqrt(i)).astype(np.float32)
+        br = (0.1 * rng.standard_normal(o)).astype(np.float32)
+        xr = rng.standard_normal((e, i)).astype(np.float32)
+        mr_ = (0.1 * rng.standard_normal(i)).astype(np.float32)
+        ir = (1.0 + rng.random(i)).astype(np.float32)
+        for act, norm in ((0, True), (1, False)):
+            out = torch.full((e, o), float("nan"), dtype=torch.float32, device="cuda")
+            if norm:
+                mr.native.policy_dense_layer(dev(wr), dev(br), dev(xr), out, act, dev(mr_), dev(ir), 1.5)
+                ref = ol.policy_dense(wr, br, xr, o, act, mr_, ir, 1.5)
+            else:
+                mr.native.policy_dense_layer(dev(wr), dev(br), dev(xr), out, act)
+                ref = ol.policy_dense(wr, br, xr, o, act)
+            assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32)), (e, i, o, act)
     # sample and score
     A = 29
     states = np.zeros(E, mr.TASK_STATE_DTYPE)
