@@ -180,6 +180,14 @@ def summarize(run, steps, tol):
     return out
 
 
+_STARTED = time.perf_counter()
+
+
+def progress(leg):
+    """One stderr line per finished leg, so a long run shows where it is."""
+    print(f"bench: {leg} done at {time.perf_counter() - _STARTED:.1f} s", file=sys.stderr, flush=True)
+
+
 def warm_start_leg(mr, args):
     """Learned warm start on the chain-and-boxes workload: capture (features, converged impulse)
     samples from the plain adaptive solve, train the MLP with the library's own kernels, then run
@@ -533,7 +541,7 @@ def main():
 
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world_size != 1:
-        print("bench.py measures one GPU only (--gpus 1): this build runs on a single GPU. Worlds are "
+        print("bench.py measures one GPU only (--gpus 1): this library runs on a single GPU. Worlds are "
               "independent, so a multi-GPU run would shard them with no exchange step.", file=sys.stderr)
         sys.exit(2)
 
@@ -698,16 +706,15 @@ def main():
         del src, dst
     world.close()
     del commands
-    if not args.skip_heterogeneous:
-        out["heterogeneous"] = heterogeneous_leg(mr, args)
-    if not args.skip_server:
-        out["server"] = server_leg(mr, args)
-    if not args.skip_warm_start:
-        out["warm_start"] = warm_start_leg(mr, args)
-    if not args.skip_task:
-        out["task"] = task_leg(mr, args)
-    if not args.skip_ccd:
-        out["ccd"] = ccd_leg(mr, args)
+    progress("headline")
+    for name, skip, leg in (("heterogeneous", args.skip_heterogeneous, heterogeneous_leg),
+                            ("server", args.skip_server, server_leg),
+                            ("warm_start", args.skip_warm_start, warm_start_leg),
+                            ("task", args.skip_task, task_leg),
+                            ("ccd", args.skip_ccd, ccd_leg)):
+        if not skip:
+            out[name] = leg(mr, args)
+            progress(name)
     if not args.skip_cpu_baseline:
         threads, how = host_cores()
         if args.cpu_threads:
@@ -715,6 +722,7 @@ def main():
         out["cpu_baseline"] = cpu_baseline(kind, args.seed, args.cpu_envs, args.cpu_preroll, args.cpu_steps,
                                            threads, lambda m: headline_config(m, args))
         out["cpu_baseline"].update(how)
+        progress("cpu_baseline")
     print(json.dumps(out), flush=True)
 
 

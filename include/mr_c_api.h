@@ -243,7 +243,7 @@ int mr_world_read_mlp_samples(const MRWorldHandle* world, float* out, size_t cou
  * reference's (include/metalrobo/c_api.h:55-235), so a caller compiled
  * against that header passes and receives the same bytes here; sizes and
  * offsets are asserted at the end of this section. Fields that have no
- * meaning for this build's physics-only synthetic sources must be zero (or
+ * meaning for this library's physics-only synthetic sources must be zero (or
  * null) and are rejected otherwise; output fields with no counterpart here
  * are written as zero. */
 
@@ -261,7 +261,7 @@ typedef struct MRTaskRolloutDynamicSphereC {
 /* c_api.h:62-88. Read here: environment_count, physics_substeps,
  * velocity_iterations, final_velocity_iterations, control_timestep_seconds
  * (0 keeps each default), seed. materialize_articulated_contact_responses
- * and disable_task_terminations are accepted with any value (this build
+ * and disable_task_terminations are accepted with any value (this library
  * always materializes the response columns and has no task terminations);
  * every other field must be zero. */
 typedef struct MRTaskRolloutConfigC {
@@ -298,7 +298,7 @@ typedef struct MRTaskVisualObservationConfigC MRTaskVisualObservationConfigC;
  * into immutable (model tables), persistent state (q, v, scene bodies,
  * manifolds) and transient (per-step scratch, commands, statuses) bytes,
  * and the GPU / submission milliseconds. Observation, motion-feature,
- * episode and the other fingerprints are zero: this build runs no task
+ * episode and the other fingerprints are zero: this library runs no task
  * program. */
 typedef struct MRTaskRolloutLayoutC {
     uint32_t environment_count;
@@ -333,7 +333,7 @@ typedef struct MRTaskRolloutLayoutC {
 } MRTaskRolloutLayoutC;
 
 /* c_api.h:160-166 robot-factory sources (not built here: creating a
- * rollout from them fails with an error), plus this build's synthetic
+ * rollout from them fails with an error), plus this library's synthetic
  * physics worlds. */
 enum MRRunManifestSourceC {
     MR_RUN_SOURCE_UNITREE_G1 = 0u,
@@ -382,7 +382,7 @@ typedef struct MRRunManifestC {
  * per-control-step records: candidate_pairs (active broadphase pairs),
  * raw_contacts, manifolds, constraint_blocks (contacts), constraint_rows
  * (3 per contact), islands, articulation_point_queries (2 per contact)
- * and operator_velocity_elements (nv). Stages this build does not have
+ * and operator_velocity_elements (nv). Stages this library does not have
  * (convex, mesh, CCD, rods, quality solver, tiles) and the island
  * reference counts stay zero. */
 typedef struct MRTaskRolloutStageHighWaterC {

@@ -232,7 +232,7 @@ int mr_locomotion_task_read_policy_outputs(MRLocomotionTaskHandle* task, float* 
 /* mr_task_rollout_write_policy_rollout_pack (c_api.h:747-755): publishes one
  * aggregated batch with the dimensions of the task and the installed policy.
  * Each MRTaskTransitionC becomes an MRLearningTransitionC as the reference
- * maps it (c_api.cpp:4420-4447). The fingerprints are this build's: the task
+ * maps it (c_api.cpp:4420-4447). The fingerprints are this library's: the task
  * program's content hash over its records and the policy's pack content
  * hash. The task defines no outcome channels. Requires an installed policy. */
 int mr_locomotion_task_write_policy_rollout_pack(const MRLocomotionTaskHandle* task,

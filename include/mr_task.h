@@ -10,7 +10,7 @@
  * (src/metal/PolicyInference.metal:78,172) evaluate the policy network. This
  * header carries the records those kernels read, laid out byte for byte as
  * include/metalrobo/task_program_types.h:445-607, and the entry points of
- * this build's HIP kernels for the locomotion subset of the task program:
+ * this library's HIP kernels for the locomotion subset of the task program:
  *   observations   root angular velocity (local), projected gravity, command,
  *                  joint position error, joint velocity, previous action,
  *                  root linear velocity (local), root height

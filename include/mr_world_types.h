@@ -232,7 +232,7 @@ typedef struct MRShapeC {
 } MRShapeC;
 
 /* Cooked geometry of convex and mesh shapes (MRGeometryHeaderGPU,
- * engine_types.h:1360-1383, in this build's own compact form). Vertices are
+ * engine_types.h:1360-1383, restated in a compact layout). Vertices are
  * local, before the shape's scale. The model compiler fills the bounds and a
  * mesh's BVH4 (node_offset, node_count); callers leave them zero. */
 #define MR_MAX_CONVEX_VERTICES 64u

@@ -509,7 +509,7 @@ MRTaskRolloutHandle* mr_create_task_rollout(const MRRunManifestC* manifest) {
         require(manifest->surface == surface,
                 "manifest.surface must name the synthetic source's own ground (TERRAIN for the humanoid, "
                 "GROUND otherwise)");
-        require(manifest->task == 0u, "manifest.task must be 0: this build runs no task program");
+        require(manifest->task == 0u, "manifest.task must be 0: this library runs no task program");
         require(manifest->urdf_path == nullptr && manifest->srdf_path == nullptr &&
                     manifest->world_pack_path == nullptr && manifest->task_pack_path == nullptr &&
                     manifest->robot_actuator_pack_path == nullptr && manifest->sensor_program_pack_path == nullptr &&
@@ -517,9 +517,9 @@ MRTaskRolloutHandle* mr_create_task_rollout(const MRRunManifestC* manifest) {
                     manifest->teacher_clip_id == nullptr,
                 "manifest pack paths must be null for a synthetic source");
         require(manifest->visual_sensor_program == nullptr && manifest->inspection_visual_program == nullptr,
-                "visual programs are not supported by this build");
+                "visual programs are not supported by this library");
         require(p.dynamic_spheres == nullptr && p.dynamic_sphere_count == 0u,
-                "profile.dynamic_spheres are not supported by this build");
+                "profile.dynamic_spheres are not supported by this library");
         require(p.interaction_reference_mode == 0u && p.interaction_student_authority == 0.0f &&
                     p.override_interaction_student_authority == 0u && p.interaction_reset_phase_fraction == 0.0f &&
                     p.override_interaction_reset_phase_fraction == 0u && p.override_difficulty_band_range == 0u &&
@@ -527,7 +527,7 @@ MRTaskRolloutHandle* mr_create_task_rollout(const MRRunManifestC* manifest) {
                     p.interaction_reset_phase_probability == 0.0f &&
                     p.override_interaction_reset_phase_probability == 0u &&
                     p.interaction_reset_maximum_phase == 0.0f && p.override_interaction_reset_maximum_phase == 0u,
-                "profile interaction and difficulty-band fields must be zero: this build runs no task program");
+                "profile interaction and difficulty-band fields must be zero: this library runs no task program");
         auto* h = new MRTaskRolloutHandle();
         try {
             h->model = mr::build_synthetic(kind, p.environment_count, p.seed);
@@ -591,7 +591,7 @@ int mr_task_rollout_advance(MRTaskRolloutHandle* handle, const float* normalized
         const auto t0 = std::chrono::steady_clock::now();
         require(handle != nullptr, "null handle");
         require(control_step_count > 0u, "task-rollout control_step_count must be positive.");
-        require(evaluate_final_policy == 0u, "this build has no policy program to evaluate");
+        require(evaluate_final_policy == 0u, "this library has no policy program to evaluate");
         mr::World& w = *handle->world;
         const mr::ModelData& m = handle->model;
         const size_t E = w.environment_count(), A = handle->action_count, nv = m.max_nv;

@@ -124,7 +124,7 @@ class WorldLayoutC(ct.Structure):
 
 
 class TaskRolloutDynamicSphereC(ct.Structure):
-    """c_api.h:55-61 (the reference's layout; not supported by this build's sources)."""
+    """c_api.h:55-61 (the reference's layout; not supported by this library's sources)."""
     _fields_ = [("position", ct.c_float * 3), ("linear_velocity", ct.c_float * 3), ("radius", ct.c_float),
                 ("mass", ct.c_float), ("launch_step", ct.c_uint32)]
 

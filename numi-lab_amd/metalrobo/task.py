@@ -5,7 +5,7 @@ humanoid_locomotion_program() builds the velocity-tracking program of the
 (actions = joint-position targets around the nominal pose; actor frame =
 root angular velocity, projected gravity, command, joint position error,
 joint velocity and previous action; tracking, regularization and
-termination operators with rate weights), on this build's synthetic model.
+termination operators with rate weights), on this library's synthetic model.
 """
 
 from __future__ import annotations

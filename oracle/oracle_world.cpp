@@ -1231,7 +1231,7 @@ void heightfield_point(const StepContext& ctx, RawBatch& out, bool hf_is_a, V3 x
 // ---- convex and mesh narrowphase (GJK, MPR, support-mapped patches, BVH4)
 //
 // Role of Collision.metal:2298-3480 (GJK, EPA/MPR, support patches) and
-// :4097-4400 (BVH4 mesh traversal), in this build's own form. A finite
+// :4097-4400 (BVH4 mesh traversal), restated independently. A finite
 // collider is reduced to a core: a point set searched by scan for support
 // points plus a radius (sphere: centre, radius r; capsule: its endpoints,
 // radius r; box: its 8 vertices; convex: its scaled, transformed vertices;
