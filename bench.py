@@ -307,7 +307,8 @@ STATUS_NAMES = {1: "fixed_budget_complete", 2: "nonfinite_input", 3: "nonfinite_
 
 def task_leg(mr, args):
     """The task layer either side of the step, driven by a device policy: 4096 humanoid worlds run
-    observe -> dense-layer policy (actor 256-128 ELU, critic of the same shape, stochastic head) ->
+    observe -> dense-layer policy (actor 256-128 ELU, stochastic head; the pack's critic of the same
+    shape runs only when rollouts are recorded, which this leg does not) ->
     action sampling and scoring -> apply_actions -> control step -> complete, all on the world's
     stream. The synthetic policy has random weights of that shape. The same worlds stepped with
     plain PD targets and no task give the layer's cost."""
@@ -360,7 +361,8 @@ def task_leg(mr, args):
     run = timed_run(plain, commands, W, K, E)
     plain.close()
     return {"workload": "humanoid29_heightfield locomotion task with a device policy", "worlds": E, "steps": K,
-            "policy": {"actor": [O, 256, 128, A], "critic": [O, 256, 128, 1], "activation": "elu",
+            "policy": {"actor": [O, 256, 128, A], "critic": [O, 256, 128, 1], "critic_evaluated": False,
+                       "activation": "elu",
                        "stochastic": True, "weights": "random (synthetic)"},
             "env_steps_per_s": round(E * K / dt, 1), "ms_per_step": round(1000.0 * dt / K, 4),
             "world_only_env_steps_per_s": round(run["env_steps_per_s"], 1),
