@@ -8,8 +8,9 @@
 //   mr_policy_dense_tile_kernel     src/metal/PolicyInference.metal:78-170
 //   mr_policy_sample_kernel         src/metal/PolicyInference.metal:172-268
 //
-// The task kernels run one thread per environment (a few hundred FLOPs per
-// environment each, beside the physics step's millions); the dense layer
+// apply_actions and complete run one thread per environment (a few hundred
+// FLOPs per environment each, beside the physics step's millions); observe
+// runs one wavefront per environment over its operators; the dense layer
 // runs register-tiled over environments x neurons with LDS-staged operands.
 
 #include "mr_device_math.h"
@@ -406,13 +407,18 @@ extern "C" __global__ void __launch_bounds__(64) mr_task_complete_kernel(mr::Tas
 // started) episodes from the reset state, raising the reset mask of the next
 // physics step, then write the actor frame into the observation history
 // (newest frame last; a new episode fills every frame).
+// One wavefront per environment, lanes over the observation operators: every
+// lane derives the environment's state, root and normalised gravity the same
+// way, lane 0 writes the records, and each lane writes its own columns of the
+// history (coalesced across the wavefront).
 extern "C" __global__ void __launch_bounds__(64) mr_task_observe_kernel(mr::TaskArgs a) {
-    const uint32_t env = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t env = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
     if (env >= a.environments) return;
     const uint32_t A = a.action_count, O = a.observation_count, H = a.history_length;
     MRTaskStateC st = a.states[env];
     const bool reset = st.status[0] == 0u || st.status[1] != 0u;
-    a.reset_mask[env] = reset ? 1u : 0u;
+    if (lane == 0u) a.reset_mask[env] = reset ? 1u : 0u;
     const float* q = (reset ? a.reset_q : a.q) + (size_t)env * a.max_nq;
     const float* v = (reset ? a.reset_v : a.v) + (size_t)env * a.max_nv;
     float* prev = a.previous_joint + (size_t)env * 2u * A;
@@ -443,11 +449,11 @@ extern "C" __global__ void __launch_bounds__(64) mr_task_observe_kernel(mr::Task
         st.air_return_tracking[1] = r0.position.z;
         st.threat_metadata[0] = MR_INVALID_INDEX;
         st.threat_metadata[3] = MR_INVALID_INDEX;
-        for (uint32_t k = 0; k < A; ++k) {
+        for (uint32_t k = lane; k < A; k += 64u) {
             prev[k] = 0.0f;
             prev[A + k] = q[a.actions[k].indices[2]];
         }
-        a.states[env] = st;
+        if (lane == 0u) a.states[env] = st;
     }
     const Root r = root_of(a, q, v);
     const float* previous_action = a.action_history + ((size_t)env * a.delay_slots + (a.delay_slots - 2u)) * A;
@@ -469,7 +475,7 @@ extern "C" __global__ void __launch_bounds__(64) mr_task_observe_kernel(mr::Task
     }
     if (have_gn) gn = normalized_or(gn, V3{0.0f, 0.0f, -1.0f});
     float* hist = a.observations_out + (size_t)env * H * O;
-    for (uint32_t i = 0; i < O; ++i) {
+    for (uint32_t i = lane; i < O; i += 64u) {
         const MRTaskObservationOperatorC& op = a.observations[i];
         float out;
         if (op.source[0] == MR_TASK_OBSERVE_PROJECTED_GRAVITY &&
@@ -567,7 +573,7 @@ mr_policy_dense_tile_kernel(const float* weights, const float* bias, const float
             ws[k][n] = (n0 + n < out_width && k < span) ? weights[(size_t)(n0 + n) * in_width + f0 + k] : 0.0f;
         }
         __syncthreads();
-        for (uint32_t k = 0; k < span; ++k) {
+        const auto feature = [&](uint32_t k) {
             const float4 xv = *reinterpret_cast<const float4*>(&xs[k][te * 4u]);
             const float4 wv = *reinterpret_cast<const float4*>(&ws[k][tn * 4u]);
             const float xa[4] = {xv.x, xv.y, xv.z, xv.w};
@@ -576,6 +582,13 @@ mr_policy_dense_tile_kernel(const float* weights, const float* bias, const float
             for (uint32_t i = 0; i < 4u; ++i)
 #pragma unroll
                 for (uint32_t j = 0; j < 4u; ++j) acc[i][j] = fmaf(wa[j], xa[i], acc[i][j]);
+        };
+        if (span == kDenseChunk) {
+            // full chunk: unrolled, so the LDS reads issue ahead of the FMAs
+#pragma unroll
+            for (uint32_t k = 0; k < kDenseChunk; ++k) feature(k);
+        } else {
+            for (uint32_t k = 0; k < span; ++k) feature(k);
         }
     }
 #pragma unroll
@@ -636,7 +649,8 @@ hipError_t launch_task_complete(const TaskArgs& a, const MRWorldStatusC* status,
 }
 
 hipError_t launch_task_observe(const TaskArgs& a, hipStream_t stream) {
-    hipLaunchKernelGGL(mr_task_observe_kernel, dim3((a.environments + 63u) / 64u), dim3(64), 0, stream, a);
+    if (a.environments == 0u) return hipSuccess;
+    hipLaunchKernelGGL(mr_task_observe_kernel, dim3(a.environments), dim3(64), 0, stream, a);
     return hipGetLastError();
 }
 
