@@ -8,9 +8,9 @@
 //   mr_policy_dense_tile_kernel     src/metal/PolicyInference.metal:78-170
 //   mr_policy_sample_kernel         src/metal/PolicyInference.metal:172-268
 //
-// apply_actions and complete run one thread per environment (a few hundred
-// FLOPs per environment each, beside the physics step's millions); observe
-// runs one wavefront per environment over its operators; the dense layer
+// complete runs one thread per environment (a few hundred FLOPs per
+// environment, beside the physics step's millions); apply_actions and observe
+// run one wavefront per environment over its actions and operators; the dense layer
 // runs register-tiled over environments x neurons with LDS-staged operands.
 
 #include "mr_device_math.h"
@@ -182,32 +182,43 @@ __device__ __forceinline__ float random_signed(const mr::TaskArgs& a, uint32_t e
 // host evaluates once), and write the joint-position
 // target default_q + scale * filtered, clamped to the binding's bounds, into
 // the implicit-drive command [E][max_nv] (every other coordinate 0).
+// One wavefront per environment. Lanes over the actions shift the delay line
+// and filter the response; after the barrier, lanes over the command's
+// coordinates write each coordinate once: the last position-type action bound
+// to it (action order), else 0.
 extern "C" __global__ void __launch_bounds__(64) mr_task_apply_actions_kernel(mr::TaskArgs a, const float* actions) {
-    const uint32_t env = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t env = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
     if (env >= a.environments) return;
     const uint32_t A = a.action_count, slots = a.delay_slots;
     float* hist = a.action_history + (size_t)env * slots * A;
     float* cmd = a.targets + (size_t)env * a.max_nv;
-    for (uint32_t j = 0; j < a.max_nv; ++j) cmd[j] = 0.0f;
     const MRTaskStateC& st = a.states[env];
     const uint32_t filter_slot = slots - 1u;
     const uint32_t raw_last = filter_slot - 1u;
     const uint32_t delay = st.schedule[2] < raw_last ? st.schedule[2] : raw_last;
     const uint32_t selected = raw_last - delay;
-    for (uint32_t k = 0; k < A; ++k) {
+    for (uint32_t k = lane; k < A; k += 64u) {
         for (uint32_t d = 0; d < raw_last; ++d) hist[d * A + k] = hist[(d + 1u) * A + k];
         const float requested = actions[(size_t)env * A + k];
         const float previous = hist[filter_slot * A + k];
-        const MRTaskActionBindingC& b = a.actions[k];
         const float fraction = a.response_fraction[k];
         hist[raw_last * A + k] = requested;
         const float delayed = hist[selected * A + k];
         hist[filter_slot * A + k] = previous + (delayed - previous) * fraction;
-        const float filtered = hist[filter_slot * A + k];
-        if (b.actuator[0] != MR_TASK_ACTUATOR_JOINT_POSITION && b.actuator[0] != MR_TASK_ACTUATOR_GRIPPER_POSITION)
-            continue;
-        const float target = a.default_q[b.indices[2]] + b.parameters[0] * filtered;
-        cmd[b.indices[3]] = fminf(fmaxf(target, b.parameters[1]), b.parameters[2]);
+    }
+    __syncthreads();
+    for (uint32_t j = lane; j < a.max_nv; j += 64u) {
+        float value = 0.0f;
+        for (uint32_t k = 0; k < A; ++k) {
+            const MRTaskActionBindingC& b = a.actions[k];
+            if (b.indices[3] != j ||
+                (b.actuator[0] != MR_TASK_ACTUATOR_JOINT_POSITION && b.actuator[0] != MR_TASK_ACTUATOR_GRIPPER_POSITION))
+                continue;
+            const float target = a.default_q[b.indices[2]] + b.parameters[0] * hist[filter_slot * A + k];
+            value = fminf(fmaxf(target, b.parameters[1]), b.parameters[2]);
+        }
+        cmd[j] = value;
     }
 }
 
@@ -638,8 +649,8 @@ mr_policy_sample_kernel(const float* mean, uint32_t mean_stride, const float* ac
 namespace mr {
 
 hipError_t launch_task_apply_actions(const TaskArgs& a, const float* actions, hipStream_t stream) {
-    hipLaunchKernelGGL(mr_task_apply_actions_kernel, dim3((a.environments + 63u) / 64u), dim3(64), 0, stream, a,
-                       actions);
+    if (a.environments == 0u) return hipSuccess;
+    hipLaunchKernelGGL(mr_task_apply_actions_kernel, dim3(a.environments), dim3(64), 0, stream, a, actions);
     return hipGetLastError();
 }
 
